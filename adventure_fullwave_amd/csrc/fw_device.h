@@ -1,0 +1,94 @@
+// fw_device.h - device helpers shared by the libfwhip kernels (gfx950).
+//
+// Complex FP64 values are double2 (re, im).  Products follow the reference's
+// ComplexMulti (lib/std/complex_adv.c:103-114): re = ac - bd, im = bc + ad.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fw {
+
+__device__ __forceinline__ double2 cz() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// acc + a*b
+__device__ __forceinline__ double2 cfma(double2 acc, double2 a, double2 b) {
+  return make_double2(acc.x + (a.x * b.x - a.y * b.y), acc.y + (a.y * b.x + a.x * b.y));
+}
+// acc - a*b
+__device__ __forceinline__ double2 cfms(double2 acc, double2 a, double2 b) {
+  return make_double2(acc.x - (a.x * b.x - a.y * b.y), acc.y - (a.y * b.x + a.x * b.y));
+}
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ double cabsv(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
+__device__ __forceinline__ bool ciszero(double2 a) { return a.x == 0.0 && a.y == 0.0; }
+
+// naive reciprocal of the reference's ComplexReciprocal
+// (lib/std/complex_adv.c:39-51): scalar interface updates use it.
+__device__ __forceinline__ double2 crecip_naive(double2 a) {
+  double d = 1.0 / (a.x * a.x + a.y * a.y);
+  return make_double2(a.x * d, -(a.y * d));
+}
+__device__ __forceinline__ double2 cdiv_naive(double2 a, double2 b) { return cmul(a, crecip_naive(b)); }
+
+// 1/d by R. L. Smith's algorithm (Comm. ACM 5(8), 1962): scaled by the
+// larger component, so it neither overflows nor underflows for pivots of
+// any magnitude.  Used for the skyline pivots, where the reference relies on
+// the C99 complex division of its host compiler.
+__device__ __forceinline__ double2 crecip_smith(double2 d) {
+  if (fabs(d.x) >= fabs(d.y)) {
+    double r = d.y / d.x;
+    double den = d.x + d.y * r;
+    return make_double2(1.0 / den, -r / den);
+  } else {
+    double r = d.x / d.y;
+    double den = d.y + d.x * r;
+    return make_double2(r / den, -1.0 / den);
+  }
+}
+
+// ---- wavefront (64-lane) reductions --------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double2 wave_csum(double2 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, 64);
+    v.y += __shfl_xor(v.y, o, 64);
+  }
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+// sum over aligned groups of G lanes (G power of two <= 64)
+template <int G>
+__device__ __forceinline__ double2 group_csum(double2 v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, G);
+    v.y += __shfl_xor(v.y, o, G);
+  }
+  return v;
+}
+
+// broadcast lane `lane` (wave-uniform) of a double through the scalar unit
+__device__ __forceinline__ double readlane_d(double x, int lane) {
+  long long b = __double_as_longlong(x);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double2 readlane_c(double2 x, int lane) {
+  return make_double2(readlane_d(x.x, lane), readlane_d(x.y, lane));
+}
+
+}  // namespace fw

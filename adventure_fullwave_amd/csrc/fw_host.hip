@@ -1,0 +1,1041 @@
+// fw_host.hip - host side of libfwhip: model setup, device buffers and the
+// C ABI declared in include/fwhip.h.
+//
+// Setup turns the reference's per-subdomain data model (fwh_mesh_desc) into
+// device-resident structures:
+//   * lower-triangle CSR pattern of every subdomain matrix and, for every
+//     entry, the ordered list of element contributions (so that assembly on
+//     the device adds them in the reference's element order, SetAe
+//     aij_matrix_func.h:260-323),
+//   * Dirichlet flags, the interior-interface couplings (A_IG) and the
+//     interface rows (A_G.) used by the Schur step,
+//   * the flat global interface numbering (part-major, owned unknowns in
+//     ln_part order) and the slot -> unknown gather lists, ordered as the
+//     reference accumulates (COM_res_sp, then the part gather),
+//   * skyline profiles, their row blocks, and the factors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fwhip.h"
+#include "fw_kernels.h"
+
+using fw::SubMeta;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(FWH_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
+                  __FILE__, __LINE__);                                                \
+  } while (0)
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct fwh_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::vector<void *> allocs;
+  bool loaded = false, first_done = false, last_done = false;
+
+  // sizes
+  int nsub = 0, npart = 0, ng = 0, nmax = 0, ngmax = 0, wmax = 0, G = 1;
+  long long ndof = 0, nslots = 0, nnz = 0, nlen = 0, nel = 0, ncpl = 0, ngrow = 0;
+  std::vector<SubMeta> meta_h;
+  std::vector<int> order_h;
+
+  // device model
+  SubMeta *meta = nullptr;
+  int *order = nullptr, *dof_sub = nullptr;
+  int *idx1 = nullptr, *rowbeg = nullptr, *blk_row = nullptr;
+  double2 *fac = nullptr, *dinv = nullptr;
+  int8_t *bcflag = nullptr;
+  int *kidx = nullptr;
+  double2 *f = nullptr;
+  int *rowptr = nullptr, *col = nullptr, *ent_row = nullptr;
+  double2 *aval = nullptr;
+  int *cp_ptr = nullptr, *cp_k = nullptr;
+  double2 *cp_val = nullptr;
+  int *gr_ptr = nullptr, *gr_col = nullptr;
+  double2 *gr_val = nullptr;
+  int *g_nfid = nullptr, *g_gid = nullptr;
+  int *gq_ptr = nullptr, *gq_slot = nullptr;
+
+  // vectors
+  double2 *P = nullptr, *Q = nullptr, *R = nullptr, *Z = nullptr, *U = nullptr, *PC = nullptr;
+  double2 *yslot = nullptr, *pcslot = nullptr, *X = nullptr, *part = nullptr;
+  fw::Scal *sc = nullptr;
+  double *hist = nullptr;
+  int hist_cap = 0;
+
+  // graph of batched iterations
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_n = 0;
+
+  fwh_stats stats{};
+  fwh_info info{};
+
+  template <class T>
+  int alloc(T **p, size_t count) {
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(FWH_E_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = (T *)q;
+    return 0;
+  }
+  template <class T>
+  int upload(T **p, const std::vector<T> &h) {
+    int rc = alloc(p, h.size());
+    if (rc) return rc;
+    if (!h.empty()) HIPCHK(hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return 0;
+  }
+  void release_graph() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    graph_n = 0;
+  }
+  void release() {
+    release_graph();
+    for (void *p : allocs) hipFree(p);
+    allocs.clear();
+  }
+  fw::SolveArgs solve_args(int mode, bool check_done) const {
+    fw::SolveArgs a{};
+    a.meta = meta;
+    a.order = order;
+    a.base = 0;
+    a.mode = mode;
+    a.done = check_done ? &sc->done : nullptr;
+    a.nmax = nmax;
+    a.ngmax = ngmax;
+    a.idx1 = idx1;
+    a.rowbeg = rowbeg;
+    a.fac = fac;
+    a.dinv = dinv;
+    a.bcflag = bcflag;
+    a.kidx = kidx;
+    a.f = f;
+    a.cp_ptr = cp_ptr;
+    a.cp_k = cp_k;
+    a.cp_val = cp_val;
+    a.gr_ptr = gr_ptr;
+    a.gr_col = gr_col;
+    a.gr_val = gr_val;
+    a.g_nfid = g_nfid;
+    a.g_gid = g_gid;
+    a.blk_row = blk_row;
+    a.P = P;
+    a.yslot = yslot;
+    a.X = X;
+    return a;
+  }
+  fw::VecArgs vec_args() const {
+    fw::VecArgs v{};
+    v.ng = ng;
+    v.gq_ptr = gq_ptr;
+    v.gq_slot = gq_slot;
+    v.yslot = yslot;
+    v.pcslot = pcslot;
+    v.P = P;
+    v.Q = Q;
+    v.R = R;
+    v.Z = Z;
+    v.U = U;
+    v.PC = PC;
+    v.part = part;
+    v.sc = sc;
+    v.hist = hist;
+    v.G = G;
+    return v;
+  }
+  // all subdomains: one launch (batched, size-ordered) or one launch per
+  // subdomain in the reference's loop order (plain)
+  hipError_t run_solve(int mode, bool plain, bool check_done) {
+    fw::SolveArgs a = solve_args(mode, check_done);
+    if (!plain) return fw::launch_solve(a, nsub, st);
+    a.order = nullptr;
+    for (int s = 0; s < nsub; s++) {
+      a.base = s;
+      a.nmax = meta_h[s].n;
+      a.ngmax = meta_h[s].ng;
+      hipError_t e = fw::launch_solve(a, 1, st);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+};
+
+extern "C" const char *fwh_last_error(void) { return g_err.c_str(); }
+
+extern "C" int fwh_create(fwh_ctx **out, int device) {
+  if (!out) return fail(FWH_E_ARG, "null ctx pointer");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(FWH_E_ARG, "device %d out of range (%d devices)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  fwh_ctx *c = new fwh_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(FWH_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" void fwh_destroy(fwh_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->st);
+  c->release();
+  (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+// ---------------------------------------------------------------------------
+// model load
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Key {
+  long long key;
+  int contrib;
+  bool operator<(const Key &o) const { return key < o.key || (key == o.key && contrib < o.contrib); }
+};
+
+int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
+  const int nsub = m->nsub, npart = m->npart;
+  if (nsub <= 0 || npart <= 0) return fail(FWH_E_ARG, "empty model (nsub=%d npart=%d)", nsub, npart);
+  if (!(m->omega > 0.0)) return fail(FWH_E_ARG, "omega must be positive");
+  for (int s = 0; s < nsub; s++) {
+    if (m->sub_part[s] < 0 || m->sub_part[s] >= npart) return fail(FWH_E_ARG, "subdomain %d: bad part", s);
+    if (s > 0 && m->sub_part[s] < m->sub_part[s - 1]) return fail(FWH_E_ARG, "subdomains must be part-major");
+    if (m->nmp[s] <= 0) return fail(FWH_E_ARG, "subdomain %d has no DOF (emessage 120)", s);
+  }
+  c->nsub = nsub;
+  c->npart = npart;
+  const long long nel = m->el_off[nsub];
+  c->nel = nel;
+
+  // ---- per subdomain offsets --------------------------------------------
+  std::vector<SubMeta> meta(nsub);
+  long long ndof = 0, nslots = 0;
+  int nmax = 0, ngmax = 0;
+  for (int s = 0; s < nsub; s++) {
+    SubMeta &M = meta[s];
+    memset(&M, 0, sizeof(M));
+    M.n = m->nmp[s];
+    M.ng = m->ib_off[s + 1] - m->ib_off[s];
+    M.dof0 = (int)ndof;
+    M.g0 = (int)nslots;
+    ndof += M.n;
+    nslots += M.ng;
+    nmax = std::max(nmax, M.n);
+    ngmax = std::max(ngmax, M.ng);
+  }
+  if (ndof > (1LL << 30) || nslots > (1LL << 30)) return fail(FWH_E_LIMIT, "model too large");
+  c->ndof = ndof;
+  c->nslots = nslots;
+  c->nmax = nmax;
+  c->ngmax = ngmax;
+  if ((size_t)(nmax + ngmax + fw::CAP) * 16 > 160 * 1024)
+    return fail(FWH_E_LIMIT, "subdomain of %d DOFs / %d interface DOFs exceeds the LDS budget", nmax, ngmax);
+
+  // ---- flags and interface slots ----------------------------------------
+  std::vector<int8_t> bcflag(ndof, 0);
+  std::vector<int> kidx(ndof, -1), dof_sub(ndof);
+  for (int s = 0; s < nsub; s++) {
+    const SubMeta &M = meta[s];
+    for (int i = 0; i < M.n; i++) dof_sub[M.dof0 + i] = s;
+    for (int k = m->bc_off[s]; k < m->bc_off[s + 1]; k++) {
+      int i = m->ibc[k];
+      if (i < 0 || i >= M.n) return fail(FWH_E_ARG, "subdomain %d: essential BC id %d out of range", s, i);
+      bcflag[M.dof0 + i] = 2;
+    }
+    for (int k = 0; k < M.ng; k++) {
+      int i = m->ib_nfid[m->ib_off[s] + k];
+      if (i < 0 || i >= M.n) return fail(FWH_E_ARG, "subdomain %d: interface id %d out of range", s, i);
+      if (bcflag[M.dof0 + i] != 0) return fail(FWH_E_ARG, "subdomain %d: edge %d listed twice as Dirichlet", s, i);
+      bcflag[M.dof0 + i] = 1;
+      kidx[M.dof0 + i] = k;
+    }
+  }
+
+  // ---- global interface numbering -----------------------------------------
+  std::vector<int> goff(npart + 1, 0);
+  for (int p = 0; p < npart; p++) goff[p + 1] = goff[p] + m->infree[p];
+  const int ng = goff[npart];
+  c->ng = ng;
+  std::vector<std::vector<int>> lnmap(npart);
+  for (int p = 0; p < npart; p++) {
+    const int tf = m->infree[p] + m->outfree[p];
+    lnmap[p].assign(tf, -1);
+    for (int ln = m->outfree[p]; ln < tf; ln++) lnmap[p][ln] = goff[p] + ln - m->outfree[p];
+  }
+  for (int p = 0; p < npart; p++)
+    for (int o = 0; o < p; o++) {
+      const int a0 = m->op_off[p * npart + o], a1 = m->op_off[p * npart + o + 1];
+      const int b0 = m->op_off[o * npart + p], b1 = m->op_off[o * npart + p + 1];
+      if (a1 - a0 != b1 - b0) return fail(FWH_E_ARG, "op lists (%d,%d) and (%d,%d) differ in length", p, o, o, p);
+      for (int j = 0; j < a1 - a0; j++) {
+        const int lp = m->op_ibid[a0 + j], lo = m->op_ibid[b0 + j];
+        if (lp < 0 || lp >= (int)lnmap[p].size() || lo < 0 || lo >= (int)lnmap[o].size())
+          return fail(FWH_E_ARG, "op list index out of range");
+        if (lnmap[p][lp] == -1 && lo >= m->outfree[o]) lnmap[p][lp] = lnmap[o][lo];
+      }
+    }
+  for (int p = 0; p < npart; p++)
+    for (size_t ln = 0; ln < lnmap[p].size(); ln++)
+      if (lnmap[p][ln] < 0) return fail(FWH_E_ARG, "part %d: interface DOF %zu has no owner", p, ln);
+  std::vector<int> g_gid(nslots), g_nfid(nslots);
+  std::vector<int> gcount(ng + 1, 0);
+  for (int s = 0; s < nsub; s++) {
+    const int p = m->sub_part[s];
+    for (int k = 0; k < meta[s].ng; k++) {
+      const int ln = m->ib_ln[m->ib_off[s] + k];
+      if (ln < 0 || ln >= (int)lnmap[p].size()) return fail(FWH_E_ARG, "subdomain %d: ln_part out of range", s);
+      const int g = lnmap[p][ln];
+      g_gid[meta[s].g0 + k] = g;
+      g_nfid[meta[s].g0 + k] = m->ib_nfid[m->ib_off[s] + k];
+      gcount[g + 1]++;
+    }
+  }
+  std::vector<int> gq_ptr(ng + 1, 0), gq_slot(nslots), gq_part(nslots);
+  for (int g = 0; g < ng; g++) gq_ptr[g + 1] = gq_ptr[g] + gcount[g + 1];
+  {
+    std::vector<int> fillp(gq_ptr.begin(), gq_ptr.end() - 1);
+    for (int s = 0; s < nsub; s++)
+      for (int k = 0; k < meta[s].ng; k++) {
+        const int sl = meta[s].g0 + k, g = g_gid[sl];
+        gq_part[fillp[g]] = m->sub_part[s];
+        gq_slot[fillp[g]++] = sl;
+      }
+    for (int g = 0; g < ng; g++) {
+      if (gq_ptr[g + 1] == gq_ptr[g]) return fail(FWH_E_ARG, "interface unknown %d has no subdomain", g);
+      for (int t = gq_ptr[g]; t < gq_ptr[g + 1]; t++)
+        if (t == gq_ptr[g] || gq_part[t] != gq_part[t - 1]) gq_slot[t] = ~gq_slot[t];
+    }
+  }
+
+  // ---- element data ---------------------------------------------------------
+  const long long nap = m->ap_off[nsub];
+  std::vector<int> el_vtx(4 * nel);
+  for (int s = 0; s < nsub; s++)
+    for (long long el = m->el_off[s]; el < m->el_off[s + 1]; el++)
+      for (int i = 0; i < 4; i++) {
+        const int lv = m->nop[10 * el + i];
+        if (lv < 0 || m->ap_off[s] + lv >= m->ap_off[s + 1]) return fail(FWH_E_ARG, "element %lld: vertex out of range", el);
+        el_vtx[4 * el + i] = m->ap_off[s] + lv;
+      }
+
+  // ---- symbolic lower-triangle pattern + assembly lists --------------------
+  std::vector<int> rowptr(ndof + 1, 0), col, ent_row, asm_ptr(1, 0), asm_src;
+  std::vector<int> f_ptr(ndof + 1, 0), f_src;
+  std::vector<int> diag_ent(ndof, -1);
+  static const int PA[21] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 5};
+  static const int PB[21] = {0, 1, 2, 3, 4, 5, 1, 2, 3, 4, 5, 2, 3, 4, 5, 3, 4, 5, 4, 5, 5};
+  {
+    std::vector<Key> keys, fkeys;
+    for (int s = 0; s < nsub; s++) {
+      const SubMeta &M = meta[s];
+      const long long n = M.n;
+      keys.clear();
+      fkeys.clear();
+      for (long long el = m->el_off[s]; el < m->el_off[s + 1]; el++) {
+        int id[6];
+        for (int k = 0; k < 6; k++) {
+          id[k] = m->nop[10 * el + 4 + k];
+          if (id[k] < 0 || id[k] >= M.n) return fail(FWH_E_ARG, "element %lld: edge id out of range", el);
+        }
+        for (int p = 0; p < 21; p++) {
+          const int ia = id[PA[p]], ib = id[PB[p]];
+          const int r = std::max(ia, ib), cc = std::min(ia, ib);
+          if (PA[p] != PB[p] && ia == ib) return fail(FWH_E_ARG, "element %lld repeats an edge", el);
+          keys.push_back({(long long)r * n + cc, (int)(27 * el + p)});
+        }
+        for (int k = 0; k < 6; k++) fkeys.push_back({(long long)id[k], (int)(27 * el + 21 + k)});
+      }
+      std::sort(keys.begin(), keys.end());
+      std::sort(fkeys.begin(), fkeys.end());
+      for (size_t t = 0; t < keys.size(); t++) {
+        if (t == 0 || keys[t].key != keys[t - 1].key) {
+          const int r = (int)(keys[t].key / n), cc = (int)(keys[t].key % n);
+          col.push_back(cc);
+          ent_row.push_back(M.dof0 + r);
+          rowptr[M.dof0 + r + 1]++;
+          asm_ptr.push_back(asm_ptr.back());
+          if (r == cc) diag_ent[M.dof0 + r] = (int)col.size() - 1;
+        }
+        asm_src.push_back(keys[t].contrib);
+        asm_ptr.back()++;
+      }
+      for (auto &k : fkeys) {
+        f_src.push_back(k.contrib);
+        f_ptr[M.dof0 + k.key + 1]++;
+      }
+      if ((long long)col.size() > (1LL << 31) - 64 || (long long)asm_src.size() > (1LL << 31) - 64)
+        return fail(FWH_E_LIMIT, "assembly lists exceed 32-bit indexing");
+    }
+    for (long long d = 0; d < ndof; d++) {
+      rowptr[d + 1] += rowptr[d];
+      f_ptr[d + 1] += f_ptr[d];
+    }
+  }
+  const long long nnz = (long long)col.size();
+  c->nnz = nnz;
+  for (int s = 0; s < nsub; s++) {
+    meta[s].nnz0 = rowptr[meta[s].dof0];
+    meta[s].nnz = rowptr[meta[s].dof0 + meta[s].n] - meta[s].nnz0;
+  }
+
+  // ---- couplings A_IG and interface rows A_G. (entry ids into aval) --------
+  std::vector<int> cp_ptr(ndof + 1, 0), cp_k, cp_ent, gr_ptr(nslots + 1, 0), gr_col, gr_ent;
+  {
+    // upper-part access: for every row d, entries (r, d) with r > d
+    std::vector<int> up_ptr(ndof + 1, 0), up_ent;
+    for (long long e = 0; e < nnz; e++) {
+      const int d = ent_row[e];
+      const int j = meta[dof_sub[d]].dof0 + col[e];
+      if (j != d) up_ptr[j + 1]++;
+    }
+    for (long long d = 0; d < ndof; d++) up_ptr[d + 1] += up_ptr[d];
+    up_ent.resize(up_ptr[ndof]);
+    {
+      std::vector<int> fp(up_ptr.begin(), up_ptr.end() - 1);
+      for (long long e = 0; e < nnz; e++) {
+        const int d = ent_row[e];
+        const int j = meta[dof_sub[d]].dof0 + col[e];
+        if (j != d) up_ent[fp[j]++] = (int)e;
+      }
+    }
+    std::vector<std::vector<std::pair<int, int>>> grow(nslots);
+    for (long long d = 0; d < ndof; d++) {
+      const int s = dof_sub[d];
+      const int dof0 = meta[s].dof0;
+      if (bcflag[d] == 0) {
+        for (int e = rowptr[d]; e < rowptr[d + 1]; e++) {
+          const int j = dof0 + col[e];
+          if (j != d && bcflag[j] == 1) {
+            cp_k.push_back(kidx[j]);
+            cp_ent.push_back(e);
+          }
+        }
+        for (int t = up_ptr[d]; t < up_ptr[d + 1]; t++) {
+          const int e = up_ent[t];
+          const int r = ent_row[e];
+          if (bcflag[r] == 1) {
+            cp_k.push_back(kidx[r]);
+            cp_ent.push_back(e);
+          }
+        }
+      } else if (bcflag[d] == 1) {
+        const int sl = meta[s].g0 + kidx[d];
+        for (int e = rowptr[d]; e < rowptr[d + 1]; e++) {
+          gr_col.push_back(col[e]);
+          gr_ent.push_back(e);
+          gr_ptr[sl + 1]++;
+        }
+        for (int t = up_ptr[d]; t < up_ptr[d + 1]; t++) {
+          const int e = up_ent[t];
+          gr_col.push_back(ent_row[e] - dof0);
+          gr_ent.push_back(e);
+          gr_ptr[sl + 1]++;
+        }
+      }
+      cp_ptr[d + 1] = (int)cp_k.size();
+    }
+    // gr entries were appended in DOF order; slots of one subdomain are in
+    // kidx order which need not follow DOF order: regroup by slot
+    std::vector<int> gp(nslots + 1, 0);
+    for (long long sl = 0; sl < nslots; sl++) gp[sl + 1] = gp[sl] + gr_ptr[sl + 1];
+    std::vector<int> gcol2(gr_col.size()), gent2(gr_ent.size());
+    {
+      size_t t = 0;
+      for (long long d = 0; d < ndof; d++) {
+        if (bcflag[d] != 1) continue;
+        const int sl = meta[dof_sub[d]].g0 + kidx[d];
+        const int cnt = gr_ptr[sl + 1];
+        for (int q = 0; q < cnt; q++, t++) {
+          gcol2[gp[sl] + q] = gr_col[t];
+          gent2[gp[sl] + q] = gr_ent[t];
+        }
+      }
+    }
+    gr_ptr = gp;
+    gr_col.swap(gcol2);
+    gr_ent.swap(gent2);
+  }
+  c->ncpl = (long long)cp_k.size();
+  c->ngrow = (long long)gr_col.size();
+  std::vector<int> diag_slot(nslots);
+  for (long long d = 0; d < ndof; d++)
+    if (bcflag[d] == 1) diag_slot[meta[dof_sub[d]].g0 + kidx[d]] = diag_ent[d];
+
+  // ---- uploads + assembly on the device -------------------------------------------
+  hipStream_t st = c->st;
+  int rc = 0;
+  int *d_el_vtx, *d_src, *d_asm_ptr, *d_asm_src, *d_f_ptr, *d_f_src, *d_cp_ent, *d_gr_ent, *d_diag_slot, *d_bad;
+  double *d_crd, *d_joe, *d_eps, *d_sig, *d_pml = nullptr;
+  double2 *d_elem;
+  std::vector<double> crd(m->crd, m->crd + 3 * nap), joe(6 * nap, 0.0);
+  if (m->joe) std::copy(m->joe, m->joe + 6 * nap, joe.begin());
+  std::vector<double> eps(m->eps, m->eps + nel), sig(m->sigma, m->sigma + nel);
+  std::vector<int> src(nel, 0);
+  if (m->src) std::copy(m->src, m->src + nel, src.begin());
+  if ((rc = c->upload(&d_el_vtx, el_vtx)) || (rc = c->upload(&d_crd, crd)) || (rc = c->upload(&d_joe, joe)) ||
+      (rc = c->upload(&d_eps, eps)) || (rc = c->upload(&d_sig, sig)) || (rc = c->upload(&d_src, src)))
+    return rc;
+  if (m->pml) {
+    std::vector<double> pml(m->pml, m->pml + 6 * nel);
+    if ((rc = c->upload(&d_pml, pml))) return rc;
+  }
+  if ((rc = c->alloc(&d_elem, 27 * (size_t)nel)) || (rc = c->alloc(&d_bad, 1))) return rc;
+  HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(int), st));
+  if ((rc = c->upload(&d_asm_ptr, asm_ptr)) || (rc = c->upload(&d_asm_src, asm_src)) ||
+      (rc = c->upload(&d_f_ptr, f_ptr)) || (rc = c->upload(&d_f_src, f_src)) ||
+      (rc = c->upload(&c->rowptr, rowptr)) || (rc = c->upload(&c->col, col)) ||
+      (rc = c->upload(&c->ent_row, ent_row)) || (rc = c->alloc(&c->aval, nnz)) || (rc = c->alloc(&c->f, ndof)) ||
+      (rc = c->upload(&c->bcflag, bcflag)) || (rc = c->upload(&c->kidx, kidx)) ||
+      (rc = c->upload(&c->dof_sub, dof_sub)) || (rc = c->upload(&c->cp_ptr, cp_ptr)) ||
+      (rc = c->upload(&c->cp_k, cp_k)) || (rc = c->upload(&d_cp_ent, cp_ent)) ||
+      (rc = c->alloc(&c->cp_val, cp_k.size())) || (rc = c->upload(&c->gr_ptr, gr_ptr)) ||
+      (rc = c->upload(&c->gr_col, gr_col)) || (rc = c->upload(&d_gr_ent, gr_ent)) ||
+      (rc = c->alloc(&c->gr_val, gr_col.size())) || (rc = c->upload(&c->g_nfid, g_nfid)) ||
+      (rc = c->upload(&c->g_gid, g_gid)) || (rc = c->upload(&c->gq_ptr, gq_ptr)) ||
+      (rc = c->upload(&c->gq_slot, gq_slot)) || (rc = c->upload(&d_diag_slot, diag_slot)))
+    return rc;
+
+  HIPCHK(hipStreamSynchronize(st));
+  double t0 = now_ms();
+  fw::AsmArgs aa{};
+  aa.nel = (int)nel;
+  aa.el_vtx = d_el_vtx;
+  aa.crd = d_crd;
+  aa.joe = d_joe;
+  aa.eps = d_eps;
+  aa.sigma = d_sig;
+  aa.pml = d_pml;
+  aa.src = d_src;
+  aa.omega = m->omega;
+  aa.elem_out = d_elem;
+  aa.bad = d_bad;
+  HIPCHK(fw::launch_element(aa, st));
+  HIPCHK(fw::launch_gather_sum((int)nnz, d_asm_ptr, d_asm_src, d_elem, c->aval, st));
+  HIPCHK(fw::launch_gather_sum((int)ndof, d_f_ptr, d_f_src, d_elem, c->f, st));
+  HIPCHK(fw::launch_copy_idx((int)cp_k.size(), d_cp_ent, c->aval, c->cp_val, st));
+  HIPCHK(fw::launch_copy_idx((int)gr_col.size(), d_gr_ent, c->aval, c->gr_val, st));
+  if ((rc = c->alloc(&c->pcslot, nslots))) return rc;
+  HIPCHK(fw::launch_pcslot((int)nslots, d_diag_slot, nullptr, c->aval, c->pcslot, st));
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (bad) return fail(FWH_E_MESH, "degenerate element (non-positive volume or zero-length edge)");
+
+  // ---- Dirichlet profile + skyline layout -----------------------------------------
+  if ((rc = c->alloc(&c->meta, nsub))) return rc;
+  HIPCHK(hipMemcpyAsync(c->meta, meta.data(), nsub * sizeof(SubMeta), hipMemcpyHostToDevice, st));
+  if ((rc = c->alloc(&c->idx1, ndof))) return rc;
+  HIPCHK(fw::launch_profile((int)ndof, c->dof_sub, c->meta, c->rowptr, c->col, c->aval, c->bcflag, c->idx1, st));
+  std::vector<int> idx1(ndof), rowbeg(ndof), blk_row;
+  HIPCHK(hipMemcpyAsync(idx1.data(), c->idx1, ndof * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  c->info.ms_assemble = now_ms() - t0;
+  long long nlen_tot = 0;
+  int wmax = 0;
+  double flops = 0.0;
+  for (int s = 0; s < nsub; s++) {
+    SubMeta &M = meta[s];
+    long long rb = 0;
+    for (int i = 0; i < M.n; i++) {
+      const int fi = idx1[M.dof0 + i];
+      if (fi < 0 || fi > i) return fail(FWH_E_ARG, "bad profile");
+      rowbeg[M.dof0 + i] = (int)rb;
+      rb += i - fi + 1;
+      wmax = std::max(wmax, i - fi);
+    }
+    if (rb > (1LL << 31) - 1) return fail(FWH_E_LIMIT, "subdomain %d skyline too long", s);
+    M.nlen = rb;
+    M.fac0 = nlen_tot;
+    nlen_tot += rb;
+    // row blocks: <= BMAX rows and <= CAP skyline entries each
+    M.blk0 = (int)blk_row.size();
+    int i0 = 0;
+    blk_row.push_back(0);
+    while (i0 < M.n) {
+      int i1 = i0;
+      while (i1 < M.n && i1 - i0 < fw::BMAX) {
+        const long long e1 = (i1 + 1 < M.n) ? rowbeg[M.dof0 + i1 + 1] : rb;
+        if (e1 - rowbeg[M.dof0 + i0] > fw::CAP) break;
+        i1++;
+      }
+      if (i1 == i0) return fail(FWH_E_LIMIT, "subdomain %d row %d wider than %d entries", s, i0, fw::CAP);
+      blk_row.push_back(i1);
+      i0 = i1;
+    }
+    M.nblk = (int)blk_row.size() - M.blk0 - 1;
+    // flops of the factorisation: 8 per complex multiply-add
+    for (int i = 0; i < M.n; i++) {
+      const int fi = idx1[M.dof0 + i];
+      for (int j = fi; j < i; j++) flops += 8.0 * (j - std::max(fi, idx1[M.dof0 + j]));
+      flops += 14.0 * (i - fi);
+    }
+  }
+  c->nlen = nlen_tot;
+  c->wmax = wmax;
+  if ((size_t)(wmax + 1 + nmax) * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "profile too wide for the factor kernel");
+  HIPCHK(hipMemcpyAsync(c->meta, meta.data(), nsub * sizeof(SubMeta), hipMemcpyHostToDevice, st));
+  if ((rc = c->upload(&c->rowbeg, rowbeg)) || (rc = c->upload(&c->blk_row, blk_row)) ||
+      (rc = c->alloc(&c->fac, nlen_tot)) || (rc = c->alloc(&c->dinv, ndof)))
+    return rc;
+  // work order: largest skyline first
+  std::vector<int> order(nsub);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return meta[x].nlen > meta[y].nlen; });
+  if ((rc = c->upload(&c->order, order))) return rc;
+  c->order_h = order;
+  c->meta_h = meta;
+
+  HIPCHK(hipStreamSynchronize(st));
+  t0 = now_ms();
+  HIPCHK(hipMemsetAsync(c->fac, 0, nlen_tot * sizeof(double2), st));
+  HIPCHK(fw::launch_skyfill((int)nnz, c->ent_row, c->dof_sub, c->meta, c->col, c->aval, c->bcflag, c->idx1,
+                            c->rowbeg, c->fac, st));
+  HIPCHK(fw::launch_skydiag_bc((int)ndof, c->dof_sub, c->meta, c->bcflag, c->rowbeg, c->idx1, c->fac, st));
+  fw::FactorArgs fa{};
+  fa.meta = c->meta;
+  fa.order = c->order;
+  fa.base = 0;
+  fa.idx1 = c->idx1;
+  fa.rowbeg = c->rowbeg;
+  fa.fac = c->fac;
+  fa.dinv = c->dinv;
+  fa.wmax = wmax;
+  fa.nmax = nmax;
+  HIPCHK(fw::launch_factor(fa, nsub, st));
+  HIPCHK(hipStreamSynchronize(st));
+  c->info.ms_factor = now_ms() - t0;
+
+  // ---- interface vectors ------------------------------------------------------------
+  c->G = std::max(1, std::min(512, (ng + fw::VT_HINT - 1) / fw::VT_HINT));
+  if ((rc = c->alloc(&c->P, ng)) || (rc = c->alloc(&c->Q, ng)) || (rc = c->alloc(&c->R, ng)) ||
+      (rc = c->alloc(&c->Z, ng)) || (rc = c->alloc(&c->U, ng)) || (rc = c->alloc(&c->PC, ng)) ||
+      (rc = c->alloc(&c->yslot, nslots)) || (rc = c->alloc(&c->X, ndof)) || (rc = c->alloc(&c->part, 3 * c->G)) ||
+      (rc = c->alloc(&c->sc, 1)))
+    return rc;
+  HIPCHK(hipMemsetAsync(c->X, 0, ndof * sizeof(double2), st));
+  HIPCHK(hipMemsetAsync(c->sc, 0, sizeof(fw::Scal), st));
+  HIPCHK(hipStreamSynchronize(st));
+
+  // ---- info ----------------------------------------------------------------------------
+  fwh_info &I = c->info;
+  I.nsub = nsub;
+  I.npart = npart;
+  I.n_iface = ng;
+  I.n_dof = ndof;
+  I.n_slots = nslots;
+  I.nnz = nnz;
+  I.nlen = nlen_tot;
+  I.n_elem = nel;
+  I.n_cpl = c->ncpl;
+  I.n_grow = c->ngrow;
+  I.max_n = nmax;
+  I.bytes_solve = 16.0 * nlen_tot + 40.0 * ndof;
+  I.bytes_iter = I.bytes_solve + 20.0 * (c->ncpl + c->ngrow) + 48.0 * nslots + 128.0 * ng;
+  I.flops_factor = flops;
+  c->loaded = true;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int fwh_load(fwh_ctx *c, const fwh_mesh_desc *m) {
+  if (!c || !m) return fail(FWH_E_ARG, "null argument");
+  if (c->loaded) return fail(FWH_E_STATE, "model already loaded");
+  HIPCHK(hipSetDevice(c->device));
+  int rc = build_model(c, m);
+  if (rc) {
+    (void)hipStreamSynchronize(c->st);
+    c->release();
+    c->loaded = false;
+  }
+  return rc;
+}
+
+extern "C" int fwh_get_info(const fwh_ctx *c, fwh_info *info) {
+  if (!c || !info) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  *info = c->info;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// HDDM steps
+// ---------------------------------------------------------------------------
+namespace {
+int ensure_hist(fwh_ctx *c, int max_loop) {
+  if (max_loop + 1 <= c->hist_cap) return 0;
+  int rc = c->alloc(&c->hist, (size_t)max_loop + 1);
+  if (rc) return rc;
+  c->hist_cap = max_loop + 1;
+  c->release_graph();  // graphs hold the old pointer
+  return 0;
+}
+int read_scal(fwh_ctx *c, fw::Scal *s) {
+  HIPCHK(hipMemcpyAsync(s, c->sc, sizeof(fw::Scal), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+}  // namespace
+
+extern "C" int fwh_first(fwh_ctx *c, const fwh_opts *o) {
+  if (!c || !o) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  if (o->hddm_max_loop < 0) return fail(FWH_E_ARG, "hddm_max_loop < 0");
+  HIPCHK(hipSetDevice(c->device));
+  int rc = ensure_hist(c, o->hddm_max_loop);
+  if (rc) return rc;
+  memset(&c->stats, 0, sizeof(c->stats));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const double t0 = now_ms();
+  HIPCHK(fw::launch_scal_init(c->sc, o->hddm_eps, o->hddm_max_loop, c->st));
+  HIPCHK(c->run_solve(fw::MODE_FIRST, o->plain != 0, false));
+  HIPCHK(fw::launch_first(c->vec_args(), c->st));
+  fw::Scal s;
+  if ((rc = read_scal(c, &s))) return rc;
+  c->stats.ms_first = now_ms() - t0;
+  c->stats.r0 = 1.0 / s.r0inv;
+  c->stats.sub_solves = c->nsub;
+  c->stats.loops = s.loop;
+  c->first_done = true;
+  c->last_done = false;
+  return 0;
+}
+
+extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
+  if (!c || !o) return fail(FWH_E_ARG, "null argument");
+  if (!c->first_done) return fail(FWH_E_STATE, "fwh_first not called");
+  HIPCHK(hipSetDevice(c->device));
+  fw::Scal s;
+  int rc;
+  if ((rc = read_scal(c, &s))) return rc;
+  const double t0 = now_ms();
+  int it = 0;
+  if (o->plain) {
+    while (it < n && !s.done) {
+      HIPCHK(c->run_solve(fw::MODE_ITER, true, true));
+      HIPCHK(fw::launch_iter(c->vec_args(), c->st));
+      if ((rc = read_scal(c, &s))) return rc;
+      it++;
+    }
+  } else if (o->graph_iters > 0) {
+    if (!c->gexec || c->graph_n != o->graph_iters) {
+      c->release_graph();
+      HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < o->graph_iters; k++) {
+        hipError_t e = c->run_solve(fw::MODE_ITER, false, true);
+        if (e == hipSuccess) e = fw::launch_iter(c->vec_args(), c->st);
+        if (e != hipSuccess) {
+          hipGraph_t g;
+          (void)hipStreamEndCapture(c->st, &g);
+          if (g) (void)hipGraphDestroy(g);
+          return fail(FWH_E_HIP, "graph capture: %s", hipGetErrorString(e));
+        }
+      }
+      HIPCHK(hipStreamEndCapture(c->st, &c->graph));
+      HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+      c->graph_n = o->graph_iters;
+    }
+    while (it < n && !s.done) {
+      HIPCHK(hipGraphLaunch(c->gexec, c->st));
+      if ((rc = read_scal(c, &s))) return rc;
+      it += c->graph_n;
+    }
+  } else {
+    while (it < n && !s.done) {
+      HIPCHK(c->run_solve(fw::MODE_ITER, false, true));
+      HIPCHK(fw::launch_iter(c->vec_args(), c->st));
+      it++;
+      if ((it & 15) == 0 || it >= n)
+        if ((rc = read_scal(c, &s))) return rc;
+    }
+  }
+  if ((rc = read_scal(c, &s))) return rc;
+  c->stats.ms_loop += now_ms() - t0;
+  c->stats.iters = s.iters;
+  c->stats.loops = s.loop;
+  c->stats.converged = s.conv;
+  c->stats.final_resid = s.resi;
+  c->stats.sub_solves = (long)c->nsub * (1 + s.iters);
+  if (done) *done = s.done;
+  return 0;
+}
+
+extern "C" int fwh_last(fwh_ctx *c) {
+  if (!c) return fail(FWH_E_ARG, "null argument");
+  if (!c->first_done) return fail(FWH_E_STATE, "fwh_first not called");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const double t0 = now_ms();
+  HIPCHK(c->run_solve(fw::MODE_LAST, false, false));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->stats.ms_last = now_ms() - t0;
+  c->stats.sub_solves += c->nsub;
+  c->last_done = true;
+  return 0;
+}
+
+extern "C" int fwh_get_stats(const fwh_ctx *c, fwh_stats *st) {
+  if (!c || !st) return fail(FWH_E_ARG, "null argument");
+  *st = c->stats;
+  return 0;
+}
+
+extern "C" int fwh_get_x(const fwh_ctx *c, double *x) {
+  if (!c || !x) return fail(FWH_E_ARG, "null argument");
+  if (!c->last_done) return fail(FWH_E_STATE, "no result (fwh_last not called)");
+  HIPCHK(hipMemcpy(x, c->X, c->ndof * sizeof(double2), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int fwh_get_u(const fwh_ctx *c, double *u) {
+  if (!c || !u) return fail(FWH_E_ARG, "null argument");
+  if (!c->first_done) return fail(FWH_E_STATE, "no solve");
+  HIPCHK(hipMemcpy(u, c->U, (size_t)c->ng * sizeof(double2), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int fwh_get_resid(const fwh_ctx *c, double *r, int nr) {
+  if (!c || !r) return fail(FWH_E_ARG, "null argument");
+  if (!c->first_done) return fail(FWH_E_STATE, "no solve");
+  if (nr > c->hist_cap) nr = c->hist_cap;
+  if (nr > 0) HIPCHK(hipMemcpy(r, c->hist, (size_t)nr * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// single operators
+// ---------------------------------------------------------------------------
+extern "C" int fwh_schur_apply(fwh_ctx *c, const double *p, double *q, int plain) {
+  if (!c || !p || !q) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->P, p, (size_t)c->ng * sizeof(double2), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c->run_solve(fw::MODE_ITER, plain != 0, false));
+  HIPCHK(fw::launch_gather_q(c->vec_args(), c->st));
+  HIPCHK(hipMemcpyAsync(q, c->Q, (size_t)c->ng * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->first_done = false;
+  return 0;
+}
+
+extern "C" int fwh_ldl_solve_all(fwh_ctx *c, double *rhs, int plain) {
+  if (!c || !rhs) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->X, rhs, c->ndof * sizeof(double2), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c->run_solve(fw::MODE_RHS, plain != 0, false));
+  HIPCHK(hipMemcpyAsync(rhs, c->X, c->ndof * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->last_done = false;
+  return 0;
+}
+
+extern "C" int fwh_time_kernels(fwh_ctx *c, int reps, double *ms_solve, double *ms_schur) {
+  if (!c || reps <= 0) return fail(FWH_E_ARG, "bad argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  HIPCHK(hipSetDevice(c->device));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float ms = 0.f;
+  // warm up
+  HIPCHK(c->run_solve(fw::MODE_RHS, false, false));
+  HIPCHK(hipEventRecord(e0, c->st));
+  for (int r = 0; r < reps; r++) HIPCHK(c->run_solve(fw::MODE_RHS, false, false));
+  HIPCHK(hipEventRecord(e1, c->st));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  if (ms_solve) *ms_solve = ms / reps;
+  HIPCHK(hipEventRecord(e0, c->st));
+  for (int r = 0; r < reps; r++) HIPCHK(c->run_solve(fw::MODE_ITER, false, false));
+  HIPCHK(hipEventRecord(e1, c->st));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  if (ms_schur) *ms_schur = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->first_done = false;
+  c->last_done = false;
+  return 0;
+}
+
+extern "C" int fwh_get_sub(const fwh_ctx *c, int s, int what, void *out) {
+  if (!c || !out) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  if (s < 0 || s >= c->nsub) return fail(FWH_E_ARG, "subdomain %d out of range", s);
+  HIPCHK(hipSetDevice(c->device));
+  const SubMeta &M = c->meta_h[s];
+  switch (what) {
+    case 0:
+      HIPCHK(hipMemcpy(out, c->aval + M.nnz0, (size_t)M.nnz * sizeof(double2), hipMemcpyDeviceToHost));
+      break;
+    case 1: {
+      std::vector<int> rp(M.n + 1);
+      HIPCHK(hipMemcpy(rp.data(), c->rowptr + M.dof0, (M.n + 1) * sizeof(int), hipMemcpyDeviceToHost));
+      for (auto &x : rp) x -= M.nnz0;
+      memcpy(out, rp.data(), rp.size() * sizeof(int));
+      break;
+    }
+    case 2:
+      HIPCHK(hipMemcpy(out, c->col + M.nnz0, (size_t)M.nnz * sizeof(int), hipMemcpyDeviceToHost));
+      break;
+    case 3:
+      HIPCHK(hipMemcpy(out, c->f + M.dof0, (size_t)M.n * sizeof(double2), hipMemcpyDeviceToHost));
+      break;
+    case 4:
+      HIPCHK(hipMemcpy(out, c->idx1 + M.dof0, (size_t)M.n * sizeof(int), hipMemcpyDeviceToHost));
+      break;
+    case 5: {
+      std::vector<int> i1(M.n), rb(M.n);
+      HIPCHK(hipMemcpy(i1.data(), c->idx1 + M.dof0, M.n * sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(rb.data(), c->rowbeg + M.dof0, M.n * sizeof(int), hipMemcpyDeviceToHost));
+      int *o = (int *)out;
+      for (int i = 0; i < M.n; i++) o[i] = rb[i] - i1[i];  // idx2[i] = rowbeg - idx1
+      break;
+    }
+    case 6:
+      HIPCHK(hipMemcpy(out, c->fac + M.fac0, (size_t)M.nlen * sizeof(double2), hipMemcpyDeviceToHost));
+      break;
+    case 7: {
+      int *o = (int *)out;
+      o[0] = M.n;
+      o[1] = M.nnz;
+      o[2] = (int)M.nlen;
+      o[3] = M.ng;
+      break;
+    }
+    default:
+      return fail(FWH_E_ARG, "unknown field %d", what);
+  }
+  return 0;
+}
+
+extern "C" int fwh_element_batch(fwh_ctx *c, int nel, const double *xyz, const double *eps,
+                                 const double *sigma, double omega, const double *pml, const int *src,
+                                 const double *joe, double *ae, double *be) {
+  if (!c || nel < 0 || !xyz || !eps || !sigma || !ae || !be) return fail(FWH_E_ARG, "bad argument");
+  if (nel == 0) return 0;
+  HIPCHK(hipSetDevice(c->device));
+  fwh_ctx tmp;  // scratch allocations, freed below
+  tmp.st = c->st;
+  int rc;
+  std::vector<int> ev(4 * (size_t)nel);
+  std::iota(ev.begin(), ev.end(), 0);
+  std::vector<double> vx(xyz, xyz + 12 * (size_t)nel), ve(eps, eps + nel), vs(sigma, sigma + nel);
+  std::vector<double> vj(24 * (size_t)nel, 0.0);
+  if (joe) std::copy(joe, joe + 24 * (size_t)nel, vj.begin());
+  std::vector<int> vsrc(nel, 0);
+  if (src) std::copy(src, src + nel, vsrc.begin());
+  int *d_ev, *d_src, *d_bad;
+  double *d_x, *d_e, *d_s, *d_j, *d_p = nullptr;
+  double2 *d_out;
+  if ((rc = tmp.upload(&d_ev, ev)) || (rc = tmp.upload(&d_x, vx)) || (rc = tmp.upload(&d_e, ve)) ||
+      (rc = tmp.upload(&d_s, vs)) || (rc = tmp.upload(&d_j, vj)) || (rc = tmp.upload(&d_src, vsrc)) ||
+      (rc = tmp.alloc(&d_out, 27 * (size_t)nel)) || (rc = tmp.alloc(&d_bad, 1))) {
+    tmp.release();
+    return rc;
+  }
+  if (pml) {
+    std::vector<double> vp(pml, pml + 6 * (size_t)nel);
+    if ((rc = tmp.upload(&d_p, vp))) {
+      tmp.release();
+      return rc;
+    }
+  }
+  (void)hipMemsetAsync(d_bad, 0, sizeof(int), c->st);
+  fw::AsmArgs aa{};
+  aa.nel = nel;
+  aa.el_vtx = d_ev;
+  aa.crd = d_x;
+  aa.joe = d_j;
+  aa.eps = d_e;
+  aa.sigma = d_s;
+  aa.pml = d_p;
+  aa.src = d_src;
+  aa.omega = omega;
+  aa.elem_out = d_out;
+  aa.bad = d_bad;
+  std::vector<double2> out(27 * (size_t)nel);
+  int bad = 0;
+  hipError_t e = fw::launch_element(aa, c->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(double2), hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  tmp.release();
+  tmp.st = nullptr;
+  if (e != hipSuccess) return fail(FWH_E_HIP, "element batch: %s", hipGetErrorString(e));
+  if (bad) return fail(FWH_E_MESH, "degenerate element in batch");
+  static const int PA[21] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 5};
+  static const int PB[21] = {0, 1, 2, 3, 4, 5, 1, 2, 3, 4, 5, 2, 3, 4, 5, 3, 4, 5, 4, 5, 5};
+  for (int el = 0; el < nel; el++) {
+    const double2 *o = out.data() + 27 * (size_t)el;
+    double *A = ae + 72 * (size_t)el;
+    for (int p = 0; p < 21; p++) {
+      const int i = PA[p], j = PB[p];
+      A[2 * (i * 6 + j)] = o[p].x;
+      A[2 * (i * 6 + j) + 1] = o[p].y;
+      A[2 * (j * 6 + i)] = o[p].x;
+      A[2 * (j * 6 + i) + 1] = o[p].y;
+    }
+    for (int k = 0; k < 6; k++) {
+      be[12 * (size_t)el + 2 * k] = o[21 + k].x;
+      be[12 * (size_t)el + 2 * k + 1] = o[21 + k].y;
+    }
+  }
+  return 0;
+}
+
+extern "C" int fwh_cocg_icc_all(fwh_ctx *c, const double *b, double *x, int *status, double eps,
+                                double ic_shift) {
+  (void)b;
+  (void)x;
+  (void)status;
+  (void)eps;
+  (void)ic_shift;
+  if (!c) return fail(FWH_E_ARG, "null argument");
+  return fail(FWH_E_STATE, "subdomain COCG/ICC not built in this library version");
+}

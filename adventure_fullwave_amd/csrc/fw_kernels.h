@@ -1,0 +1,135 @@
+// fw_kernels.h - kernel argument blocks and launchers of libfwhip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fw {
+
+// per-subdomain metadata, one record per subdomain
+struct SubMeta {
+  int n;          // edge DOFs
+  int ng;         // interface DOFs (slots)
+  int nblk;       // row blocks of the skyline sweeps
+  int dof0;       // offset into per-DOF arrays
+  int g0;         // offset into per-slot arrays
+  int blk0;       // offset into blk_row (nblk+1 entries)
+  long long fac0; // offset into the skyline array (complex entries)
+  long long nlen; // skyline length
+  int nnz0;       // offset into the lower-triangle CSR arrays (entries)
+  int nnz;        // entries
+};
+
+enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3 };
+
+// solve kernel limits: rows per block, prefetch registers per thread
+constexpr int SOLVE_THREADS = 256;
+constexpr int BMAX = 16;
+constexpr int PFK = 4;
+constexpr int CAP = SOLVE_THREADS * PFK;  // skyline entries per row block
+constexpr int VT_HINT = 256;              // threads per block of the vector kernels
+
+struct SolveArgs {
+  const SubMeta *meta;
+  const int *order;      // work order (may be null: identity)
+  int base;              // first work item of this launch
+  int mode;
+  const int *done;       // iteration mode: skip when *done != 0 (may be null)
+  int nmax, ngmax;       // LDS sizing
+  const int *idx1;       // per DOF, local first column of the profile
+  const int *rowbeg;     // per DOF, start of row i within the subdomain's skyline
+  const double2 *fac;
+  const double2 *dinv;   // per DOF, 1/d_i
+  const int8_t *bcflag;  // per DOF: 0 interior, 1 interface, 2 essential
+  const int *kidx;       // per DOF: interface slot (local) or -1
+  const double2 *f;      // per DOF load
+  const int *cp_ptr;     // per DOF (n_dof+1): interior-interface couplings
+  const int *cp_k;
+  const double2 *cp_val;
+  const int *gr_ptr;     // per slot (n_slots+1): interface rows of A
+  const int *gr_col;
+  const double2 *gr_val;
+  const int *g_nfid;     // per slot: local DOF
+  const int *g_gid;      // per slot: global interface index
+  const int *blk_row;    // per subdomain nblk+1 block start rows
+  const double2 *P;      // global interface values
+  double2 *yslot;        // per slot output
+  double2 *X;            // per DOF solution (MODE_LAST, MODE_RHS in/out)
+};
+
+struct FactorArgs {
+  const SubMeta *meta;
+  const int *order;
+  int base;
+  const int *idx1;
+  const int *rowbeg;
+  double2 *fac;
+  double2 *dinv;
+  int wmax;              // LDS row buffer length
+  int nmax;
+};
+
+struct AsmArgs {
+  int nel;
+  const int *el_vtx;     // [nel*4] global vertex index
+  const double *crd;     // [nv*3]
+  const double *joe;     // [nv*6]
+  const double *eps, *sigma, *pml;
+  const int *src;
+  double omega;
+  double2 *elem_out;     // [nel*27]: 21 lower pairs + 6 loads
+  int *bad;              // set when an element is degenerate
+};
+
+// scalars of the interface COCG
+struct Scal {
+  double2 rz[2];   // (r,z) by loop parity
+  double r0inv;
+  double resi;
+  double eps;
+  int loop;
+  int done;
+  int conv;
+  int iters;
+  int max_loop;
+  int pad;
+};
+
+struct VecArgs {
+  int ng;                 // N_Gamma
+  const int *gq_ptr;      // gather CSR over interface unknowns
+  const int *gq_slot;     // slot ids, ~slot marks the first slot of a part group
+  const double2 *yslot;
+  const double2 *pcslot;
+  double2 *P, *Q, *R, *Z, *U, *PC;
+  double2 *part;          // [3*G] partial sums
+  Scal *sc;
+  double *hist;
+  int G;                  // grid size of the vector kernels
+};
+
+// launchers (fw_kernels.hip)
+hipError_t launch_element(const AsmArgs &a, hipStream_t st);
+hipError_t launch_gather_sum(int n, const int *ptr, const int *src, const double2 *in,
+                             double2 *out, hipStream_t st);
+hipError_t launch_copy_idx(int n, const int *slot, const double2 *in, double2 *out,
+                           hipStream_t st);
+hipError_t launch_profile(int ndof, const int *dof_sub, const SubMeta *meta, const int *rowptr,
+                          const int *col, const double2 *val, const int8_t *bcflag, int *idx1,
+                          hipStream_t st);
+hipError_t launch_skyfill(int nnz_tot, const int *ent_row, const int *dof_sub, const SubMeta *meta,
+                          const int *col, const double2 *val, const int8_t *bcflag,
+                          const int *idx1, const int *rowbeg, double2 *fac, hipStream_t st);
+hipError_t launch_skydiag_bc(int ndof, const int *dof_sub, const SubMeta *meta,
+                             const int8_t *bcflag, const int *rowbeg, const int *idx1, double2 *fac,
+                             hipStream_t st);
+hipError_t launch_factor(const FactorArgs &a, int nwork, hipStream_t st);
+hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st);
+hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, const double2 *val,
+                         double2 *pcslot, hipStream_t st);
+hipError_t launch_first(const VecArgs &a, hipStream_t st);
+hipError_t launch_iter(const VecArgs &a, hipStream_t st);
+hipError_t launch_gather_q(const VecArgs &a, hipStream_t st);
+hipError_t launch_fill(double2 *p, long long n, double2 v, hipStream_t st);
+hipError_t launch_scal_init(Scal *sc, double eps, int max_loop, hipStream_t st);
+
+}  // namespace fw

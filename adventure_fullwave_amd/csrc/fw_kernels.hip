@@ -1,0 +1,861 @@
+// fw_kernels.hip - HIP kernels of libfwhip for gfx950 (CDNA4, wave64).
+//
+// Hot path of ADVENTURE_FullWave's HDDM full-wave solve:
+//   * element kernel          tet_MakeElement_EM_E   lib/fem/make_elem_em.c:14-400
+//   * assembly gather         mymt_CS_AIJ_SetAe      lib/mymt_aij_solver/aij_matrix_func.h:260-323
+//   * Dirichlet profile       SetBC + skyindex       aij_matrix_func.h:60-136, advlas_skyline.c:12-60
+//   * skyline LDL^T factor    advlas_skyline_ldl_decomposite  lib/hddm/advlas_skyline.c:79-128
+//   * skyline solve + Schur   advlas_skyline_ldl_solve :134-167, hddm_solver.h:559-622
+//   * interface COCG          hddm_solver_s.h:158-400, hddm_pc.h:52-137
+#include "fw_kernels.h"
+#include "fw_device.h"
+
+namespace fw {
+
+constexpr double NEARLY_ZERO = 1.0e-60;  // lib/std/defs.h:105
+constexpr double NU = 7.957747e+05;      // lib/fem/make_elem_em.c:37
+
+// edge k: vertices EDGE[k][0] -> EDGE[k][1]; opposite edge EDGE[k][2] -> EDGE[k][3]
+// (the reference's tet_n table, lib/fem/tetrahedral.h:19-24)
+__constant__ int EDGE[6][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2},
+                               {1, 2, 0, 3}, {1, 3, 2, 0}, {2, 3, 0, 1}};
+
+// ---------------------------------------------------------------------------
+// element kernel: one thread per tetrahedron
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_element(AsmArgs a) {
+  const int el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= a.nel) return;
+  double v[12];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int g = a.el_vtx[4 * el + i];
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[3 * i + c] = a.crd[3 * (long long)g + c];
+  }
+  double r1[3], r2[3], r3[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    r1[c] = v[3 + c] - v[c];
+    r2[c] = v[6 + c] - v[c];
+    r3[c] = v[9 + c] - v[c];
+  }
+  // G[i] = D * grad(lambda_i)
+  double G[4][3];
+  G[1][0] = r2[1] * r3[2] - r2[2] * r3[1];
+  G[1][1] = r2[2] * r3[0] - r2[0] * r3[2];
+  G[1][2] = r2[0] * r3[1] - r2[1] * r3[0];
+  G[2][0] = r3[1] * r1[2] - r3[2] * r1[1];
+  G[2][1] = r3[2] * r1[0] - r3[0] * r1[2];
+  G[2][2] = r3[0] * r1[1] - r3[1] * r1[0];
+  G[3][0] = r1[1] * r2[2] - r1[2] * r2[1];
+  G[3][1] = r1[2] * r2[0] - r1[0] * r2[2];
+  G[3][2] = r1[0] * r2[1] - r1[1] * r2[0];
+#pragma unroll
+  for (int c = 0; c < 3; c++) G[0][c] = -(G[1][c] + G[2][c] + G[3][c]);
+  const double D = r1[0] * G[1][0] + r1[1] * G[1][1] + r1[2] * G[1][2];
+  double g[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) g[c] = (v[c] + v[3 + c] + v[6 + c] + v[9 + c]) * 0.25;
+
+  double ls[6], e[6][3], F[6][3];
+  bool bad = !(D > 0.0);
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int ia = EDGE[k][0], ib = EDGE[k][1], ic = EDGE[k][2], id = EDGE[k][3];
+    double dl[3], vc[3], vd[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      dl[c] = v[3 * ib + c] - v[3 * ia + c];
+      vc[c] = v[3 * ic + c];
+      vd[c] = v[3 * id + c];
+      e[k][c] = vd[c] - vc[c];
+    }
+    double l = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+    double si = dl[0] > 0.0 ? 1.0 : dl[0] < 0.0 ? -1.0
+              : dl[1] > 0.0 ? 1.0 : dl[1] < 0.0 ? -1.0
+              : dl[2] > 0.0 ? 1.0 : dl[2] < 0.0 ? -1.0 : 0.0;
+    if (si == 0.0) bad = true;
+    ls[k] = l * si;
+    double w0 = vc[1] * vd[2] - vc[2] * vd[1];
+    double w1 = vc[2] * vd[0] - vc[0] * vd[2];
+    double w2 = vc[0] * vd[1] - vc[1] * vd[0];
+    F[k][0] = w0 + e[k][1] * g[2] - e[k][2] * g[1];
+    F[k][1] = w1 + e[k][2] * g[0] - e[k][0] * g[2];
+    F[k][2] = w2 + e[k][0] * g[1] - e[k][1] * g[0];
+  }
+  if (bad) {
+    atomicOr(a.bad, 1);
+    return;
+  }
+  // centroid second moments C = (1/20) sum (v_i - g)(v_i - g)^T
+  double C[3][3];
+#pragma unroll
+  for (int p = 0; p < 3; p++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) C[p][q] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    double pp[3] = {v[3 * i] - g[0], v[3 * i + 1] - g[1], v[3 * i + 2] - g[2]};
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+      for (int q = 0; q < 3; q++) C[p][q] += pp[p] * pp[q];
+  }
+#pragma unroll
+  for (int p = 0; p < 3; p++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) C[p][q] *= 0.05;
+
+  const double drr = 2.0 / (3.0 * D), dss = 1.0 / (6.0 * D);
+  const double nu_omega = NU / a.omega;
+  const double dd = -(a.eps[el] * a.omega);
+  const double sig = a.sigma[el];
+  double2 lam[3], ilam[3];
+  const bool stretch = a.pml != nullptr;
+  if (stretch) {
+    double2 s[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) s[c] = make_double2(a.pml[6 * (long long)el + 2 * c], a.pml[6 * (long long)el + 2 * c + 1]);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      lam[c] = cdiv_naive(cmul(s[(c + 1) % 3], s[(c + 2) % 3]), s[c]);
+      ilam[c] = crecip_naive(lam[c]);
+    }
+  }
+  double2 *out = a.elem_out + 27LL * el;
+  int p = 0;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+#pragma unroll
+    for (int j = i; j < 6; j++, p++) {
+      double rr[3], mm[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const int b = (c + 1) % 3, cc = (c + 2) % 3;
+        double q = e[i][b] * e[j][b] * C[cc][cc] - e[i][b] * e[j][cc] * C[cc][b] -
+                   e[i][cc] * e[j][b] * C[b][cc] + e[i][cc] * e[j][cc] * C[b][b];
+        rr[c] = drr * ls[i] * ls[j] * e[i][c] * e[j][c];
+        mm[c] = dss * ls[i] * ls[j] * (F[i][c] * F[j][c] + q);
+      }
+      double2 t;
+      if (!stretch) {
+        const double rsum = rr[0] + rr[1] + rr[2];
+        const double ssum = mm[0] + mm[1] + mm[2];
+        t.x = 0.0;
+        t.x += nu_omega * rsum;
+        t.x += dd * ssum;
+        t.y = sig * ssum;
+      } else {
+        t = cz();
+        const double2 epsc = make_double2(dd, sig);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          t = cadd(t, cscale(ilam[c], nu_omega * rr[c]));
+          t = cadd(t, cscale(cmul(epsc, lam[c]), mm[c]));
+        }
+      }
+      out[p] = t;
+    }
+  }
+  // current source: exact integral of N_i . sum_j lambda_j J_j
+  // (tetNedelec_JoA_Complex, lib/fem/tetrahedral.c:794-858)
+  if (a.src[el]) {
+    double2 J[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      long long vb = 6LL * a.el_vtx[4 * el + i];
+#pragma unroll
+      for (int c = 0; c < 3; c++) J[i][c] = make_double2(a.joe[vb + 2 * c], a.joe[vb + 2 * c + 1]);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const int ia = EDGE[k][0], ib = EDGE[k][1];
+      const double base = ls[k] / 120.0;
+      double2 acc = cz();
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int jv = 0; jv < 4; jv++) {
+          double pn = base * ((1.0 + (jv == ia)) * G[ib][c] - (1.0 + (jv == ib)) * G[ia][c]);
+          acc = cadd(acc, cscale(J[jv][c], pn));
+        }
+      out[21 + k] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; k++) out[21 + k] = cz();
+  }
+}
+
+hipError_t launch_element(const AsmArgs &a, hipStream_t st) {
+  if (a.nel <= 0) return hipSuccess;
+  k_element<<<(a.nel + 127) / 128, 128, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// assembly: out[i] = sum of in[src[k]] for k in ptr[i]..ptr[i+1]  (the order
+// of the list is the reference's element order, so sums are accumulated in
+// the order SetAe/SetFe add them)
+// ---------------------------------------------------------------------------
+__global__ void k_gather_sum(int n, const int *__restrict__ ptr, const int *__restrict__ src,
+                             const double2 *__restrict__ in, double2 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 acc = cz();
+  for (int k = ptr[i]; k < ptr[i + 1]; k++) acc = cadd(acc, in[src[k]]);
+  out[i] = acc;
+}
+hipError_t launch_gather_sum(int n, const int *ptr, const int *src, const double2 *in,
+                             double2 *out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  k_gather_sum<<<(n + 255) / 256, 256, 0, st>>>(n, ptr, src, in, out);
+  return hipGetLastError();
+}
+
+__global__ void k_copy_idx(int n, const int *__restrict__ slot, const double2 *__restrict__ in,
+                           double2 *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[slot[i]];
+}
+hipError_t launch_copy_idx(int n, const int *slot, const double2 *in, double2 *out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  k_copy_idx<<<(n + 255) / 256, 256, 0, st>>>(n, slot, in, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Dirichlet profile: first non-zero column of every row after SetBC +
+// DeleteZero (aij_matrix_func.h:60-136, 342-391; advlas_skyline.c:12-25).
+// Dirichlet rows become identity rows (first column = the row itself).
+// ---------------------------------------------------------------------------
+__global__ void k_profile(int ndof, const int *__restrict__ dof_sub, const SubMeta *__restrict__ meta,
+                          const int *__restrict__ rowptr, const int *__restrict__ col,
+                          const double2 *__restrict__ val, const int8_t *__restrict__ bcflag,
+                          int *__restrict__ idx1) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndof) return;
+  const int s = dof_sub[d];
+  const int dof0 = meta[s].dof0;
+  const int i = d - dof0;
+  int first = i;
+  if (bcflag[d] == 0) {
+    for (int k = rowptr[d]; k < rowptr[d + 1]; k++) {
+      const int j = col[k];
+      if (bcflag[dof0 + j] == 0 && !ciszero(val[k])) {
+        first = j;
+        break;
+      }
+    }
+  }
+  idx1[d] = first;
+}
+hipError_t launch_profile(int ndof, const int *dof_sub, const SubMeta *meta, const int *rowptr,
+                          const int *col, const double2 *val, const int8_t *bcflag, int *idx1,
+                          hipStream_t st) {
+  if (ndof <= 0) return hipSuccess;
+  k_profile<<<(ndof + 255) / 256, 256, 0, st>>>(ndof, dof_sub, meta, rowptr, col, val, bcflag, idx1);
+  return hipGetLastError();
+}
+
+// skyline values of the Dirichlet matrix (advlas_copy_nzmat_to_skymat, :49-60)
+__global__ void k_skyfill(int nnz_tot, const int *__restrict__ ent_row, const int *__restrict__ dof_sub,
+                          const SubMeta *__restrict__ meta, const int *__restrict__ col,
+                          const double2 *__restrict__ val, const int8_t *__restrict__ bcflag,
+                          const int *__restrict__ idx1, const int *__restrict__ rowbeg,
+                          double2 *__restrict__ fac) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz_tot) return;
+  const int d = ent_row[k];
+  if (bcflag[d] != 0) return;
+  const int s = dof_sub[d];
+  const SubMeta m = meta[s];
+  const int j = col[k];
+  if (bcflag[m.dof0 + j] != 0) return;
+  const double2 v = val[k];
+  if (ciszero(v)) return;
+  fac[m.fac0 + rowbeg[d] + (j - idx1[d])] = v;
+}
+hipError_t launch_skyfill(int nnz_tot, const int *ent_row, const int *dof_sub, const SubMeta *meta,
+                          const int *col, const double2 *val, const int8_t *bcflag,
+                          const int *idx1, const int *rowbeg, double2 *fac, hipStream_t st) {
+  if (nnz_tot <= 0) return hipSuccess;
+  k_skyfill<<<(nnz_tot + 255) / 256, 256, 0, st>>>(nnz_tot, ent_row, dof_sub, meta, col, val, bcflag,
+                                                  idx1, rowbeg, fac);
+  return hipGetLastError();
+}
+
+__global__ void k_skydiag_bc(int ndof, const int *__restrict__ dof_sub, const SubMeta *__restrict__ meta,
+                             const int8_t *__restrict__ bcflag, const int *__restrict__ rowbeg,
+                             const int *__restrict__ idx1, double2 *__restrict__ fac) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndof || bcflag[d] == 0) return;
+  const SubMeta m = meta[dof_sub[d]];
+  const int i = d - m.dof0;
+  fac[m.fac0 + rowbeg[d] + (i - idx1[d])] = make_double2(1.0, 0.0);
+}
+hipError_t launch_skydiag_bc(int ndof, const int *dof_sub, const SubMeta *meta, const int8_t *bcflag,
+                             const int *rowbeg, const int *idx1, double2 *fac, hipStream_t st) {
+  if (ndof <= 0) return hipSuccess;
+  k_skydiag_bc<<<(ndof + 255) / 256, 256, 0, st>>>(ndof, dof_sub, meta, bcflag, rowbeg, idx1, fac);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// skyline LDL^T (advlas_skyline_ldl_decomposite, advlas_skyline.c:79-128):
+// one wavefront per subdomain, row by row.  Row i is built in LDS:
+//   u_ij = a_ij - sum_{k=max(f_i,f_j)}^{j-1} u_ik l_jk      (j ascending)
+//   d_i  = a_ii - sum_k u_ik^2 / d_k,  l_ik = u_ik / d_k
+// and written back; finished rows are re-read through L2 (agent-scope
+// acquire after every row), the pivots 1/d_k stay in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_factor(FactorArgs a) {
+  const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
+  const SubMeta m = a.meta[w];
+  extern __shared__ double2 lds[];
+  double2 *u = lds;              // [wmax+1]
+  double2 *dg = lds + a.wmax + 1;  // [nmax]
+  const int lane = threadIdx.x;
+  const int *idx1 = a.idx1 + m.dof0;
+  const int *rowbeg = a.rowbeg + m.dof0;
+  double2 *S = a.fac + m.fac0;
+  for (int i = 0; i < m.n; i++) {
+    const int fi = idx1[i];
+    const int rb = rowbeg[i];
+    const int wi = i - fi;
+    for (int t = lane; t <= wi; t += 64) u[t] = S[rb + t];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    for (int j = fi; j < i; j++) {
+      const int fj = idx1[j];
+      const int rbj = rowbeg[j];
+      const int k0 = fi > fj ? fi : fj;
+      double2 acc = cz();
+      for (int k = k0 + lane; k < j; k += 64) acc = cfma(acc, u[k - fi], S[rbj + (k - fj)]);
+      acc = wave_csum(acc);
+      if (lane == 0) u[j - fi] = csub(u[j - fi], acc);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    }
+    double2 r = cz();
+    for (int k = fi + lane; k < i; k += 64) {
+      const double2 uk = u[k - fi];
+      const double2 dk = dg[k];
+      r = cadd(r, cmul(cmul(uk, uk), dk));
+      u[k - fi] = cmul(uk, dk);
+    }
+    r = wave_csum(r);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) {
+      const double2 piv = crecip_smith(csub(u[wi], r));
+      dg[i] = piv;
+      u[wi] = piv;
+      a.dinv[m.dof0 + i] = piv;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    for (int t = lane; t <= wi; t += 64) S[rb + t] = u[t];
+    // make the finished row visible to this wave's later loads
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+  }
+}
+hipError_t launch_factor(const FactorArgs &a, int nwork, hipStream_t st) {
+  if (nwork <= 0) return hipSuccess;
+  size_t lds = sizeof(double2) * (size_t)(a.wmax + 1 + a.nmax);
+  k_factor<<<nwork, 64, lds, st>>>(a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Subdomain Dirichlet solve (hddm_solver.h:461-693, advlas_skyline.c:134-167),
+// one 256-thread workgroup per subdomain:
+//   1. load: v_I = f_I (first/last step) - A_IG v_G, v_G = interface values
+//      (SetBC, aij_matrix_func.h:60-136; renewal_bc_sp hddm_solver_sp.h:58-69)
+//   2. forward sweep  v_i -= sum_j L_ij v_j      (row blocks of <= BMAX rows)
+//   3. v_i *= 1/d_i
+//   4. backward sweep v_j -= L_ij v_i, i descending
+//   5. y_G = (A x)_G (- f_G)   (MxV_TMP_AIJ on the original matrix, rows of G)
+// The skyline is streamed once per sweep: each row block (<= CAP entries,
+// contiguous) is prefetched into registers while the previous block is
+// processed from LDS.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void blk_range(const int *blk, const int *rowbeg, int n, long long nlen,
+                                          int b, int &i0, int &i1, int &e0, int &e1) {
+  i0 = blk[b];
+  i1 = blk[b + 1];
+  e0 = rowbeg[i0];
+  e1 = i1 < n ? rowbeg[i1] : (int)nlen;
+}
+
+__global__ __launch_bounds__(SOLVE_THREADS) void k_solve(SolveArgs a) {
+  if (a.mode == MODE_ITER && a.done && *a.done) return;
+  const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
+  const SubMeta m = a.meta[w];
+  extern __shared__ double2 lds[];
+  double2 *v = lds;                  // [nmax]
+  double2 *vg = lds + a.nmax;        // [ngmax]
+  double2 *stg = vg + a.ngmax;       // [CAP]
+  __shared__ double2 part[BMAX];
+  __shared__ int bf[BMAX], brb[BMAX];
+  __shared__ int s_jlo;
+  const int tid = threadIdx.x;
+  const int n = m.n;
+  const int *idx1 = a.idx1 + m.dof0;
+  const int *rowbeg = a.rowbeg + m.dof0;
+  const double2 *fac = a.fac + m.fac0;
+  const int *blk = a.blk_row + m.blk0;
+
+  // ---- 1. load vector -------------------------------------------------------
+  for (int k = tid; k < m.ng; k += SOLVE_THREADS)
+    vg[k] = (a.mode == MODE_FIRST || a.mode == MODE_RHS) ? cz() : a.P[a.g_gid[m.g0 + k]];
+  __syncthreads();
+  for (int i = tid; i < n; i += SOLVE_THREADS) {
+    const int d = m.dof0 + i;
+    double2 b;
+    if (a.mode == MODE_RHS) {
+      b = a.X[d];
+    } else {
+      const int8_t fl = a.bcflag[d];
+      if (fl == 1) {
+        b = vg[a.kidx[d]];
+      } else if (fl == 2) {
+        b = cz();
+      } else {
+        b = (a.mode == MODE_ITER) ? cz() : a.f[d];
+        for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++) b = cfms(b, a.cp_val[c], vg[a.cp_k[c]]);
+      }
+    }
+    v[i] = b;
+  }
+
+  // ---- 2. forward sweep -------------------------------------------------------
+  double2 pre[PFK];
+  {
+    int i0, i1, e0, e1;
+    if (m.nblk > 0) {
+      blk_range(blk, rowbeg, n, m.nlen, 0, i0, i1, e0, e1);
+#pragma unroll
+      for (int k = 0; k < PFK; k++) {
+        const int e = e0 + tid + k * SOLVE_THREADS;
+        pre[k] = e < e1 ? fac[e] : cz();
+      }
+    }
+  }
+  for (int b = 0; b < m.nblk; b++) {
+    int i0, i1, e0, e1;
+    blk_range(blk, rowbeg, n, m.nlen, b, i0, i1, e0, e1);
+    const int nb = i1 - i0;
+    __syncthreads();  // previous block finished with stg / v
+#pragma unroll
+    for (int k = 0; k < PFK; k++) {
+      const int e = tid + k * SOLVE_THREADS;
+      if (e < e1 - e0) stg[e] = pre[k];
+    }
+    if (tid < nb) {
+      bf[tid] = idx1[i0 + tid];
+      brb[tid] = rowbeg[i0 + tid] - e0;
+    }
+    __syncthreads();
+    if (b + 1 < m.nblk) {
+      int n0, n1, f0, f1;
+      blk_range(blk, rowbeg, n, m.nlen, b + 1, n0, n1, f0, f1);
+#pragma unroll
+      for (int k = 0; k < PFK; k++) {
+        const int e = f0 + tid + k * SOLVE_THREADS;
+        pre[k] = e < f1 ? fac[e] : cz();
+      }
+    }
+    // A: contributions of columns before the block, 16 lanes per row
+    {
+      const int r = tid >> 4, l = tid & 15;
+      double2 acc = cz();
+      if (r < nb) {
+        const int i = i0 + r;
+        const int fi = bf[r], rb = brb[r];
+        const int jend = i0 < i ? i0 : i;
+        for (int j = fi + l; j < jend; j += 16) acc = cfma(acc, stg[rb + (j - fi)], v[j]);
+      }
+      acc = group_csum<16>(acc);
+      if (r < nb && l == 0) part[r] = acc;
+    }
+    __syncthreads();
+    // B: the block's own lower triangle, one lane per row (wave 0)
+    if (tid < 64) {
+      const int r = tid;
+      double2 s = cz();
+      double2 L[BMAX];
+#pragma unroll
+      for (int q = 0; q < BMAX; q++) L[q] = cz();
+      if (r < nb) {
+        const int i = i0 + r;
+        s = csub(v[i], part[r]);
+        const int fi = bf[r], rb = brb[r];
+#pragma unroll
+        for (int q = 0; q < BMAX; q++) {
+          const int j = i0 + q;
+          if (q < r && j >= fi) L[q] = stg[rb + (j - fi)];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < BMAX; q++) {
+        if (q < nb) {
+          const double2 xq = readlane_c(s, q);
+          if (r > q) s = cfms(s, L[q], xq);
+        }
+      }
+      if (r < nb) v[i0 + r] = s;
+    }
+  }
+  __syncthreads();
+
+  // ---- 3. diagonal ---------------------------------------------------------------
+  for (int i = tid; i < n; i += SOLVE_THREADS) v[i] = cmul(v[i], a.dinv[m.dof0 + i]);
+
+  // ---- 4. backward sweep -----------------------------------------------------------
+  if (m.nblk > 0) {
+    int i0, i1, e0, e1;
+    blk_range(blk, rowbeg, n, m.nlen, m.nblk - 1, i0, i1, e0, e1);
+#pragma unroll
+    for (int k = 0; k < PFK; k++) {
+      const int e = e0 + tid + k * SOLVE_THREADS;
+      pre[k] = e < e1 ? fac[e] : cz();
+    }
+  }
+  for (int b = m.nblk - 1; b >= 0; b--) {
+    int i0, i1, e0, e1;
+    blk_range(blk, rowbeg, n, m.nlen, b, i0, i1, e0, e1);
+    const int nb = i1 - i0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PFK; k++) {
+      const int e = tid + k * SOLVE_THREADS;
+      if (e < e1 - e0) stg[e] = pre[k];
+    }
+    if (tid < nb) {
+      bf[tid] = idx1[i0 + tid];
+      brb[tid] = rowbeg[i0 + tid] - e0;
+    }
+    if (tid == 0) {
+      int jl = i0;
+      for (int r = 0; r < nb; r++) jl = min(jl, idx1[i0 + r]);
+      s_jlo = jl;
+    }
+    __syncthreads();
+    if (b > 0) {
+      int n0, n1, f0, f1;
+      blk_range(blk, rowbeg, n, m.nlen, b - 1, n0, n1, f0, f1);
+#pragma unroll
+      for (int k = 0; k < PFK; k++) {
+        const int e = f0 + tid + k * SOLVE_THREADS;
+        pre[k] = e < f1 ? fac[e] : cz();
+      }
+    }
+    // A: the block's triangle, one lane per column q (wave 0)
+    if (tid < 64) {
+      const int q = tid;
+      double2 x = q < nb ? v[i0 + q] : cz();
+      double2 Lc[BMAX];
+#pragma unroll
+      for (int r = 0; r < BMAX; r++) {
+        Lc[r] = cz();
+        if (r > q && r < nb && i0 + q >= bf[r]) Lc[r] = stg[brb[r] + (i0 + q - bf[r])];
+      }
+#pragma unroll
+      for (int r = BMAX - 1; r >= 0; r--) {
+        if (r < nb) {
+          const double2 xr = readlane_c(x, r);
+          if (q < r) x = cfms(x, Lc[r], xr);
+        }
+      }
+      if (q < nb) v[i0 + q] = x;
+    }
+    __syncthreads();
+    // B: update columns before the block with all rows of the block
+    for (int j = s_jlo + tid; j < i0; j += SOLVE_THREADS) {
+      double2 acc = cz();
+      for (int r = 0; r < nb; r++) {
+        const int f = bf[r];
+        if (j >= f) acc = cfma(acc, stg[brb[r] + (j - f)], v[i0 + r]);
+      }
+      v[j] = csub(v[j], acc);
+    }
+  }
+  __syncthreads();
+
+  // ---- 5. outputs -----------------------------------------------------------------------
+  if (a.mode == MODE_LAST || a.mode == MODE_RHS)
+    for (int i = tid; i < n; i += SOLVE_THREADS) a.X[m.dof0 + i] = v[i];
+  if (a.mode != MODE_RHS) {
+    for (int k = tid; k < m.ng; k += SOLVE_THREADS) {
+      const int sl = m.g0 + k;
+      double2 y = cz();
+      for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], v[a.gr_col[c]]);
+      if (a.mode != MODE_ITER) y = csub(y, a.f[m.dof0 + a.g_nfid[sl]]);
+      a.yslot[sl] = y;
+    }
+  }
+}
+
+hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
+  if (nwork <= 0) return hipSuccess;
+  size_t lds = sizeof(double2) * ((size_t)a.nmax + a.ngmax + CAP);
+  k_solve<<<nwork, SOLVE_THREADS, lds, st>>>(a);
+  return hipGetLastError();
+}
+
+// diagonal of A at every slot, zero -> 1 (hddm_pc_CS_DIAG_make, hddm_pc.h:52-86)
+__global__ void k_pcslot(int nslots, const int *__restrict__ diag_ent, const int *__restrict__ unused,
+                         const double2 *__restrict__ val, double2 *__restrict__ pcslot) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nslots) return;
+  const int e = diag_ent[k];
+  double2 d = e >= 0 ? val[e] : cz();
+  if (ciszero(d)) d = make_double2(1.0, 0.0);
+  pcslot[k] = d;
+}
+hipError_t launch_pcslot(int nslots, const int *diag_ent, const int *unused, const double2 *val,
+                         double2 *pcslot, hipStream_t st) {
+  if (nslots <= 0) return hipSuccess;
+  k_pcslot<<<(nslots + 255) / 256, 256, 0, st>>>(nslots, diag_ent, unused, val, pcslot);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// interface vectors
+// ---------------------------------------------------------------------------
+// gather of per-subdomain slot values into interface unknowns, in the
+// reference's order: within a part, subdomains in order (COM_res_sp,
+// hddm_solver_sp.h:41-54); parts summed owner-first, ascending
+// (hddm_solver_s.h:404-424)
+__device__ __forceinline__ double2 gather_slots(const int *__restrict__ ptr, const int *__restrict__ slot,
+                                                const double2 *__restrict__ y, int g) {
+  double2 acc = cz(), part = cz();
+  const int c0 = ptr[g], c1 = ptr[g + 1];
+  for (int c = c0; c < c1; c++) {
+    int sl = slot[c];
+    if (sl < 0) {
+      sl = ~sl;
+      if (c != c0) acc = cadd(acc, part);
+      part = y[sl];
+    } else {
+      part = cadd(part, y[sl]);
+    }
+  }
+  return c1 > c0 ? cadd(acc, part) : acc;
+}
+
+constexpr int VT = 256;
+
+__device__ __forceinline__ double2 block_csum(double2 v, double2 *sh) {
+  v = wave_csum(v);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double2 t = cz();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < VT / 64; k++) t = cadd(t, sh[k]);
+    sh[VT / 64] = t;
+  }
+  __syncthreads();
+  return sh[VT / 64];
+}
+__device__ __forceinline__ double block_max(double v, double *sh) {
+  v = wave_max(v);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = sh[0];
+#pragma unroll
+    for (int k = 1; k < VT / 64; k++) t = fmax(t, sh[k]);
+    sh[VT / 64] = t;
+  }
+  __syncthreads();
+  return sh[VT / 64];
+}
+// identical fixed-order reduction of G partials in every block
+__device__ __forceinline__ double2 reduce_parts(const double2 *__restrict__ p, int G, double2 *sh) {
+  double2 t = cz();
+  for (int k = threadIdx.x; k < G; k += VT) t = cadd(t, p[k]);
+  return block_csum(t, sh);
+}
+__device__ __forceinline__ double reduce_parts_max(const double2 *__restrict__ p, int G, double *sh) {
+  double t = 0.0;
+  for (int k = threadIdx.x; k < G; k += VT) t = fmax(t, p[k].x);
+  return block_max(t, sh);
+}
+
+// first step: Q = gather(y), PC = 1/gather(diag) (hddm_pc.h:108-137),
+// R = Q_owned, Z = R PC, P = Z, U = 0 (hddm_solver_s.h:158-256)
+__global__ __launch_bounds__(VT) void k_first_a(VecArgs a) {
+  __shared__ double2 shc[VT / 64 + 1];
+  __shared__ double shm[VT / 64 + 1];
+  double2 rz = cz();
+  double mx = 0.0;
+  for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT) {
+    const double2 q = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
+    const double2 ps = gather_slots(a.gq_ptr, a.gq_slot, a.pcslot, g);
+    const double2 pc = ciszero(ps) ? make_double2(1.0, 0.0) : cdiv_naive(make_double2(1.0, 0.0), ps);
+    const double2 z = cmul(q, pc);
+    a.Q[g] = q;
+    a.PC[g] = pc;
+    a.R[g] = q;
+    a.Z[g] = z;
+    a.P[g] = z;
+    a.U[g] = cz();
+    rz = cadd(rz, cmul(q, z));
+    mx = fmax(mx, cabsv(q));
+  }
+  rz = block_csum(rz, shc);
+  mx = block_max(mx, shm);
+  if (threadIdx.x == 0) {
+    a.part[a.G + blockIdx.x] = rz;
+    a.part[2 * a.G + blockIdx.x] = make_double2(mx, 0.0);
+  }
+}
+__global__ __launch_bounds__(VT) void k_first_b(VecArgs a) {
+  __shared__ double2 shc[VT / 64 + 1];
+  __shared__ double shm[VT / 64 + 1];
+  const double2 rz = reduce_parts(a.part + a.G, a.G, shc);
+  const double r0 = reduce_parts_max(a.part + 2 * a.G, a.G, shm);
+  if (threadIdx.x == 0) {
+    Scal *s = a.sc;
+    s->done = (r0 < NEARLY_ZERO) ? 1 : 0;
+    s->r0inv = 1.0 / r0;
+    s->rz[1] = rz;
+    s->rz[0] = rz;
+    s->loop = 1;
+    s->iters = 0;
+    s->conv = 0;
+    s->resi = 1.0;
+    a.hist[0] = 1.0;
+  }
+}
+hipError_t launch_first(const VecArgs &a, hipStream_t st) {
+  k_first_a<<<a.G, VT, 0, st>>>(a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_first_b<<<1, VT, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+// iteration (hddm_solver_s.h:260-400)
+__global__ __launch_bounds__(VT) void k_iter_a(VecArgs a) {
+  if (a.sc->done) return;
+  __shared__ double2 shc[VT / 64 + 1];
+  double2 pq = cz();
+  for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT) {
+    const double2 q = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
+    a.Q[g] = q;
+    pq = cadd(pq, cmul(a.P[g], q));
+  }
+  pq = block_csum(pq, shc);
+  if (threadIdx.x == 0) a.part[blockIdx.x] = pq;
+}
+__global__ __launch_bounds__(VT) void k_iter_b(VecArgs a) {
+  const Scal *s = a.sc;
+  if (s->done) return;
+  __shared__ double2 shc[VT / 64 + 1];
+  __shared__ double shm[VT / 64 + 1];
+  const double2 pq = reduce_parts(a.part, a.G, shc);
+  const double2 rz = s->rz[s->loop & 1];
+  double2 malpha = cz();
+  if (!(cabsv(rz) < NEARLY_ZERO && cabsv(pq) < NEARLY_ZERO)) malpha = csub(cz(), cdiv_naive(rz, pq));
+  double2 rzn = cz();
+  double mx = 0.0;
+  for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT) {
+    const double2 u = cadd(a.U[g], cmul(malpha, a.P[g]));
+    const double2 r = cadd(a.R[g], cmul(malpha, a.Q[g]));
+    const double2 z = cmul(r, a.PC[g]);
+    a.U[g] = u;
+    a.R[g] = r;
+    a.Z[g] = z;
+    mx = fmax(mx, cabsv(r));
+    rzn = cadd(rzn, cmul(r, z));
+  }
+  rzn = block_csum(rzn, shc);
+  mx = block_max(mx, shm);
+  if (threadIdx.x == 0) {
+    a.part[a.G + blockIdx.x] = rzn;
+    a.part[2 * a.G + blockIdx.x] = make_double2(mx, 0.0);
+  }
+}
+__global__ __launch_bounds__(VT) void k_iter_c(VecArgs a) {
+  const Scal *s = a.sc;
+  if (s->done) return;
+  __shared__ double2 shc[VT / 64 + 1];
+  __shared__ double shm[VT / 64 + 1];
+  const double2 rzn = reduce_parts(a.part + a.G, a.G, shc);
+  const double mx = reduce_parts_max(a.part + 2 * a.G, a.G, shm);
+  const int loop = s->loop;
+  const double resi = mx * s->r0inv;
+  const double2 rzo = s->rz[loop & 1];
+  double2 beta = cz();
+  if (!(cabsv(rzn) < NEARLY_ZERO && cabsv(rzo) < NEARLY_ZERO)) beta = cdiv_naive(rzn, rzo);
+  const bool conv = resi < s->eps;
+  for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT)
+    a.P[g] = conv ? a.U[g] : cadd(a.Z[g], cmul(beta, a.P[g]));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Scal *ws = a.sc;
+    a.hist[loop] = resi;
+    ws->resi = resi;
+    ws->conv = conv ? 1 : 0;
+    ws->rz[(loop + 1) & 1] = rzn;
+  }
+}
+// status machine of the main loop (hddm_solver.h:125-143)
+__global__ void k_iter_d(Scal *s) {
+  if (s->done) return;
+  s->iters += 1;
+  if (s->conv) s->done = 1;
+  if (s->loop == s->max_loop) s->done = 1;
+  s->loop += 1;
+}
+hipError_t launch_iter(const VecArgs &a, hipStream_t st) {
+  k_iter_a<<<a.G, VT, 0, st>>>(a);
+  k_iter_b<<<a.G, VT, 0, st>>>(a);
+  k_iter_c<<<a.G, VT, 0, st>>>(a);
+  k_iter_d<<<1, 1, 0, st>>>(a.sc);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(VT) void k_gather_q(VecArgs a) {
+  for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT)
+    a.Q[g] = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
+}
+hipError_t launch_gather_q(const VecArgs &a, hipStream_t st) {
+  k_gather_q<<<a.G, VT, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+__global__ void k_fill(double2 *p, long long n, double2 v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+hipError_t launch_fill(double2 *p, long long n, double2 v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long nb = (n + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  k_fill<<<(int)nb, 256, 0, st>>>(p, n, v);
+  return hipGetLastError();
+}
+
+__global__ void k_scal_init(Scal *s, double eps, int max_loop) {
+  s->eps = eps;
+  s->max_loop = max_loop;
+  s->done = 0;
+  s->conv = 0;
+  s->loop = 0;
+  s->iters = 0;
+}
+hipError_t launch_scal_init(Scal *sc, double eps, int max_loop, hipStream_t st) {
+  k_scal_init<<<1, 1, 0, st>>>(sc, eps, max_loop);
+  return hipGetLastError();
+}
+
+}  // namespace fw

@@ -1,0 +1,239 @@
+"""ctypes front end of libfwhip.so (C ABI: include/fwhip.h).
+
+Python-side mirror of the reference's solver entry point for this path,
+hddm_solver_CS(PartMesh*, OPTIONS, set_hddm_func) (lib/hddm/hddm_solver.h:25-183):
+`Solver(model).solve(eps, max_loop)` runs LOOP_1ST / main loop / LOOP_LAST on
+one GPU and returns the per-subdomain solutions, the owned interface values
+and the residual history the reference logs to calc_log/log_g_HDDM_Full_Wave.
+
+The library is loaded from this package directory; there is no CPU fallback:
+if libfwhip.so is missing or no GPU is visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfwhip.so")
+_LIB = None
+
+FWH_E = {-1: "FWH_E_ARG", -2: "FWH_E_HIP", -3: "FWH_E_MESH", -4: "FWH_E_STATE", -5: "FWH_E_LIMIT"}
+
+EXPORTS = ("fwh_create", "fwh_destroy", "fwh_last_error", "fwh_load", "fwh_get_info", "fwh_solve",
+           "fwh_first", "fwh_iterate", "fwh_last", "fwh_get_stats", "fwh_get_x", "fwh_get_u",
+           "fwh_get_resid", "fwh_schur_apply", "fwh_ldl_solve_all", "fwh_time_kernels", "fwh_get_sub",
+           "fwh_element_batch", "fwh_cocg_icc_all")
+
+
+class FwhOpts(ctypes.Structure):
+    _fields_ = [("hddm_eps", ctypes.c_double), ("hddm_max_loop", ctypes.c_int),
+                ("plain", ctypes.c_int), ("graph_iters", ctypes.c_int)]
+
+
+class FwhStats(ctypes.Structure):
+    _fields_ = [("loops", ctypes.c_int), ("iters", ctypes.c_int), ("converged", ctypes.c_int),
+                ("r0", ctypes.c_double), ("final_resid", ctypes.c_double), ("ms_first", ctypes.c_double),
+                ("ms_loop", ctypes.c_double), ("ms_last", ctypes.c_double), ("sub_solves", ctypes.c_long)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FwhInfo(ctypes.Structure):
+    _fields_ = [("nsub", ctypes.c_int), ("npart", ctypes.c_int), ("n_iface", ctypes.c_int),
+                ("n_dof", ctypes.c_long), ("n_slots", ctypes.c_long), ("nnz", ctypes.c_long),
+                ("nlen", ctypes.c_long), ("n_elem", ctypes.c_long), ("n_cpl", ctypes.c_long),
+                ("n_grow", ctypes.c_long), ("max_n", ctypes.c_int), ("ms_assemble", ctypes.c_double),
+                ("ms_factor", ctypes.c_double), ("bytes_solve", ctypes.c_double),
+                ("bytes_iter", ctypes.c_double), ("flops_factor", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FwhError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libfwhip.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise FwhError(f"{LIB_PATH} not built: run adventure_fullwave_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        D = ctypes.c_double
+        L.fwh_last_error.restype = ctypes.c_char_p
+        L.fwh_create.argtypes = [ctypes.POINTER(P), I]
+        L.fwh_destroy.argtypes = [P]
+        L.fwh_destroy.restype = None
+        L.fwh_load.argtypes = [P, P]
+        L.fwh_get_info.argtypes = [P, P]
+        L.fwh_solve.argtypes = [P, P, P]
+        L.fwh_first.argtypes = [P, P]
+        L.fwh_iterate.argtypes = [P, P, I, ctypes.POINTER(I)]
+        L.fwh_last.argtypes = [P]
+        L.fwh_get_stats.argtypes = [P, P]
+        L.fwh_get_x.argtypes = [P, P]
+        L.fwh_get_u.argtypes = [P, P]
+        L.fwh_get_resid.argtypes = [P, P, I]
+        L.fwh_schur_apply.argtypes = [P, P, P, I]
+        L.fwh_ldl_solve_all.argtypes = [P, P, I]
+        L.fwh_time_kernels.argtypes = [P, I, ctypes.POINTER(D), ctypes.POINTER(D)]
+        L.fwh_get_sub.argtypes = [P, I, I, P]
+        L.fwh_element_batch.argtypes = [P, I, P, P, P, D, P, P, P, P, P]
+        L.fwh_cocg_icc_all.argtypes = [P, P, P, P, D, D]
+        for name in EXPORTS:
+            if name not in ("fwh_destroy", "fwh_last_error"):
+                getattr(L, name).restype = I
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().fwh_last_error().decode(errors="replace")
+        raise FwhError(f"{what}: {FWH_E.get(rc, rc)}: {msg}")
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    """One device context (fwh_create / fwh_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self.h = ctypes.c_void_p()
+        _check(lib().fwh_create(ctypes.byref(self.h), device), "fwh_create")
+
+    def close(self):
+        if self.h:
+            lib().fwh_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def element_batch(self, xyz, eps, sigma, omega, pml=None, src=None, joe=None):
+        nel = len(eps)
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(nel, 12)
+        eps = np.ascontiguousarray(eps, np.float64)
+        sigma = np.ascontiguousarray(sigma, np.float64)
+        pml = None if pml is None else np.ascontiguousarray(pml, np.float64).reshape(nel, 6)
+        src = None if src is None else np.ascontiguousarray(src, np.int32)
+        joe = None if joe is None else np.ascontiguousarray(joe, np.float64).reshape(nel, 24)
+        ae = np.zeros((nel, 72))
+        be = np.zeros((nel, 12))
+        _check(lib().fwh_element_batch(self.h, nel, _ptr(xyz), _ptr(eps), _ptr(sigma), float(omega),
+                                       _ptr(pml), _ptr(src), _ptr(joe), _ptr(ae), _ptr(be)),
+               "fwh_element_batch")
+        return ae.view(np.complex128).reshape(nel, 6, 6), be.view(np.complex128).reshape(nel, 6)
+
+
+class Solver:
+    """A decomposed model resident on the GPU (fwh_load) and its HDDM solve."""
+
+    def __init__(self, model, device: int = 0, ctx: Optional[Context] = None):
+        self.model = model
+        self.ctx = ctx or Context(device)
+        self._desc = model.desc()
+        _check(lib().fwh_load(self.ctx.h, ctypes.byref(self._desc)), "fwh_load")
+        inf = FwhInfo()
+        _check(lib().fwh_get_info(self.ctx.h, ctypes.byref(inf)), "fwh_get_info")
+        self.info = inf.as_dict()
+
+    @staticmethod
+    def opts(eps=1e-8, max_loop=100000, plain=False, graph_iters=16):
+        return FwhOpts(float(eps), int(max_loop), int(bool(plain)), int(graph_iters))
+
+    def solve(self, eps=1e-8, max_loop=100000, plain=False, graph_iters=16):
+        o = self.opts(eps, max_loop, plain, graph_iters)
+        st = FwhStats()
+        _check(lib().fwh_solve(self.ctx.h, ctypes.byref(o), ctypes.byref(st)), "fwh_solve")
+        self.last_opts = o
+        return st.as_dict()
+
+    def first(self, eps=1e-8, max_loop=100000, plain=False, graph_iters=16):
+        self.last_opts = self.opts(eps, max_loop, plain, graph_iters)
+        _check(lib().fwh_first(self.ctx.h, ctypes.byref(self.last_opts)), "fwh_first")
+
+    def iterate(self, n):
+        done = ctypes.c_int()
+        _check(lib().fwh_iterate(self.ctx.h, ctypes.byref(self.last_opts), int(n), ctypes.byref(done)),
+               "fwh_iterate")
+        return bool(done.value)
+
+    def last(self):
+        _check(lib().fwh_last(self.ctx.h), "fwh_last")
+
+    def stats(self):
+        st = FwhStats()
+        _check(lib().fwh_get_stats(self.ctx.h, ctypes.byref(st)), "fwh_get_stats")
+        return st.as_dict()
+
+    def x(self):
+        out = np.zeros(int(self.info["n_dof"]), np.complex128)
+        _check(lib().fwh_get_x(self.ctx.h, _ptr(out)), "fwh_get_x")
+        return out
+
+    def x_list(self):
+        x = self.x()
+        off = np.concatenate([[0], np.cumsum(np.asarray(self.model.nmp, np.int64))])
+        return [x[off[s]:off[s + 1]] for s in range(self.model.nsub)]
+
+    def u(self):
+        out = np.zeros(int(self.info["n_iface"]), np.complex128)
+        _check(lib().fwh_get_u(self.ctx.h, _ptr(out)), "fwh_get_u")
+        return out
+
+    def resid(self, n):
+        out = np.zeros(int(n))
+        _check(lib().fwh_get_resid(self.ctx.h, _ptr(out), int(n)), "fwh_get_resid")
+        return out
+
+    def schur_apply(self, p, plain=False):
+        p = np.ascontiguousarray(p, np.complex128)
+        q = np.zeros_like(p)
+        _check(lib().fwh_schur_apply(self.ctx.h, _ptr(p), _ptr(q), int(bool(plain))), "fwh_schur_apply")
+        return q
+
+    def ldl_solve_all(self, rhs, plain=False):
+        v = np.ascontiguousarray(rhs, np.complex128).copy()
+        _check(lib().fwh_ldl_solve_all(self.ctx.h, _ptr(v), int(bool(plain))), "fwh_ldl_solve_all")
+        return v
+
+    def time_kernels(self, reps=10):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().fwh_time_kernels(self.ctx.h, int(reps), ctypes.byref(a), ctypes.byref(b)),
+               "fwh_time_kernels")
+        return a.value, b.value
+
+    def sub(self, s):
+        L = lib()
+        sz = np.zeros(4, np.int32)
+        _check(L.fwh_get_sub(self.ctx.h, s, 7, _ptr(sz)), "fwh_get_sub")
+        n, nnz, nlen, ng = (int(x) for x in sz)
+        val = np.zeros(max(nnz, 1), np.complex128)
+        rowptr = np.zeros(n + 1, np.int32)
+        col = np.zeros(max(nnz, 1), np.int32)
+        f = np.zeros(n, np.complex128)
+        idx1 = np.zeros(n, np.int32)
+        idx2 = np.zeros(n, np.int32)
+        fac = np.zeros(max(nlen, 1), np.complex128)
+        for what, arr in ((0, val), (1, rowptr), (2, col), (3, f), (4, idx1), (5, idx2), (6, fac)):
+            _check(L.fwh_get_sub(self.ctx.h, s, what, _ptr(arr)), "fwh_get_sub")
+        return dict(n=n, nnz=nnz, nlen=nlen, ng=ng, val=val[:nnz], rowptr=rowptr, col=col[:nnz], f=f,
+                    idx1=idx1, idx2=idx2, factor=fac[:nlen])
+
+    def close(self):
+        self.ctx.close()

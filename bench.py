@@ -1,0 +1,123 @@
+"""Benchmark of the HDDM full-wave hot path on one MI355X.
+
+Metric: COCG interface iterations/s (and subdomain LDL^T solves/s) of the
+batched GPU form, solving to ||r||inf/||r0||inf < 1e-8, next to the plain
+one-subdomain-per-launch GPU form and the CPU oracle.
+
+A step is one complete interface solve of the workload (first step with the
+load vector, COCG iterations to 1e-8, last step), with the model already
+assembled and factored in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "COCG interface iters/s + subdomain LDL solves/s, batched vs plain, at 1e-8 res"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cfg", type=int, default=2)
+    ap.add_argument("--eps", type=float, default=1e-8)
+    ap.add_argument("--graph-iters", type=int, default=16)
+    ap.add_argument("--plain-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from adventure_fullwave_amd import fwhip, mesh
+
+    t0 = time.time()
+    model = mesh.make_config(args.cfg)
+    log(f"[bench] cfg{args.cfg}: {model.n_edges} edges, {model.n_tets} tets, {model.nsub} subdomains, "
+        f"{model.n_interface} interface unknowns ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    sol = fwhip.Solver(model)
+    info = sol.info
+    log(f"[bench] loaded: assemble {info['ms_assemble']:.1f} ms, factor {info['ms_factor']:.1f} ms, "
+        f"sum nlen {info['nlen']} ({time.time() - t0:.1f}s)")
+
+    # warmup + timed batched solves
+    for _ in range(args.warmup):
+        st = sol.solve(eps=args.eps, max_loop=200000, graph_iters=args.graph_iters)
+        log(f"[bench] warmup: {st['iters']} iterations, converged={st['converged']}")
+    iters = 0
+    solves = 0
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        st = sol.solve(eps=args.eps, max_loop=200000, graph_iters=args.graph_iters)
+        iters += st["iters"]
+        solves += st["sub_solves"]
+        log(f"[bench] step {k}: {st['iters']} iterations, resid {st['final_resid']:.3e}")
+    dt = time.perf_counter() - t_start
+    iters_per_s = iters / dt
+    solves_per_s = solves / dt
+
+    # plain form on a bounded window of iterations
+    sol.first(eps=1e-30, max_loop=args.plain_iters + 1, plain=True, graph_iters=0)
+    tp = time.perf_counter()
+    sol.iterate(args.plain_iters)
+    dtp = time.perf_counter() - tp
+    plain_iters_per_s = args.plain_iters / dtp
+    log(f"[bench] plain: {plain_iters_per_s:.2f} iters/s")
+
+    # dominant kernel: the batched subdomain Schur/LDL^T kernel (HIP events)
+    ms_solve, ms_schur = sol.time_kernels(reps=20)
+    bytes_iter_kernel = info["bytes_iter"] - 128.0 * info["n_iface"]
+    achieved = bytes_iter_kernel / (ms_schur * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_solve (MODE_ITER, all subdomains)", "ms_per_launch": round(ms_schur, 4),
+                "bytes_per_launch": bytes_iter_kernel,
+                "ldl_only": {"ms": round(ms_solve, 4), "bytes": info["bytes_solve"],
+                             "GBs": round(info["bytes_solve"] / (ms_solve * 1e-3) / 1e9, 1)}}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import fw_oracle as fo
+        tc, ns = fo.time_iterations(model, args.cpu_iters)
+        cpu = {"value": round(args.cpu_iters / tc, 4), "unit": "iters/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_iters} interface iterations of the same workload "
+                         f"({ns} subdomain solves, {tc:.1f}s) with the C oracle, one core",
+               "ldl_solves_per_s": round(ns / tc, 1)}
+    out = {
+        "metric": METRIC, "value": round(iters_per_s, 2), "unit": "iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000 * dt / args.steps, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "complex128",
+        "data": "synthetic",
+        "config": {"workload": f"cfg{args.cfg}: {model.meta.get('dims')} cell box, {model.n_edges} edge DOFs, "
+                               f"{model.nsub} subdomains in {model.npart} parts, PML {model.meta.get('pml_mode')}, "
+                               f"COCG+diag to {args.eps:g}",
+                   "n_iface": info["n_iface"], "iters_per_solve": iters // max(args.steps, 1)},
+        "ldl_solves_per_s": round(solves_per_s, 1),
+        "plain": {"iters_per_s": round(plain_iters_per_s, 3), "window_iters": args.plain_iters},
+        "batched_over_plain": round(iters_per_s / plain_iters_per_s, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "setup_ms": {"assemble": round(info["ms_assemble"], 2), "factor": round(info["ms_factor"], 2)},
+    }
+    print(json.dumps(out), flush=True)
+    sol.close()
+
+
+if __name__ == "__main__":
+    main()

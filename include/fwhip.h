@@ -1,0 +1,170 @@
+/*
+ * fwhip.h - C ABI of the MI355X (gfx950) HDDM full-wave library libfwhip.so.
+ *
+ * Plain C linkage, plain pointers and sizes; no HIP or torch types cross this
+ * boundary.  Complex values are interleaved doubles (re, im), the layout of
+ * the reference's Complex struct (lib/std/types.h:15-18) and of C99
+ * double _Complex.  Every entry point returns 0 on success or a negative
+ * FWH_E_* code; fwh_last_error() returns the message of the last failure in
+ * the calling thread.  The reference turns failures into
+ * emessage(code, abort, str) -> COM_Exit (lib/std/errormessage.h:13); a
+ * reference-side hook does the same with these codes.
+ *
+ * Each entry point names the reference interface it replaces.  The reference
+ * plugs its subdomain work into the HDDM_FUNC hook table
+ * (lib/std/types.h:629-730) filled by set_hddm_func_fullwave
+ * (main/fullwave/fullwave.c:23-146) and driven by hddm_solver_CS
+ * (lib/hddm/hddm_solver.h:25-183); this library keeps that call structure
+ * (first step / iterations / last step) but runs every subdomain of every
+ * part on one device.
+ */
+#ifndef FWHIP_H
+#define FWHIP_H
+
+#include "fwh_mesh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FWH_OK 0
+#define FWH_E_ARG (-1)     /* invalid argument / inconsistent model        */
+#define FWH_E_HIP (-2)     /* HIP runtime failure                           */
+#define FWH_E_MESH (-3)    /* degenerate element (six-times volume <= 0,
+                              zero-length edge: emessage 2050 in
+                              lib/fem/tetrahedral.c:257-258)                */
+#define FWH_E_STATE (-4)   /* call out of order (e.g. iterate before load)  */
+#define FWH_E_LIMIT (-5)   /* a size exceeds a compiled kernel limit        */
+
+typedef struct fwh_ctx fwh_ctx;
+
+/* Solver options: the reference's HDDM_Opt / SolvOpt fields on this path
+ * (lib/std/types.h:116-141; defaults lib/std/defs.h:120-205).            */
+typedef struct fwh_opts {
+  double hddm_eps;      /* -hddm-eps: stop when ||r||inf/||r0||inf < eps   */
+  int    hddm_max_loop; /* -hddm-max-loop (0: not executed)                */
+  int    plain;         /* 1: one subdomain per launch, reference order;
+                           0: batched (all subdomains per launch, graph)   */
+  int    graph_iters;   /* batched: iterations per captured graph (0=no
+                           graph, plain launches)                          */
+} fwh_opts;
+
+typedef struct fwh_stats {
+  int    loops;         /* hddm_sw.loop at exit                            */
+  int    iters;         /* interface iterations executed                   */
+  int    converged;     /* stop test fired                                 */
+  double r0;            /* ||r0||inf                                       */
+  double final_resid;   /* last ||r||inf/||r0||inf                         */
+  double ms_first;      /* first step (solve + PC) wall ms                 */
+  double ms_loop;       /* main loop wall ms                               */
+  double ms_last;       /* last step wall ms                               */
+  long   sub_solves;    /* subdomain LDL^T solves performed                */
+} fwh_stats;
+
+typedef struct fwh_info {
+  int    nsub, npart;
+  int    n_iface;       /* global interface unknowns N_Gamma               */
+  long   n_dof;         /* sum of subdomain DOFs                           */
+  long   n_slots;       /* sum of subdomain interface DOFs                 */
+  long   nnz;           /* sum of assembled lower-triangle entries         */
+  long   nlen;          /* sum of skyline lengths                          */
+  long   n_elem;        /* elements                                        */
+  long   n_cpl;         /* interior-interface coupling entries (A_I,G)     */
+  long   n_grow;        /* entries of the interface rows (A_G,.)           */
+  int    max_n;         /* largest subdomain                               */
+  double ms_assemble;   /* setup timings                                   */
+  double ms_factor;
+  double bytes_solve;   /* algorithmic bytes of one LDL^T solve of every
+                           subdomain: sum 16 nlen + 40 n                   */
+  double bytes_iter;    /* algorithmic bytes of one interface iteration    */
+  double flops_factor;  /* complex-FP64 flops of the factorisation         */
+} fwh_info;
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart (the reference has no device).                  */
+int  fwh_create(fwh_ctx **ctx, int device);
+void fwh_destroy(fwh_ctx *ctx);
+const char *fwh_last_error(void);
+
+/* ---- model load: assembly + Dirichlet structure + skyline + factor ------
+ * Replaces, for every subdomain of every part, HDDM_make_TMP_AIJ_em_E
+ * (main/fullwave/fullwave.c:221-259 -> tet_MakeElement_EM_E,
+ * lib/fem/make_elem_em.c:14-400, mymt_CS_AIJ_Set/sort/DeleteZero) and the
+ * factorisation half of hddm_solver_CS_solve_dom_1st_d
+ * (lib/hddm/hddm_solver.h:485-514: Copy_TMP_AIJ, SetBC,
+ * advlas_copy_nzindex_to_skyindex, advlas_copy_nzmat_to_skymat,
+ * advlas_skyline_ldl_decomposite).  The model description is copied. */
+int  fwh_load(fwh_ctx *ctx, const fwh_mesh_desc *m);
+int  fwh_get_info(const fwh_ctx *ctx, fwh_info *info);
+
+/* ---- the HDDM solve -----------------------------------------------------
+ * fwh_solve replaces __HDDM_SOLVER_PARENT__ in s-mode
+ * (lib/hddm/hddm_solver.h:59-183 with hddm_solver_s.h:7-438,
+ * hddm_solver_sp.h:8-69, hddm_pc.h:52-137): LOOP_1ST, the main loop with
+ * its status machine, LOOP_LAST.  The three steps are also exported:
+ *   fwh_first   = LOOP_1ST + iteration_s_1st (hddm_solver_s.h:7-52,158-256)
+ *   fwh_iterate = n x (LOOP + iteration_s)   (hddm_solver_s.h:69-112,260-400)
+ *   fwh_last    = LOOP_LAST + get_result_sp  (hddm_solver_s.h:116-154,
+ *                 main/fullwave/fullwave.c:500-514)                       */
+int  fwh_solve(fwh_ctx *ctx, const fwh_opts *opts, fwh_stats *stats);
+int  fwh_first(fwh_ctx *ctx, const fwh_opts *opts);
+/* runs up to n iterations; *done = 1 once the status is LAST             */
+int  fwh_iterate(fwh_ctx *ctx, const fwh_opts *opts, int n, int *done);
+int  fwh_last(fwh_ctx *ctx);
+int  fwh_get_stats(const fwh_ctx *ctx, fwh_stats *stats);
+
+/* Results.  x: per subdomain solution (dm->x_c), concatenated in subdomain
+ * order, 2*nmp doubles each.  u: interface values of the owned DOFs, part
+ * by part in ln_part order (HDDM_VEC.u, 2*infree each).  resid: residual
+ * history ||r||inf/||r0||inf for loops 0..nresid-1 (the values the
+ * reference writes to calc_log/log_g_HDDM_Full_Wave).                    */
+int  fwh_get_x(const fwh_ctx *ctx, double *x);
+int  fwh_get_u(const fwh_ctx *ctx, double *u);
+int  fwh_get_resid(const fwh_ctx *ctx, double *resid, int nresid);
+
+/* ---- single operators (parity tests, benchmarks) ----------------------- */
+/* One Schur-complement application q = S p over all subdomains
+ * (renewal_bc_sp + solve_dom_d + COM_res_sp + gather,
+ * hddm_solver_sp.h:28-69, hddm_solver.h:559-622, hddm_solver_s.h:404-424).
+ * p, q: owned interface values, 2*n_iface doubles, part-major.           */
+int  fwh_schur_apply(fwh_ctx *ctx, const double *p, double *q, int plain);
+/* advlas_skyline_ldl_solve (lib/hddm/advlas_skyline.c:134-167) applied to
+ * every subdomain at once: rhs/x are 2*n_dof doubles (in place).          */
+int  fwh_ldl_solve_all(fwh_ctx *ctx, double *rhs, int plain);
+/* Device time (HIP events on the library stream) of one batched LDL^T solve
+ * of all subdomains and of one batched Schur-step kernel (Dirichlet load,
+ * solve, interface rows of A x), averaged over `reps` launches, in ms.    */
+int  fwh_time_kernels(fwh_ctx *ctx, int reps, double *ms_solve,
+                      double *ms_schur);
+
+/* Subdomain arrays for inspection.  what: 0 = assembled lower-triangle
+ * values (2*nnz_s doubles, CSR order), 1 = CSR row pointer (n_s+1 ints),
+ * 2 = CSR columns (nnz_s ints), 3 = load f (2*n_s doubles),
+ * 4 = skyline idx1 (n_s ints), 5 = skyline idx2 (n_s ints),
+ * 6 = factor (2*nlen_s doubles; L strictly lower, 1/d on the diagonal,
+ * advlas_skyline.c:123-125), 7 = sizes {n_s, nnz_s, nlen_s, ninbc_s} ints */
+int  fwh_get_sub(const fwh_ctx *ctx, int s, int what, void *out);
+
+/* Element kernel alone (tet_MakeElement_EM_E, make_elem_em.c:14-400) on a
+ * batch: xyz [nel*12], eps/sigma [nel], pml [nel*6] or NULL, src [nel],
+ * joe [nel*24] (4 vertices x 3 complex).  ae: [nel*72] (6x6 complex,
+ * row-major), be: [nel*12].                                               */
+int  fwh_element_batch(fwh_ctx *ctx, int nel, const double *xyz,
+                       const double *eps, const double *sigma, double omega,
+                       const double *pml, const int *src, const double *joe,
+                       double *ae, double *be);
+
+/* Subdomain COCG with IC(0) (mymt_CS_AIJ_Solver_COCG,
+ * lib/mymt_aij_solver/aij_solver_cg.h:9-44,113-355 with
+ * mymt_CS_AIJ_PC_ICC_make/set, aij_pc_func.h:36-167) on every subdomain's
+ * Dirichlet matrix with load b (2*n_dof doubles, concatenated), zero
+ * initial guess.  x: 2*n_dof doubles; status: per subdomain the reference
+ * return value (>=0 iterations, <0 error).                                */
+int  fwh_cocg_icc_all(fwh_ctx *ctx, const double *b, double *x, int *status,
+                      double eps, double ic_shift);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FWHIP_H */

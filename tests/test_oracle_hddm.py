@@ -1,0 +1,119 @@
+"""Pins the oracle's s-mode HDDM (lib/hddm/hddm_solver.h:59-183,
+hddm_solver_s.h:7-438, hddm_pc.h:52-137) against independent mathematics:
+the converged solution against a global sparse direct solve of the assembled
+system, the subdomain Schur step against a dense Schur complement, the
+COCG+ICC subdomain solver against dense solves, and the reference's status
+machine (max-loop 0, max-loop reached)."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spl
+
+from oracle import fw_oracle as fo
+from tests.helpers import dense_lower, global_system, to_global
+
+
+@pytest.fixture(scope="module")
+def om(small_model):
+    return fo.OracleModel(small_model)
+
+
+def test_hddm_matches_direct_solve(small_model, om):
+    A, F, free = global_system(small_model, om)
+    xd = np.zeros(small_model.n_edges, complex)
+    xd[free] = spl.spsolve(A[free][:, free].tocsc(), F[free])
+    res = fo.hddm_solve(small_model, eps=1e-10, max_loop=5000)
+    assert res["stats"]["converged"] == 1
+    x = to_global(small_model, res["x"])
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-8
+    # shared edges agree between subdomains
+    off = 0
+    seen = {}
+    for s in range(small_model.nsub):
+        n = small_model.nmp[s]
+        for gid, val in zip(small_model.ledge_global[off:off + n], res["x"][s]):
+            if gid in seen:
+                assert abs(seen[gid] - val) <= 1e-12 * np.abs(xd).max()
+            seen[gid] = val
+        off += n
+    # residual history starts at 1 and ends below eps
+    r = res["resid"]
+    assert r[0] == 1.0 and r[-1] < 1e-10
+
+
+def test_schur_step_vs_dense(small_model, om):
+    s = 3
+    g = om.get(s)
+    n = g["n"]
+    A = dense_lower(g["rowptr"], g["col"], g["val"], n)
+    ib = small_model.ib_nfid[small_model.ib_off[s]:small_model.ib_off[s + 1]]
+    ess = small_model.ibc[small_model.bc_off[s]:small_model.bc_off[s + 1]]
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal(len(ib)) + 1j * rng.standard_normal(len(ib))
+    x, y = om.schur_sub(s, p)
+    inter = np.setdiff1d(np.arange(n), np.concatenate([ib, ess]))
+    xd = np.zeros(n, complex)
+    xd[ib] = p
+    xd[inter] = np.linalg.solve(A[np.ix_(inter, inter)], -A[np.ix_(inter, ib)] @ p)
+    assert np.abs(x - xd).max() < 1e-10 * np.abs(xd).max()
+    S = A[np.ix_(ib, ib)] - A[np.ix_(ib, inter)] @ np.linalg.solve(A[np.ix_(inter, inter)], A[np.ix_(inter, ib)])
+    assert np.allclose(S, S.T, rtol=1e-10, atol=1e-12 * np.abs(S).max())
+    assert np.abs(y[ib] - S @ p).max() < 1e-9 * np.abs(S @ p).max()
+
+
+def test_profile_after_setbc(small_model, om):
+    """Dirichlet rows are identity rows; the profile starts at the first
+    non-zero non-Dirichlet column (SetBC + DeleteZero + skyindex)."""
+    s = 1
+    g = om.get(s)
+    n = g["n"]
+    bc = np.zeros(n, bool)
+    bc[small_model.ib_nfid[small_model.ib_off[s]:small_model.ib_off[s + 1]]] = True
+    bc[small_model.ibc[small_model.bc_off[s]:small_model.bc_off[s + 1]]] = True
+    for i in range(n):
+        cols = g["col"][g["rowptr"][i]:g["rowptr"][i + 1]]
+        vals = g["val"][g["rowptr"][i]:g["rowptr"][i + 1]]
+        if bc[i]:
+            assert g["idx1"][i] == i
+            assert g["factor"][g["idx2"][i] + i] == 1.0
+        else:
+            keep = cols[(~bc[cols]) & (vals != 0)]
+            assert g["idx1"][i] == keep.min()
+
+
+def test_max_loop_zero_not_executed(small_model, capfd):
+    res = fo.hddm_solve(small_model, eps=1e-8, max_loop=0)
+    assert res["stats"]["iters"] == 0
+
+
+def test_max_loop_reached_stops(small_model):
+    res = fo.hddm_solve(small_model, eps=1e-30, max_loop=7)
+    assert res["stats"]["iters"] == 7 and res["stats"]["converged"] == 0
+    assert res["stats"]["loops"] == 8
+
+
+def test_cocg_icc_subdomain(small_model, om):
+    """COCG+ICC on a subdomain's interior matrix converges to the dense
+    solution; ICC of a matrix without fill is exact (one iteration)."""
+    s = 0
+    g = om.get(s)
+    n = g["n"]
+    A = dense_lower(g["rowptr"], g["col"], g["val"], n)
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    st, x, hist = fo.cocg_icc(g["rowptr"], g["col"], g["val"], b, eps=1e-10, ic_shift=1.2)
+    assert st > 0
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-7
+    # tridiagonal complex-symmetric: IC(0) = exact LDL^T -> converges at once
+    m = 40
+    rp = np.concatenate([[0], np.cumsum([1] + [2] * (m - 1))]).astype(np.int32)
+    col, val = [], []
+    for i in range(m):
+        if i > 0:
+            col.append(i - 1)
+            val.append(0.3 - 0.1j)
+        col.append(i)
+        val.append(2.0 + 0.5j)
+    b = np.ones(m, complex)
+    st, x, hist = fo.cocg_icc(rp, np.array(col), np.array(val), b, eps=1e-12, ic_shift=1.0)
+    assert st == 1
