@@ -87,6 +87,9 @@ struct fwh_ctx {
   double2 *gr_val = nullptr;
   int *g_nfid = nullptr, *g_gid = nullptr;
   int *gq_ptr = nullptr, *gq_slot = nullptr;
+  // Dirichlet CSR + IC(0) for the subdomain COCG (built on first use)
+  int *dptr = nullptr, *dcol = nullptr, *fptr = nullptr, *fcol = nullptr;
+  double2 *dval = nullptr, *dL = nullptr, *ddg = nullptr, *fval = nullptr;
 
   // vectors
   double2 *P = nullptr, *Q = nullptr, *R = nullptr, *Z = nullptr, *U = nullptr, *PC = nullptr;
@@ -1029,13 +1032,110 @@ extern "C" int fwh_element_batch(fwh_ctx *c, int nel, const double *xyz, const d
   return 0;
 }
 
+// Dirichlet matrix of every subdomain as the reference's AIJ_MAT after
+// SetBC + DeleteZero + Assem (aij_matrix_func.h:60-235): Dirichlet rows are
+// identity rows, Dirichlet columns and exact zeros are dropped; rows sorted,
+// diagonal last.  Built once from the assembled values.
+namespace {
+int build_dirichlet_csr(fwh_ctx *c) {
+  if (c->dptr) return 0;
+  std::vector<double2> val(c->nnz);
+  std::vector<int> rowptr(c->ndof + 1), col(c->nnz), ent_row(c->nnz), dsub(c->ndof);
+  std::vector<int8_t> bc(c->ndof);
+  HIPCHK(hipMemcpy(val.data(), c->aval, c->nnz * sizeof(double2), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(rowptr.data(), c->rowptr, (c->ndof + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(col.data(), c->col, c->nnz * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(bc.data(), c->bcflag, c->ndof * sizeof(int8_t), hipMemcpyDeviceToHost));
+  std::vector<int> dptr(c->ndof + 1, 0), dcol;
+  std::vector<double2> dval;
+  std::vector<std::vector<std::pair<int, double2>>> up(c->nmax);
+  std::vector<int> fptr(c->ndof + 1, 0), fcol;
+  std::vector<double2> fval;
+  for (int s = 0; s < c->nsub; s++) {
+    const SubMeta &M = c->meta_h[s];
+    for (int i = 0; i < M.n; i++) up[i].clear();
+    for (int i = 0; i < M.n; i++) {
+      const long long d = M.dof0 + i;
+      if (bc[d]) {
+        dcol.push_back(i);
+        dval.push_back(make_double2(1.0, 0.0));
+      } else {
+        for (int e = rowptr[d]; e < rowptr[d + 1]; e++) {
+          const int j = col[e];
+          if (bc[M.dof0 + j] || (val[e].x == 0.0 && val[e].y == 0.0)) continue;
+          dcol.push_back(j);
+          dval.push_back(val[e]);
+          if (j != i) up[j].push_back({i, val[e]});
+        }
+      }
+      dptr[d + 1] = (int)dcol.size();
+    }
+    for (int i = 0; i < M.n; i++) {
+      const long long d = M.dof0 + i;
+      for (int e = dptr[d]; e < dptr[d + 1]; e++) {
+        fcol.push_back(dcol[e]);
+        fval.push_back(dval[e]);
+      }
+      for (auto &pv : up[i]) {
+        fcol.push_back(pv.first);
+        fval.push_back(pv.second);
+      }
+      fptr[d + 1] = (int)fcol.size();
+    }
+  }
+  int rc;
+  if ((rc = c->upload(&c->dptr, dptr)) || (rc = c->upload(&c->dcol, dcol)) || (rc = c->upload(&c->dval, dval)) ||
+      (rc = c->alloc(&c->dL, dval.size())) || (rc = c->alloc(&c->ddg, c->ndof)) ||
+      (rc = c->upload(&c->fptr, fptr)) || (rc = c->upload(&c->fcol, fcol)) || (rc = c->upload(&c->fval, fval)))
+    return rc;
+  return 0;
+}
+}  // namespace
+
 extern "C" int fwh_cocg_icc_all(fwh_ctx *c, const double *b, double *x, int *status, double eps,
                                 double ic_shift) {
-  (void)b;
-  (void)x;
-  (void)status;
-  (void)eps;
-  (void)ic_shift;
-  if (!c) return fail(FWH_E_ARG, "null argument");
-  return fail(FWH_E_STATE, "subdomain COCG/ICC not built in this library version");
+  if (!c || !b || !x || !status) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  if ((size_t)5 * c->nmax * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "subdomain too large for the COCG kernel");
+  HIPCHK(hipSetDevice(c->device));
+  int rc = build_dirichlet_csr(c);
+  if (rc) return rc;
+  double2 *d_b = nullptr;
+  int *d_st = nullptr;
+  fwh_ctx tmp;
+  tmp.st = c->st;
+  std::vector<double2> hb((const double2 *)b, (const double2 *)b + c->ndof);
+  if ((rc = tmp.upload(&d_b, hb)) || (rc = tmp.alloc(&d_st, c->nsub))) {
+    tmp.release();
+    return rc;
+  }
+  fw::IccArgs a{};
+  a.meta = c->meta;
+  a.dptr = c->dptr;
+  a.dcol = c->dcol;
+  a.dval = c->dval;
+  a.L = c->dL;
+  a.dg = c->ddg;
+  a.fptr = c->fptr;
+  a.fcol = c->fcol;
+  a.fval = c->fval;
+  // acceleration factor outside [1, 3] is replaced by 1.05 (aij_pc_func.h:66-73)
+  a.af = (ic_shift > 3.0 || ic_shift < 1.0) ? 1.05 : ic_shift;
+  a.b = d_b;
+  a.x = c->X;
+  a.status = d_st;
+  a.eps = eps;
+  a.div = 1.0e+10;  // DEFAULT_SOLVER_DIV, lib/std/defs.h
+  a.nmax = c->nmax;
+  hipError_t e = hipMemsetAsync(c->X, 0, c->ndof * sizeof(double2), c->st);
+  if (e == hipSuccess) e = fw::launch_icc_make(a, c->nsub, c->st);
+  if (e == hipSuccess) e = fw::launch_cocg(a, c->nsub, c->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(x, c->X, c->ndof * sizeof(double2), hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(status, d_st, c->nsub * sizeof(int), hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  tmp.release();
+  tmp.st = nullptr;
+  c->last_done = false;
+  if (e != hipSuccess) return fail(FWH_E_HIP, "cocg/icc: %s", hipGetErrorString(e));
+  return 0;
 }

@@ -133,3 +133,29 @@ hipError_t launch_fill(double2 *p, long long n, double2 v, hipStream_t st);
 hipError_t launch_scal_init(Scal *sc, double eps, int max_loop, hipStream_t st);
 
 }  // namespace fw
+
+namespace fw {
+
+// subdomain COCG with IC(0) (mymt_CS_AIJ_Solver_COCG, aij_solver_cg.h:113-355;
+// mymt_CS_AIJ_PC_ICC_make/set, aij_pc_func.h:36-167)
+struct IccArgs {
+  const SubMeta *meta;
+  const int *dptr;        // Dirichlet lower CSR, per DOF (n_dof+1, global entry ids)
+  const int *dcol;        // local columns (sorted, diagonal last)
+  const double2 *dval;
+  double2 *L;             // IC(0) factor, same layout as dval
+  double2 *dg;            // per DOF 1/L_ii
+  const int *fptr;        // full symmetric rows of the Dirichlet matrix (n_dof+1)
+  const int *fcol;
+  const double2 *fval;
+  double af;              // ICC acceleration factor
+  const double2 *b;       // per DOF load
+  double2 *x;             // per DOF solution (initial guess in)
+  int *status;            // per subdomain: >=0 iterations, <0 reference status
+  double eps, div;
+  int nmax;
+};
+hipError_t launch_icc_make(const IccArgs &a, int nsub, hipStream_t st);
+hipError_t launch_cocg(const IccArgs &a, int nsub, hipStream_t st);
+
+}  // namespace fw

@@ -859,3 +859,214 @@ hipError_t launch_scal_init(Scal *sc, double eps, int max_loop, hipStream_t st) 
 }
 
 }  // namespace fw
+
+namespace fw {
+
+// ---------------------------------------------------------------------------
+// IC(0) factor of every subdomain's Dirichlet matrix (aij_pc_func.h:36-108),
+// one wavefront per subdomain, row by row:
+//   L_ik = a_ik - sum_{c<k in row i} d_c L_ic L_kc   (entries of row i in order)
+//   L_ii = af a_ii - sum_c d_c L_ic^2,   d_i = 1 / L_ii
+// L_kc is looked up in the finished row k (binary search; zero when (k,c) is
+// outside the pattern, as getAIJ returns).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 lookup_row(const int *__restrict__ col, const double2 *L, int b, int e, int c) {
+  int lo = b, hi = e;  // [lo, hi)
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int cm = col[mid];
+    if (cm == c) return L[mid];
+    if (cm < c) lo = mid + 1; else hi = mid;
+  }
+  return cz();
+}
+
+__global__ __launch_bounds__(64) void k_icc_make(IccArgs a) {
+  const SubMeta m = a.meta[blockIdx.x];
+  extern __shared__ double2 lds[];
+  double2 *dgl = lds;           // [nmax]
+  double2 *u = lds + a.nmax;    // current row
+  const int lane = threadIdx.x;
+  for (int i = 0; i < m.n; i++) {
+    const int d = m.dof0 + i;
+    const int ii = a.dptr[d], jj = a.dptr[d + 1] - 1;
+    for (int t = lane; t <= jj - ii; t += 64) u[t] = a.dval[ii + t];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    for (int row = ii; row < jj; row++) {
+      const int k = a.dcol[row];
+      const int kb = a.dptr[m.dof0 + k], ke = a.dptr[m.dof0 + k + 1];
+      double2 acc = cz();
+      for (int t = ii + lane; t < row; t += 64) {
+        const int c = a.dcol[t];
+        const double2 lkc = lookup_row(a.dcol, a.L, kb, ke, c);
+        acc = cadd(acc, cmul(dgl[c], cmul(u[t - ii], lkc)));
+      }
+      acc = wave_csum(acc);
+      if (lane == 0) u[row - ii] = csub(a.dval[row], acc);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    }
+    double2 acc = cz();
+    for (int t = ii + lane; t < jj; t += 64) {
+      const double2 q = u[t - ii];
+      acc = cadd(acc, cmul(dgl[a.dcol[t]], cmul(q, q)));
+    }
+    acc = wave_csum(acc);
+    if (lane == 0) {
+      const double2 ljj = csub(cscale(a.dval[jj], a.af), acc);
+      u[jj - ii] = ljj;
+      const double2 di = cdiv_naive(make_double2(1.0, 0.0), ljj);
+      dgl[i] = di;
+      a.dg[d] = di;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    for (int t = lane; t <= jj - ii; t += 64) a.L[ii + t] = u[t];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+  }
+}
+hipError_t launch_icc_make(const IccArgs &a, int nsub, hipStream_t st) {
+  if (nsub <= 0) return hipSuccess;
+  // row buffer: the longest Dirichlet row is at most nmax entries
+  k_icc_make<<<nsub, 64, sizeof(double2) * 2 * (size_t)a.nmax, st>>>(a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// COCG with scaled search directions and ICC (aij_solver_cg.h:113-355):
+//   r = b - A x, q = v/(v,r) with v = M^-1 r, then
+//   alpha = 1/(q,Aq), x += alpha q, r -= alpha Aq, stop on ||r||_2/||b||_2,
+//   q += v/(v,r).   Bilinear (unconjugated) products in the recurrences.
+// One 256-thread workgroup per subdomain, vectors in LDS; the triangular
+// ICC solves run in wavefront 0.
+// ---------------------------------------------------------------------------
+constexpr int CT = 256;
+
+__device__ __forceinline__ double2 bsum(double2 v, double2 *sh) {
+  v = wave_csum(v);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double2 t = cz();
+#pragma unroll
+  for (int k = 0; k < CT / 64; k++) t = cadd(t, sh[k]);
+  __syncthreads();
+  return t;
+}
+
+__device__ void icc_apply(const IccArgs &a, const SubMeta &m, const double2 *r, double2 *v) {
+  // forward: v_i = (r_i - sum_j v_j L_ij) / L_ii        (aij_pc_func.h:135-145)
+  // backward: v_c -= v_i L_ic d_c, i descending          (aij_pc_func.h:148-156)
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    for (int i = 0; i < m.n; i++) {
+      const int d = m.dof0 + i;
+      const int ii = a.dptr[d], jj = a.dptr[d + 1] - 1;
+      double2 acc = cz();
+      for (int t = ii + lane; t < jj; t += 64) acc = cadd(acc, cmul(v[a.dcol[t]], a.L[t]));
+      acc = wave_csum(acc);
+      if (lane == 0) v[i] = cdiv_naive(csub(r[i], acc), a.L[jj]);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    }
+    for (int i = m.n - 1; i >= 0; i--) {
+      const int d = m.dof0 + i;
+      const int ii = a.dptr[d], jj = a.dptr[d + 1] - 1;
+      const double2 vi = v[i];
+      for (int t = ii + lane; t < jj; t += 64) {
+        const int c = a.dcol[t];
+        v[c] = csub(v[c], cmul(vi, cmul(a.L[t], a.dg[m.dof0 + c])));
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void full_mxv(const IccArgs &a, const SubMeta &m, const double2 *in, double2 *out) {
+  for (int i = threadIdx.x; i < m.n; i += CT) {
+    const int d = m.dof0 + i;
+    double2 acc = cz();
+    for (int t = a.fptr[d]; t < a.fptr[d + 1]; t++) acc = cfma(acc, a.fval[t], in[a.fcol[t]]);
+    out[i] = acc;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(CT) void k_cocg(IccArgs a) {
+  const SubMeta m = a.meta[blockIdx.x];
+  extern __shared__ double2 lds[];
+  double2 *x = lds, *r = x + a.nmax, *v = r + a.nmax, *q = v + a.nmax, *aq = q + a.nmax;
+  __shared__ double2 sh[CT / 64];
+  const int n = m.n;
+  int status = -1;
+  for (int i = threadIdx.x; i < n; i += CT) x[i] = a.x[m.dof0 + i];
+  __syncthreads();
+  full_mxv(a, m, x, aq);
+  double2 bb = cz(), rr = cz();
+  for (int i = threadIdx.x; i < n; i += CT) {
+    const double2 bi = a.b[m.dof0 + i];
+    r[i] = csub(bi, aq[i]);
+    bb = cadd(bb, make_double2(cabs2(bi), 0.0));
+    rr = cadd(rr, make_double2(cabs2(r[i]), 0.0));
+  }
+  bb = bsum(bb, sh);
+  rr = bsum(rr, sh);
+  double bnorm = sqrt(bb.x);
+  if (bnorm < NEARLY_ZERO) {
+    status = 0;
+  } else {
+    bnorm = 1.0 / bnorm;
+    double res = sqrt(rr.x) * bnorm;
+    if (res <= a.eps) {
+      status = 0;
+    } else {
+      icc_apply(a, m, r, v);
+      double2 vr = cz();
+      for (int i = threadIdx.x; i < n; i += CT) vr = cfma(vr, v[i], r[i]);
+      double2 rv = crecip_naive(bsum(vr, sh));
+      for (int i = threadIdx.x; i < n; i += CT) q[i] = cmul(v[i], rv);
+      __syncthreads();
+      const int max_ite = 10 * n;
+      status = -3;  // SOLVER_STATUS_REACHED_MAX_ITE
+      for (int it = 1; it < max_ite; it++) {
+        full_mxv(a, m, q, aq);
+        double2 qa = cz();
+        for (int i = threadIdx.x; i < n; i += CT) qa = cfma(qa, q[i], aq[i]);
+        double2 alpha = bsum(qa, sh);
+        if (ciszero(alpha)) { status = -5; break; }
+        alpha = cdiv_naive(make_double2(1.0, 0.0), alpha);
+        const double2 malpha = csub(cz(), alpha);
+        double2 r2 = cz();
+        for (int i = threadIdx.x; i < n; i += CT) {
+          x[i] = cadd(x[i], cmul(alpha, q[i]));
+          r[i] = cadd(r[i], cmul(malpha, aq[i]));
+          r2 = cadd(r2, make_double2(cabs2(r[i]), 0.0));
+        }
+        r2 = bsum(r2, sh);
+        res = sqrt(r2.x) * bnorm;
+        if (res <= a.eps) { status = it; break; }
+        if (res >= a.div) { status = -2; break; }
+        icc_apply(a, m, r, v);
+        vr = cz();
+        for (int i = threadIdx.x; i < n; i += CT) vr = cfma(vr, v[i], r[i]);
+        rv = crecip_naive(bsum(vr, sh));
+        for (int i = threadIdx.x; i < n; i += CT) q[i] = cadd(q[i], cmul(v[i], rv));
+        __syncthreads();
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += CT) a.x[m.dof0 + i] = x[i];
+  if (threadIdx.x == 0) a.status[blockIdx.x] = status;
+}
+hipError_t launch_cocg(const IccArgs &a, int nsub, hipStream_t st) {
+  if (nsub <= 0) return hipSuccess;
+  k_cocg<<<nsub, CT, sizeof(double2) * 5 * (size_t)a.nmax, st>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace fw

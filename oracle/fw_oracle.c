@@ -577,7 +577,7 @@ static int cocg_run(int nf, const int *ai, const int *aij, const fwo_c *vij,
   for (n = 1; n < max_ite; n++) {
     csr_mxv(nf, ai, aij, vij, q, aq);
     fwo_c alpha = cbilinear(nf, q, aq);
-    if (ciszero(alpha)) { ret = -4; goto END; } /* reference exits here */
+    if (ciszero(alpha)) { ret = -5; goto END; } /* the reference exits the process here */
     alpha = cdiv(C1, alpha);
     for (int i = 0; i < nf; i++) x[i] = cadd(x[i], cmul(alpha, q[i]));
     alpha = csub(C0, alpha);

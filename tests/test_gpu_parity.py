@@ -172,3 +172,49 @@ def test_max_loop_semantics(fw, small_model):
     st0 = sol.solve(eps=1e-8, max_loop=0)
     assert st0["iters"] == 0
     sol.close()
+
+
+def dirichlet_csr(model, om, s):
+    """The reference's AIJ_MAT after SetBC + DeleteZero + Assem."""
+    g = om.get(s)
+    n = g["n"]
+    bc = np.zeros(n, bool)
+    bc[model.ib_nfid[model.ib_off[s]:model.ib_off[s + 1]]] = True
+    bc[model.ibc[model.bc_off[s]:model.bc_off[s + 1]]] = True
+    rp, cl, vl = [0], [], []
+    for i in range(n):
+        if bc[i]:
+            cl.append(i)
+            vl.append(1.0 + 0j)
+        else:
+            for k in range(g["rowptr"][i], g["rowptr"][i + 1]):
+                j = g["col"][k]
+                if not bc[j] and g["val"][k] != 0:
+                    cl.append(j)
+                    vl.append(g["val"][k])
+        rp.append(len(cl))
+    return np.array(rp, np.int32), np.array(cl, np.int32), np.array(vl)
+
+
+def test_cocg_icc_all(small, small_oracle, small_model):
+    say("[gpu] subdomain cocg+icc")
+    rng = np.random.default_rng(6)
+    n = int(small.info["n_dof"])
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    from adventure_fullwave_amd import fwhip
+    x = np.zeros(n, np.complex128)
+    st = np.zeros(small_model.nsub, np.int32)
+    bb = np.ascontiguousarray(b)
+    fwhip._check(fwhip.lib().fwh_cocg_icc_all(small.ctx.h, bb.ctypes.data, x.ctypes.data, st.ctypes.data,
+                                               1e-9, 1.2), "fwh_cocg_icc_all")
+    off = 0
+    for s in range(small_model.nsub):
+        rp, cl, vl = dirichlet_csr(small_model, small_oracle, s)
+        ns = len(rp) - 1
+        sto, xo, _ = fo.cocg_icc(rp, cl, vl, b[off:off + ns], eps=1e-9, ic_shift=1.2)
+        assert sto > 0 and st[s] > 0
+        assert abs(int(st[s]) - sto) <= max(1, 0.02 * sto)
+        from tests.helpers import dense_lower
+        xd = np.linalg.solve(dense_lower(rp, cl, vl, ns), b[off:off + ns])
+        assert np.linalg.norm(x[off:off + ns] - xd) / np.linalg.norm(xd) < 1e-7
+        off += ns
