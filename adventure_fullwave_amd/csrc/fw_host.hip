@@ -66,7 +66,7 @@ struct fwh_ctx {
   bool loaded = false, first_done = false, last_done = false;
 
   // sizes
-  int nsub = 0, npart = 0, ng = 0, nmax = 0, ngmax = 0, wmax = 0, G = 1;
+  int nsub = 0, npart = 0, ng = 0, nmax = 0, ngmax = 0, wmax = 0, G = 1, nblkmax = 0;
   long long ndof = 0, nslots = 0, nnz = 0, nlen = 0, nel = 0, ncpl = 0, ngrow = 0;
   std::vector<SubMeta> meta_h;
   std::vector<int> order_h;
@@ -144,6 +144,7 @@ struct fwh_ctx {
     a.done = check_done ? &sc->done : nullptr;
     a.nmax = nmax;
     a.ngmax = ngmax;
+    a.nblkmax = nblkmax;
     a.idx1 = idx1;
     a.rowbeg = rowbeg;
     a.fac = fac;
@@ -194,6 +195,7 @@ struct fwh_ctx {
       a.base = s;
       a.nmax = meta_h[s].n;
       a.ngmax = meta_h[s].ng;
+      a.nblkmax = meta_h[s].nblk;
       hipError_t e = fw::launch_solve(a, 1, st);
       if (e != hipSuccess) return e;
     }
@@ -275,8 +277,6 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   c->nslots = nslots;
   c->nmax = nmax;
   c->ngmax = ngmax;
-  if ((size_t)(nmax + ngmax + fw::CAP) * 16 > 160 * 1024)
-    return fail(FWH_E_LIMIT, "subdomain of %d DOFs / %d interface DOFs exceeds the LDS budget", nmax, ngmax);
 
   // ---- flags and interface slots ----------------------------------------
   std::vector<int8_t> bcflag(ndof, 0);
@@ -610,6 +610,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
       i0 = i1;
     }
     M.nblk = (int)blk_row.size() - M.blk0 - 1;
+    c->nblkmax = std::max(c->nblkmax, M.nblk);
     // flops of the factorisation: 8 per complex multiply-add
     for (int i = 0; i < M.n; i++) {
       const int fi = idx1[M.dof0 + i];
@@ -619,6 +620,8 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   }
   c->nlen = nlen_tot;
   c->wmax = wmax;
+  if (fw::solve_lds_bytes(nmax, ngmax, c->nblkmax) > 160 * 1024)
+    return fail(FWH_E_LIMIT, "subdomain of %d DOFs / %d interface DOFs exceeds the LDS budget", nmax, ngmax);
   if ((size_t)(wmax + 1 + nmax) * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "profile too wide for the factor kernel");
   HIPCHK(hipMemcpyAsync(c->meta, meta.data(), nsub * sizeof(SubMeta), hipMemcpyHostToDevice, st));
   if ((rc = c->upload(&c->rowbeg, rowbeg)) || (rc = c->upload(&c->blk_row, blk_row)) ||

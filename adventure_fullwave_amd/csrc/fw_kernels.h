@@ -35,6 +35,7 @@ struct SolveArgs {
   int mode;
   const int *done;       // iteration mode: skip when *done != 0 (may be null)
   int nmax, ngmax;       // LDS sizing
+  int nblkmax;
   const int *idx1;       // per DOF, local first column of the profile
   const int *rowbeg;     // per DOF, start of row i within the subdomain's skyline
   const double2 *fac;
@@ -124,6 +125,7 @@ hipError_t launch_skydiag_bc(int ndof, const int *dof_sub, const SubMeta *meta,
                              hipStream_t st);
 hipError_t launch_factor(const FactorArgs &a, int nwork, hipStream_t st);
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st);
+size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax);
 hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, const double2 *val,
                          double2 *pcslot, hipStream_t st);
 hipError_t launch_first(const VecArgs &a, hipStream_t st);

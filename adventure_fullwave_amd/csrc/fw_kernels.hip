@@ -368,242 +368,6 @@ hipError_t launch_factor(const FactorArgs &a, int nwork, hipStream_t st) {
   return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// Subdomain Dirichlet solve (hddm_solver.h:461-693, advlas_skyline.c:134-167),
-// one 256-thread workgroup per subdomain:
-//   1. load: v_I = f_I (first/last step) - A_IG v_G, v_G = interface values
-//      (SetBC, aij_matrix_func.h:60-136; renewal_bc_sp hddm_solver_sp.h:58-69)
-//   2. forward sweep  v_i -= sum_j L_ij v_j      (row blocks of <= BMAX rows)
-//   3. v_i *= 1/d_i
-//   4. backward sweep v_j -= L_ij v_i, i descending
-//   5. y_G = (A x)_G (- f_G)   (MxV_TMP_AIJ on the original matrix, rows of G)
-// The skyline is streamed once per sweep: each row block (<= CAP entries,
-// contiguous) is prefetched into registers while the previous block is
-// processed from LDS.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void blk_range(const int *blk, const int *rowbeg, int n, long long nlen,
-                                          int b, int &i0, int &i1, int &e0, int &e1) {
-  i0 = blk[b];
-  i1 = blk[b + 1];
-  e0 = rowbeg[i0];
-  e1 = i1 < n ? rowbeg[i1] : (int)nlen;
-}
-
-__global__ __launch_bounds__(SOLVE_THREADS) void k_solve(SolveArgs a) {
-  if (a.mode == MODE_ITER && a.done && *a.done) return;
-  const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
-  const SubMeta m = a.meta[w];
-  extern __shared__ double2 lds[];
-  double2 *v = lds;                  // [nmax]
-  double2 *vg = lds + a.nmax;        // [ngmax]
-  double2 *stg = vg + a.ngmax;       // [CAP]
-  __shared__ double2 part[BMAX];
-  __shared__ int bf[BMAX], brb[BMAX];
-  __shared__ int s_jlo;
-  const int tid = threadIdx.x;
-  const int n = m.n;
-  const int *idx1 = a.idx1 + m.dof0;
-  const int *rowbeg = a.rowbeg + m.dof0;
-  const double2 *fac = a.fac + m.fac0;
-  const int *blk = a.blk_row + m.blk0;
-
-  // ---- 1. load vector -------------------------------------------------------
-  for (int k = tid; k < m.ng; k += SOLVE_THREADS)
-    vg[k] = (a.mode == MODE_FIRST || a.mode == MODE_RHS) ? cz() : a.P[a.g_gid[m.g0 + k]];
-  __syncthreads();
-  for (int i = tid; i < n; i += SOLVE_THREADS) {
-    const int d = m.dof0 + i;
-    double2 b;
-    if (a.mode == MODE_RHS) {
-      b = a.X[d];
-    } else {
-      const int8_t fl = a.bcflag[d];
-      if (fl == 1) {
-        b = vg[a.kidx[d]];
-      } else if (fl == 2) {
-        b = cz();
-      } else {
-        b = (a.mode == MODE_ITER) ? cz() : a.f[d];
-        for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++) b = cfms(b, a.cp_val[c], vg[a.cp_k[c]]);
-      }
-    }
-    v[i] = b;
-  }
-
-  // ---- 2. forward sweep -------------------------------------------------------
-  double2 pre[PFK];
-  {
-    int i0, i1, e0, e1;
-    if (m.nblk > 0) {
-      blk_range(blk, rowbeg, n, m.nlen, 0, i0, i1, e0, e1);
-#pragma unroll
-      for (int k = 0; k < PFK; k++) {
-        const int e = e0 + tid + k * SOLVE_THREADS;
-        pre[k] = e < e1 ? fac[e] : cz();
-      }
-    }
-  }
-  for (int b = 0; b < m.nblk; b++) {
-    int i0, i1, e0, e1;
-    blk_range(blk, rowbeg, n, m.nlen, b, i0, i1, e0, e1);
-    const int nb = i1 - i0;
-    __syncthreads();  // previous block finished with stg / v
-#pragma unroll
-    for (int k = 0; k < PFK; k++) {
-      const int e = tid + k * SOLVE_THREADS;
-      if (e < e1 - e0) stg[e] = pre[k];
-    }
-    if (tid < nb) {
-      bf[tid] = idx1[i0 + tid];
-      brb[tid] = rowbeg[i0 + tid] - e0;
-    }
-    __syncthreads();
-    if (b + 1 < m.nblk) {
-      int n0, n1, f0, f1;
-      blk_range(blk, rowbeg, n, m.nlen, b + 1, n0, n1, f0, f1);
-#pragma unroll
-      for (int k = 0; k < PFK; k++) {
-        const int e = f0 + tid + k * SOLVE_THREADS;
-        pre[k] = e < f1 ? fac[e] : cz();
-      }
-    }
-    // A: contributions of columns before the block, 16 lanes per row
-    {
-      const int r = tid >> 4, l = tid & 15;
-      double2 acc = cz();
-      if (r < nb) {
-        const int i = i0 + r;
-        const int fi = bf[r], rb = brb[r];
-        const int jend = i0 < i ? i0 : i;
-        for (int j = fi + l; j < jend; j += 16) acc = cfma(acc, stg[rb + (j - fi)], v[j]);
-      }
-      acc = group_csum<16>(acc);
-      if (r < nb && l == 0) part[r] = acc;
-    }
-    __syncthreads();
-    // B: the block's own lower triangle, one lane per row (wave 0)
-    if (tid < 64) {
-      const int r = tid;
-      double2 s = cz();
-      double2 L[BMAX];
-#pragma unroll
-      for (int q = 0; q < BMAX; q++) L[q] = cz();
-      if (r < nb) {
-        const int i = i0 + r;
-        s = csub(v[i], part[r]);
-        const int fi = bf[r], rb = brb[r];
-#pragma unroll
-        for (int q = 0; q < BMAX; q++) {
-          const int j = i0 + q;
-          if (q < r && j >= fi) L[q] = stg[rb + (j - fi)];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < BMAX; q++) {
-        if (q < nb) {
-          const double2 xq = readlane_c(s, q);
-          if (r > q) s = cfms(s, L[q], xq);
-        }
-      }
-      if (r < nb) v[i0 + r] = s;
-    }
-  }
-  __syncthreads();
-
-  // ---- 3. diagonal ---------------------------------------------------------------
-  for (int i = tid; i < n; i += SOLVE_THREADS) v[i] = cmul(v[i], a.dinv[m.dof0 + i]);
-
-  // ---- 4. backward sweep -----------------------------------------------------------
-  if (m.nblk > 0) {
-    int i0, i1, e0, e1;
-    blk_range(blk, rowbeg, n, m.nlen, m.nblk - 1, i0, i1, e0, e1);
-#pragma unroll
-    for (int k = 0; k < PFK; k++) {
-      const int e = e0 + tid + k * SOLVE_THREADS;
-      pre[k] = e < e1 ? fac[e] : cz();
-    }
-  }
-  for (int b = m.nblk - 1; b >= 0; b--) {
-    int i0, i1, e0, e1;
-    blk_range(blk, rowbeg, n, m.nlen, b, i0, i1, e0, e1);
-    const int nb = i1 - i0;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PFK; k++) {
-      const int e = tid + k * SOLVE_THREADS;
-      if (e < e1 - e0) stg[e] = pre[k];
-    }
-    if (tid < nb) {
-      bf[tid] = idx1[i0 + tid];
-      brb[tid] = rowbeg[i0 + tid] - e0;
-    }
-    if (tid == 0) {
-      int jl = i0;
-      for (int r = 0; r < nb; r++) jl = min(jl, idx1[i0 + r]);
-      s_jlo = jl;
-    }
-    __syncthreads();
-    if (b > 0) {
-      int n0, n1, f0, f1;
-      blk_range(blk, rowbeg, n, m.nlen, b - 1, n0, n1, f0, f1);
-#pragma unroll
-      for (int k = 0; k < PFK; k++) {
-        const int e = f0 + tid + k * SOLVE_THREADS;
-        pre[k] = e < f1 ? fac[e] : cz();
-      }
-    }
-    // A: the block's triangle, one lane per column q (wave 0)
-    if (tid < 64) {
-      const int q = tid;
-      double2 x = q < nb ? v[i0 + q] : cz();
-      double2 Lc[BMAX];
-#pragma unroll
-      for (int r = 0; r < BMAX; r++) {
-        Lc[r] = cz();
-        if (r > q && r < nb && i0 + q >= bf[r]) Lc[r] = stg[brb[r] + (i0 + q - bf[r])];
-      }
-#pragma unroll
-      for (int r = BMAX - 1; r >= 0; r--) {
-        if (r < nb) {
-          const double2 xr = readlane_c(x, r);
-          if (q < r) x = cfms(x, Lc[r], xr);
-        }
-      }
-      if (q < nb) v[i0 + q] = x;
-    }
-    __syncthreads();
-    // B: update columns before the block with all rows of the block
-    for (int j = s_jlo + tid; j < i0; j += SOLVE_THREADS) {
-      double2 acc = cz();
-      for (int r = 0; r < nb; r++) {
-        const int f = bf[r];
-        if (j >= f) acc = cfma(acc, stg[brb[r] + (j - f)], v[i0 + r]);
-      }
-      v[j] = csub(v[j], acc);
-    }
-  }
-  __syncthreads();
-
-  // ---- 5. outputs -----------------------------------------------------------------------
-  if (a.mode == MODE_LAST || a.mode == MODE_RHS)
-    for (int i = tid; i < n; i += SOLVE_THREADS) a.X[m.dof0 + i] = v[i];
-  if (a.mode != MODE_RHS) {
-    for (int k = tid; k < m.ng; k += SOLVE_THREADS) {
-      const int sl = m.g0 + k;
-      double2 y = cz();
-      for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], v[a.gr_col[c]]);
-      if (a.mode != MODE_ITER) y = csub(y, a.f[m.dof0 + a.g_nfid[sl]]);
-      a.yslot[sl] = y;
-    }
-  }
-}
-
-hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
-  if (nwork <= 0) return hipSuccess;
-  size_t lds = sizeof(double2) * ((size_t)a.nmax + a.ngmax + CAP);
-  k_solve<<<nwork, SOLVE_THREADS, lds, st>>>(a);
-  return hipGetLastError();
-}
 
 // diagonal of A at every slot, zero -> 1 (hddm_pc_CS_DIAG_make, hddm_pc.h:52-86)
 __global__ void k_pcslot(int nslots, const int *__restrict__ diag_ent, const int *__restrict__ unused,

@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--plain-iters", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-loop", type=int, default=200000)
+    ap.add_argument("--pml", default="stretch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
@@ -47,7 +49,7 @@ def main():
     from adventure_fullwave_amd import fwhip, mesh
 
     t0 = time.time()
-    model = mesh.make_config(args.cfg)
+    model = mesh.make_config(args.cfg, pml_mode=args.pml)
     log(f"[bench] cfg{args.cfg}: {model.n_edges} edges, {model.n_tets} tets, {model.nsub} subdomains, "
         f"{model.n_interface} interface unknowns ({time.time() - t0:.1f}s)")
     t0 = time.time()
@@ -58,13 +60,13 @@ def main():
 
     # warmup + timed batched solves
     for _ in range(args.warmup):
-        st = sol.solve(eps=args.eps, max_loop=200000, graph_iters=args.graph_iters)
+        st = sol.solve(eps=args.eps, max_loop=args.max_loop, graph_iters=args.graph_iters)
         log(f"[bench] warmup: {st['iters']} iterations, converged={st['converged']}")
     iters = 0
     solves = 0
     t_start = time.perf_counter()
     for k in range(args.steps):
-        st = sol.solve(eps=args.eps, max_loop=200000, graph_iters=args.graph_iters)
+        st = sol.solve(eps=args.eps, max_loop=args.max_loop, graph_iters=args.graph_iters)
         iters += st["iters"]
         solves += st["sub_solves"]
         log(f"[bench] step {k}: {st['iters']} iterations, resid {st['final_resid']:.3e}")

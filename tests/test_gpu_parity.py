@@ -213,7 +213,11 @@ def test_cocg_icc_all(small, small_oracle, small_model):
         ns = len(rp) - 1
         sto, xo, _ = fo.cocg_icc(rp, cl, vl, b[off:off + ns], eps=1e-9, ic_shift=1.2)
         assert sto > 0 and st[s] > 0
-        assert abs(int(st[s]) - sto) <= max(1, 0.02 * sto)
+        # subdomain COCG counts at eps 1e-9 move with the summation order of
+        # the products: a CPU COCG with permuted dot-product order gives
+        # 157-200 iterations where the oracle gives 164-201 on these
+        # subdomains (up to 9 of 179), so the bar is 6 %
+        assert abs(int(st[s]) - sto) <= max(2, 0.06 * sto)
         from tests.helpers import dense_lower
         xd = np.linalg.solve(dense_lower(rp, cl, vl, ns), b[off:off + ns])
         assert np.linalg.norm(x[off:off + ns] - xd) / np.linalg.norm(xd) < 1e-7
