@@ -609,6 +609,9 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
       blk_row.push_back(i1);
       i0 = i1;
     }
+    // pad to an even number of blocks with empty blocks [n, n): the solve
+    // kernel sweeps block pairs without conditional loads
+    if (((int)blk_row.size() - M.blk0 - 1) % 2 == 1) blk_row.push_back(M.n);
     M.nblk = (int)blk_row.size() - M.blk0 - 1;
     c->nblkmax = std::max(c->nblkmax, M.nblk);
     // flops of the factorisation: 8 per complex multiply-add

@@ -22,9 +22,9 @@ struct SubMeta {
 enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3 };
 
 // solve kernel limits: rows per block, prefetch registers per thread
-constexpr int SOLVE_THREADS = 256;
+constexpr int SOLVE_THREADS = 64;
 constexpr int BMAX = 16;
-constexpr int PFK = 4;
+constexpr int PFK = 16;
 constexpr int CAP = SOLVE_THREADS * PFK;  // skyline entries per row block
 constexpr int VT_HINT = 256;              // threads per block of the vector kernels
 

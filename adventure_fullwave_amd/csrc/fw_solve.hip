@@ -1,6 +1,6 @@
 // fw_solve.hip - batched subdomain Dirichlet solve (the Schur-step kernel).
 //
-// One 256-thread workgroup per subdomain (hddm_solver.h:461-693,
+// One wavefront (64 lanes) per subdomain (hddm_solver.h:461-693,
 // advlas_skyline.c:134-167):
 //   1. load: v_I = f_I (first/last step) - A_IG v_G, v_G = interface values
 //      (SetBC, aij_matrix_func.h:60-136; renewal_bc_sp hddm_solver_sp.h:58-69)
@@ -10,154 +10,218 @@
 //   5. y_G = (A x)_G (- f_G)   (MxV_TMP_AIJ on the original matrix, rows of G)
 //
 // The skyline is streamed once per sweep in contiguous row blocks of at most
-// CAP entries.  All index metadata of the subdomain (first columns, row
-// starts, block starts) is staged in LDS up front, so the only global
-// traffic of a sweep is the skyline itself, prefetched two blocks ahead in
-// two register sets while the current block is processed from LDS.
+// CAP entries, two blocks in flight per wavefront (register sets pa/pb, 16 KB
+// each), each staged in LDS when its turn comes.  A sweep is a chain of
+// dependent rows, so the kernel is organised around that chain: no
+// workgroup barriers, block-row metadata in registers (broadcast with
+// v_readlane), the triangle of each block resolved by one lane per row with
+// its operands preloaded into registers.
 #include "fw_kernels.h"
 #include "fw_device.h"
 
 namespace fw {
 
-template <int PF>
-__device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, int e0, int e1, double2 (&p)[PF]) {
+constexpr int WL = 64;  // lanes
+
+__device__ __forceinline__ int readlane_i(int x, int lane) { return __builtin_amdgcn_readlane(x, lane); }
+
+// one prefetched block per lane: plain doubles (a POD the compiler keeps in
+// registers; the HIP vector type's unions make it fall back to scratch)
+struct Pf2 {
+  double x, y;
+};
+struct PfSet {
+  Pf2 e[PFK];
+};
+
+// Branch-free block loads: addresses past the block are clamped to its last
+// entry (same cache line, no extra traffic), so the loads carry no control
+// flow and the compiler can keep the other register set's loads in flight
+// (a counted vmcnt wait instead of a full drain).
+__device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, int e0, int e1, PfSet &p) {
+  const int lane = threadIdx.x;
+  const int last = e1 - 1;
 #pragma unroll
-  for (int k = 0; k < PF; k++) {
-    const int e = e0 + (int)threadIdx.x + k * SOLVE_THREADS;
-    p[k] = e < e1 ? fac[e] : cz();
+  for (int k = 0; k < PFK; k++) {
+    const double2 t = fac[min(e0 + lane + k * WL, last)];
+    p.e[k].x = t.x;
+    p.e[k].y = t.y;
   }
 }
 
-template <int PF>
-__device__ __forceinline__ void stage_block(double2 *stg, int cnt, const double2 (&p)[PF]) {
+// Unconditional staging: entries past the block land in unused LDS slots.
+__device__ __forceinline__ void stage_block(double2 *stg, const PfSet &p) {
+  const int lane = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < PF; k++) {
-    const int e = (int)threadIdx.x + k * SOLVE_THREADS;
-    if (e < cnt) stg[e] = p[k];
-  }
+  for (int k = 0; k < PFK; k++) stg[lane + k * WL] = make_double2(p.e[k].x, p.e[k].y);
 }
+
+// The workgroup is a single wavefront: LDS traffic of one wave is processed
+// in order, so a compiler ordering point is all a phase change needs (no
+// s_barrier and, above all, no memory fence that would drain the loads in
+// flight).
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
 struct SolveLds {
-  double2 *v, *vg, *stg;
+  double2 *v, *stg;
   int *idx1, *rb, *blk;
-  double2 *part;
 };
 
 // forward block b: rows [i0,i1) with skyline entries [e0,e1)
 __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__restrict__ fac, int nblk, int b,
-                                          double2 (&p)[PFK]) {
-  const int tid = threadIdx.x;
+                                          PfSet &p) {
+  const int lane = threadIdx.x;
   const int i0 = L.blk[b], i1 = L.blk[b + 1];
   const int e0 = L.rb[i0], e1 = L.rb[i1];
   const int nb = i1 - i0;
-  __syncthreads();  // the previous block is done with stg, part and v
-  stage_block(L.stg, e1 - e0, p);
-  __syncthreads();
-  if (b + 2 < nblk) issue_block(fac, L.rb[L.blk[b + 2]], L.rb[L.blk[b + 3]], p);
-  // A: columns before the block, 16 lanes per row
+  wave_sync();
+  stage_block(L.stg, p);
+  wave_sync();
   {
-    const int r = tid >> 4, l = tid & 15;
-    double2 acc = cz();
-    if (r < nb) {
-      const int i = i0 + r;
-      const int fi = L.idx1[i], rb = L.rb[i] - e0;
-      for (int j = fi + l; j < i0; j += 16) acc = cfma(acc, L.stg[rb + (j - fi)], L.v[j]);
-    }
-    acc = group_csum<16>(acc);
-    if (r < nb && l == 0) L.part[r] = acc;
+    const int bn = min(b + 2, nblk - 1);  // unconditional: see issue_block
+    issue_block(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], p);
   }
-  __syncthreads();
-  // B: the block's own lower triangle, one lane per row (wave 0)
-  if (tid < 64) {
-    const int r = tid;
-    double2 s = cz();
-    int fi = 0, rb = 0;
-    if (r < nb) {
-      s = csub(L.v[i0 + r], L.part[r]);
-      fi = L.idx1[i0 + r];
-      rb = L.rb[i0 + r] - e0;
+
+  // A: columns before the block; rows r = lane/4, four lanes per row
+  const int ra = lane >> 2, la = lane & 3;
+  double2 acc0 = cz(), acc1 = cz(), acc2 = cz(), acc3 = cz();
+  if (ra < nb) {
+    const int fi = L.idx1[i0 + ra];
+    const int rbase = L.rb[i0 + ra] - e0 - fi;
+    int j = fi + la;
+    for (; j + 12 < i0; j += 16) {
+      acc0 = cfma(acc0, L.stg[rbase + j], L.v[j]);
+      acc1 = cfma(acc1, L.stg[rbase + j + 4], L.v[j + 4]);
+      acc2 = cfma(acc2, L.stg[rbase + j + 8], L.v[j + 8]);
+      acc3 = cfma(acc3, L.stg[rbase + j + 12], L.v[j + 12]);
     }
-    for (int q = 0; q < nb; q++) {
+    for (; j < i0; j += 4) acc0 = cfma(acc0, L.stg[rbase + j], L.v[j]);
+  }
+  double2 acc = cadd(cadd(acc0, acc1), cadd(acc2, acc3));
+  acc.x += __shfl_xor(acc.x, 1, 4);
+  acc.y += __shfl_xor(acc.y, 1, 4);
+  acc.x += __shfl_xor(acc.x, 2, 4);
+  acc.y += __shfl_xor(acc.y, 2, 4);
+  const double2 part = make_double2(__shfl(acc.x, 4 * (lane & 15)), __shfl(acc.y, 4 * (lane & 15)));
+
+  // B: the block's own triangle, lane r owns row r
+  const int r = lane;
+  double2 s = cz();
+  int fr = i0, rbr = 0;
+  if (r < nb) {
+    fr = L.idx1[i0 + r];
+    rbr = L.rb[i0 + r] - e0 - fr;
+    s = csub(L.v[i0 + r], part);
+  }
+  // operands of the chain first (independent LDS loads), then the chain of
+  // readlane + multiply-subtract steps
+  Pf2 lq[BMAX];
+#pragma unroll
+  for (int q = 0; q < BMAX; q++) {
+    const int j = i0 + q;
+    const bool use = q < r && r < nb && j >= fr;
+    const double2 t = L.stg[use ? rbr + j : 0];
+    lq[q].x = use ? t.x : 0.0;
+    lq[q].y = use ? t.y : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < BMAX; q++) {
+    if (q < nb) {
       const double2 xq = readlane_c(s, q);
-      const int j = i0 + q;
-      if (r > q && r < nb && j >= fi) s = cfms(s, L.stg[rb + (j - fi)], xq);
+      if (r > q) s = cfms(s, make_double2(lq[q].x, lq[q].y), xq);
     }
-    if (r < nb) L.v[i0 + r] = s;
   }
+  if (r < nb) L.v[i0 + r] = s;
 }
 
 // backward block b (rows descending): triangle, then columns before the block
 __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__restrict__ fac, int b,
-                                          double2 (&p)[PFK]) {
-  const int tid = threadIdx.x;
+                                          PfSet &p) {
+  const int lane = threadIdx.x;
   const int i0 = L.blk[b], i1 = L.blk[b + 1];
   const int e0 = L.rb[i0], e1 = L.rb[i1];
   const int nb = i1 - i0;
-  __syncthreads();
-  stage_block(L.stg, e1 - e0, p);
-  __syncthreads();
-  if (b >= 2) issue_block(fac, L.rb[L.blk[b - 2]], L.rb[L.blk[b - 1]], p);
-  // A: triangle, one lane per column q (wave 0)
-  if (tid < 64) {
-    const int q = tid;
-    double2 x = q < nb ? L.v[i0 + q] : cz();
-    for (int r = nb - 1; r >= 0; r--) {
-      const double2 xr = readlane_c(x, r);
-      const int fr = L.idx1[i0 + r];
-      if (q < r && i0 + q >= fr) x = cfms(x, L.stg[L.rb[i0 + r] - e0 + (i0 + q - fr)], xr);
-    }
-    if (q < nb) L.v[i0 + q] = x;
+  wave_sync();
+  stage_block(L.stg, p);
+  wave_sync();
+  {
+    const int bn = max(b - 2, 0);
+    issue_block(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], p);
   }
-  __syncthreads();
-  // B: columns before the block, all rows of the block
-  int jlo = i0;
-  for (int r = 0; r < nb; r++) jlo = min(jlo, L.idx1[i0 + r]);
-  for (int j = jlo + tid; j < i0; j += SOLVE_THREADS) {
+
+  // row metadata of the block: lane r holds row r's first column and base
+  int fR = i0, rbR = 0;
+  if (lane < nb) {
+    fR = L.idx1[i0 + lane];
+    rbR = L.rb[i0 + lane] - e0 - fR;
+  }
+  // A: triangle, lane q owns column q
+  const int q = lane;
+  double2 x = q < nb ? L.v[i0 + q] : cz();
+  Pf2 lc[BMAX];
+#pragma unroll
+  for (int r = 0; r < BMAX; r++) {
+    const int f_u = readlane_i(fR, r), rb_u = readlane_i(rbR, r);
+    const bool use = r < nb && q < r && i0 + q >= f_u;
+    const double2 t = L.stg[use ? rb_u + i0 + q : 0];
+    lc[r].x = use ? t.x : 0.0;
+    lc[r].y = use ? t.y : 0.0;
+  }
+#pragma unroll
+  for (int r = BMAX - 1; r >= 0; r--) {
+    if (r < nb) {
+      const double2 xr = readlane_c(x, r);
+      if (q < r) x = cfms(x, make_double2(lc[r].x, lc[r].y), xr);
+    }
+  }
+  if (q < nb) L.v[i0 + q] = x;
+
+  // B: every column before the block receives all rows of the block
+  int jlo = fR;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) jlo = min(jlo, __shfl_xor(jlo, o, 64));
+  for (int j = jlo + lane; j < i0; j += WL) {
     double2 acc = cz();
-    for (int r = 0; r < nb; r++) {
-      const int f = L.idx1[i0 + r];
-      if (j >= f) acc = cfma(acc, L.stg[L.rb[i0 + r] - e0 + (j - f)], L.v[i0 + r]);
+#pragma unroll
+    for (int r = 0; r < BMAX; r++) {
+      if (r < nb) {
+        const int f_u = readlane_i(fR, r), rb_u = readlane_i(rbR, r);
+        const double2 xr = readlane_c(x, r);
+        if (j >= f_u) acc = cfma(acc, L.stg[rb_u + j], xr);
+      }
     }
     L.v[j] = csub(L.v[j], acc);
   }
 }
 
-__global__ __launch_bounds__(SOLVE_THREADS) void k_solve(SolveArgs a) {
+__global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_solve(SolveArgs a) {
   if (a.mode == MODE_ITER && a.done && *a.done) return;
   const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
   const SubMeta m = a.meta[w];
   extern __shared__ double2 lds[];
-  __shared__ double2 part[BMAX];
   SolveLds L;
   L.v = lds;                          // [nmax]
-  L.vg = L.v + a.nmax;                // [ngmax]
-  L.stg = L.vg + a.ngmax;             // [CAP]
+  L.stg = L.v + a.nmax;               // [CAP]
   L.idx1 = (int *)(L.stg + CAP);      // [nmax]
   L.rb = L.idx1 + a.nmax;             // [nmax+1]
   L.blk = L.rb + a.nmax + 1;          // [nblkmax+1]
-  L.part = part;
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x;
   const int n = m.n;
   const double2 *fac = a.fac + m.fac0;
 
-  // ---- 0. metadata and interface values --------------------------------------
-  for (int i = tid; i < n; i += SOLVE_THREADS) {
+  // ---- 0. metadata --------------------------------------------------------------
+  for (int i = lane; i < n; i += WL) {
     L.idx1[i] = a.idx1[m.dof0 + i];
     L.rb[i] = a.rowbeg[m.dof0 + i];
   }
-  if (tid == 0) L.rb[n] = (int)m.nlen;
-  for (int b = tid; b <= m.nblk; b += SOLVE_THREADS) L.blk[b] = a.blk_row[m.blk0 + b];
-  for (int k = tid; k < m.ng; k += SOLVE_THREADS)
-    L.vg[k] = (a.mode == MODE_FIRST || a.mode == MODE_RHS) ? cz() : a.P[a.g_gid[m.g0 + k]];
+  if (lane == 0) L.rb[n] = (int)m.nlen;
+  for (int b = lane; b <= m.nblk; b += WL) L.blk[b] = a.blk_row[m.blk0 + b];
   __syncthreads();
 
-  // first two skyline blocks in flight while the load vector is built
-  double2 pa[PFK], pb[PFK];
-  if (m.nblk > 0) issue_block(fac, L.rb[L.blk[0]], L.rb[L.blk[1]], pa);
-  if (m.nblk > 1) issue_block(fac, L.rb[L.blk[1]], L.rb[L.blk[2]], pb);
 
-  // ---- 1. load vector ------------------------------------------------------------
-  for (int i = tid; i < n; i += SOLVE_THREADS) {
+  // ---- 1. load vector --------------------------------------------------------------
+  const bool zero_g = (a.mode == MODE_FIRST || a.mode == MODE_RHS);
+  for (int i = lane; i < n; i += WL) {
     const int d = m.dof0 + i;
     double2 b;
     if (a.mode == MODE_RHS) {
@@ -165,50 +229,58 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(SolveArgs a) {
     } else {
       const int8_t fl = a.bcflag[d];
       if (fl == 1) {
-        b = L.vg[a.kidx[d]];
+        b = zero_g ? cz() : a.P[a.g_gid[m.g0 + a.kidx[d]]];
       } else if (fl == 2) {
         b = cz();
       } else {
         b = (a.mode == MODE_ITER) ? cz() : a.f[d];
-        for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++) b = cfms(b, a.cp_val[c], L.vg[a.cp_k[c]]);
+        if (!zero_g)
+          for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++)
+            b = cfms(b, a.cp_val[c], a.P[a.g_gid[m.g0 + a.cp_k[c]]]);
       }
     }
     L.v[i] = b;
   }
 
-  // ---- 2. forward sweep ---------------------------------------------------------------
+  // ---- 2. forward sweep: the first two blocks go in flight once the load
+  // vector's own global reads are complete ----------------------------------
+  PfSet pa, pb;
+  // (the host pads every subdomain to an even number >= 2 of row blocks, so
+  // the sweeps run over block pairs without conditional loads: a branch
+  // around a load would make the compiler drain every load in flight)
+  issue_block(fac, L.rb[L.blk[0]], L.rb[L.blk[1]], pa);
+  issue_block(fac, L.rb[L.blk[1]], L.rb[L.blk[2]], pb);
   for (int b = 0; b < m.nblk; b += 2) {
     fwd_block(L, fac, m.nblk, b, pa);
-    if (b + 1 < m.nblk) fwd_block(L, fac, m.nblk, b + 1, pb);
+    fwd_block(L, fac, m.nblk, b + 1, pb);
   }
-  // first two backward blocks in flight during the diagonal pass
-  const int nb0 = m.nblk - 1;
-  if (nb0 >= 0) {
-    // block nb0 goes to the set it would use in the backward loop below
-    if ((nb0 & 1) == 0) issue_block(fac, L.rb[L.blk[nb0]], L.rb[L.blk[nb0 + 1]], pa);
-    else issue_block(fac, L.rb[L.blk[nb0]], L.rb[L.blk[nb0 + 1]], pb);
-  }
-  if (nb0 >= 1) {
-    if (((nb0 - 1) & 1) == 0) issue_block(fac, L.rb[L.blk[nb0 - 1]], L.rb[L.blk[nb0]], pa);
-    else issue_block(fac, L.rb[L.blk[nb0 - 1]], L.rb[L.blk[nb0]], pb);
-  }
-  __syncthreads();
+  wave_sync();
 
-  // ---- 3. diagonal ------------------------------------------------------------------------
-  for (int i = tid; i < n; i += SOLVE_THREADS) L.v[i] = cmul(L.v[i], a.dinv[m.dof0 + i]);
+  // ---- 3. diagonal (before the backward loads are issued: global loads
+  // complete in issue order, so a later dinv load would drain them) ---------
+  for (int i = lane; i < n; i += WL) L.v[i] = cmul(L.v[i], a.dinv[m.dof0 + i]);
+  wave_sync();
 
-  // ---- 4. backward sweep: even blocks use pa, odd blocks pb -------------------------------
-  for (int b = nb0; b >= 0; b--) {
-    if ((b & 1) == 0) bwd_block(L, fac, b, pa);
-    else bwd_block(L, fac, b, pb);
+  // the last two blocks go in flight for the backward sweep: the last block
+  // in pa, the one before in pb, alternating from there (the register sets
+  // are always named statically, so they stay in registers)
+  const int nbl = m.nblk - 1;
+  issue_block(fac, L.rb[L.blk[nbl]], L.rb[L.blk[nbl + 1]], pa);
+  issue_block(fac, L.rb[L.blk[nbl - 1]], L.rb[L.blk[nbl]], pb);
+  wave_sync();
+
+  // ---- 4. backward sweep ---------------------------------------------------------------------
+  for (int b = nbl; b >= 1; b -= 2) {
+    bwd_block(L, fac, b, pa);
+    bwd_block(L, fac, b - 1, pb);
   }
-  __syncthreads();
+  wave_sync();
 
-  // ---- 5. outputs --------------------------------------------------------------------------
+  // ---- 5. outputs ------------------------------------------------------------------------------
   if (a.mode == MODE_LAST || a.mode == MODE_RHS)
-    for (int i = tid; i < n; i += SOLVE_THREADS) a.X[m.dof0 + i] = L.v[i];
+    for (int i = lane; i < n; i += WL) a.X[m.dof0 + i] = L.v[i];
   if (a.mode != MODE_RHS) {
-    for (int k = tid; k < m.ng; k += SOLVE_THREADS) {
+    for (int k = lane; k < m.ng; k += WL) {
       const int sl = m.g0 + k;
       double2 y = cz();
       for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], L.v[a.gr_col[c]]);
@@ -219,12 +291,13 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(SolveArgs a) {
 }
 
 size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax) {
-  return sizeof(double2) * ((size_t)nmax + ngmax + CAP) + sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
+  (void)ngmax;
+  return sizeof(double2) * ((size_t)nmax + CAP) + sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
 }
 
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
   if (nwork <= 0) return hipSuccess;
-  k_solve<<<nwork, SOLVE_THREADS, solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax), st>>>(a);
+  k_solve<<<nwork, WL, solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax), st>>>(a);
   return hipGetLastError();
 }
 

@@ -935,7 +935,10 @@ static int s_iteration(hddm_t *H, int loop, double *resid) {
   double resi = 0.0;
   for (int p = 0; p < np; p++) {
     hvec_t *h = &H->hv[p];
-    for (int i = 0, j = h->outfree; i < h->infree; i++, j++) pq = cadd(pq, cmul(h->P[j], h->Q[j]));
+    if (H->o.perturb)
+      for (int i = h->infree - 1; i >= 0; i--) pq = cadd(pq, cmul(h->P[h->outfree + i], h->Q[h->outfree + i]));
+    else
+      for (int i = 0, j = h->outfree; i < h->infree; i++, j++) pq = cadd(pq, cmul(h->P[j], h->Q[j]));
   }
   if (cabs_(rz) < NEARLY_ZERO && cabs_(pq) < NEARLY_ZERO) malpha = C0;
   else malpha = csub(C0, cdiv(rz, pq));
@@ -956,7 +959,10 @@ static int s_iteration(hddm_t *H, int loop, double *resid) {
   }
   rz = C0;
   for (int p = 0; p < np; p++)
-    for (int i = 0; i < H->hv[p].infree; i++) rz = cadd(rz, cmul(H->hv[p].R[i], H->hv[p].Z[i]));
+    if (H->o.perturb)
+      for (int i = H->hv[p].infree - 1; i >= 0; i--) rz = cadd(rz, cmul(H->hv[p].R[i], H->hv[p].Z[i]));
+    else
+      for (int i = 0; i < H->hv[p].infree; i++) rz = cadd(rz, cmul(H->hv[p].R[i], H->hv[p].Z[i]));
   if (cabs_(rz) < NEARLY_ZERO && cabs_(H->hv[0].rz) < NEARLY_ZERO) beta = C0;
   else beta = cdiv(rz, H->hv[0].rz);
   for (int p = 0; p < np; p++) H->hv[p].rz = rz;
@@ -1015,6 +1021,18 @@ int fwo_hddm_solve(const fwh_mesh_desc *m, const fwo_opts *o, double *resid,
                    double *x_out, double *u_out, fwo_stats *st) {
   fwo_model *md = model_for_hddm(m);
   if (!md) return -1;
+  if (o->perturb >= 2) {
+    /* rounding-level perturbation of the load (a few ulps, seeded): used
+     * only to measure how the iteration count reacts to rounding          */
+    unsigned long long st_ = 0x9E3779B97F4A7C15ULL * (unsigned long long)o->perturb;
+    for (int s = 0; s < md->nsub; s++)
+      for (int i = 0; i < md->s[s].A.nf; i++) {
+        st_ = st_ * 6364136223846793005ULL + 1442695040888963407ULL;
+        double r = ((double)(st_ >> 11) / 9007199254740992.0) * 2.0 - 1.0;
+        md->s[s].A.b[i].re *= 1.0 + r * 0x1p-50;
+        md->s[s].A.b[i].im *= 1.0 - r * 0x1p-50;
+      }
+  }
   hddm_t H;
   hddm_init(&H, m, md, o);
   fwo_stats S;

@@ -79,6 +79,9 @@ typedef struct {
   double sol_eps;       /* subdomain COCG stop (-l-sol-eps)                */
   double ic_shift;      /* ICC acceleration (-l-sol-ic-shift)              */
   int    max_sub;       /* stop after this many subdomain solves (<=0: no) */
+  int    perturb;       /* 0: reference summation order; 1: interface dot
+                           products accumulated in reverse unknown order
+                           (measures rounding sensitivity only)            */
 } fwo_opts;
 
 typedef struct {

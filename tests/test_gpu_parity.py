@@ -155,7 +155,11 @@ def test_hddm_cfg1_matches_oracle(fw, cfg1_model):
     st = sol.solve(eps=1e-8, max_loop=20000, graph_iters=16)
     it_ref = ref["stats"]["iters"]
     say(f"[gpu] cfg1 iters gpu={st['iters']} oracle={it_ref}")
-    assert abs(st["iters"] - it_ref) <= max(1, 0.02 * it_ref)
+    # The oracle's own count on this model moves between 3196 and 3263
+    # (2.1 %) when its load vector is perturbed by a few ulps
+    # (tests/test_oracle_hddm.py::test_iteration_count_rounding_spread), so
+    # the bar is 3 % rather than the nominal 2 %.
+    assert abs(st["iters"] - it_ref) <= max(1, 0.03 * it_ref)
     x = sol.x()
     assert np.linalg.norm(x - ref["x_flat"]) / np.linalg.norm(ref["x_flat"]) <= 1e-7
     sol.close()
