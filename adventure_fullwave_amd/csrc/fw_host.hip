@@ -98,10 +98,23 @@ struct fwh_ctx {
   double *hist = nullptr;
   int hist_cap = 0;
 
+  // element data kept for post-processing
+  int *el_vtx = nullptr, *el_dof = nullptr;
+  double *crd = nullptr, *sig = nullptr, *post_nu = nullptr;
+  double2 *post_out = nullptr;
+  double omega = 0.0;
+
+  // explicit Schur complements (formed on first use)
+  double2 *smat = nullptr;
+  long long *sm_off = nullptr;
+  int nbmax = 0;
+  bool schur_built = false;
+
   // graph of batched iterations
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_n = 0;
+  int graph_schur = 0;
 
   fwh_stats stats{};
   fwh_info info{};
@@ -124,15 +137,15 @@ struct fwh_ctx {
     return 0;
   }
   void release_graph() {
-    if (gexec) hipGraphExecDestroy(gexec);
-    if (graph) hipGraphDestroy(graph);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     gexec = nullptr;
     graph = nullptr;
     graph_n = 0;
   }
   void release() {
     release_graph();
-    for (void *p : allocs) hipFree(p);
+    for (void *p : allocs) (void)hipFree(p);
     allocs.clear();
   }
   fw::SolveArgs solve_args(int mode, bool check_done) const {
@@ -164,6 +177,23 @@ struct fwh_ctx {
     a.P = P;
     a.yslot = yslot;
     a.X = X;
+    a.unit_k = 0;
+    a.smat = smat;
+    a.sm_off = sm_off;
+    return a;
+  }
+  fw::SmvArgs smv_args(bool check_done) const {
+    fw::SmvArgs a{};
+    a.meta = meta;
+    a.order = order;
+    a.base = 0;
+    a.done = check_done ? &sc->done : nullptr;
+    a.nbmax = nbmax;
+    a.smat = smat;
+    a.sm_off = sm_off;
+    a.g_gid = g_gid;
+    a.P = P;
+    a.yslot = yslot;
     return a;
   }
   fw::VecArgs vec_args() const {
@@ -201,6 +231,20 @@ struct fwh_ctx {
     }
     return hipSuccess;
   }
+  // the Schur step of an interface iteration (y_slot = S_s p_s): implicit
+  // (Dirichlet solves through the factors) or explicit (formed S_s)
+  hipError_t run_schur(bool expl, bool plain, bool check_done) {
+    if (!expl) return run_solve(fw::MODE_ITER, plain, check_done);
+    fw::SmvArgs a = smv_args(check_done);
+    if (!plain) return fw::launch_smv(a, nsub, st);
+    a.order = nullptr;
+    for (int s = 0; s < nsub; s++) {
+      a.base = s;
+      hipError_t e = fw::launch_smv(a, 1, st);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
 };
 
 extern "C" const char *fwh_last_error(void) { return g_err.c_str(); }
@@ -224,8 +268,8 @@ extern "C" int fwh_create(fwh_ctx **out, int device) {
 
 extern "C" void fwh_destroy(fwh_ctx *c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  hipStreamSynchronize(c->st);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->st);
   c->release();
   (void)hipStreamDestroy(c->st);
   delete c;
@@ -356,14 +400,20 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
 
   // ---- element data ---------------------------------------------------------
   const long long nap = m->ap_off[nsub];
-  std::vector<int> el_vtx(4 * nel);
+  std::vector<int> el_vtx(4 * nel), el_dof(6 * nel);
   for (int s = 0; s < nsub; s++)
-    for (long long el = m->el_off[s]; el < m->el_off[s + 1]; el++)
+    for (long long el = m->el_off[s]; el < m->el_off[s + 1]; el++) {
       for (int i = 0; i < 4; i++) {
         const int lv = m->nop[10 * el + i];
         if (lv < 0 || m->ap_off[s] + lv >= m->ap_off[s + 1]) return fail(FWH_E_ARG, "element %lld: vertex out of range", el);
         el_vtx[4 * el + i] = m->ap_off[s] + lv;
       }
+      for (int k = 0; k < 6; k++) {
+        const int le = m->nop[10 * el + 4 + k];
+        if (le < 0 || le >= m->nmp[s]) return fail(FWH_E_ARG, "element %lld: edge out of range", el);
+        el_dof[6 * el + k] = meta[s].dof0 + le;
+      }
+    }
 
   // ---- symbolic lower-triangle pattern + assembly lists --------------------
   std::vector<int> rowptr(ndof + 1, 0), col, ent_row, asm_ptr(1, 0), asm_src;
@@ -520,8 +570,13 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   std::vector<int> src(nel, 0);
   if (m->src) std::copy(m->src, m->src + nel, src.begin());
   if ((rc = c->upload(&d_el_vtx, el_vtx)) || (rc = c->upload(&d_crd, crd)) || (rc = c->upload(&d_joe, joe)) ||
-      (rc = c->upload(&d_eps, eps)) || (rc = c->upload(&d_sig, sig)) || (rc = c->upload(&d_src, src)))
+      (rc = c->upload(&d_eps, eps)) || (rc = c->upload(&d_sig, sig)) || (rc = c->upload(&d_src, src)) ||
+      (rc = c->upload(&c->el_dof, el_dof)))
     return rc;
+  c->el_vtx = d_el_vtx;
+  c->crd = d_crd;
+  c->sig = d_sig;
+  c->omega = m->omega;
   if (m->pml) {
     std::vector<double> pml(m->pml, m->pml + 6 * nel);
     if ((rc = c->upload(&d_pml, pml))) return rc;
@@ -728,7 +783,54 @@ int read_scal(fwh_ctx *c, fw::Scal *s) {
   HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }
+// Form every S_s = A_GG - A_GI A_II^-1 A_IG once: column k of all subdomains
+// in one batched launch of the Schur step on the unit vector e_k (the same
+// kernel, factors and summation order as the implicit iteration).
+int build_schur(fwh_ctx *c) {
+  if (c->schur_built) return 0;
+  std::vector<long long> off(c->nsub);
+  long long tot = 0;
+  int nbmax = 1;
+  for (int s = 0; s < c->nsub; s++) {
+    const long long nb = (c->meta_h[s].ng + 15) / 16;
+    off[s] = tot;
+    tot += nb * (nb + 1) / 2 * fw::SMV_TILE;
+    nbmax = std::max(nbmax, (int)nb);
+  }
+  if (fw::smv_lds_bytes(nbmax) > 160 * 1024)
+    return fail(FWH_E_LIMIT, "a subdomain of %d interface DOFs exceeds the explicit Schur LDS budget", c->ngmax);
+  HIPCHK(hipStreamSynchronize(c->st));
+  const double t0 = now_ms();
+  int rc;
+  if ((rc = c->alloc(&c->smat, tot)) || (rc = c->upload(&c->sm_off, off))) return rc;
+  c->nbmax = nbmax;
+  HIPCHK(fw::launch_fill(c->smat, tot, make_double2(0.0, 0.0), c->st));
+  fw::SolveArgs a = c->solve_args(fw::MODE_UNIT, false);
+  for (int k = 0; k < c->ngmax; k++) {
+    a.unit_k = k;
+    HIPCHK(fw::launch_solve(a, c->nsub, c->st));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->info.ms_schur_form = now_ms() - t0;
+  c->info.smat_bytes = 16.0 * (double)tot;
+  double packed = 0.0;
+  for (int s = 0; s < c->nsub; s++) {
+    const double g = c->meta_h[s].ng;
+    packed += 16.0 * g * (g + 1.0) / 2.0;
+  }
+  // packed lower triangle + p gather (value + index) + y out, per slot
+  c->info.bytes_smv = packed + 36.0 * (double)c->nslots;
+  c->schur_built = true;
+  return 0;
+}
 }  // namespace
+
+extern "C" int fwh_schur_form(fwh_ctx *c) {
+  if (!c) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  HIPCHK(hipSetDevice(c->device));
+  return build_schur(c);
+}
 
 extern "C" int fwh_first(fwh_ctx *c, const fwh_opts *o) {
   if (!c || !o) return fail(FWH_E_ARG, "null argument");
@@ -737,6 +839,7 @@ extern "C" int fwh_first(fwh_ctx *c, const fwh_opts *o) {
   HIPCHK(hipSetDevice(c->device));
   int rc = ensure_hist(c, o->hddm_max_loop);
   if (rc) return rc;
+  if (o->schur && (rc = build_schur(c))) return rc;
   memset(&c->stats, 0, sizeof(c->stats));
   HIPCHK(hipStreamSynchronize(c->st));
   const double t0 = now_ms();
@@ -763,19 +866,21 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   if ((rc = read_scal(c, &s))) return rc;
   const double t0 = now_ms();
   int it = 0;
+  const bool expl = o->schur != 0;
+  if (expl && !c->schur_built) return fail(FWH_E_STATE, "explicit Schur mode requested after fwh_first without it");
   if (o->plain) {
     while (it < n && !s.done) {
-      HIPCHK(c->run_solve(fw::MODE_ITER, true, true));
+      HIPCHK(c->run_schur(expl, true, true));
       HIPCHK(fw::launch_iter(c->vec_args(), c->st));
       if ((rc = read_scal(c, &s))) return rc;
       it++;
     }
   } else if (o->graph_iters > 0) {
-    if (!c->gexec || c->graph_n != o->graph_iters) {
+    if (!c->gexec || c->graph_n != o->graph_iters || c->graph_schur != o->schur) {
       c->release_graph();
       HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < o->graph_iters; k++) {
-        hipError_t e = c->run_solve(fw::MODE_ITER, false, true);
+        hipError_t e = c->run_schur(expl, false, true);
         if (e == hipSuccess) e = fw::launch_iter(c->vec_args(), c->st);
         if (e != hipSuccess) {
           hipGraph_t g;
@@ -787,6 +892,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
       HIPCHK(hipStreamEndCapture(c->st, &c->graph));
       HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
       c->graph_n = o->graph_iters;
+      c->graph_schur = o->schur;
     }
     while (it < n && !s.done) {
       HIPCHK(hipGraphLaunch(c->gexec, c->st));
@@ -795,7 +901,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
     }
   } else {
     while (it < n && !s.done) {
-      HIPCHK(c->run_solve(fw::MODE_ITER, false, true));
+      HIPCHK(c->run_schur(expl, false, true));
       HIPCHK(fw::launch_iter(c->vec_args(), c->st));
       it++;
       if ((it & 15) == 0 || it >= n)
@@ -808,7 +914,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   c->stats.loops = s.loop;
   c->stats.converged = s.conv;
   c->stats.final_resid = s.resi;
-  c->stats.sub_solves = (long)c->nsub * (1 + s.iters);
+  c->stats.sub_solves = (long)c->nsub * (1 + (expl ? 0 : s.iters));  // explicit: no solves per iteration
   if (done) *done = s.done;
   return 0;
 }
@@ -855,15 +961,52 @@ extern "C" int fwh_get_resid(const fwh_ctx *c, double *r, int nr) {
   return 0;
 }
 
+// Element post-processing of the last solution (HDDM_post_dom_em_E,
+// main/fullwave/fullwave.c:607-635, per element HDDM_post_em_E :640-763).
+extern "C" int fwh_post(fwh_ctx *c, const double *nu, double *B, double *E, double *J) {
+  if (!c || !B || !E || !J) return fail(FWH_E_ARG, "null argument");
+  if (!c->last_done) return fail(FWH_E_STATE, "no result (fwh_last not called)");
+  HIPCHK(hipSetDevice(c->device));
+  int rc;
+  const size_t nel = (size_t)c->nel;
+  if (!c->post_out && (rc = c->alloc(&c->post_out, 9 * nel))) return rc;
+  if (nu) {
+    if (!c->post_nu && (rc = c->alloc(&c->post_nu, nel))) return rc;
+    HIPCHK(hipMemcpyAsync(c->post_nu, nu, nel * sizeof(double), hipMemcpyHostToDevice, c->st));
+  }
+  fw::PostArgs a{};
+  a.nel = (int)c->nel;
+  a.el_vtx = c->el_vtx;
+  a.el_dof = c->el_dof;
+  a.crd = c->crd;
+  a.sigma = c->sig;
+  a.nu = nu ? c->post_nu : nullptr;
+  a.omega = c->omega;
+  a.X = c->X;
+  a.B = c->post_out;
+  a.E = c->post_out + 3 * nel;
+  a.J = c->post_out + 6 * nel;
+  HIPCHK(fw::launch_post(a, c->st));
+  HIPCHK(hipMemcpyAsync(B, a.B, 3 * nel * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(E, a.E, 3 * nel * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(J, a.J, 3 * nel * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // single operators
 // ---------------------------------------------------------------------------
-extern "C" int fwh_schur_apply(fwh_ctx *c, const double *p, double *q, int plain) {
+extern "C" int fwh_schur_apply(fwh_ctx *c, const double *p, double *q, int mode) {
   if (!c || !p || !q) return fail(FWH_E_ARG, "null argument");
   if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  if (mode < 0 || mode > 3) return fail(FWH_E_ARG, "bad mode %d", mode);
   HIPCHK(hipSetDevice(c->device));
+  const bool expl = mode >= 2;
+  int rc;
+  if (expl && (rc = build_schur(c))) return rc;
   HIPCHK(hipMemcpyAsync(c->P, p, (size_t)c->ng * sizeof(double2), hipMemcpyHostToDevice, c->st));
-  HIPCHK(c->run_solve(fw::MODE_ITER, plain != 0, false));
+  HIPCHK(c->run_schur(expl, (mode & 1) != 0, false));
   HIPCHK(fw::launch_gather_q(c->vec_args(), c->st));
   HIPCHK(hipMemcpyAsync(q, c->Q, (size_t)c->ng * sizeof(double2), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -883,7 +1026,7 @@ extern "C" int fwh_ldl_solve_all(fwh_ctx *c, double *rhs, int plain) {
   return 0;
 }
 
-extern "C" int fwh_time_kernels(fwh_ctx *c, int reps, double *ms_solve, double *ms_schur) {
+extern "C" int fwh_time_kernels(fwh_ctx *c, int reps, double *ms_solve, double *ms_schur, double *ms_smv) {
   if (!c || reps <= 0) return fail(FWH_E_ARG, "bad argument");
   if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
   HIPCHK(hipSetDevice(c->device));
@@ -905,6 +1048,18 @@ extern "C" int fwh_time_kernels(fwh_ctx *c, int reps, double *ms_solve, double *
   HIPCHK(hipEventSynchronize(e1));
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   if (ms_schur) *ms_schur = ms / reps;
+  if (ms_smv) {
+    *ms_smv = -1.0;
+    if (c->schur_built) {
+      HIPCHK(c->run_schur(true, false, false));
+      HIPCHK(hipEventRecord(e0, c->st));
+      for (int r = 0; r < reps; r++) HIPCHK(c->run_schur(true, false, false));
+      HIPCHK(hipEventRecord(e1, c->st));
+      HIPCHK(hipEventSynchronize(e1));
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      *ms_smv = ms / reps;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->first_done = false;

@@ -19,7 +19,13 @@ struct SubMeta {
   int nnz;        // entries
 };
 
-enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3 };
+// MODE_UNIT: the Schur step applied to interface unit vector unit_k, output
+// written into the explicit Schur-complement tiles (fw_schur.hip)
+enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3, MODE_UNIT = 4 };
+
+// explicit Schur complements: 16x16 tiles of the lower triangle
+constexpr int SMV_TILE = 256;
+constexpr int SMV_THREADS = 256;
 
 // solve kernel limits: rows per block, prefetch registers per thread
 constexpr int SOLVE_THREADS = 64;
@@ -55,6 +61,22 @@ struct SolveArgs {
   const double2 *P;      // global interface values
   double2 *yslot;        // per slot output
   double2 *X;            // per DOF solution (MODE_LAST, MODE_RHS in/out)
+  int unit_k;            // MODE_UNIT: local interface index of the unit vector
+  double2 *smat;         // MODE_UNIT: Schur-complement tiles
+  const long long *sm_off;  // per subdomain first entry of its tiles
+};
+
+struct SmvArgs {
+  const SubMeta *meta;
+  const int *order;
+  int base;
+  const int *done;
+  int nbmax;             // largest number of 16-row blocks (LDS sizing)
+  const double2 *smat;
+  const long long *sm_off;
+  const int *g_gid;
+  const double2 *P;
+  double2 *yslot;
 };
 
 struct FactorArgs {
@@ -79,6 +101,19 @@ struct AsmArgs {
   double omega;
   double2 *elem_out;     // [nel*27]: 21 lower pairs + 6 loads
   int *bad;              // set when an element is degenerate
+};
+
+// element post-processing (HDDM_post_em_E)
+struct PostArgs {
+  int nel;
+  const int *el_vtx;     // [nel*4] global vertex index
+  const int *el_dof;     // [nel*6] global DOF of each local edge
+  const double *crd;     // [nv*3]
+  const double *sigma;   // [nel]
+  const double *nu;      // [nel] 1/mu of the element's material, or null (all 7.957747e5)
+  double omega;
+  const double2 *X;      // per DOF solution
+  double2 *B, *E, *J;    // [nel*3] each
 };
 
 // scalars of the interface COCG
@@ -110,6 +145,7 @@ struct VecArgs {
 
 // launchers (fw_kernels.hip)
 hipError_t launch_element(const AsmArgs &a, hipStream_t st);
+hipError_t launch_post(const PostArgs &a, hipStream_t st);
 hipError_t launch_gather_sum(int n, const int *ptr, const int *src, const double2 *in,
                              double2 *out, hipStream_t st);
 hipError_t launch_copy_idx(int n, const int *slot, const double2 *in, double2 *out,
@@ -126,6 +162,8 @@ hipError_t launch_skydiag_bc(int ndof, const int *dof_sub, const SubMeta *meta,
 hipError_t launch_factor(const FactorArgs &a, int nwork, hipStream_t st);
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st);
 size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax);
+hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st);
+size_t smv_lds_bytes(int nbmax);
 hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, const double2 *val,
                          double2 *pcslot, hipStream_t st);
 hipError_t launch_first(const VecArgs &a, hipStream_t st);

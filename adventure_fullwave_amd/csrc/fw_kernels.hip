@@ -23,16 +23,16 @@ __constant__ int EDGE[6][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2},
 // ---------------------------------------------------------------------------
 // element kernel: one thread per tetrahedron
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(128) void k_element(AsmArgs a) {
-  const int el = blockIdx.x * blockDim.x + threadIdx.x;
-  if (el >= a.nel) return;
-  double v[12];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    int g = a.el_vtx[4 * el + i];
-#pragma unroll
-    for (int c = 0; c < 3; c++) v[3 * i + c] = a.crd[3 * (long long)g + c];
-  }
+// Geometry of one tetrahedron (v: 4 vertices x 3): D = 6V, centroid g, and
+// per local edge k the signed length ls_k = l_k si_k (orientation from the
+// coordinate order x, then y, then z: tetNedelec_Direction,
+// tetrahedral.c:225-263), the opposite-edge vector e_k = v_d - v_c
+// (tetNedelec_mk, tetrahedral.c:267-281) and F_k = v_c x v_d + e_k x g, so
+// that the edge function is N_k(r) = (ls_k / D)(v_c x v_d + e_k x r)
+// (tetNedelec_km_mk / _Fxyz, tetrahedral.c:305-340, without the ls factor).
+// Returns false for a degenerate or inverted element.
+__device__ __forceinline__ bool tet_geom(const double v[12], double &D, double g[3], double ls[6], double e[6][3],
+                                         double F[6][3]) {
   double r1[3], r2[3], r3[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -40,26 +40,14 @@ __global__ __launch_bounds__(128) void k_element(AsmArgs a) {
     r2[c] = v[6 + c] - v[c];
     r3[c] = v[9 + c] - v[c];
   }
-  // G[i] = D * grad(lambda_i)
-  double G[4][3];
-  G[1][0] = r2[1] * r3[2] - r2[2] * r3[1];
-  G[1][1] = r2[2] * r3[0] - r2[0] * r3[2];
-  G[1][2] = r2[0] * r3[1] - r2[1] * r3[0];
-  G[2][0] = r3[1] * r1[2] - r3[2] * r1[1];
-  G[2][1] = r3[2] * r1[0] - r3[0] * r1[2];
-  G[2][2] = r3[0] * r1[1] - r3[1] * r1[0];
-  G[3][0] = r1[1] * r2[2] - r1[2] * r2[1];
-  G[3][1] = r1[2] * r2[0] - r1[0] * r2[2];
-  G[3][2] = r1[0] * r2[1] - r1[1] * r2[0];
-#pragma unroll
-  for (int c = 0; c < 3; c++) G[0][c] = -(G[1][c] + G[2][c] + G[3][c]);
-  const double D = r1[0] * G[1][0] + r1[1] * G[1][1] + r1[2] * G[1][2];
-  double g[3];
+  // first row of the adjugate: D = r1 . (r2 x r3)
+  const double G0 = r2[1] * r3[2] - r2[2] * r3[1];
+  const double G1 = r2[2] * r3[0] - r2[0] * r3[2];
+  const double G2 = r2[0] * r3[1] - r2[1] * r3[0];
+  D = r1[0] * G0 + r1[1] * G1 + r1[2] * G2;
 #pragma unroll
   for (int c = 0; c < 3; c++) g[c] = (v[c] + v[3 + c] + v[6 + c] + v[9 + c]) * 0.25;
-
-  double ls[6], e[6][3], F[6][3];
-  bool bad = !(D > 0.0);
+  bool ok = D > 0.0;
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     const int ia = EDGE[k][0], ib = EDGE[k][1], ic = EDGE[k][2], id = EDGE[k][3];
@@ -75,7 +63,7 @@ __global__ __launch_bounds__(128) void k_element(AsmArgs a) {
     double si = dl[0] > 0.0 ? 1.0 : dl[0] < 0.0 ? -1.0
               : dl[1] > 0.0 ? 1.0 : dl[1] < 0.0 ? -1.0
               : dl[2] > 0.0 ? 1.0 : dl[2] < 0.0 ? -1.0 : 0.0;
-    if (si == 0.0) bad = true;
+    if (si == 0.0) ok = false;
     ls[k] = l * si;
     double w0 = vc[1] * vd[2] - vc[2] * vd[1];
     double w1 = vc[2] * vd[0] - vc[0] * vd[2];
@@ -84,7 +72,47 @@ __global__ __launch_bounds__(128) void k_element(AsmArgs a) {
     F[k][1] = w1 + e[k][2] * g[0] - e[k][0] * g[2];
     F[k][2] = w2 + e[k][0] * g[1] - e[k][1] * g[0];
   }
-  if (bad) {
+  return ok;
+}
+
+// G[i] = D grad(lambda_i): the columns of the adjugate of the edge matrix
+__device__ __forceinline__ void tet_grads(const double v[12], double G[4][3]) {
+  double r1[3], r2[3], r3[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    r1[c] = v[3 + c] - v[c];
+    r2[c] = v[6 + c] - v[c];
+    r3[c] = v[9 + c] - v[c];
+  }
+  G[1][0] = r2[1] * r3[2] - r2[2] * r3[1];
+  G[1][1] = r2[2] * r3[0] - r2[0] * r3[2];
+  G[1][2] = r2[0] * r3[1] - r2[1] * r3[0];
+  G[2][0] = r3[1] * r1[2] - r3[2] * r1[1];
+  G[2][1] = r3[2] * r1[0] - r3[0] * r1[2];
+  G[2][2] = r3[0] * r1[1] - r3[1] * r1[0];
+  G[3][0] = r1[1] * r2[2] - r1[2] * r2[1];
+  G[3][1] = r1[2] * r2[0] - r1[0] * r2[2];
+  G[3][2] = r1[0] * r2[1] - r1[1] * r2[0];
+#pragma unroll
+  for (int c = 0; c < 3; c++) G[0][c] = -(G[1][c] + G[2][c] + G[3][c]);
+}
+
+__device__ __forceinline__ void load_tet(const int *el_vtx, const double *crd, int el, double v[12]) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int g = el_vtx[4 * el + i];
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[3 * i + c] = crd[3 * (long long)g + c];
+  }
+}
+
+__global__ __launch_bounds__(128) void k_element(AsmArgs a) {
+  const int el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= a.nel) return;
+  double v[12];
+  load_tet(a.el_vtx, a.crd, el, v);
+  double D, g[3], ls[6], e[6][3], F[6][3];
+  if (!tet_geom(v, D, g, ls, e, F)) {
     atomicOr(a.bad, 1);
     return;
   }
@@ -161,6 +189,8 @@ __global__ __launch_bounds__(128) void k_element(AsmArgs a) {
   // current source: exact integral of N_i . sum_j lambda_j J_j
   // (tetNedelec_JoA_Complex, lib/fem/tetrahedral.c:794-858)
   if (a.src[el]) {
+    double G[4][3];
+    tet_grads(v, G);
     double2 J[4][3];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -191,6 +221,52 @@ __global__ __launch_bounds__(128) void k_element(AsmArgs a) {
 hipError_t launch_element(const AsmArgs &a, hipStream_t st) {
   if (a.nel <= 0) return hipSuccess;
   k_element<<<(a.nel + 127) / 128, 128, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// element post-processing, one thread per tetrahedron
+// (HDDM_post_em_E, main/fullwave/fullwave.c:640-763):
+//   xe_k  = (nu / omega) x_c[edge k]                      (:721)
+//   rotE  = (2/D)     sum_k (i xe_k) ls_k e_k             (:716-729)
+//   E     = (2/D)     sum_k xe_k ls_k F_k(g)              (:733-748)
+//   Je    = (sigma/D) sum_k xe_k ls_k F_k(g)              (:750-762)
+// nu is 7.957747e5 for elements whose flag is a listed material and 0
+// otherwise (:670-675); with N_k(r) = (ls_k/D)(v_c x v_d + e_k x r) these are
+// rotE = i (nu/omega) curl E, E = 2 (nu/omega) E(g), Je = sigma (nu/omega) E(g),
+// the reference's scalings kept as they are.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_post(PostArgs a) {
+  const int el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= a.nel) return;
+  double v[12];
+  load_tet(a.el_vtx, a.crd, el, v);
+  double D, g[3], ls[6], e[6][3], F[6][3];
+  (void)tet_geom(v, D, g, ls, e, F);
+  const double s = (a.nu ? a.nu[el] : NU) / a.omega;
+  double2 B[3] = {cz(), cz(), cz()}, E[3] = {cz(), cz(), cz()};
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const double2 x = cscale(a.X[a.el_dof[6 * (long long)el + k]], s);
+    const double2 ix = make_double2(-x.y, x.x);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      B[c] = cadd(B[c], cscale(ix, ls[k] * e[k][c]));
+      E[c] = cadd(E[c], cscale(x, ls[k] * F[k][c]));
+    }
+  }
+  const double dd = 2.0 / D, ds = a.sigma[el] / D;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    a.B[3 * (long long)el + c] = cscale(B[c], dd);
+    a.E[3 * (long long)el + c] = cscale(E[c], dd);
+    a.J[3 * (long long)el + c] = cscale(E[c], ds);
+  }
+}
+
+hipError_t launch_post(const PostArgs &a, hipStream_t st) {
+  if (a.nel <= 0) return hipSuccess;
+  k_post<<<(a.nel + 127) / 128, 128, 0, st>>>(a);
   return hipGetLastError();
 }
 

@@ -1,0 +1,181 @@
+// fw_schur.hip - the interface matvec q = S p with explicit subdomain Schur
+// complements.
+//
+// The Schur step of the reference (hddm_solver.h:559-622 with
+// renewal_bc_sp, hddm_solver_sp.h:58-69, and COM_res_sp,
+// hddm_solver_sp.h:28-54) applies, per subdomain and per interface
+// iteration, y_G = (A x)_G with A_II x_I = -A_IG p_G, x_G = p_G; that is
+// y = S_s p_s with the complex-symmetric Schur complement
+// S_s = A_GG - A_GI A_II^-1 A_IG.  In explicit mode the S_s are formed once
+// from the skyline factors (the same Schur step applied to the unit vectors
+// of the interface, MODE_UNIT of k_solve) and each iteration streams them
+// once through k_smv: a batched, HBM-bound symmetric matvec instead of two
+// dependent triangular sweeps per subdomain.
+//
+// Storage of S_s: the lower triangle in 16x16 tiles, tile (I,J) with J <= I
+// at tile index I(I+1)/2 + J, 256 complex entries per tile.  Inside a tile
+// entry (row r, column c) sits at ((r>>3)*2 + (c>>3))*64 + ((r&7) | (c&7)<<3),
+// so each of the four 64-lane loads of a tile reads 1 KB contiguously and
+// lane l holds rows l&7, (l&7)+8 and columns l>>3, (l>>3)+8.  Diagonal tiles
+// hold the strict lower part and half the diagonal (upper part zero): every
+// tile T is applied as T + T^T, which restores S_s exactly.
+#include "fw_kernels.h"
+#include "fw_device.h"
+
+namespace fw {
+
+namespace {
+
+constexpr int SMV_WAVES = SMV_THREADS / 64;
+
+struct Ent {
+  double x, y;
+};
+struct Tile {
+  Ent e[4];  // e[a*2+b]: row (l&7)+8a, column (l>>3)+8b
+};
+
+__device__ __forceinline__ void load_tile(const double2 *__restrict__ base, long long t, int lane, Tile &T) {
+  const double2 *p = base + t * SMV_TILE + lane;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const double2 v = p[k * 64];
+    T.e[k].x = v.x;
+    T.e[k].y = v.y;
+  }
+}
+
+__device__ __forceinline__ double2 to_c(const Ent &e) { return make_double2(e.x, e.y); }
+
+// a double moved between lanes by a DPP pattern (two 32-bit moves)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// sum over each aligned group of 8 lanes, in registers: quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], then row_half_mirror (lane i <- lane 7-i of its
+// 8-lane half-row).  Every lane of a group ends with the same sum.
+__device__ __forceinline__ double sum8(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  return v;
+}
+__device__ __forceinline__ double2 csum8(double2 v) { return make_double2(sum8(v.x), sum8(v.y)); }
+
+struct SmvState {
+  int I, J;
+  double2 q0, q1;    // row partials (rows r8, r8+8 of block I) over this lane's columns
+  double2 pI0, pI1;  // p at rows r8, r8+8 of block I
+};
+
+// one tile: row contributions accumulate in registers until the row block
+// ends; column contributions are summed over the 8 row lanes at once and
+// added to the wave's private accumulator
+__device__ __forceinline__ void tile_step(const Tile &T, SmvState &st, const double2 *pv, double2 *my, int nb,
+                                          bool last, int lane) {
+  const int r8 = lane & 7, c8 = lane >> 3;
+  const double2 pJ0 = pv[st.J * 16 + c8], pJ1 = pv[st.J * 16 + c8 + 8];
+  const double2 s00 = to_c(T.e[0]), s01 = to_c(T.e[1]), s10 = to_c(T.e[2]), s11 = to_c(T.e[3]);
+  st.q0 = cfma(cfma(st.q0, s00, pJ0), s01, pJ1);
+  st.q1 = cfma(cfma(st.q1, s10, pJ0), s11, pJ1);
+  double2 u0 = cfma(cmul(s00, st.pI0), s10, st.pI1);  // column c8
+  double2 u1 = cfma(cmul(s01, st.pI0), s11, st.pI1);  // column c8 + 8
+  u0 = csum8(u0);
+  u1 = csum8(u1);
+  if (r8 == 0) {
+    double2 *d = my + st.J * 16 + c8;
+    d[0] = cadd(d[0], u0);
+    d[8] = cadd(d[8], u1);
+  }
+  st.J++;
+  if (st.J > st.I || last) {
+    double2 q0 = st.q0, q1 = st.q1;
+#pragma unroll
+    for (int o = 8; o < 64; o <<= 1) {
+      q0.x += __shfl_xor(q0.x, o, 64);
+      q0.y += __shfl_xor(q0.y, o, 64);
+      q1.x += __shfl_xor(q1.x, o, 64);
+      q1.y += __shfl_xor(q1.y, o, 64);
+    }
+    if (c8 == 0) {
+      double2 *d = my + st.I * 16 + r8;
+      d[0] = cadd(d[0], q0);
+      d[8] = cadd(d[8], q1);
+    }
+    st.q0 = cz();
+    st.q1 = cz();
+    st.I++;
+    st.J = 0;
+    if (st.I < nb) {
+      st.pI0 = pv[st.I * 16 + r8];
+      st.pI1 = pv[st.I * 16 + r8 + 8];
+    }
+  }
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(SMV_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_smv(SmvArgs a) {
+  if (a.done && *a.done) return;
+  const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
+  const SubMeta m = a.meta[w];
+  const int ng = m.ng, nb = (ng + 15) >> 4, npad = nb * 16, stride = a.nbmax * 16;
+  extern __shared__ double2 lds[];
+  double2 *pv = lds;          // [nbmax*16] interface values of the subdomain
+  double2 *acc = pv + stride; // [SMV_WAVES][nbmax*16] per-wave accumulators
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+
+  for (int k = tid; k < npad; k += SMV_THREADS) pv[k] = k < ng ? a.P[a.g_gid[m.g0 + k]] : cz();
+  for (int k = tid; k < SMV_WAVES * stride; k += SMV_THREADS) acc[k] = cz();
+  __syncthreads();
+
+  // contiguous, equal shares of the tile sequence (row-block major)
+  const long long ntile = (long long)nb * (nb + 1) / 2;
+  const long long t0 = ntile * wave / SMV_WAVES, t1 = ntile * (wave + 1) / SMV_WAVES;
+  if (t0 < t1) {
+    double2 *my = acc + wave * stride;
+    const double2 *base = a.smat + a.sm_off[w];
+    SmvState st;
+    st.I = (int)((sqrt(8.0 * (double)t0 + 1.0) - 1.0) * 0.5);
+    while ((long long)(st.I + 1) * (st.I + 2) / 2 <= t0) st.I++;
+    while ((long long)st.I * (st.I + 1) / 2 > t0) st.I--;
+    st.J = (int)(t0 - (long long)st.I * (st.I + 1) / 2);
+    st.q0 = cz();
+    st.q1 = cz();
+    st.pI0 = pv[st.I * 16 + (lane & 7)];
+    st.pI1 = pv[st.I * 16 + (lane & 7) + 8];
+    // two tiles in flight in two statically named register sets; the loads
+    // are unconditional (clamped to the wave's last tile) so the compiler can
+    // wait for one set while the other stays in flight
+    const long long tl = t1 - 1;
+    Tile A, B;
+    load_tile(base, t0, lane, A);
+    load_tile(base, t0 + 1 < tl ? t0 + 1 : tl, lane, B);
+    for (long long t = t0; t < t1; t += 2) {
+      tile_step(A, st, pv, my, nb, t == tl, lane);
+      load_tile(base, t + 2 < tl ? t + 2 : tl, lane, A);
+      if (t + 1 < t1) tile_step(B, st, pv, my, nb, t + 1 == tl, lane);
+      load_tile(base, t + 3 < tl ? t + 3 : tl, lane, B);
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < ng; k += SMV_THREADS) {
+    double2 y = cadd(acc[k], acc[stride + k]);
+    for (int v = 2; v < SMV_WAVES; v++) y = cadd(y, acc[v * stride + k]);
+    a.yslot[m.g0 + k] = y;
+  }
+}
+
+size_t smv_lds_bytes(int nbmax) { return sizeof(double2) * (size_t)(1 + SMV_WAVES) * nbmax * 16; }
+
+hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st) {
+  if (nwork <= 0) return hipSuccess;
+  k_smv<<<nwork, SMV_THREADS, smv_lds_bytes(a.nbmax), st>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace fw

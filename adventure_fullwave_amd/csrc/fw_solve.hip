@@ -198,6 +198,7 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   if (a.mode == MODE_ITER && a.done && *a.done) return;
   const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
   const SubMeta m = a.meta[w];
+  if (a.mode == MODE_UNIT && a.unit_k >= m.ng) return;
   extern __shared__ double2 lds[];
   SolveLds L;
   L.v = lds;                          // [nmax]
@@ -226,6 +227,15 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     double2 b;
     if (a.mode == MODE_RHS) {
       b = a.X[d];
+    } else if (a.mode == MODE_UNIT) {
+      const int8_t fl = a.bcflag[d];
+      b = cz();
+      if (fl == 1) {
+        if (a.kidx[d] == a.unit_k) b.x = 1.0;
+      } else if (fl == 0) {
+        for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++)
+          if (a.cp_k[c] == a.unit_k) b = csub(b, a.cp_val[c]);
+      }
     } else {
       const int8_t fl = a.bcflag[d];
       if (fl == 1) {
@@ -279,7 +289,21 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   // ---- 5. outputs ------------------------------------------------------------------------------
   if (a.mode == MODE_LAST || a.mode == MODE_RHS)
     for (int i = lane; i < n; i += WL) a.X[m.dof0 + i] = L.v[i];
-  if (a.mode != MODE_RHS) {
+  if (a.mode == MODE_UNIT) {
+    // column unit_k of S_s: rows k >= unit_k into the lower-triangle tiles
+    // (layout in fw_schur.hip), the diagonal halved
+    const int j = a.unit_k, J = j >> 4, cj = j & 15;
+    double2 *dst = a.smat + a.sm_off[w];
+    for (int k = j + lane; k < m.ng; k += WL) {
+      const int sl = m.g0 + k;
+      double2 y = cz();
+      for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], L.v[a.gr_col[c]]);
+      if (k == j) y = cscale(y, 0.5);
+      const int I = k >> 4, ri = k & 15;
+      const long long t = (long long)I * (I + 1) / 2 + J;
+      dst[t * SMV_TILE + ((ri >> 3) * 2 + (cj >> 3)) * 64 + ((ri & 7) | ((cj & 7) << 3))] = y;
+    }
+  } else if (a.mode != MODE_RHS) {
     for (int k = lane; k < m.ng; k += WL) {
       const int sl = m.g0 + k;
       double2 y = cz();

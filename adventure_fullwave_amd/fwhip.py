@@ -26,12 +26,12 @@ FWH_E = {-1: "FWH_E_ARG", -2: "FWH_E_HIP", -3: "FWH_E_MESH", -4: "FWH_E_STATE", 
 EXPORTS = ("fwh_create", "fwh_destroy", "fwh_last_error", "fwh_load", "fwh_get_info", "fwh_solve",
            "fwh_first", "fwh_iterate", "fwh_last", "fwh_get_stats", "fwh_get_x", "fwh_get_u",
            "fwh_get_resid", "fwh_schur_apply", "fwh_ldl_solve_all", "fwh_time_kernels", "fwh_get_sub",
-           "fwh_element_batch", "fwh_cocg_icc_all")
+           "fwh_element_batch", "fwh_cocg_icc_all", "fwh_schur_form", "fwh_post")
 
 
 class FwhOpts(ctypes.Structure):
     _fields_ = [("hddm_eps", ctypes.c_double), ("hddm_max_loop", ctypes.c_int),
-                ("plain", ctypes.c_int), ("graph_iters", ctypes.c_int)]
+                ("plain", ctypes.c_int), ("graph_iters", ctypes.c_int), ("schur", ctypes.c_int)]
 
 
 class FwhStats(ctypes.Structure):
@@ -49,7 +49,9 @@ class FwhInfo(ctypes.Structure):
                 ("nlen", ctypes.c_long), ("n_elem", ctypes.c_long), ("n_cpl", ctypes.c_long),
                 ("n_grow", ctypes.c_long), ("max_n", ctypes.c_int), ("ms_assemble", ctypes.c_double),
                 ("ms_factor", ctypes.c_double), ("bytes_solve", ctypes.c_double),
-                ("bytes_iter", ctypes.c_double), ("flops_factor", ctypes.c_double)]
+                ("bytes_iter", ctypes.c_double), ("flops_factor", ctypes.c_double),
+                ("ms_schur_form", ctypes.c_double), ("smat_bytes", ctypes.c_double),
+                ("bytes_smv", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -85,7 +87,9 @@ def lib():
         L.fwh_get_resid.argtypes = [P, P, I]
         L.fwh_schur_apply.argtypes = [P, P, P, I]
         L.fwh_ldl_solve_all.argtypes = [P, P, I]
-        L.fwh_time_kernels.argtypes = [P, I, ctypes.POINTER(D), ctypes.POINTER(D)]
+        L.fwh_time_kernels.argtypes = [P, I, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)]
+        L.fwh_schur_form.argtypes = [P]
+        L.fwh_post.argtypes = [P, P, P, P, P]
         L.fwh_get_sub.argtypes = [P, I, I, P]
         L.fwh_element_batch.argtypes = [P, I, P, P, P, D, P, P, P, P, P]
         L.fwh_cocg_icc_all.argtypes = [P, P, P, P, D, D]
@@ -153,19 +157,31 @@ class Solver:
         self.info = inf.as_dict()
 
     @staticmethod
-    def opts(eps=1e-8, max_loop=100000, plain=False, graph_iters=16):
-        return FwhOpts(float(eps), int(max_loop), int(bool(plain)), int(graph_iters))
+    def opts(eps=1e-8, max_loop=100000, plain=False, graph_iters=16, schur=False):
+        return FwhOpts(float(eps), int(max_loop), int(bool(plain)), int(graph_iters), int(bool(schur)))
 
-    def solve(self, eps=1e-8, max_loop=100000, plain=False, graph_iters=16):
-        o = self.opts(eps, max_loop, plain, graph_iters)
+    def solve(self, eps=1e-8, max_loop=100000, plain=False, graph_iters=16, schur=False):
+        """schur=True: explicit subdomain Schur complements (formed once)."""
+        o = self.opts(eps, max_loop, plain, graph_iters, schur)
         st = FwhStats()
         _check(lib().fwh_solve(self.ctx.h, ctypes.byref(o), ctypes.byref(st)), "fwh_solve")
         self.last_opts = o
+        self._refresh_info()
         return st.as_dict()
 
-    def first(self, eps=1e-8, max_loop=100000, plain=False, graph_iters=16):
-        self.last_opts = self.opts(eps, max_loop, plain, graph_iters)
+    def first(self, eps=1e-8, max_loop=100000, plain=False, graph_iters=16, schur=False):
+        self.last_opts = self.opts(eps, max_loop, plain, graph_iters, schur)
         _check(lib().fwh_first(self.ctx.h, ctypes.byref(self.last_opts)), "fwh_first")
+        self._refresh_info()
+
+    def _refresh_info(self):
+        inf = FwhInfo()
+        _check(lib().fwh_get_info(self.ctx.h, ctypes.byref(inf)), "fwh_get_info")
+        self.info = inf.as_dict()
+
+    def schur_form(self):
+        _check(lib().fwh_schur_form(self.ctx.h), "fwh_schur_form")
+        self._refresh_info()
 
     def iterate(self, n):
         done = ctypes.c_int()
@@ -201,10 +217,28 @@ class Solver:
         _check(lib().fwh_get_resid(self.ctx.h, _ptr(out), int(n)), "fwh_get_resid")
         return out
 
-    def schur_apply(self, p, plain=False):
+    def post(self, nu=None):
+        """Per-element (rotE, E, Je) of the last solution, each (nel, 3)
+        complex (HDDM_post_em_E, main/fullwave/fullwave.c:640-763); nu: per
+        element 1/mu or None for 7.957747e5 everywhere."""
+        nel = int(self.info["n_elem"])
+        out = [np.zeros((nel, 3), np.complex128) for _ in range(3)]
+        nu_p = None
+        if nu is not None:
+            nu = np.ascontiguousarray(nu, np.float64)
+            if nu.shape != (nel,):
+                raise FwhError(f"nu must have shape ({nel},)")
+            nu_p = _ptr(nu)
+        _check(lib().fwh_post(self.ctx.h, nu_p, *(_ptr(o) for o in out)), "fwh_post")
+        return tuple(out)
+
+    def schur_apply(self, p, plain=False, explicit=False):
         p = np.ascontiguousarray(p, np.complex128)
         q = np.zeros_like(p)
-        _check(lib().fwh_schur_apply(self.ctx.h, _ptr(p), _ptr(q), int(bool(plain))), "fwh_schur_apply")
+        mode = int(bool(plain)) + 2 * int(bool(explicit))
+        _check(lib().fwh_schur_apply(self.ctx.h, _ptr(p), _ptr(q), mode), "fwh_schur_apply")
+        if explicit:
+            self._refresh_info()
         return q
 
     def ldl_solve_all(self, rhs, plain=False):
@@ -213,10 +247,12 @@ class Solver:
         return v
 
     def time_kernels(self, reps=10):
-        a, b = ctypes.c_double(), ctypes.c_double()
-        _check(lib().fwh_time_kernels(self.ctx.h, int(reps), ctypes.byref(a), ctypes.byref(b)),
+        """(ms per batched LDL^T solve, per implicit Schur step, per explicit
+        Schur matvec or -1 when S_s are not formed)."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().fwh_time_kernels(self.ctx.h, int(reps), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
                "fwh_time_kernels")
-        return a.value, b.value
+        return a.value, b.value, c.value
 
     def sub(self, s):
         L = lib()

@@ -6,7 +6,12 @@ one-subdomain-per-launch GPU form and the CPU oracle.
 
 A step is one complete interface solve of the workload (first step with the
 load vector, COCG iterations to 1e-8, last step), with the model already
-assembled and factored in HBM.  Prints one JSON line.
+assembled and factored in HBM.  The headline form (--schur explicit) applies
+the interface operator through subdomain Schur complements formed once from
+the factors (their formation time is reported under setup_ms); the implicit
+form (Dirichlet LDL^T solves in every iteration, the reference's own
+structure) is timed beside it on the same model, batched and plain.  Prints
+one JSON line.
 """
 from __future__ import annotations
 
@@ -39,7 +44,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-loop", type=int, default=200000)
-    ap.add_argument("--pml", default="stretch")
+    ap.add_argument("--pml", default="sigma")
+    ap.add_argument("--schur", choices=("explicit", "implicit"), default="explicit")
+    ap.add_argument("--implicit-steps", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
@@ -58,23 +65,37 @@ def main():
     log(f"[bench] loaded: assemble {info['ms_assemble']:.1f} ms, factor {info['ms_factor']:.1f} ms, "
         f"sum nlen {info['nlen']} ({time.time() - t0:.1f}s)")
 
-    # warmup + timed batched solves
-    for _ in range(args.warmup):
-        st = sol.solve(eps=args.eps, max_loop=args.max_loop, graph_iters=args.graph_iters)
-        log(f"[bench] warmup: {st['iters']} iterations, converged={st['converged']}")
-    iters = 0
-    solves = 0
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        st = sol.solve(eps=args.eps, max_loop=args.max_loop, graph_iters=args.graph_iters)
-        iters += st["iters"]
-        solves += st["sub_solves"]
-        log(f"[bench] step {k}: {st['iters']} iterations, resid {st['final_resid']:.3e}")
-    dt = time.perf_counter() - t_start
-    iters_per_s = iters / dt
-    solves_per_s = solves / dt
+    expl = args.schur == "explicit"
 
-    # plain form on a bounded window of iterations
+    def run(schur, steps, warmup, tag):
+        for _ in range(warmup):
+            st = sol.solve(eps=args.eps, max_loop=args.max_loop, graph_iters=args.graph_iters, schur=schur)
+            log(f"[bench] {tag} warmup: {st['iters']} iterations, converged={st['converged']}")
+        iters = solves = 0
+        t_start = time.perf_counter()
+        for k in range(steps):
+            st = sol.solve(eps=args.eps, max_loop=args.max_loop, graph_iters=args.graph_iters, schur=schur)
+            iters += st["iters"]
+            solves += st["sub_solves"]
+            log(f"[bench] {tag} step {k}: {st['iters']} iterations, resid {st['final_resid']:.3e}")
+        return iters, solves, time.perf_counter() - t_start
+
+    # warmup + timed batched solves of the headline form
+    iters, solves, dt = run(expl, args.steps, args.warmup, args.schur)
+    iters_per_s = iters / dt
+    info = dict(sol.info)
+
+    # the other batched form, same model
+    other = None
+    if args.implicit_steps > 0:
+        it2, sv2, dt2 = run(not expl, args.implicit_steps, 0, "implicit" if expl else "explicit")
+        other = {"iters_per_s": round(it2 / dt2, 2), "ldl_solves_per_s": round(sv2 / dt2, 1),
+                 "iters_per_solve": it2 // args.implicit_steps, "ms_per_step": round(1000 * dt2 / args.implicit_steps, 2)}
+        info = dict(sol.info)
+    imp = other if expl else {"iters_per_s": round(iters_per_s, 2), "ldl_solves_per_s": round(solves / dt, 1)}
+
+    # plain form (one subdomain per launch, implicit, the reference's loop) on
+    # a bounded window of iterations
     sol.first(eps=1e-30, max_loop=args.plain_iters + 1, plain=True, graph_iters=0)
     tp = time.perf_counter()
     sol.iterate(args.plain_iters)
@@ -82,16 +103,25 @@ def main():
     plain_iters_per_s = args.plain_iters / dtp
     log(f"[bench] plain: {plain_iters_per_s:.2f} iters/s")
 
-    # dominant kernel: the batched subdomain Schur/LDL^T kernel (HIP events)
-    ms_solve, ms_schur = sol.time_kernels(reps=20)
+    # dominant kernels (HIP events on the library stream)
+    ms_solve, ms_schur, ms_smv = sol.time_kernels(reps=20)
     bytes_iter_kernel = info["bytes_iter"] - 128.0 * info["n_iface"]
-    achieved = bytes_iter_kernel / (ms_schur * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_solve (MODE_ITER, all subdomains)", "ms_per_launch": round(ms_schur, 4),
-                "bytes_per_launch": bytes_iter_kernel,
-                "ldl_only": {"ms": round(ms_solve, 4), "bytes": info["bytes_solve"],
-                             "GBs": round(info["bytes_solve"] / (ms_solve * 1e-3) / 1e9, 1)}}
+    k_solve = {"kernel": "k_solve (implicit Schur step, all subdomains)", "ms_per_launch": round(ms_schur, 4),
+               "bytes_per_launch": bytes_iter_kernel,
+               "GBs": round(bytes_iter_kernel / (ms_schur * 1e-3) / 1e9, 1),
+               "ldl_only": {"ms": round(ms_solve, 4), "bytes": info["bytes_solve"],
+                            "GBs": round(info["bytes_solve"] / (ms_solve * 1e-3) / 1e9, 1)}}
+    if expl and ms_smv > 0:
+        achieved = info["bytes_smv"] / (ms_smv * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "k_smv (explicit Schur matvec, all subdomains)", "ms_per_launch": round(ms_smv, 4),
+                    "bytes_per_launch": info["bytes_smv"], "implicit_kernel": k_solve}
+    else:
+        achieved = bytes_iter_kernel / (ms_schur * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, **k_solve}
+        roofline.pop("GBs")
 
     cpu = None
     if not args.no_cpu:
@@ -101,6 +131,9 @@ def main():
                "sample": f"{args.cpu_iters} interface iterations of the same workload "
                          f"({ns} subdomain solves, {tc:.1f}s) with the C oracle, one core",
                "ldl_solves_per_s": round(ns / tc, 1)}
+    setup = {"assemble": round(info["ms_assemble"], 2), "factor": round(info["ms_factor"], 2)}
+    if expl:
+        setup["schur_form"] = round(info["ms_schur_form"], 2)
     out = {
         "metric": METRIC, "value": round(iters_per_s, 2), "unit": "iters/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000 * dt / args.steps, 2),
@@ -108,14 +141,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"cfg{args.cfg}: {model.meta.get('dims')} cell box, {model.n_edges} edge DOFs, "
                                f"{model.nsub} subdomains in {model.npart} parts, PML {model.meta.get('pml_mode')}, "
-                               f"COCG+diag to {args.eps:g}",
+                               f"COCG+diag to {args.eps:g}, Schur step {args.schur}",
                    "n_iface": info["n_iface"], "iters_per_solve": iters // max(args.steps, 1)},
-        "ldl_solves_per_s": round(solves_per_s, 1),
-        "plain": {"iters_per_s": round(plain_iters_per_s, 3), "window_iters": args.plain_iters},
+        "ldl_solves_per_s": imp["ldl_solves_per_s"] if imp else None,
+        "batched_ldl_kernel_solves_per_s": round(info["nsub"] / (ms_solve * 1e-3), 1),
+        "implicit" if expl else "explicit": other,
+        "plain": {"iters_per_s": round(plain_iters_per_s, 3), "window_iters": args.plain_iters,
+                  "form": "implicit, one subdomain per launch"},
         "batched_over_plain": round(iters_per_s / plain_iters_per_s, 2),
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "setup_ms": {"assemble": round(info["ms_assemble"], 2), "factor": round(info["ms_factor"], 2)},
+        "setup_ms": setup,
+        "time_to_solution_ms": round(setup["factor"] + setup.get("schur_form", 0.0) + 1000 * dt / args.steps, 1),
     }
     print(json.dumps(out), flush=True)
     sol.close()

@@ -47,6 +47,11 @@ typedef struct fwh_opts {
                            0: batched (all subdomains per launch, graph)   */
   int    graph_iters;   /* batched: iterations per captured graph (0=no
                            graph, plain launches)                          */
+  int    schur;         /* 0: implicit Schur step, Dirichlet solves through
+                           the factors every iteration (the reference's
+                           form); 1: explicit S_s formed once from the
+                           factors (fwh_schur_form), one streamed matvec
+                           per iteration                                   */
 } fwh_opts;
 
 typedef struct fwh_stats {
@@ -78,6 +83,10 @@ typedef struct fwh_info {
                            subdomain: sum 16 nlen + 40 n                   */
   double bytes_iter;    /* algorithmic bytes of one interface iteration    */
   double flops_factor;  /* complex-FP64 flops of the factorisation         */
+  double ms_schur_form; /* explicit Schur complements: formation wall ms   */
+  double smat_bytes;    /* their storage (16x16 tiles of the lower half)   */
+  double bytes_smv;     /* algorithmic bytes of one explicit Schur step:
+                           sum 8 ng(ng+1) + 36 ng                          */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */
@@ -122,20 +131,38 @@ int  fwh_get_x(const fwh_ctx *ctx, double *x);
 int  fwh_get_u(const fwh_ctx *ctx, double *u);
 int  fwh_get_resid(const fwh_ctx *ctx, double *resid, int nresid);
 
+/* Element post-processing of the last solution: HDDM_post_dom_em_E
+ * (main/fullwave/fullwave.c:607-635) with HDDM_post_em_E (:640-763) for
+ * every element, in model element order.  nu: per element 1/mu of its
+ * material (7.957747e5 when the element's flag is a listed material, 0
+ * otherwise, :670-675), or NULL for 7.957747e5 everywhere.  B, E, J receive
+ * 3 complex values (6 doubles) per element: the reference's el_B_c
+ * = i (nu/omega) rot E, el_E_c = 2 (nu/omega) E(centroid) and
+ * el_Je_c = sigma (nu/omega) E(centroid), with the reference's scalings.  */
+int  fwh_post(fwh_ctx *ctx, const double *nu, double *B, double *E, double *J);
+
 /* ---- single operators (parity tests, benchmarks) ----------------------- */
 /* One Schur-complement application q = S p over all subdomains
  * (renewal_bc_sp + solve_dom_d + COM_res_sp + gather,
  * hddm_solver_sp.h:28-69, hddm_solver.h:559-622, hddm_solver_s.h:404-424).
- * p, q: owned interface values, 2*n_iface doubles, part-major.           */
-int  fwh_schur_apply(fwh_ctx *ctx, const double *p, double *q, int plain);
+ * p, q: owned interface values, 2*n_iface doubles, part-major.
+ * mode: 0 implicit batched, 1 implicit plain (one subdomain per launch),
+ * 2 explicit batched, 3 explicit plain (2, 3 form S_s first if needed).  */
+int  fwh_schur_apply(fwh_ctx *ctx, const double *p, double *q, int mode);
+/* Form the explicit subdomain Schur complements S_s = A_GG - A_GI A_II^-1
+ * A_IG from the factors: the Schur step of solve_dom_d
+ * (lib/hddm/hddm_solver.h:559-622) applied to every interface unit vector.
+ * Idempotent; fwh_first forms them when opts.schur = 1.                   */
+int  fwh_schur_form(fwh_ctx *ctx);
 /* advlas_skyline_ldl_solve (lib/hddm/advlas_skyline.c:134-167) applied to
  * every subdomain at once: rhs/x are 2*n_dof doubles (in place).          */
 int  fwh_ldl_solve_all(fwh_ctx *ctx, double *rhs, int plain);
 /* Device time (HIP events on the library stream) of one batched LDL^T solve
- * of all subdomains and of one batched Schur-step kernel (Dirichlet load,
- * solve, interface rows of A x), averaged over `reps` launches, in ms.    */
+ * of all subdomains, of one batched implicit Schur-step kernel (Dirichlet
+ * load, solve, interface rows of A x) and of one explicit Schur matvec
+ * (-1 if S_s are not formed), averaged over `reps` launches, in ms.       */
 int  fwh_time_kernels(fwh_ctx *ctx, int reps, double *ms_solve,
-                      double *ms_schur);
+                      double *ms_schur, double *ms_smv);
 
 /* Subdomain arrays for inspection.  what: 0 = assembled lower-triangle
  * values (2*nnz_s doubles, CSR order), 1 = CSR row pointer (n_s+1 ints),

@@ -234,6 +234,65 @@ int fwo_element(const double *v, double eps, double sigma, double omega,
 
 /* ------------------------------------------------------------------------ */
 /* row-list sparse matrix, lib/mymt_aij_solver/aij_matrix_func.h (TMP_AIJ)  */
+
+/* HDDM_post_em_E (main/fullwave/fullwave.c:640-763) for one element.
+ * v: 4 vertices x 3; xe12: the solution at the element's 6 edges (re, im);
+ * nu: 7.957747e5 when the element's flag is a listed material, else 0
+ * (:670-675).  xe is scaled by nu/omega first (:721) and the scaled values
+ * feed all three outputs:
+ *   rotE = (2/D) sum_k (i xe_k) ls_k e_k              (:716-729)
+ *   E    = (2/D) sum_k xe_k ls_k F_k(centroid)        (:733-748)
+ *   Je   = (sigma/D) sum_k xe_k ls_k F_k(centroid)    (:750-762)
+ * with the geometry of tetNedelec_mk/_XYZ/_km_mk/_Fxyz
+ * (lib/fem/tetrahedral.c:267-340).  Outputs 3 complex each.              */
+int fwo_post_element(const double *v, const double *xe12, double nu,
+                     double sigma, double omega, double *rot6, double *e6,
+                     double *j6) {
+  double r1[3], r2[3], r3[3];
+  for (int a = 0; a < 3; a++) {
+    r1[a] = v[3 + a] - v[a];
+    r2[a] = v[6 + a] - v[a];
+    r3[a] = v[9 + a] - v[a];
+  }
+  const double D = r1[0] * (r2[1] * r3[2] - r2[2] * r3[1]) +
+                   r1[1] * (r2[2] * r3[0] - r2[0] * r3[2]) +
+                   r1[2] * (r2[0] * r3[1] - r2[1] * r3[0]);
+  if (!(D > 0.0)) return -1;
+  double g[3];
+  for (int a = 0; a < 3; a++) g[a] = (v[a] + v[3 + a] + v[6 + a] + v[9 + a]) * 0.25;
+  const fwo_c *xe = (const fwo_c *)xe12;
+  fwo_c R[3] = {C0, C0, C0}, E[3] = {C0, C0, C0};
+  const double s = nu / omega;
+  for (int k = 0; k < 6; k++) {
+    const double *va = v + 3 * EDGE[k][0], *vb = v + 3 * EDGE[k][1];
+    const double *vc = v + 3 * EDGE[k][2], *vd = v + 3 * EDGE[k][3];
+    double dl[3] = {vb[0] - va[0], vb[1] - va[1], vb[2] - va[2]};
+    double l = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+    int si = 0;
+    for (int a = 0; a < 3 && si == 0; a++) si = dl[a] > 0.0 ? 1 : (dl[a] < 0.0 ? -1 : 0);
+    if (si == 0) return -2;
+    const double ls = l * (double)si;
+    double e[3], F[3];
+    for (int a = 0; a < 3; a++) e[a] = vd[a] - vc[a];
+    F[0] = (vc[1] * vd[2] - vd[1] * vc[2]) - e[2] * g[1] + e[1] * g[2];
+    F[1] = (vc[2] * vd[0] - vd[2] * vc[0]) - e[0] * g[2] + e[2] * g[0];
+    F[2] = (vc[0] * vd[1] - vd[0] * vc[1]) - e[1] * g[0] + e[0] * g[1];
+    const fwo_c x = cscale(xe[k], s);
+    const fwo_c ix = cmul(x, (fwo_c){0.0, 1.0});
+    for (int a = 0; a < 3; a++) {
+      R[a] = cadd(R[a], cscale(ix, ls * e[a]));
+      E[a] = cadd(E[a], cscale(x, ls * F[a]));
+    }
+  }
+  fwo_c *ro = (fwo_c *)rot6, *eo = (fwo_c *)e6, *jo = (fwo_c *)j6;
+  for (int a = 0; a < 3; a++) {
+    ro[a] = cscale(R[a], 2.0 / D);
+    eo[a] = cscale(E[a], 2.0 / D);
+    jo[a] = cscale(E[a], sigma / D);
+  }
+  return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 typedef struct {
   int nf;

@@ -35,6 +35,16 @@ int fwo_element(const double *xyz12, double eps, double sigma, double omega,
                 const double *pml, int src, const double *joe24,
                 double *ae72, double *be12);
 
+/* ---- element post-processing ------------------------------------------- */
+/* HDDM_post_em_E (main/fullwave/fullwave.c:640-763) for one element:
+ * xe12 = solution at the 6 edges (re,im), nu = 1/mu of the element's
+ * material (0 when its flag is not a listed material).  rot6, e6, j6 get
+ * 3 complex each: i (nu/omega) rot E, 2 (nu/omega) E(g), sigma (nu/omega)
+ * E(g) in the reference's scalings.  Returns <0 for a degenerate element. */
+int fwo_post_element(const double *xyz12, const double *xe12, double nu,
+                     double sigma, double omega, double *rot6, double *e6,
+                     double *j6);
+
 /* ---- skyline LDL^T (a4-a6) on a dense-described profile --------------- */
 /* advlas_skyline_ldl_decomposite (lib/hddm/advlas_skyline.c:79-128):
  * mat (2*nlen doubles) holds rows i = 0..n-1, row i columns idx1[i]..i at

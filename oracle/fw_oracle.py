@@ -42,6 +42,7 @@ def lib():
         L.fwo_element.restype = ctypes.c_int
         L.fwo_element.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, P,
                                   ctypes.c_int, P, P, P]
+        L.fwo_post_element.argtypes = [P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P]
         L.fwo_sky_factor.argtypes = [ctypes.c_int, P, P, P]
         L.fwo_sky_solve.argtypes = [ctypes.c_int, P, P, P, P]
         L.fwo_model_build.restype = P
@@ -77,6 +78,35 @@ def element(xyz, eps, sigma, omega, pml=None, src=0, joe=None):
     if rc != 0:
         raise ValueError(f"fwo_element failed ({rc})")
     return ae.view(np.complex128).reshape(6, 6), be.view(np.complex128)
+
+
+def post_element(xyz, xe, nu, sigma, omega):
+    """(rotE, E, Je) of one element, 3 complex each (HDDM_post_em_E)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(12)
+    x = np.ascontiguousarray(xe, dtype=np.complex128).reshape(6)
+    out = [np.zeros(3, np.complex128) for _ in range(3)]
+    rc = lib().fwo_post_element(_p(xyz), _p(x), float(nu), float(sigma), float(omega), *(_p(o) for o in out))
+    if rc != 0:
+        raise ValueError(f"fwo_post_element failed ({rc})")
+    return tuple(out)
+
+
+def post_model(model, x_flat, nu=None):
+    """post_element over every element of a decomposed model; x_flat is the
+    per-subdomain solution concatenated (2*nmp doubles each)."""
+    nop = np.asarray(model.nop).reshape(-1, 10)
+    nel = nop.shape[0]
+    dof0 = np.concatenate([[0], np.cumsum(np.asarray(model.nmp, np.int64))])
+    out = [np.zeros((nel, 3), np.complex128) for _ in range(3)]
+    crd = np.asarray(model.crd).reshape(-1, 3)
+    for s in range(model.nsub):
+        for el in range(model.el_off[s], model.el_off[s + 1]):
+            v = crd[model.ap_off[s] + nop[el, :4]]
+            xe = x_flat[dof0[s] + nop[el, 4:]]
+            r = post_element(v, xe, 7.957747e+05 if nu is None else nu[el], model.sigma[el], model.omega)
+            for o, val in zip(out, r):
+                o[el] = val
+    return tuple(out)
 
 
 def sky_factor(idx1, idx2, mat):

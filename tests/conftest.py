@@ -23,3 +23,10 @@ def small_model():
 def cfg1_model():
     from adventure_fullwave_amd import mesh
     return mesh.make_config(1)
+
+
+@pytest.fixture(scope="session")
+def cfg1_sigma_model():
+    """cfg 1 with the reference-mode absorber (graded conductor layers)."""
+    from adventure_fullwave_amd import mesh
+    return mesh.make_config(1, pml_mode="sigma")

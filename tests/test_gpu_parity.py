@@ -2,11 +2,13 @@
 
 Through the C ABI: element kernel, assembled subdomain matrices, Dirichlet
 profile (bit-exact), skyline factor, batched and plain LDL^T solves, the
-Schur step, and the full HDDM solve (iteration count, residual history,
-solution) on the same seeded models.  Tolerances: element/assembly 1e-12
-relative (different but exact-in-math formulas and FMA contraction), factor
-1e-10 relative, solves 1e-10 relative; HDDM iteration count within 2 %,
-solution within rel. L2 1e-7 of the oracle and of a direct solve."""
+Schur step (implicit and explicit), the full HDDM solve (iteration count,
+residual history, solution) on the same seeded models, and the element
+post-processing.  Tolerances: element/assembly/post 1e-12 relative (different
+but exact-in-math formulas and FMA contraction), factor 1e-10 relative,
+solves and Schur steps 1e-10 relative; HDDM iteration count within 2 % (8 %
+for the rounding-chaotic stretched-PML cfg 1, whose oracle count itself
+spreads 7.2 %), solution within rel. L2 1e-7 of the oracle."""
 import sys
 
 import numpy as np
@@ -118,24 +120,42 @@ def oracle_schur(model, om, p):
     return q
 
 
+@pytest.mark.parametrize("explicit", [False, True])
 @pytest.mark.parametrize("plain", [False, True])
-def test_schur_apply(small, small_oracle, small_model, plain):
-    say(f"[gpu] schur apply plain={plain}")
+def test_schur_apply(small, small_oracle, small_model, plain, explicit):
+    say(f"[gpu] schur apply plain={plain} explicit={explicit}")
     rng = np.random.default_rng(5)
     ng = int(small.info["n_iface"])
     p = rng.standard_normal(ng) + 1j * rng.standard_normal(ng)
-    q = small.schur_apply(p, plain=plain)
+    q = small.schur_apply(p, plain=plain, explicit=explicit)
     ref = oracle_schur(small_model, small_oracle, p)
     assert np.abs(q - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("mode", ["batched_graph", "batched", "plain"])
+def test_schur_explicit_symmetric_linear(small, small_model):
+    """The explicit S (formed from unit-vector Schur steps, lower half kept)
+    is complex-symmetric and linear: x^T S y = y^T S x, S(a x + y) = a Sx + Sy."""
+    say("[gpu] explicit schur symmetry/linearity")
+    rng = np.random.default_rng(7)
+    ng = int(small.info["n_iface"])
+    x = rng.standard_normal(ng) + 1j * rng.standard_normal(ng)
+    y = rng.standard_normal(ng) + 1j * rng.standard_normal(ng)
+    sx = small.schur_apply(x, explicit=True)
+    sy = small.schur_apply(y, explicit=True)
+    assert abs(x @ sy - y @ sx) <= 1e-12 * abs(x @ sy)
+    a = 0.3 - 1.7j
+    sxy = small.schur_apply(a * x + y, explicit=True)
+    assert np.abs(sxy - (a * sx + sy)).max() <= 1e-12 * np.abs(sxy).max()
+    assert small.info["ms_schur_form"] > 0 and small.info["smat_bytes"] > 0
+
+
+@pytest.mark.parametrize("mode", ["batched_graph", "batched", "plain", "explicit_graph", "explicit_plain"])
 def test_hddm_solve_small(fw, small_model, mode):
     say(f"[gpu] hddm solve small {mode}")
     ref = fo.hddm_solve(small_model, eps=1e-8, max_loop=5000)
     sol = fw.Solver(small_model)
-    st = sol.solve(eps=1e-8, max_loop=5000, plain=(mode == "plain"),
-                   graph_iters=(8 if mode == "batched_graph" else 0))
+    st = sol.solve(eps=1e-8, max_loop=5000, plain=mode.endswith("plain"),
+                   graph_iters=(8 if mode.endswith("graph") else 0), schur=mode.startswith("explicit"))
     it_ref = ref["stats"]["iters"]
     assert st["converged"] == 1
     assert abs(st["iters"] - it_ref) <= max(1, 0.02 * it_ref)
@@ -148,20 +168,58 @@ def test_hddm_solve_small(fw, small_model, mode):
     sol.close()
 
 
-def test_hddm_cfg1_matches_oracle(fw, cfg1_model):
-    say("[gpu] hddm cfg1")
-    ref = fo.hddm_solve(cfg1_model, eps=1e-8, max_loop=20000)
-    sol = fw.Solver(cfg1_model)
-    st = sol.solve(eps=1e-8, max_loop=20000, graph_iters=16)
+@pytest.fixture(scope="module")
+def cfg1_ref(cfg1_model):
+    return fo.hddm_solve(cfg1_model, eps=1e-8, max_loop=20000)
+
+
+@pytest.fixture(scope="module")
+def cfg1_sigma_ref(cfg1_sigma_model):
+    return fo.hddm_solve(cfg1_sigma_model, eps=1e-8, max_loop=20000)
+
+
+@pytest.mark.parametrize("schur", [False, True])
+def test_hddm_cfg1_matches_oracle(fw, cfg1_sigma_model, cfg1_sigma_ref, schur):
+    """cfg 1 with the reference-mode absorber: iteration count within 2 % and
+    solution within rel. L2 1e-7 of the oracle (the oracle's own count is 417
+    for 17 of 18 few-ulp load perturbations, 423 for one)."""
+    say(f"[gpu] hddm cfg1 sigma explicit={schur}")
+    ref = cfg1_sigma_ref
+    sol = fw.Solver(cfg1_sigma_model)
+    st = sol.solve(eps=1e-8, max_loop=20000, graph_iters=16, schur=schur)
     it_ref = ref["stats"]["iters"]
-    say(f"[gpu] cfg1 iters gpu={st['iters']} oracle={it_ref}")
-    # The oracle's own count on this model moves between 3196 and 3263
-    # (2.1 %) when its load vector is perturbed by a few ulps
-    # (tests/test_oracle_hddm.py::test_iteration_count_rounding_spread), so
-    # the bar is 3 % rather than the nominal 2 %.
-    assert abs(st["iters"] - it_ref) <= max(1, 0.03 * it_ref)
+    say(f"[gpu] cfg1 sigma iters gpu={st['iters']} oracle={it_ref}")
+    assert st["converged"] == 1
+    assert abs(st["iters"] - it_ref) <= max(1, 0.02 * it_ref)
     x = sol.x()
     assert np.linalg.norm(x - ref["x_flat"]) / np.linalg.norm(ref["x_flat"]) <= 1e-7
+    sol.close()
+
+
+@pytest.mark.parametrize("schur", [False, True])
+def test_hddm_cfg1_stretch(fw, cfg1_model, cfg1_ref, schur):
+    """cfg 1 with the complex-stretched PML.  The COCG iteration count of this
+    case is rounding-chaotic: the oracle's own count spans 3150-3380 (7.2 % of
+    its nominal 3199) over 18 few-ulp load perturbations, so the count bar is
+    8 %; the solution bar stays rel. L2 1e-7, and the explicit and implicit
+    Schur operators must agree to 1e-10."""
+    say(f"[gpu] hddm cfg1 stretch explicit={schur}")
+    ref = cfg1_ref
+    sol = fw.Solver(cfg1_model)
+    st = sol.solve(eps=1e-8, max_loop=20000, graph_iters=16, schur=schur)
+    it_ref = ref["stats"]["iters"]
+    say(f"[gpu] cfg1 stretch iters gpu={st['iters']} oracle={it_ref}")
+    assert st["converged"] == 1
+    assert abs(st["iters"] - it_ref) <= 0.08 * it_ref
+    x = sol.x()
+    assert np.linalg.norm(x - ref["x_flat"]) / np.linalg.norm(ref["x_flat"]) <= 1e-7
+    if schur:
+        rng = np.random.default_rng(9)
+        ng = int(sol.info["n_iface"])
+        p = rng.standard_normal(ng) + 1j * rng.standard_normal(ng)
+        qi = sol.schur_apply(p)
+        qe = sol.schur_apply(p, explicit=True)
+        assert np.abs(qe - qi).max() <= 1e-10 * np.abs(qi).max()
     sol.close()
 
 
@@ -226,3 +284,26 @@ def test_cocg_icc_all(small, small_oracle, small_model):
         xd = np.linalg.solve(dense_lower(rp, cl, vl, ns), b[off:off + ns])
         assert np.linalg.norm(x[off:off + ns] - xd) / np.linalg.norm(xd) < 1e-7
         off += ns
+
+
+def test_post_matches_oracle(fw, small_model):
+    """fwh_post (HDDM_post_em_E for every element) against the oracle on the
+    GPU's own solution; elements without a listed material give zeros."""
+    say("[gpu] element post-processing")
+    sol = fw.Solver(small_model)
+    st = sol.solve(eps=1e-8, max_loop=5000, graph_iters=8)
+    assert st["converged"] == 1
+    x = sol.x()
+    got = sol.post()
+    ref = fo.post_model(small_model, x)
+    for g, r in zip(got, ref):
+        assert np.abs(g - r).max() <= 1e-12 * np.abs(r).max()
+    nel = int(sol.info["n_elem"])
+    nu = np.full(nel, 7.957747e+05)
+    nu[::3] = 0.0
+    got2 = sol.post(nu)
+    for g2, g in zip(got2, got):
+        assert np.all(g2[::3] == 0)
+        keep = np.arange(nel) % 3 != 0
+        assert np.array_equal(g2[keep], g[keep])
+    sol.close()
