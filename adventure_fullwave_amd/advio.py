@@ -259,7 +259,7 @@ def build_full_wave_model(parts: List[dict], mtrl: dict, antenna_pp: Dict[int, L
     joe_g = np.stack([Jc.real, Jc.imag], axis=2).reshape(nglob, 6)
 
     nop_all, ap_off, crd_all, joe_all, nmp = [], [0], [], [], []
-    bc_off, ib_off, ib_nfid_all, ib_ln_all, ledge = [0], [0], [], [], []
+    bc_off, ib_off, ib_nfid_all, ib_ln_all, ledge, mp2sn = [0], [0], [], [], [], []
     eps_all, sig_all, src_all, el_off = [], [], [], [0]
     for (p, d, gn, fl, s2g) in subs:
         vert = np.zeros(len(s2g), bool)
@@ -279,6 +279,7 @@ def build_full_wave_model(parts: List[dict], mtrl: dict, antenna_pp: Dict[int, L
         mid_g = s2g[~vert]
         nmp.append(len(mid_g))
         ledge.append(mid_g)
+        mp2sn.append(np.nonzero(~vert)[0])
         bc_off.append(bc_off[-1])
         ifc = np.nonzero(is_iface[mid_g])[0]
         ib_nfid_all.append(ifc)
@@ -301,7 +302,7 @@ def build_full_wave_model(parts: List[dict], mtrl: dict, antenna_pp: Dict[int, L
         infree=infree.astype(np.int32), outfree=outfree.astype(np.int32),
         op_off=np.array(op_off, np.int32), op_ibid=cat(op_ibid).astype(np.int32),
         ledge_global=cat(ledge).astype(np.int64), n_edges=int(is_iface.size), n_tets=int(el_off[-1]),
-        name=name)
+        mp2sn=cat(mp2sn).astype(np.int32), name=name)
 
 
 def load_npz_model(path: str) -> mesh.Model:
@@ -310,6 +311,8 @@ def load_npz_model(path: str) -> mesh.Model:
     kw = {k: z[k] for k in ("sub_part", "el_off", "nop", "eps", "sigma", "src", "ap_off", "crd", "joe", "nmp",
                             "bc_off", "ibc", "ib_off", "ib_nfid", "ib_ln", "infree", "outfree", "op_off",
                             "op_ibid", "ledge_global")}
+    if "mp2sn" in z.files:
+        kw["mp2sn"] = z["mp2sn"]
     return mesh.Model(npart=int(z["npart"]), nsub=int(z["nsub"]), omega=float(z["omega"]), pml=None,
                       n_edges=int(z["n_edges"]), n_tets=int(z["n_tets"]), name=os.path.basename(path), **kw)
 
@@ -325,3 +328,122 @@ def load_reference_model(model_dir: str) -> mesh.Model:
     for flag, (mode, fname) in mtrl["antenna"].items():
         pp[flag] = read_parallelpiped(os.path.join(model_dir, fname))
     return build_full_wave_model(parts, mtrl, pp)
+
+
+# ---------------------------------------------------------------------------
+# result files (write_outdata_file_HDDM_FW, lib/std/fileio.c:2930-3224)
+# ---------------------------------------------------------------------------
+RESULT_UNITS = {"MagneticField": "T", "ElectricField": "T", "CurrentDensity": "A/m^2"}
+
+
+def _doc_bytes(doc_id: str, props: List[tuple], payload: bytes) -> bytes:
+    """One AdvDoc document; props in the order given."""
+    pb = [struct.pack("<I", len(doc_id)), doc_id.encode(), struct.pack("<I", len(props))]
+    for k, v in props:
+        kb, vb = k.encode(), str(v).encode()
+        pb += [struct.pack("<I", len(kb)), kb, struct.pack("<I", len(vb)), vb]
+    pbytes = b"".join(pb)
+    return b"AdvDoc\0" + struct.pack("<Q", len(payload)) + payload + struct.pack("<Q", len(pbytes)) + pbytes
+
+
+def result_documents(model, x, B, E, J, part: int, id_prefix: str = "fwhip"):
+    """The documents of part `part`'s result file, as (props, payload) pairs
+    in the reference's order: MagneticVectorPotential (per subdomain nmp,
+    then (mpid2snid, re, im) per mid-node), then the real parts of
+    MagneticField, ElectricField, CurrentDensity, then their imaginary parts
+    (per subdomain nel, then 3 values per element; fileio.c:2962-3219).
+    x: per-subdomain solutions concatenated; B, E, J: (nel, 3) complex."""
+    if model.mp2sn is None:
+        raise ValueError("the model has no mid-node -> subdomain node map (mp2sn)")
+    subs = np.nonzero(np.asarray(model.sub_part) == part)[0]
+    dof0 = np.concatenate([[0], np.cumsum(np.asarray(model.nmp, np.int64))])
+    rec = np.dtype([("id", "<i4"), ("re", "<f8"), ("im", "<f8")])
+    out = []
+    chunks = []
+    for s in subs:
+        n = int(model.nmp[s])
+        r = np.empty(n, rec)
+        r["id"] = model.mp2sn[dof0[s]:dof0[s + 1]]
+        xs = np.asarray(x[dof0[s]:dof0[s + 1]])
+        r["re"], r["im"] = xs.real, xs.imag
+        chunks += [struct.pack("<i", n), r.tobytes()]
+    out.append(([("index_byte", 4), ("label", "MagneticVectorPotential"), ("format", "f8f8"),
+                 ("fega_type", "NodeVariable"), ("num_subdomains", len(subs)),
+                 ("content_type", "HDDM_FEGenericAttribute")], b"".join(chunks)))
+    nel_all = int(model.el_off[-1])
+    for comp, tag in ((np.real, "Real"), (np.imag, "Imaginary")):
+        for name, F in (("MagneticField", B), ("ElectricField", E), ("CurrentDensity", J)):
+            chunks = []
+            for s in subs:
+                e0, e1 = int(model.el_off[s]), int(model.el_off[s + 1])
+                chunks += [struct.pack("<i", e1 - e0), np.ascontiguousarray(comp(F[e0:e1]), "<f8").tobytes()]
+            out.append(([("index_byte", 4), ("unit", RESULT_UNITS[name]), ("label", name + tag),
+                         ("format", "f8f8f8"), ("fega_type", "AllElementVariable"),
+                         ("num_items_orig", nel_all), ("num_subdomains", len(subs)),
+                         ("content_type", "HDDM_FEGenericAttribute")], b"".join(chunks)))
+    return out
+
+
+def write_result(out_dir: str, model, x, B, E, J, id_prefix: str = "fwhip"):
+    """advhddm_out_P.adv for every part plus the options file advhddm_out.adv
+    (write_outdata_file_HDDM_FW and write_outdata_file_options_FW,
+    fileio.c:2930-3224).  Document ids are this writer's own (the reference
+    draws unique ids from ADVENTURE_IO)."""
+    os.makedirs(out_dir, exist_ok=True)
+    paths = []
+    opts = [("Work", "HDDM_ANALYSIS"), ("Formulation", "EMETHOD"), ("Analysis", "Mag_Full_Wavecurrent"),
+            ("index_byte", 4), ("label", "Options"), ("fega_type", "AllElementConstant"), ("format", ""),
+            ("num_items", 0), ("content_type", "FEGenericAttribute")]
+    p = os.path.join(out_dir, "advhddm_out.adv")
+    with open(p, "wb") as f:
+        f.write(_doc_bytes(f"{id_prefix}-options", opts, b""))
+    paths.append(p)
+    for part in range(model.npart):
+        p = os.path.join(out_dir, f"advhddm_out_{part}.adv")
+        with open(p, "wb") as f:
+            for k, (props, payload) in enumerate(result_documents(model, x, B, E, J, part)):
+                f.write(_doc_bytes(f"{id_prefix}-{part}-{k}", props, payload))
+        paths.append(p)
+    return paths
+
+
+def read_result(path: str) -> Dict[str, list]:
+    """Decode a part result file: {'x': [complex per subdomain], 'MagneticField':
+    [(nel,3) complex per subdomain], 'ElectricField': ..., 'CurrentDensity': ...,
+    'ids': [mpid2snid per subdomain], 'props': {label: props}}."""
+    docs = {d["props"]["label"]: d for d in read_adv(path)}
+    out = {"props": {k: d["props"] for k, d in docs.items()}}
+    d = docs["MagneticVectorPotential"]["data"]
+    rec = np.dtype([("id", "<i4"), ("re", "<f8"), ("im", "<f8")])
+    off, xs, ids = 0, [], []
+    for _ in range(int(docs["MagneticVectorPotential"]["props"]["num_subdomains"])):
+        (n,) = struct.unpack_from("<i", d, off)
+        off += 4
+        r = np.frombuffer(d, rec, count=n, offset=off)
+        off += rec.itemsize * n
+        xs.append(_cplx(r["re"], r["im"]))
+        ids.append(r["id"].copy())
+    out["x"], out["ids"] = xs, ids
+    for name in ("MagneticField", "ElectricField", "CurrentDensity"):
+        parts = []
+        for tag in ("Real", "Imaginary"):
+            d = docs[name + tag]["data"]
+            off, vals = 0, []
+            for _ in range(int(docs[name + tag]["props"]["num_subdomains"])):
+                (n,) = struct.unpack_from("<i", d, off)
+                off += 4
+                vals.append(np.frombuffer(d, "<f8", count=3 * n, offset=off).reshape(n, 3))
+                off += 24 * n
+            parts.append(vals)
+        out[name] = [_cplx(re, im) for re, im in zip(*parts)]
+    return out
+
+
+def _cplx(re, im):
+    """complex array from its parts, keeping signed zeros (re + 1j*im would
+    turn -0.0 into +0.0)"""
+    out = np.empty(np.shape(re), np.complex128)
+    out.real = re
+    out.imag = im
+    return out
+

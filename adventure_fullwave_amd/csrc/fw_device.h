@@ -22,6 +22,11 @@ __device__ __forceinline__ double2 cfma(double2 acc, double2 a, double2 b) {
 __device__ __forceinline__ double2 cfms(double2 acc, double2 a, double2 b) {
   return make_double2(acc.x - (a.x * b.x - a.y * b.y), acc.y - (a.y * b.x + a.x * b.y));
 }
+// z + b*p with explicit fused multiply-adds, so that every kernel that forms
+// the next search direction obtains the same bits
+__device__ __forceinline__ double2 zbp(double2 z, double2 b, double2 p) {
+  return make_double2(__fma_rn(b.x, p.x, __fma_rn(-b.y, p.y, z.x)), __fma_rn(b.x, p.y, __fma_rn(b.y, p.x, z.y)));
+}
 __device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
 __device__ __forceinline__ double cabsv(double2 a) { return sqrt(a.x * a.x + a.y * a.y); }
 __device__ __forceinline__ bool ciszero(double2 a) { return a.x == 0.0 && a.y == 0.0; }

@@ -63,7 +63,7 @@ struct fwh_ctx {
   int device = 0;
   hipStream_t st = nullptr;
   std::vector<void *> allocs;
-  bool loaded = false, first_done = false, last_done = false;
+  bool loaded = false, first_done = false, last_done = false, p_final = false;
 
   // sizes
   int nsub = 0, npart = 0, ng = 0, nmax = 0, ngmax = 0, wmax = 0, G = 1, nblkmax = 0;
@@ -148,7 +148,9 @@ struct fwh_ctx {
     for (void *p : allocs) (void)hipFree(p);
     allocs.clear();
   }
-  fw::SolveArgs solve_args(int mode, bool check_done) const {
+  // fused: the interface values are the next search direction z + beta p of
+  // the running interface iteration (formed by the kernels from Z, P, beta)
+  fw::SolveArgs solve_args(int mode, bool check_done, bool fused = false) const {
     fw::SolveArgs a{};
     a.meta = meta;
     a.order = order;
@@ -175,6 +177,8 @@ struct fwh_ctx {
     a.g_gid = g_gid;
     a.blk_row = blk_row;
     a.P = P;
+    a.Z = fused ? Z : nullptr;
+    a.sc = sc;
     a.yslot = yslot;
     a.X = X;
     a.unit_k = 0;
@@ -182,7 +186,7 @@ struct fwh_ctx {
     a.sm_off = sm_off;
     return a;
   }
-  fw::SmvArgs smv_args(bool check_done) const {
+  fw::SmvArgs smv_args(bool check_done, bool fused) const {
     fw::SmvArgs a{};
     a.meta = meta;
     a.order = order;
@@ -193,6 +197,8 @@ struct fwh_ctx {
     a.sm_off = sm_off;
     a.g_gid = g_gid;
     a.P = P;
+    a.Z = fused ? Z : nullptr;
+    a.sc = sc;
     a.yslot = yslot;
     return a;
   }
@@ -217,8 +223,8 @@ struct fwh_ctx {
   }
   // all subdomains: one launch (batched, size-ordered) or one launch per
   // subdomain in the reference's loop order (plain)
-  hipError_t run_solve(int mode, bool plain, bool check_done) {
-    fw::SolveArgs a = solve_args(mode, check_done);
+  hipError_t run_solve(int mode, bool plain, bool check_done, bool fused = false) {
+    fw::SolveArgs a = solve_args(mode, check_done, fused);
     if (!plain) return fw::launch_solve(a, nsub, st);
     a.order = nullptr;
     for (int s = 0; s < nsub; s++) {
@@ -233,9 +239,9 @@ struct fwh_ctx {
   }
   // the Schur step of an interface iteration (y_slot = S_s p_s): implicit
   // (Dirichlet solves through the factors) or explicit (formed S_s)
-  hipError_t run_schur(bool expl, bool plain, bool check_done) {
-    if (!expl) return run_solve(fw::MODE_ITER, plain, check_done);
-    fw::SmvArgs a = smv_args(check_done);
+  hipError_t run_schur(bool expl, bool plain, bool check_done, bool fused) {
+    if (!expl) return run_solve(fw::MODE_ITER, plain, check_done, fused);
+    fw::SmvArgs a = smv_args(check_done, fused);
     if (!plain) return fw::launch_smv(a, nsub, st);
     a.order = nullptr;
     for (int s = 0; s < nsub; s++) {
@@ -818,7 +824,8 @@ int build_schur(fwh_ctx *c) {
     const double g = c->meta_h[s].ng;
     packed += 16.0 * g * (g + 1.0) / 2.0;
   }
-  // packed lower triangle + p gather (value + index) + y out, per slot
+  // packed lower triangle + p gather (value + index) + y out, per slot (the
+  // kernel also reads z to form p = z + beta p: not counted)
   c->info.bytes_smv = packed + 36.0 * (double)c->nslots;
   c->schur_built = true;
   return 0;
@@ -854,6 +861,7 @@ extern "C" int fwh_first(fwh_ctx *c, const fwh_opts *o) {
   c->stats.loops = s.loop;
   c->first_done = true;
   c->last_done = false;
+  c->p_final = false;
   return 0;
 }
 
@@ -870,7 +878,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   if (expl && !c->schur_built) return fail(FWH_E_STATE, "explicit Schur mode requested after fwh_first without it");
   if (o->plain) {
     while (it < n && !s.done) {
-      HIPCHK(c->run_schur(expl, true, true));
+      HIPCHK(c->run_schur(expl, true, true, true));
       HIPCHK(fw::launch_iter(c->vec_args(), c->st));
       if ((rc = read_scal(c, &s))) return rc;
       it++;
@@ -880,7 +888,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
       c->release_graph();
       HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < o->graph_iters; k++) {
-        hipError_t e = c->run_schur(expl, false, true);
+        hipError_t e = c->run_schur(expl, false, true, true);
         if (e == hipSuccess) e = fw::launch_iter(c->vec_args(), c->st);
         if (e != hipSuccess) {
           hipGraph_t g;
@@ -901,7 +909,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
     }
   } else {
     while (it < n && !s.done) {
-      HIPCHK(c->run_schur(expl, false, true));
+      HIPCHK(c->run_schur(expl, false, true, true));
       HIPCHK(fw::launch_iter(c->vec_args(), c->st));
       it++;
       if ((it & 15) == 0 || it >= n)
@@ -925,6 +933,10 @@ extern "C" int fwh_last(fwh_ctx *c) {
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->st));
   const double t0 = now_ms();
+  if (!c->p_final) {
+    HIPCHK(fw::launch_finalize_p(c->vec_args(), c->st));
+    c->p_final = true;
+  }
   HIPCHK(c->run_solve(fw::MODE_LAST, false, false));
   HIPCHK(hipStreamSynchronize(c->st));
   c->stats.ms_last = now_ms() - t0;
@@ -1006,7 +1018,7 @@ extern "C" int fwh_schur_apply(fwh_ctx *c, const double *p, double *q, int mode)
   int rc;
   if (expl && (rc = build_schur(c))) return rc;
   HIPCHK(hipMemcpyAsync(c->P, p, (size_t)c->ng * sizeof(double2), hipMemcpyHostToDevice, c->st));
-  HIPCHK(c->run_schur(expl, (mode & 1) != 0, false));
+  HIPCHK(c->run_schur(expl, (mode & 1) != 0, false, false));
   HIPCHK(fw::launch_gather_q(c->vec_args(), c->st));
   HIPCHK(hipMemcpyAsync(q, c->Q, (size_t)c->ng * sizeof(double2), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -1043,7 +1055,7 @@ extern "C" int fwh_time_kernels(fwh_ctx *c, int reps, double *ms_solve, double *
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   if (ms_solve) *ms_solve = ms / reps;
   HIPCHK(hipEventRecord(e0, c->st));
-  for (int r = 0; r < reps; r++) HIPCHK(c->run_solve(fw::MODE_ITER, false, false));
+  for (int r = 0; r < reps; r++) HIPCHK(c->run_solve(fw::MODE_ITER, false, false, true));
   HIPCHK(hipEventRecord(e1, c->st));
   HIPCHK(hipEventSynchronize(e1));
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
@@ -1051,9 +1063,9 @@ extern "C" int fwh_time_kernels(fwh_ctx *c, int reps, double *ms_solve, double *
   if (ms_smv) {
     *ms_smv = -1.0;
     if (c->schur_built) {
-      HIPCHK(c->run_schur(true, false, false));
+      HIPCHK(c->run_schur(true, false, false, true));
       HIPCHK(hipEventRecord(e0, c->st));
-      for (int r = 0; r < reps; r++) HIPCHK(c->run_schur(true, false, false));
+      for (int r = 0; r < reps; r++) HIPCHK(c->run_schur(true, false, false, true));
       HIPCHK(hipEventRecord(e1, c->st));
       HIPCHK(hipEventSynchronize(e1));
       HIPCHK(hipEventElapsedTime(&ms, e0, e1));

@@ -5,6 +5,8 @@
 
 namespace fw {
 
+struct Scal;
+
 // per-subdomain metadata, one record per subdomain
 struct SubMeta {
   int n;          // edge DOFs
@@ -59,6 +61,8 @@ struct SolveArgs {
   const int *g_gid;      // per slot: global interface index
   const int *blk_row;    // per subdomain nblk+1 block start rows
   const double2 *P;      // global interface values
+  const double2 *Z;      // MODE_ITER: p = Z + beta P (null: p = P)
+  const Scal *sc;        // beta of the interface iteration
   double2 *yslot;        // per slot output
   double2 *X;            // per DOF solution (MODE_LAST, MODE_RHS in/out)
   int unit_k;            // MODE_UNIT: local interface index of the unit vector
@@ -76,6 +80,8 @@ struct SmvArgs {
   const long long *sm_off;
   const int *g_gid;
   const double2 *P;
+  const double2 *Z;      // p = Z + beta P (null: p = P)
+  const Scal *sc;
   double2 *yslot;
 };
 
@@ -118,7 +124,8 @@ struct PostArgs {
 
 // scalars of the interface COCG
 struct Scal {
-  double2 rz[2];   // (r,z) by loop parity
+  double2 rz;      // (r,z) of the current residual
+  double2 beta;    // last beta: the next search direction is z + beta p
   double r0inv;
   double resi;
   double eps;
@@ -168,6 +175,7 @@ hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, con
                          double2 *pcslot, hipStream_t st);
 hipError_t launch_first(const VecArgs &a, hipStream_t st);
 hipError_t launch_iter(const VecArgs &a, hipStream_t st);
+hipError_t launch_finalize_p(const VecArgs &a, hipStream_t st);
 hipError_t launch_gather_q(const VecArgs &a, hipStream_t st);
 hipError_t launch_fill(double2 *p, long long n, double2 v, hipStream_t st);
 hipError_t launch_scal_init(Scal *sc, double eps, int max_loop, hipStream_t st);

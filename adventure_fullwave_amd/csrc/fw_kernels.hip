@@ -567,8 +567,8 @@ __global__ __launch_bounds__(VT) void k_first_b(VecArgs a) {
     Scal *s = a.sc;
     s->done = (r0 < NEARLY_ZERO) ? 1 : 0;
     s->r0inv = 1.0 / r0;
-    s->rz[1] = rz;
-    s->rz[0] = rz;
+    s->rz = rz;
+    s->beta = cz();  // p_1 = z_0 + 0 p, with P = Z already
     s->loop = 1;
     s->iters = 0;
     s->conv = 0;
@@ -584,15 +584,33 @@ hipError_t launch_first(const VecArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 
-// iteration (hddm_solver_s.h:260-400)
+// Interface iteration (hddm_solver_s.h:260-400): two passes over the
+// unknowns and one single-block scalar step.  Every reduction sums the
+// per-block partials in one fixed order, so results do not depend on
+// scheduling; no kernel writes a scalar that another block of the same
+// kernel reads.
+//   k_iter_a: p_k = z_k + beta p_{k-1} (written back to P), q = gather(y),
+//             partials of (p,q)
+//   k_iter_b: alpha from the (p,q) partials (every block, same order,
+//             :286-295), u += alpha p, r -= alpha q, z = PC r, partials of
+//             (r,z) and max|r|
+//   k_iter_c: one block: beta (:340-350), the stop test (:311-318, :370-376)
+//             and the status machine (hddm_solver.h:125-143)
+// The Schur step of the next iteration forms p = z + beta p itself (zbp,
+// same fused operations as k_iter_a); the final P (U when converged,
+// hddm_solver_s.h:370-376) is set by k_finalize_p before the last solve.
 __global__ __launch_bounds__(VT) void k_iter_a(VecArgs a) {
-  if (a.sc->done) return;
+  const Scal *s = a.sc;
+  if (s->done) return;
   __shared__ double2 shc[VT / 64 + 1];
+  const double2 beta = s->beta;
   double2 pq = cz();
   for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT) {
     const double2 q = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
+    const double2 p = zbp(a.Z[g], beta, a.P[g]);
     a.Q[g] = q;
-    pq = cadd(pq, cmul(a.P[g], q));
+    a.P[g] = p;
+    pq = cadd(pq, cmul(p, q));
   }
   pq = block_csum(pq, shc);
   if (threadIdx.x == 0) a.part[blockIdx.x] = pq;
@@ -603,7 +621,7 @@ __global__ __launch_bounds__(VT) void k_iter_b(VecArgs a) {
   __shared__ double2 shc[VT / 64 + 1];
   __shared__ double shm[VT / 64 + 1];
   const double2 pq = reduce_parts(a.part, a.G, shc);
-  const double2 rz = s->rz[s->loop & 1];
+  const double2 rz = s->rz;
   double2 malpha = cz();
   if (!(cabsv(rz) < NEARLY_ZERO && cabsv(pq) < NEARLY_ZERO)) malpha = csub(cz(), cdiv_naive(rz, pq));
   double2 rzn = cz();
@@ -626,41 +644,48 @@ __global__ __launch_bounds__(VT) void k_iter_b(VecArgs a) {
   }
 }
 __global__ __launch_bounds__(VT) void k_iter_c(VecArgs a) {
-  const Scal *s = a.sc;
+  Scal *s = a.sc;
   if (s->done) return;
   __shared__ double2 shc[VT / 64 + 1];
   __shared__ double shm[VT / 64 + 1];
-  const double2 rzn = reduce_parts(a.part + a.G, a.G, shc);
-  const double mx = reduce_parts_max(a.part + 2 * a.G, a.G, shm);
-  const int loop = s->loop;
-  const double resi = mx * s->r0inv;
-  const double2 rzo = s->rz[loop & 1];
-  double2 beta = cz();
-  if (!(cabsv(rzn) < NEARLY_ZERO && cabsv(rzo) < NEARLY_ZERO)) beta = cdiv_naive(rzn, rzo);
-  const bool conv = resi < s->eps;
-  for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT)
-    a.P[g] = conv ? a.U[g] : cadd(a.Z[g], cmul(beta, a.P[g]));
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    Scal *ws = a.sc;
+  const double2 rzs = reduce_parts(a.part + a.G, a.G, shc);
+  const double mxs = reduce_parts_max(a.part + 2 * a.G, a.G, shm);
+  if (threadIdx.x == 0) {
+    const int loop = s->loop;
+    const double resi = mxs * s->r0inv;
+    const double2 rzo = s->rz;
+    double2 beta = cz();
+    if (!(cabsv(rzs) < NEARLY_ZERO && cabsv(rzo) < NEARLY_ZERO)) beta = cdiv_naive(rzs, rzo);
+    const bool conv = resi < s->eps;
     a.hist[loop] = resi;
-    ws->resi = resi;
-    ws->conv = conv ? 1 : 0;
-    ws->rz[(loop + 1) & 1] = rzn;
+    s->resi = resi;
+    s->conv = conv ? 1 : 0;
+    s->rz = rzs;
+    s->beta = beta;
+    s->iters += 1;
+    if (conv) s->done = 1;
+    if (loop == s->max_loop) s->done = 1;
+    s->loop = loop + 1;
   }
-}
-// status machine of the main loop (hddm_solver.h:125-143)
-__global__ void k_iter_d(Scal *s) {
-  if (s->done) return;
-  s->iters += 1;
-  if (s->conv) s->done = 1;
-  if (s->loop == s->max_loop) s->done = 1;
-  s->loop += 1;
 }
 hipError_t launch_iter(const VecArgs &a, hipStream_t st) {
   k_iter_a<<<a.G, VT, 0, st>>>(a);
   k_iter_b<<<a.G, VT, 0, st>>>(a);
-  k_iter_c<<<a.G, VT, 0, st>>>(a);
-  k_iter_d<<<1, 1, 0, st>>>(a.sc);
+  k_iter_c<<<1, VT, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+// P for the last solve: U when the stop test fired, else the next search
+// direction z + beta p (max_loop reached, hddm_solver.h:125-143)
+__global__ __launch_bounds__(VT) void k_finalize_p(VecArgs a) {
+  const Scal *s = a.sc;
+  const bool conv = s->conv != 0;
+  const double2 beta = s->beta;
+  for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT)
+    a.P[g] = conv ? a.U[g] : zbp(a.Z[g], beta, a.P[g]);
+}
+hipError_t launch_finalize_p(const VecArgs &a, hipStream_t st) {
+  k_finalize_p<<<a.G, VT, 0, st>>>(a);
   return hipGetLastError();
 }
 
@@ -692,6 +717,7 @@ __global__ void k_scal_init(Scal *s, double eps, int max_loop) {
   s->conv = 0;
   s->loop = 0;
   s->iters = 0;
+  s->beta = cz();
 }
 hipError_t launch_scal_init(Scal *sc, double eps, int max_loop, hipStream_t st) {
   k_scal_init<<<1, 1, 0, st>>>(sc, eps, max_loop);

@@ -222,6 +222,13 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
   // ---- 1. load vector --------------------------------------------------------------
   const bool zero_g = (a.mode == MODE_FIRST || a.mode == MODE_RHS);
+  // interface values: P, or the next search direction z + beta p while the
+  // interface iteration runs (formed here, not stored, see k_iter_a)
+  const double2 beta = (a.mode == MODE_ITER && a.Z) ? a.sc->beta : cz();
+  auto pval = [&](int gid) -> double2 {
+    const double2 p = a.P[gid];
+    return a.Z ? zbp(a.Z[gid], beta, p) : p;
+  };
   for (int i = lane; i < n; i += WL) {
     const int d = m.dof0 + i;
     double2 b;
@@ -239,14 +246,14 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     } else {
       const int8_t fl = a.bcflag[d];
       if (fl == 1) {
-        b = zero_g ? cz() : a.P[a.g_gid[m.g0 + a.kidx[d]]];
+        b = zero_g ? cz() : pval(a.g_gid[m.g0 + a.kidx[d]]);
       } else if (fl == 2) {
         b = cz();
       } else {
         b = (a.mode == MODE_ITER) ? cz() : a.f[d];
         if (!zero_g)
           for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++)
-            b = cfms(b, a.cp_val[c], a.P[a.g_gid[m.g0 + a.cp_k[c]]]);
+            b = cfms(b, a.cp_val[c], pval(a.g_gid[m.g0 + a.cp_k[c]]));
       }
     }
     L.v[i] = b;

@@ -129,6 +129,8 @@ class Model:
     op_ibid: np.ndarray
     # bookkeeping (not part of the C struct)
     ledge_global: np.ndarray = None      # [sum nmp] global edge id of each local edge
+    mp2sn: np.ndarray = None             # [sum nmp] subdomain node id of each mid-node
+                                         # (dm->mpid2snid; reference models only)
     n_edges: int = 0
     n_tets: int = 0
     name: str = ""
