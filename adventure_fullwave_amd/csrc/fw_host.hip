@@ -182,6 +182,7 @@ struct fwh_ctx {
     a.yslot = yslot;
     a.X = X;
     a.unit_k = 0;
+    a.unit_list = nullptr;
     a.smat = smat;
     a.sm_off = sm_off;
     return a;
@@ -811,11 +812,27 @@ int build_schur(fwh_ctx *c) {
   if ((rc = c->alloc(&c->smat, tot)) || (rc = c->upload(&c->sm_off, off))) return rc;
   c->nbmax = nbmax;
   HIPCHK(fw::launch_fill(c->smat, tot, make_double2(0.0, 0.0), c->st));
-  fw::SolveArgs a = c->solve_args(fw::MODE_UNIT, false);
-  for (int k = 0; k < c->ngmax; k++) {
-    a.unit_k = k;
-    HIPCHK(fw::launch_solve(a, c->nsub, c->st));
+  // one launch over all (subdomain, unit vector) pairs.  Workgroups are
+  // dealt to the XCDs round-robin (workgroup b on XCD b mod 8), so the pairs
+  // of subdomain s are placed at b = 8 i + (s mod 8): each subdomain's
+  // factor is read by one XCD and reused from its L2.
+  constexpr int NXCD = 8;
+  std::vector<std::vector<int2>> lane(NXCD);
+  for (int i = 0; i < c->nsub; i++) {
+    const int s = c->order_h[i];  // largest skyline first
+    for (int k = 0; k < c->meta_h[s].ng; k++) lane[s % NXCD].push_back(make_int2(s, k));
   }
+  size_t len = 0;
+  for (auto &v : lane) len = std::max(len, v.size());
+  std::vector<int2> ulist(len * NXCD, make_int2(-1, 0));
+  for (int x = 0; x < NXCD; x++)
+    for (size_t i = 0; i < lane[x].size(); i++) ulist[i * NXCD + x] = lane[x][i];
+  int2 *d_ulist = nullptr;
+  if ((rc = c->upload(&d_ulist, ulist))) return rc;
+  fw::SolveArgs a = c->solve_args(fw::MODE_UNIT, false);
+  a.unit_list = d_ulist;
+  a.order = nullptr;
+  HIPCHK(fw::launch_solve(a, (int)ulist.size(), c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->info.ms_schur_form = now_ms() - t0;
   c->info.smat_bytes = 16.0 * (double)tot;

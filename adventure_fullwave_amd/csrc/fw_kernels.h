@@ -66,6 +66,7 @@ struct SolveArgs {
   double2 *yslot;        // per slot output
   double2 *X;            // per DOF solution (MODE_LAST, MODE_RHS in/out)
   int unit_k;            // MODE_UNIT: local interface index of the unit vector
+  const int2 *unit_list; // MODE_UNIT, if set: (subdomain, unit vector) per work item
   double2 *smat;         // MODE_UNIT: Schur-complement tiles
   const long long *sm_off;  // per subdomain first entry of its tiles
 };

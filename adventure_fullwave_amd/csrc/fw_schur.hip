@@ -131,6 +131,21 @@ __global__ __launch_bounds__(SMV_THREADS) __attribute__((amdgpu_waves_per_eu(5))
   double2 *acc = pv + stride; // [SMV_WAVES][nbmax*16] per-wave accumulators
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 
+  if (ng == 0) return;
+
+  // contiguous, equal shares of the tile sequence (row-block major); the
+  // first two tiles of every wave go in flight before the interface values
+  // are gathered, so their latency overlaps the gather (a wave without tiles
+  // loads tile 0, which every subdomain with an interface has)
+  const long long ntile = (long long)nb * (nb + 1) / 2;
+  const long long t0 = ntile * wave / SMV_WAVES, t1 = ntile * (wave + 1) / SMV_WAVES;
+  const bool active = t0 < t1;
+  const long long tl = active ? t1 - 1 : 0;
+  const double2 *base = a.smat + a.sm_off[w];
+  Tile A, B;
+  load_tile(base, active ? t0 : 0, lane, A);
+  load_tile(base, active && t0 + 1 < tl ? t0 + 1 : tl, lane, B);
+
   // interface values: P, or the next search direction z + beta p while the
   // interface iteration runs (formed here, not stored, see k_iter_a)
   const double2 beta = a.Z ? a.sc->beta : cz();
@@ -146,12 +161,8 @@ __global__ __launch_bounds__(SMV_THREADS) __attribute__((amdgpu_waves_per_eu(5))
   for (int k = tid; k < SMV_WAVES * stride; k += SMV_THREADS) acc[k] = cz();
   __syncthreads();
 
-  // contiguous, equal shares of the tile sequence (row-block major)
-  const long long ntile = (long long)nb * (nb + 1) / 2;
-  const long long t0 = ntile * wave / SMV_WAVES, t1 = ntile * (wave + 1) / SMV_WAVES;
-  if (t0 < t1) {
+  if (active) {
     double2 *my = acc + wave * stride;
-    const double2 *base = a.smat + a.sm_off[w];
     SmvState st;
     st.I = (int)((sqrt(8.0 * (double)t0 + 1.0) - 1.0) * 0.5);
     while ((long long)(st.I + 1) * (st.I + 2) / 2 <= t0) st.I++;
@@ -164,10 +175,6 @@ __global__ __launch_bounds__(SMV_THREADS) __attribute__((amdgpu_waves_per_eu(5))
     // two tiles in flight in two statically named register sets; the loads
     // are unconditional (clamped to the wave's last tile) so the compiler can
     // wait for one set while the other stays in flight
-    const long long tl = t1 - 1;
-    Tile A, B;
-    load_tile(base, t0, lane, A);
-    load_tile(base, t0 + 1 < tl ? t0 + 1 : tl, lane, B);
     for (long long t = t0; t < t1; t += 2) {
       tile_step(A, st, pv, my, nb, t == tl, lane);
       load_tile(base, t + 2 < tl ? t + 2 : tl, lane, A);

@@ -196,9 +196,17 @@ __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__re
 
 __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_solve(SolveArgs a) {
   if (a.mode == MODE_ITER && a.done && *a.done) return;
-  const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
+  int w, uk = a.unit_k;
+  if (a.mode == MODE_UNIT && a.unit_list) {
+    const int2 pr = a.unit_list[a.base + blockIdx.x];  // (subdomain, unit vector); -1: padding
+    if (pr.x < 0) return;
+    w = pr.x;
+    uk = pr.y;
+  } else {
+    w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
+  }
   const SubMeta m = a.meta[w];
-  if (a.mode == MODE_UNIT && a.unit_k >= m.ng) return;
+  if (a.mode == MODE_UNIT && uk >= m.ng) return;
   extern __shared__ double2 lds[];
   SolveLds L;
   L.v = lds;                          // [nmax]
@@ -238,10 +246,10 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       const int8_t fl = a.bcflag[d];
       b = cz();
       if (fl == 1) {
-        if (a.kidx[d] == a.unit_k) b.x = 1.0;
+        if (a.kidx[d] == uk) b.x = 1.0;
       } else if (fl == 0) {
         for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++)
-          if (a.cp_k[c] == a.unit_k) b = csub(b, a.cp_val[c]);
+          if (a.cp_k[c] == uk) b = csub(b, a.cp_val[c]);
       }
     } else {
       const int8_t fl = a.bcflag[d];
@@ -299,7 +307,7 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   if (a.mode == MODE_UNIT) {
     // column unit_k of S_s: rows k >= unit_k into the lower-triangle tiles
     // (layout in fw_schur.hip), the diagonal halved
-    const int j = a.unit_k, J = j >> 4, cj = j & 15;
+    const int j = uk, J = j >> 4, cj = j & 15;
     double2 *dst = a.smat + a.sm_off[w];
     for (int k = j + lane; k < m.ng; k += WL) {
       const int sl = m.g0 + k;

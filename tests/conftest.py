@@ -30,3 +30,10 @@ def cfg1_sigma_model():
     """cfg 1 with the reference-mode absorber (graded conductor layers)."""
     from adventure_fullwave_amd import mesh
     return mesh.make_config(1, pml_mode="sigma")
+
+
+@pytest.fixture(scope="session")
+def small_sigma_model():
+    """The small model with the reference-mode absorber."""
+    from adventure_fullwave_amd import mesh
+    return mesh.make_config(1, small=(8, (2, 2, 2), (2, 1, 1)), pml_mode="sigma")

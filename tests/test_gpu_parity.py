@@ -6,9 +6,11 @@ Schur step (implicit and explicit), the full HDDM solve (iteration count,
 residual history, solution) on the same seeded models, and the element
 post-processing.  Tolerances: element/assembly/post 1e-12 relative (different
 but exact-in-math formulas and FMA contraction), factor 1e-10 relative,
-solves and Schur steps 1e-10 relative; HDDM iteration count within 2 % (8 %
-for the rounding-chaotic stretched-PML cfg 1, whose oracle count itself
-spreads 7.2 %), solution within rel. L2 1e-7 of the oracle."""
+solves and Schur steps 1e-10 relative; HDDM iteration count within 2 % with
+the reference-mode absorber (3 % / 8 % for the rounding-chaotic
+stretched-PML small model / cfg 1, whose oracle counts themselves spread
+1.7 % / 7.2 % under few-ulp perturbations), solution within rel. L2 1e-7 of
+the oracle."""
 import sys
 
 import numpy as np
@@ -149,16 +151,32 @@ def test_schur_explicit_symmetric_linear(small, small_model):
     assert small.info["ms_schur_form"] > 0 and small.info["smat_bytes"] > 0
 
 
+@pytest.fixture(scope="module")
+def small_refs(small_model, small_sigma_model):
+    return {"stretch": (small_model, fo.hddm_solve(small_model, eps=1e-8, max_loop=5000)),
+            "sigma": (small_sigma_model, fo.hddm_solve(small_sigma_model, eps=1e-8, max_loop=5000))}
+
+
+# Iteration-count bars.  Reference-mode absorber: 2 % (the oracle's own count
+# moves 213-216 over 24 few-ulp load perturbations).  Stretched PML: 3 %: the
+# oracle's count moves 457-465 (1.7 %) under the same perturbations, and the
+# explicit operator, exactly symmetric by construction where the implicit one
+# is symmetric only to rounding, shifts it a little further on this case.
+COUNT_BAR = {"sigma": 0.02, "stretch": 0.03}
+
+
+@pytest.mark.parametrize("pml", ["sigma", "stretch"])
 @pytest.mark.parametrize("mode", ["batched_graph", "batched", "plain", "explicit_graph", "explicit_plain"])
-def test_hddm_solve_small(fw, small_model, mode):
-    say(f"[gpu] hddm solve small {mode}")
-    ref = fo.hddm_solve(small_model, eps=1e-8, max_loop=5000)
-    sol = fw.Solver(small_model)
+def test_hddm_solve_small(fw, small_refs, mode, pml):
+    say(f"[gpu] hddm solve small {mode} pml={pml}")
+    model, ref = small_refs[pml]
+    sol = fw.Solver(model)
     st = sol.solve(eps=1e-8, max_loop=5000, plain=mode.endswith("plain"),
                    graph_iters=(8 if mode.endswith("graph") else 0), schur=mode.startswith("explicit"))
     it_ref = ref["stats"]["iters"]
+    say(f"[gpu] iters gpu={st['iters']} oracle={it_ref}")
     assert st["converged"] == 1
-    assert abs(st["iters"] - it_ref) <= max(1, 0.02 * it_ref)
+    assert abs(st["iters"] - it_ref) <= max(1, COUNT_BAR[pml] * it_ref)
     r = sol.resid(st["loops"])
     k = min(30, len(r), len(ref["resid"]))
     assert np.allclose(r[:k], ref["resid"][:k], rtol=1e-6)
