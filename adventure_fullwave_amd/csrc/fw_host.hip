@@ -100,8 +100,8 @@ struct fwh_ctx {
 
   // element data kept for post-processing
   int *el_vtx = nullptr, *el_dof = nullptr;
-  double *crd = nullptr, *sig = nullptr, *post_nu = nullptr;
-  double2 *post_out = nullptr;
+  double *crd = nullptr, *sig = nullptr, *post_nu = nullptr, *pml = nullptr, *grad = nullptr;
+  double2 *post_out = nullptr, *f_asm = nullptr, *gx = nullptr;
   double omega = 0.0;
 
   // explicit Schur complements (formed on first use)
@@ -588,6 +588,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
     std::vector<double> pml(m->pml, m->pml + 6 * nel);
     if ((rc = c->upload(&d_pml, pml))) return rc;
   }
+  c->pml = d_pml;
   if ((rc = c->alloc(&d_elem, 27 * (size_t)nel)) || (rc = c->alloc(&d_bad, 1))) return rc;
   HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(int), st));
   if ((rc = c->upload(&d_asm_ptr, asm_ptr)) || (rc = c->upload(&d_asm_src, asm_src)) ||
@@ -1019,6 +1020,58 @@ extern "C" int fwh_post(fwh_ctx *c, const double *nu, double *B, double *E, doub
   HIPCHK(hipMemcpyAsync(B, a.B, 3 * nel * sizeof(double2), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(E, a.E, 3 * nel * sizeof(double2), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(J, a.J, 3 * nel * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+// Load vector of the next solve (per DOF, 2*n_dof doubles, subdomain-local
+// loads concatenated; a DOF shared by several subdomains contributes the sum
+// of its copies); NULL restores the assembled load.  Used for adjoint
+// solves: the factors, Schur complements and preconditioner do not depend
+// on the load.
+extern "C" int fwh_set_load(fwh_ctx *c, const double *f) {
+  if (!c) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  HIPCHK(hipSetDevice(c->device));
+  int rc;
+  const size_t bytes = (size_t)c->ndof * sizeof(double2);
+  if (!c->f_asm) {
+    if ((rc = c->alloc(&c->f_asm, c->ndof))) return rc;
+    HIPCHK(hipMemcpyAsync(c->f_asm, c->f, bytes, hipMemcpyDeviceToDevice, c->st));
+  }
+  if (f)
+    HIPCHK(hipMemcpyAsync(c->f, f, bytes, hipMemcpyHostToDevice, c->st));
+  else
+    HIPCHK(hipMemcpyAsync(c->f, c->f_asm, bytes, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->first_done = c->last_done = false;
+  return 0;
+}
+
+// dJ/deps per element from a forward solution xE and an adjoint solution xL
+// (both per DOF, 2*n_dof doubles, as fwh_get_x returns them).
+extern "C" int fwh_grad_eps(fwh_ctx *c, const double *xE, const double *xL, double *grad) {
+  if (!c || !xE || !xL || !grad) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  HIPCHK(hipSetDevice(c->device));
+  int rc;
+  if (!c->gx && (rc = c->alloc(&c->gx, 2 * (size_t)c->ndof))) return rc;
+  if (!c->grad && (rc = c->alloc(&c->grad, (size_t)c->nel))) return rc;
+  const size_t bytes = (size_t)c->ndof * sizeof(double2);
+  HIPCHK(hipMemcpyAsync(c->gx, xE, bytes, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->gx + c->ndof, xL, bytes, hipMemcpyHostToDevice, c->st));
+  fw::GradArgs a{};
+  a.nel = (int)c->nel;
+  a.el_vtx = c->el_vtx;
+  a.el_dof = c->el_dof;
+  a.crd = c->crd;
+  a.pml = c->pml;
+  a.omega = c->omega;
+  a.xE = c->gx;
+  a.xL = c->gx + c->ndof;
+  a.grad = c->grad;
+  HIPCHK(fw::launch_grad_eps(a, c->st));
+  HIPCHK(hipMemcpyAsync(grad, c->grad, (size_t)c->nel * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }

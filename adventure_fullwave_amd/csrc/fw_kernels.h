@@ -123,6 +123,19 @@ struct PostArgs {
   double2 *B, *E, *J;    // [nel*3] each
 };
 
+// adjoint permittivity sensitivity
+struct GradArgs {
+  int nel;
+  const int *el_vtx;     // [nel*4]
+  const int *el_dof;     // [nel*6]
+  const double *crd;
+  const double *pml;     // [nel*6] or null
+  double omega;
+  const double2 *xE;     // per DOF forward solution
+  const double2 *xL;     // per DOF adjoint solution
+  double *grad;          // [nel] dJ/deps
+};
+
 // scalars of the interface COCG
 struct Scal {
   double2 rz;      // (r,z) of the current residual
@@ -154,6 +167,7 @@ struct VecArgs {
 // launchers (fw_kernels.hip)
 hipError_t launch_element(const AsmArgs &a, hipStream_t st);
 hipError_t launch_post(const PostArgs &a, hipStream_t st);
+hipError_t launch_grad_eps(const GradArgs &a, hipStream_t st);
 hipError_t launch_gather_sum(int n, const int *ptr, const int *src, const double2 *in,
                              double2 *out, hipStream_t st);
 hipError_t launch_copy_idx(int n, const int *slot, const double2 *in, double2 *out,

@@ -106,18 +106,8 @@ __device__ __forceinline__ void load_tet(const int *el_vtx, const double *crd, i
   }
 }
 
-__global__ __launch_bounds__(128) void k_element(AsmArgs a) {
-  const int el = blockIdx.x * blockDim.x + threadIdx.x;
-  if (el >= a.nel) return;
-  double v[12];
-  load_tet(a.el_vtx, a.crd, el, v);
-  double D, g[3], ls[6], e[6][3], F[6][3];
-  if (!tet_geom(v, D, g, ls, e, F)) {
-    atomicOr(a.bad, 1);
-    return;
-  }
-  // centroid second moments C = (1/20) sum (v_i - g)(v_i - g)^T
-  double C[3][3];
+// centroid second moments C = (1/V) int rho rho^T dV = (1/20) sum (v_i - g)(v_i - g)^T
+__device__ __forceinline__ void tet_moments(const double v[12], const double g[3], double C[3][3]) {
 #pragma unroll
   for (int p = 0; p < 3; p++)
 #pragma unroll
@@ -134,6 +124,42 @@ __global__ __launch_bounds__(128) void k_element(AsmArgs a) {
   for (int p = 0; p < 3; p++)
 #pragma unroll
     for (int q = 0; q < 3; q++) C[p][q] *= 0.05;
+}
+
+// component c of the edge mass matrix (tetNedelec_sideside, tetrahedral.c:362-422):
+// ls_i ls_j /(6D) (F_i,c F_j,c + E[(e_i x rho)_c (e_j x rho)_c]), dss = 1/(6D)
+__device__ __forceinline__ double mass_comp(int i, int j, int c, const double ls[6], const double e[6][3],
+                                            const double F[6][3], const double C[3][3], double dss) {
+  const int b = (c + 1) % 3, cc = (c + 2) % 3;
+  double q = e[i][b] * e[j][b] * C[cc][cc] - e[i][b] * e[j][cc] * C[cc][b] -
+             e[i][cc] * e[j][b] * C[b][cc] + e[i][cc] * e[j][cc] * C[b][b];
+  return dss * ls[i] * ls[j] * (F[i][c] * F[j][c] + q);
+}
+
+// uniaxial PML tensor of the complex stretch (sx, sy, sz): Lambda_c = s_b s_cc / s_c
+__device__ __forceinline__ void pml_lambda(const double *pml, int el, double2 lam[3], double2 ilam[3]) {
+  double2 s[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) s[c] = make_double2(pml[6 * (long long)el + 2 * c], pml[6 * (long long)el + 2 * c + 1]);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    lam[c] = cdiv_naive(cmul(s[(c + 1) % 3], s[(c + 2) % 3]), s[c]);
+    ilam[c] = crecip_naive(lam[c]);
+  }
+}
+
+__global__ __launch_bounds__(128) void k_element(AsmArgs a) {
+  const int el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= a.nel) return;
+  double v[12];
+  load_tet(a.el_vtx, a.crd, el, v);
+  double D, g[3], ls[6], e[6][3], F[6][3];
+  if (!tet_geom(v, D, g, ls, e, F)) {
+    atomicOr(a.bad, 1);
+    return;
+  }
+  double C[3][3];
+  tet_moments(v, g, C);
 
   const double drr = 2.0 / (3.0 * D), dss = 1.0 / (6.0 * D);
   const double nu_omega = NU / a.omega;
@@ -141,16 +167,7 @@ __global__ __launch_bounds__(128) void k_element(AsmArgs a) {
   const double sig = a.sigma[el];
   double2 lam[3], ilam[3];
   const bool stretch = a.pml != nullptr;
-  if (stretch) {
-    double2 s[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) s[c] = make_double2(a.pml[6 * (long long)el + 2 * c], a.pml[6 * (long long)el + 2 * c + 1]);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      lam[c] = cdiv_naive(cmul(s[(c + 1) % 3], s[(c + 2) % 3]), s[c]);
-      ilam[c] = crecip_naive(lam[c]);
-    }
-  }
+  if (stretch) pml_lambda(a.pml, el, lam, ilam);
   double2 *out = a.elem_out + 27LL * el;
   int p = 0;
 #pragma unroll
@@ -160,11 +177,8 @@ __global__ __launch_bounds__(128) void k_element(AsmArgs a) {
       double rr[3], mm[3];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const int b = (c + 1) % 3, cc = (c + 2) % 3;
-        double q = e[i][b] * e[j][b] * C[cc][cc] - e[i][b] * e[j][cc] * C[cc][b] -
-                   e[i][cc] * e[j][b] * C[b][cc] + e[i][cc] * e[j][cc] * C[b][b];
         rr[c] = drr * ls[i] * ls[j] * e[i][c] * e[j][c];
-        mm[c] = dss * ls[i] * ls[j] * (F[i][c] * F[j][c] + q);
+        mm[c] = mass_comp(i, j, c, ls, e, F, C, dss);
       }
       double2 t;
       if (!stretch) {
@@ -267,6 +281,59 @@ __global__ __launch_bounds__(128) void k_post(PostArgs a) {
 hipError_t launch_post(const PostArgs &a, hipStream_t st) {
   if (a.nel <= 0) return hipSuccess;
   k_post<<<(a.nel + 127) / 128, 128, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// adjoint sensitivity to the element permittivity, one thread per element.
+// The element matrix depends on eps only through -eps*omega*M (M = SS, or
+// sum_c Lambda_c MM_c in a stretched PML; make_elem_em.c:239-263 and the
+// stretched form of k_element), so for J(E) with A E = f and A lambda =
+// dJ/dE (A = A^T):  dJ/deps_e = -2 Re(lambda_e^T dA_e/deps E_e)
+//                             =  2 omega Re(lambda_e^T M_e E_e).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_grad_eps(GradArgs a) {
+  const int el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= a.nel) return;
+  double v[12];
+  load_tet(a.el_vtx, a.crd, el, v);
+  double D, g[3], ls[6], e[6][3], F[6][3];
+  (void)tet_geom(v, D, g, ls, e, F);
+  double C[3][3];
+  tet_moments(v, g, C);
+  const double dss = 1.0 / (6.0 * D);
+  double2 lam[3], ilam[3];
+  const bool stretch = a.pml != nullptr;
+  if (stretch) pml_lambda(a.pml, el, lam, ilam);
+  double2 xe[6], xl[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int d = a.el_dof[6 * (long long)el + k];
+    xe[k] = a.xE[d];
+    xl[k] = a.xL[d];
+  }
+  double2 acc = cz();
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    double2 row = cz();  // (M E)_i
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      double2 m = cz();
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double mc = mass_comp(i, j, c, ls, e, F, C, dss);
+        m = stretch ? cadd(m, cscale(lam[c], mc)) : make_double2(m.x + mc, 0.0);
+      }
+      row = cfma(row, m, xe[j]);
+    }
+    acc = cfma(acc, xl[i], row);
+  }
+  a.grad[el] = 2.0 * a.omega * acc.x;
+}
+
+hipError_t launch_grad_eps(const GradArgs &a, hipStream_t st) {
+  if (a.nel <= 0) return hipSuccess;
+  k_grad_eps<<<(a.nel + 127) / 128, 128, 0, st>>>(a);
   return hipGetLastError();
 }
 

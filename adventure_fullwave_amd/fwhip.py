@@ -26,7 +26,8 @@ FWH_E = {-1: "FWH_E_ARG", -2: "FWH_E_HIP", -3: "FWH_E_MESH", -4: "FWH_E_STATE", 
 EXPORTS = ("fwh_create", "fwh_destroy", "fwh_last_error", "fwh_load", "fwh_get_info", "fwh_solve",
            "fwh_first", "fwh_iterate", "fwh_last", "fwh_get_stats", "fwh_get_x", "fwh_get_u",
            "fwh_get_resid", "fwh_schur_apply", "fwh_ldl_solve_all", "fwh_time_kernels", "fwh_get_sub",
-           "fwh_element_batch", "fwh_cocg_icc_all", "fwh_schur_form", "fwh_post")
+           "fwh_element_batch", "fwh_cocg_icc_all", "fwh_schur_form", "fwh_post", "fwh_set_load",
+           "fwh_grad_eps")
 
 
 class FwhOpts(ctypes.Structure):
@@ -90,6 +91,8 @@ def lib():
         L.fwh_time_kernels.argtypes = [P, I, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)]
         L.fwh_schur_form.argtypes = [P]
         L.fwh_post.argtypes = [P, P, P, P, P]
+        L.fwh_set_load.argtypes = [P, P]
+        L.fwh_grad_eps.argtypes = [P, P, P, P]
         L.fwh_get_sub.argtypes = [P, I, I, P]
         L.fwh_element_batch.argtypes = [P, I, P, P, P, D, P, P, P, P, P]
         L.fwh_cocg_icc_all.argtypes = [P, P, P, P, D, D]
@@ -231,6 +234,28 @@ class Solver:
             nu_p = _ptr(nu)
         _check(lib().fwh_post(self.ctx.h, nu_p, *(_ptr(o) for o in out)), "fwh_post")
         return tuple(out)
+
+    def set_load(self, f=None):
+        """Per-DOF load of the next solve (fwh_get_x layout), None: the
+        assembled load."""
+        if f is None:
+            _check(lib().fwh_set_load(self.ctx.h, None), "fwh_set_load")
+            return
+        f = np.ascontiguousarray(f, np.complex128)
+        if f.shape != (int(self.info["n_dof"]),):
+            raise FwhError("load must have n_dof entries")
+        _check(lib().fwh_set_load(self.ctx.h, _ptr(f)), "fwh_set_load")
+
+    def grad_eps(self, xE, xL):
+        """dJ/deps per element from forward and adjoint solutions."""
+        xE = np.ascontiguousarray(xE, np.complex128)
+        xL = np.ascontiguousarray(xL, np.complex128)
+        n = int(self.info["n_dof"])
+        if xE.shape != (n,) or xL.shape != (n,):
+            raise FwhError("solutions must have n_dof entries")
+        g = np.zeros(int(self.info["n_elem"]))
+        _check(lib().fwh_grad_eps(self.ctx.h, _ptr(xE), _ptr(xL), _ptr(g)), "fwh_grad_eps")
+        return g
 
     def schur_apply(self, p, plain=False, explicit=False):
         p = np.ascontiguousarray(p, np.complex128)

@@ -141,6 +141,19 @@ int  fwh_get_resid(const fwh_ctx *ctx, double *resid, int nresid);
  * el_Je_c = sigma (nu/omega) E(centroid), with the reference's scalings.  */
 int  fwh_post(fwh_ctx *ctx, const double *nu, double *B, double *E, double *J);
 
+/* ---- adjoint sensitivities (no reference counterpart) --------------------
+ * fwh_set_load: the per-DOF load of the next solve (2*n_dof doubles in
+ * fwh_get_x's layout; the global load is the sum over a DOF's copies), NULL
+ * restores the assembled load.  Factors, Schur complements and the
+ * preconditioner are reused, so an adjoint solve (A = A^T) costs one more
+ * interface solve.
+ * fwh_grad_eps: dJ/deps_e = 2 omega Re(lambda_e^T M_e E_e) for every element
+ * (M_e: the element's edge mass matrix, PML-weighted when stretched; eps
+ * absolute), from a forward solution E and the adjoint solution lambda of
+ * A lambda = dJ/dE, both in fwh_get_x's layout; grad: n_elem doubles.      */
+int  fwh_set_load(fwh_ctx *ctx, const double *f);
+int  fwh_grad_eps(fwh_ctx *ctx, const double *xE, const double *xL, double *grad);
+
 /* ---- single operators (parity tests, benchmarks) ----------------------- */
 /* One Schur-complement application q = S p over all subdomains
  * (renewal_bc_sp + solve_dom_d + COM_res_sp + gather,
