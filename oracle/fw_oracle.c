@@ -11,6 +11,9 @@
  * double _Complex in the skyline factor/solve (lib/hddm/advlas_skyline.c).
  */
 #include "fw_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include <complex.h>
 #include <math.h>
@@ -1146,7 +1149,7 @@ int fwo_hddm_solve(const fwh_mesh_desc *m, const fwo_opts *o, double *resid,
   return 0;
 }
 
-double fwo_hddm_time_iterations(const fwh_mesh_desc *m, int n_iter, long *sub_solves) {
+double fwo_hddm_time_iterations(const fwh_mesh_desc *m, int n_iter, long *sub_solves, int nthreads) {
   fwo_opts o;
   memset(&o, 0, sizeof(o));
   o.hddm_eps = 0.0; /* never converges inside the window */
@@ -1158,12 +1161,37 @@ double fwo_hddm_time_iterations(const fwh_mesh_desc *m, int n_iter, long *sub_so
   for (int s = 0; s < m->nsub; s++) { sub_factor(&md->s[s], 1); s_solve(&H, s, 0); }
   s_iteration_1st(&H, NULL);
   long before = H.sub_solves;
+  /* the subdomain Schur steps of an iteration are independent: with
+   * nthreads > 1 they run in parallel into per-subdomain buffers and are
+   * accumulated afterwards in the reference's order (same sums as s_solve) */
+  fwo_c **ybuf = (fwo_c **)calloc(m->nsub, sizeof(fwo_c *));
+  for (int s = 0; s < m->nsub; s++) ybuf[s] = (fwo_c *)malloc(md->s[s].A.nf * sizeof(fwo_c));
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
   double t0 = now_s();
   for (int it = 1; it <= n_iter; it++) {
-    for (int s = 0; s < m->nsub; s++) s_solve(&H, s, 1);
+    const int ns = m->nsub;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int s = 0; s < ns; s++) {
+      sub_t *u = &md->s[s];
+      hvec_t *hv = &H.hv[u->part];
+      for (int i = 0, j = u->nbc - u->ninbc; i < u->ninbc; i++, j++) u->vbc[j] = hv->P[u->ln[i]];
+      sub_solve_dom(u, 1, 1, ybuf[s]);
+    }
+    for (int s = 0; s < ns; s++) {
+      sub_t *u = &md->s[s];
+      hvec_t *hv = &H.hv[u->part];
+      for (int i = 0; i < u->ninbc; i++) hv->Q[u->ln[i]] = cadd(hv->Q[u->ln[i]], ybuf[s][u->nfid[i]]);
+      H.sub_solves++;
+    }
     s_iteration(&H, it, NULL);
   }
   double dt = now_s() - t0;
+  for (int s = 0; s < m->nsub; s++) free(ybuf[s]);
+  free(ybuf);
   if (sub_solves) *sub_solves = H.sub_solves - before;
   hddm_free(&H);
   fwo_model_free(md);

@@ -111,9 +111,10 @@ int fwo_hddm_solve(const fwh_mesh_desc *m, const fwo_opts *o, double *resid,
                    double *x_out, double *u_out, fwo_stats *st);
 
 /* Time `n_iter` main-loop interface iterations on a built model (the
- * CPU baseline); returns seconds.                                         */
+ * CPU baseline) with `nthreads` OpenMP threads over the subdomain Schur
+ * steps (accumulated in the reference's order); returns seconds.         */
 double fwo_hddm_time_iterations(const fwh_mesh_desc *m, int n_iter,
-                                long *sub_solves);
+                                long *sub_solves, int nthreads);
 
 /* ---- subdomain COCG + ICC (a9) ----------------------------------------- */
 /* mymt_CS_AIJ_Solver_COCG (aij_solver_cg.h:9-44, 113-355) with ICC

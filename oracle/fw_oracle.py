@@ -54,7 +54,7 @@ def lib():
         L.fwo_hddm_solve.restype = ctypes.c_int
         L.fwo_hddm_solve.argtypes = [P, P, P, P, P, P]
         L.fwo_hddm_time_iterations.restype = ctypes.c_double
-        L.fwo_hddm_time_iterations.argtypes = [P, ctypes.c_int, P]
+        L.fwo_hddm_time_iterations.argtypes = [P, ctypes.c_int, P, ctypes.c_int]
         L.fwo_cocg_icc.restype = ctypes.c_int
         L.fwo_cocg_icc.argtypes = [ctypes.c_int, P, P, P, P, P, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_int, P]
@@ -189,10 +189,12 @@ def hddm_solve(model, eps=1e-8, max_loop=100000, dom_solver=0, sol_eps=1e-9, ic_
                            seconds_loop=st.seconds_loop, sub_solves=st.sub_solves))
 
 
-def time_iterations(model, n_iter):
+def time_iterations(model, n_iter, threads=1):
+    """seconds for n_iter interface iterations (subdomain Schur steps on
+    `threads` OpenMP threads) and the number of subdomain solves"""
     d = model.desc()
     ns = ctypes.c_long()
-    t = lib().fwo_hddm_time_iterations(ctypes.byref(d), n_iter, ctypes.byref(ns))
+    t = lib().fwo_hddm_time_iterations(ctypes.byref(d), n_iter, ctypes.byref(ns), int(threads))
     return t, ns.value
 
 
