@@ -108,7 +108,22 @@ struct fwh_ctx {
   double2 *smat = nullptr;
   long long *sm_off = nullptr;
   int nbmax = 0;
+  int smv_waves = 4;
   bool schur_built = false;
+
+  // size buckets of the batched LDL^T kernel: subdomains grouped by how many
+  // of their workgroups fit a CU's LDS, one launch per bucket on its own
+  // stream (fork/join on the library stream, so graphs capture them as
+  // parallel branches); use_buckets is picked at load by timing
+  struct Bucket {
+    int first, count, nmax, ngmax, nblkmax;
+  };
+  std::vector<Bucket> buckets;
+  int *border = nullptr;          // subdomains bucket by bucket, largest first
+  bool use_buckets = false;
+  hipStream_t bst[8] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[8] = {};
+  double ms_single = 0.0, ms_bucketed = 0.0;
 
   // graph of batched iterations
   hipGraphExec_t gexec = nullptr;
@@ -147,6 +162,33 @@ struct fwh_ctx {
     release_graph();
     for (void *p : allocs) (void)hipFree(p);
     allocs.clear();
+    for (int b = 0; b < 8; b++) {
+      if (bst[b]) (void)hipStreamDestroy(bst[b]);
+      if (ev_join[b]) (void)hipEventDestroy(ev_join[b]);
+      bst[b] = nullptr;
+      ev_join[b] = nullptr;
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    ev_fork = nullptr;
+  }
+  // one batched k_solve launch over all subdomains, or one per size bucket
+  hipError_t launch_solve_all(fw::SolveArgs a, bool bucketed) {
+    if (!bucketed || buckets.size() <= 1) return fw::launch_solve(a, nsub, st);
+    hipError_t e = hipEventRecord(ev_fork, st);
+    for (size_t b = 0; b < buckets.size() && e == hipSuccess; b++) {
+      const Bucket &B = buckets[b];
+      fw::SolveArgs ab = a;
+      ab.order = border;
+      ab.base = B.first;
+      ab.nmax = B.nmax;
+      ab.ngmax = B.ngmax;
+      ab.nblkmax = B.nblkmax;
+      e = hipStreamWaitEvent(bst[b], ev_fork, 0);
+      if (e == hipSuccess) e = fw::launch_solve(ab, B.count, bst[b]);
+      if (e == hipSuccess) e = hipEventRecord(ev_join[b], bst[b]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(st, ev_join[b], 0);
+    }
+    return e;
   }
   // fused: the interface values are the next search direction z + beta p of
   // the running interface iteration (formed by the kernels from Z, P, beta)
@@ -194,6 +236,7 @@ struct fwh_ctx {
     a.base = 0;
     a.done = check_done ? &sc->done : nullptr;
     a.nbmax = nbmax;
+    a.waves = smv_waves;
     a.smat = smat;
     a.sm_off = sm_off;
     a.g_gid = g_gid;
@@ -226,7 +269,7 @@ struct fwh_ctx {
   // subdomain in the reference's loop order (plain)
   hipError_t run_solve(int mode, bool plain, bool check_done, bool fused = false) {
     fw::SolveArgs a = solve_args(mode, check_done, fused);
-    if (!plain) return fw::launch_solve(a, nsub, st);
+    if (!plain) return launch_solve_all(a, use_buckets);
     a.order = nullptr;
     for (int s = 0; s < nsub; s++) {
       a.base = s;
@@ -699,6 +742,36 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return meta[x].nlen > meta[y].nlen; });
   if ((rc = c->upload(&c->order, order))) return rc;
   c->order_h = order;
+  // size buckets by LDS occupancy of the solve kernel (at most 8 waves of it
+  // fit a CU's registers)
+  {
+    std::vector<int> klass(nsub);
+    for (int s = 0; s < nsub; s++) {
+      const size_t lds = fw::solve_lds_bytes(meta[s].n, meta[s].ng, meta[s].nblk);
+      klass[s] = (int)std::min<size_t>(8, (160 * 1024) / std::max<size_t>(lds, 1));
+    }
+    std::vector<int> bo(order);
+    std::stable_sort(bo.begin(), bo.end(), [&](int x, int y) { return klass[x] < klass[y]; });
+    c->buckets.clear();
+    for (int i = 0; i < nsub; i++) {
+      const int s = bo[i];
+      if (c->buckets.empty() || klass[bo[c->buckets.back().first]] != klass[s])
+        c->buckets.push_back({i, 0, 0, 0, 0});
+      fwh_ctx::Bucket &B = c->buckets.back();
+      B.count++;
+      B.nmax = std::max(B.nmax, meta[s].n);
+      B.ngmax = std::max(B.ngmax, meta[s].ng);
+      B.nblkmax = std::max(B.nblkmax, meta[s].nblk);
+    }
+    if ((rc = c->upload(&c->border, bo))) return rc;
+    if (c->buckets.size() > 1) {
+      HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+      for (size_t b = 0; b < c->buckets.size(); b++) {
+        HIPCHK(hipStreamCreateWithFlags(&c->bst[b], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_join[b], hipEventDisableTiming));
+      }
+    }
+  }
   c->meta_h = meta;
 
   HIPCHK(hipStreamSynchronize(st));
@@ -748,6 +821,40 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   I.bytes_solve = 16.0 * nlen_tot + 40.0 * ndof;
   I.bytes_iter = I.bytes_solve + 20.0 * (c->ncpl + c->ngrow) + 48.0 * nslots + 128.0 * ng;
   I.flops_factor = flops;
+
+  // the batched LDL^T kernel: one launch or one per size bucket, whichever
+  // is faster on this model (one warm-up, then the best of three)
+  c->use_buckets = false;
+  if (c->buckets.size() > 1) {
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best[2] = {1e30, 1e30};
+    for (int v = 0; v < 2; v++) {
+      fw::SolveArgs a = c->solve_args(fw::MODE_RHS, false);
+      HIPCHK(c->launch_solve_all(a, v == 1));
+      for (int r = 0; r < 3; r++) {
+        float ms = 0.f;
+        HIPCHK(hipEventRecord(e0, st));
+        HIPCHK(c->launch_solve_all(a, v == 1));
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        best[v] = std::min(best[v], (double)ms);
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->ms_single = best[0];
+    c->ms_bucketed = best[1];
+    c->use_buckets = best[1] < best[0];
+    HIPCHK(hipMemsetAsync(c->X, 0, ndof * sizeof(double2), st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  I.solve_buckets = (int)c->buckets.size();
+  I.solve_bucketed = c->use_buckets ? 1 : 0;
+  I.ms_solve_single = c->ms_single;
+  I.ms_solve_bucketed = c->ms_bucketed;
   c->loaded = true;
   return 0;
 }
@@ -805,7 +912,7 @@ int build_schur(fwh_ctx *c) {
     tot += nb * (nb + 1) / 2 * fw::SMV_TILE;
     nbmax = std::max(nbmax, (int)nb);
   }
-  if (fw::smv_lds_bytes(nbmax) > 160 * 1024)
+  if (fw::smv_lds_bytes(nbmax, 2) > 160 * 1024)
     return fail(FWH_E_LIMIT, "a subdomain of %d interface DOFs exceeds the explicit Schur LDS budget", c->ngmax);
   HIPCHK(hipStreamSynchronize(c->st));
   const double t0 = now_ms();
@@ -817,26 +924,96 @@ int build_schur(fwh_ctx *c) {
   // dealt to the XCDs round-robin (workgroup b on XCD b mod 8), so the pairs
   // of subdomain s are placed at b = 8 i + (s mod 8): each subdomain's
   // factor is read by one XCD and reused from its L2.
+  // With size buckets in use, every bucket gets its own list and launch
+  // (on the bucket streams, with that bucket's LDS size).
   constexpr int NXCD = 8;
-  std::vector<std::vector<int2>> lane(NXCD);
-  for (int i = 0; i < c->nsub; i++) {
-    const int s = c->order_h[i];  // largest skyline first
-    for (int k = 0; k < c->meta_h[s].ng; k++) lane[s % NXCD].push_back(make_int2(s, k));
+  std::vector<int2> ulist;
+  std::vector<size_t> ufirst;
+  const bool bk = c->use_buckets && c->buckets.size() > 1;
+  std::vector<std::vector<int>> groups;
+  if (bk) {
+    std::vector<int> bo(c->nsub);
+    HIPCHK(hipMemcpy(bo.data(), c->border, c->nsub * sizeof(int), hipMemcpyDeviceToHost));
+    for (auto &B : c->buckets) groups.emplace_back(bo.begin() + B.first, bo.begin() + B.first + B.count);
+  } else {
+    groups.push_back(c->order_h);  // largest skyline first
   }
-  size_t len = 0;
-  for (auto &v : lane) len = std::max(len, v.size());
-  std::vector<int2> ulist(len * NXCD, make_int2(-1, 0));
-  for (int x = 0; x < NXCD; x++)
-    for (size_t i = 0; i < lane[x].size(); i++) ulist[i * NXCD + x] = lane[x][i];
+  for (auto &grp : groups) {
+    std::vector<std::vector<int2>> lane(NXCD);
+    for (int s : grp)
+      for (int k = 0; k < c->meta_h[s].ng; k++) lane[s % NXCD].push_back(make_int2(s, k));
+    size_t len = 0;
+    for (auto &v : lane) len = std::max(len, v.size());
+    ufirst.push_back(ulist.size());
+    ulist.resize(ulist.size() + len * NXCD, make_int2(-1, 0));
+    for (int x = 0; x < NXCD; x++)
+      for (size_t i = 0; i < lane[x].size(); i++) ulist[ufirst.back() + i * NXCD + x] = lane[x][i];
+  }
+  ufirst.push_back(ulist.size());
   int2 *d_ulist = nullptr;
   if ((rc = c->upload(&d_ulist, ulist))) return rc;
   fw::SolveArgs a = c->solve_args(fw::MODE_UNIT, false);
   a.unit_list = d_ulist;
   a.order = nullptr;
-  HIPCHK(fw::launch_solve(a, (int)ulist.size(), c->st));
+  if (!bk) {
+    HIPCHK(fw::launch_solve(a, (int)ulist.size(), c->st));
+  } else {
+    HIPCHK(hipEventRecord(c->ev_fork, c->st));
+    for (size_t b = 0; b < c->buckets.size(); b++) {
+      fw::SolveArgs ab = a;
+      ab.base = (int)ufirst[b];
+      ab.nmax = c->buckets[b].nmax;
+      ab.ngmax = c->buckets[b].ngmax;
+      ab.nblkmax = c->buckets[b].nblkmax;
+      HIPCHK(hipStreamWaitEvent(c->bst[b], c->ev_fork, 0));
+      HIPCHK(fw::launch_solve(ab, (int)(ufirst[b + 1] - ufirst[b]), c->bst[b]));
+      HIPCHK(hipEventRecord(c->ev_join[b], c->bst[b]));
+      HIPCHK(hipStreamWaitEvent(c->st, c->ev_join[b], 0));
+    }
+  }
   HIPCHK(hipStreamSynchronize(c->st));
   c->info.ms_schur_form = now_ms() - t0;
   c->info.smat_bytes = 16.0 * (double)tot;
+  // waves per workgroup of the matvec: 2, 4 or 8, whichever streams fastest
+  // on this model (LDS permitting; one warm-up, then the best of three).  The
+  // count sets how each subdomain's sum is split into partial sums, so 4 is
+  // kept unless another count is more than 5 % faster: the choice, and with
+  // it the rounding, does not flip between runs on timing noise.
+  {
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int cand[3] = {2, 4, 8};
+    double best = 1e30;
+    for (int v = 0; v < 3; v++) {
+      c->info.ms_smv_tune[v] = -1.0;
+      if (fw::smv_lds_bytes(nbmax, cand[v]) > 160 * 1024) continue;
+      c->smv_waves = cand[v];
+      fw::SmvArgs sa = c->smv_args(false, false);
+      HIPCHK(fw::launch_smv(sa, c->nsub, c->st));
+      double t = 1e30;
+      for (int r = 0; r < 3; r++) {
+        float ms = 0.f;
+        HIPCHK(hipEventRecord(e0, c->st));
+        HIPCHK(fw::launch_smv(sa, c->nsub, c->st));
+        HIPCHK(hipEventRecord(e1, c->st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        t = std::min(t, (double)ms);
+      }
+      c->info.ms_smv_tune[v] = t;
+    }
+    best = c->info.ms_smv_tune[1] > 0 ? c->info.ms_smv_tune[1] : 1e30;
+    c->info.smv_waves = 4;
+    for (int v = 0; v < 3; v++)
+      if (c->info.ms_smv_tune[v] > 0 && c->info.ms_smv_tune[v] < 0.95 * best) {
+        best = c->info.ms_smv_tune[v];
+        c->info.smv_waves = cand[v];
+      }
+    c->smv_waves = c->info.smv_waves;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
   double packed = 0.0;
   for (int s = 0; s < c->nsub; s++) {
     const double g = c->meta_h[s].ng;

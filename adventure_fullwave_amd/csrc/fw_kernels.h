@@ -27,7 +27,6 @@ enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3, MOD
 
 // explicit Schur complements: 16x16 tiles of the lower triangle
 constexpr int SMV_TILE = 256;
-constexpr int SMV_THREADS = 256;
 
 // solve kernel limits: rows per block, prefetch registers per thread
 constexpr int SOLVE_THREADS = 64;
@@ -77,6 +76,7 @@ struct SmvArgs {
   int base;
   const int *done;
   int nbmax;             // largest number of 16-row blocks (LDS sizing)
+  int waves;             // waves per workgroup: 2, 4 or 8
   const double2 *smat;
   const long long *sm_off;
   const int *g_gid;
@@ -185,7 +185,7 @@ hipError_t launch_factor(const FactorArgs &a, int nwork, hipStream_t st);
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st);
 size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax);
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st);
-size_t smv_lds_bytes(int nbmax);
+size_t smv_lds_bytes(int nbmax, int waves);
 hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, const double2 *val,
                          double2 *pcslot, hipStream_t st);
 hipError_t launch_first(const VecArgs &a, hipStream_t st);

@@ -26,8 +26,6 @@ namespace fw {
 
 namespace {
 
-constexpr int SMV_WAVES = SMV_THREADS / 64;
-
 struct Ent {
   double x, y;
 };
@@ -121,7 +119,9 @@ __device__ __forceinline__ void tile_step(const Tile &T, SmvState &st, const dou
 
 }  // namespace
 
-__global__ __launch_bounds__(SMV_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_smv(SmvArgs a) {
+template <int W>  // waves per workgroup
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_smv(SmvArgs a) {
+  constexpr int SMV_WAVES = W, NT = 64 * W;
   if (a.done && *a.done) return;
   const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
   const SubMeta m = a.meta[w];
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(SMV_THREADS) __attribute__((amdgpu_waves_per_eu(5))
   // interface values: P, or the next search direction z + beta p while the
   // interface iteration runs (formed here, not stored, see k_iter_a)
   const double2 beta = a.Z ? a.sc->beta : cz();
-  for (int k = tid; k < npad; k += SMV_THREADS) {
+  for (int k = tid; k < npad; k += NT) {
     double2 p = cz();
     if (k < ng) {
       const int gid = a.g_gid[m.g0 + k];
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(SMV_THREADS) __attribute__((amdgpu_waves_per_eu(5))
     }
     pv[k] = p;
   }
-  for (int k = tid; k < SMV_WAVES * stride; k += SMV_THREADS) acc[k] = cz();
+  for (int k = tid; k < SMV_WAVES * stride; k += NT) acc[k] = cz();
   __syncthreads();
 
   if (active) {
@@ -183,18 +183,23 @@ __global__ __launch_bounds__(SMV_THREADS) __attribute__((amdgpu_waves_per_eu(5))
     }
   }
   __syncthreads();
-  for (int k = tid; k < ng; k += SMV_THREADS) {
+  for (int k = tid; k < ng; k += NT) {
     double2 y = cadd(acc[k], acc[stride + k]);
     for (int v = 2; v < SMV_WAVES; v++) y = cadd(y, acc[v * stride + k]);
     a.yslot[m.g0 + k] = y;
   }
 }
 
-size_t smv_lds_bytes(int nbmax) { return sizeof(double2) * (size_t)(1 + SMV_WAVES) * nbmax * 16; }
+size_t smv_lds_bytes(int nbmax, int waves) { return sizeof(double2) * (size_t)(1 + waves) * nbmax * 16; }
 
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st) {
   if (nwork <= 0) return hipSuccess;
-  k_smv<<<nwork, SMV_THREADS, smv_lds_bytes(a.nbmax), st>>>(a);
+  const size_t lds = smv_lds_bytes(a.nbmax, a.waves);
+  switch (a.waves) {
+    case 2: k_smv<2><<<nwork, 128, lds, st>>>(a); break;
+    case 8: k_smv<8><<<nwork, 512, lds, st>>>(a); break;
+    default: k_smv<4><<<nwork, 256, lds, st>>>(a); break;
+  }
   return hipGetLastError();
 }
 

@@ -52,10 +52,17 @@ class FwhInfo(ctypes.Structure):
                 ("ms_factor", ctypes.c_double), ("bytes_solve", ctypes.c_double),
                 ("bytes_iter", ctypes.c_double), ("flops_factor", ctypes.c_double),
                 ("ms_schur_form", ctypes.c_double), ("smat_bytes", ctypes.c_double),
-                ("bytes_smv", ctypes.c_double)]
+                ("bytes_smv", ctypes.c_double), ("solve_buckets", ctypes.c_int),
+                ("solve_bucketed", ctypes.c_int), ("ms_solve_single", ctypes.c_double),
+                ("ms_solve_bucketed", ctypes.c_double), ("smv_waves", ctypes.c_int),
+                ("ms_smv_tune", ctypes.c_double * 3)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v) if isinstance(v, ctypes.Array) else v
+        return out
 
 
 class FwhError(RuntimeError):

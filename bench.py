@@ -173,6 +173,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "setup_ms": setup,
+        "autotune": {"smv_waves": info["smv_waves"], "ms_smv_by_waves_2_4_8": [round(v, 4) for v in info["ms_smv_tune"]],
+                     "solve_buckets": info["solve_buckets"], "solve_bucketed": info["solve_bucketed"],
+                     "ms_solve_single_vs_bucketed": [round(info["ms_solve_single"], 4),
+                                                     round(info["ms_solve_bucketed"], 4)]},
         "time_to_solution_ms": round(setup["factor"] + setup.get("schur_form", 0.0) + 1000 * dt / args.steps, 1),
     }
     print(json.dumps(out), flush=True)

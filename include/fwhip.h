@@ -87,6 +87,13 @@ typedef struct fwh_info {
   double smat_bytes;    /* their storage (16x16 tiles of the lower half)   */
   double bytes_smv;     /* algorithmic bytes of one explicit Schur step:
                            sum 8 ng(ng+1) + 36 ng                          */
+  int    solve_buckets; /* size buckets of the batched LDL^T kernel        */
+  int    solve_bucketed;/* 1: launched per bucket (faster at load), 0: one
+                           launch                                          */
+  double ms_solve_single, ms_solve_bucketed; /* the timings of that choice */
+  int    smv_waves;     /* waves per workgroup of the explicit Schur matvec,
+                           picked by timing when the S_s are formed        */
+  double ms_smv_tune[3];/* its timings for 2, 4, 8 waves                   */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */
