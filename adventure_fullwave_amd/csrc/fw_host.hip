@@ -904,12 +904,12 @@ int read_scal(fwh_ctx *c, fw::Scal *s) {
 int build_schur(fwh_ctx *c) {
   if (c->schur_built) return 0;
   std::vector<long long> off(c->nsub);
-  long long tot = 0;
+  long long tot = 64;  // 64 zero entries first (the missing sub-block of diagonal tiles)
   int nbmax = 1;
   for (int s = 0; s < c->nsub; s++) {
     const long long nb = (c->meta_h[s].ng + 15) / 16;
     off[s] = tot;
-    tot += nb * (nb + 1) / 2 * fw::SMV_TILE;
+    tot += fw::tile_off((int)nb, 0);
     nbmax = std::max(nbmax, (int)nb);
   }
   if (fw::smv_lds_bytes(nbmax, 2) > 160 * 1024)

@@ -25,8 +25,13 @@ struct SubMeta {
 // written into the explicit Schur-complement tiles (fw_schur.hip)
 enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3, MODE_UNIT = 4 };
 
-// explicit Schur complements: 16x16 tiles of the lower triangle
+// explicit Schur complements: 16x16 tiles of the lower triangle, row block
+// by row block; off-diagonal tiles 256 entries, diagonal tiles 192 (their
+// upper 8x8 sub-block is not stored)
 constexpr int SMV_TILE = 256;
+__host__ __device__ __forceinline__ long long tile_off(int I, int J) {
+  return 128LL * I * (I - 1) + 192LL * I + 256LL * J;
+}
 
 // solve kernel limits: rows per block, prefetch registers per thread
 constexpr int SOLVE_THREADS = 64;

@@ -12,13 +12,15 @@
 // once through k_smv: a batched, HBM-bound symmetric matvec instead of two
 // dependent triangular sweeps per subdomain.
 //
-// Storage of S_s: the lower triangle in 16x16 tiles, tile (I,J) with J <= I
-// at tile index I(I+1)/2 + J, 256 complex entries per tile.  Inside a tile
-// entry (row r, column c) sits at ((r>>3)*2 + (c>>3))*64 + ((r&7) | (c&7)<<3),
-// so each of the four 64-lane loads of a tile reads 1 KB contiguously and
-// lane l holds rows l&7, (l&7)+8 and columns l>>3, (l>>3)+8.  Diagonal tiles
-// hold the strict lower part and half the diagonal (upper part zero): every
-// tile T is applied as T + T^T, which restores S_s exactly.
+// Storage of S_s: the lower triangle in 16x16 tiles, row block by row block;
+// tile (I,J), J <= I, starts at tile_off(I,J).  Inside a tile, entry (row r,
+// column c) is in 8x8 sub-block k = (r>>3)*2 + (c>>3) at k*64 +
+// ((r&7) | (c&7)<<3), so each of the four 64-lane loads of a tile reads 1 KB
+// contiguously and lane l holds rows l&7, (l&7)+8 and columns l>>3, (l>>3)+8.
+// Diagonal tiles hold the strict lower part and half the diagonal; their
+// sub-block 1 (rows 0-7, columns 8-15) is entirely upper and not stored, so
+// they take 192 entries (sub-blocks 0, 2, 3).  Every tile T is applied as
+// T + T^T, which restores S_s exactly.
 #include "fw_kernels.h"
 #include "fw_device.h"
 
@@ -36,12 +38,43 @@ struct Tile {
 // The tiles are streamed once per application and do not fit any cache
 // level (hundreds of MB): non-temporal loads keep them from evicting the
 // interface vectors and measured 12 % faster than ordinary loads.
-__device__ __forceinline__ void load_tile(const double2 *__restrict__ base, long long t, int lane, Tile &T) {
-  const double2 *p = base + t * SMV_TILE + lane;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    T.e[k].x = __builtin_nontemporal_load(&p[k * 64].x);
-    T.e[k].y = __builtin_nontemporal_load(&p[k * 64].y);
+// row block and column block of the t-th tile of the row-block-major sequence
+__device__ __forceinline__ void tile_ij(long long t, int &I, int &J) {
+  int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((long long)(i + 1) * (i + 2) / 2 <= t) i++;
+  while ((long long)i * (i + 1) / 2 > t) i--;
+  I = i;
+  J = (int)(t - (long long)i * (i + 1) / 2);
+}
+
+// Tiles are loaded in sequence through a cursor (wave-uniform, scalar
+// registers) that stops at the wave's last tile, so every load is
+// unconditional.  A diagonal tile's missing sub-block 1 is read from the
+// 64 zero entries at the head of the tile array (cache-resident: no HBM
+// traffic, no select).
+struct Cursor {
+  long long t, tl;  // next tile, last tile of the wave
+  int I, J;         // its row block and column block
+};
+__device__ __forceinline__ void load_next(const double2 *__restrict__ base, const double2 *__restrict__ zero,
+                                          Cursor &c, int lane, Tile &T) {
+  const bool dg = c.I == c.J;
+  const double2 *p = base + tile_off(c.I, c.J) + lane;
+  const double2 *q1 = dg ? zero + lane : p + 64, *q2 = p + (dg ? 64 : 128), *q3 = p + (dg ? 128 : 192);
+  T.e[0].x = __builtin_nontemporal_load(&p->x);
+  T.e[0].y = __builtin_nontemporal_load(&p->y);
+  T.e[1].x = __builtin_nontemporal_load(&q1->x);
+  T.e[1].y = __builtin_nontemporal_load(&q1->y);
+  T.e[2].x = __builtin_nontemporal_load(&q2->x);
+  T.e[2].y = __builtin_nontemporal_load(&q2->y);
+  T.e[3].x = __builtin_nontemporal_load(&q3->x);
+  T.e[3].y = __builtin_nontemporal_load(&q3->y);
+  if (c.t < c.tl) {
+    c.t++;
+    if (++c.J > c.I) {
+      c.I++;
+      c.J = 0;
+    }
   }
 }
 
@@ -129,7 +162,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   extern __shared__ double2 lds[];
   double2 *pv = lds;          // [nbmax*16] interface values of the subdomain
   double2 *acc = pv + stride; // [SMV_WAVES][nbmax*16] per-wave accumulators
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  // the wave index is uniform within a wave: readfirstlane keeps it, the
+  // tile ranges and the load cursor in scalar registers
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
 
   if (ng == 0) return;
 
@@ -142,9 +177,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   const bool active = t0 < t1;
   const long long tl = active ? t1 - 1 : 0;
   const double2 *base = a.smat + a.sm_off[w];
+  Cursor cur;
+  cur.t = active ? t0 : 0;
+  cur.tl = tl;
+  tile_ij(cur.t, cur.I, cur.J);
+  const double2 *zero = a.smat;  // the 64 zero entries ahead of every subdomain's tiles
   Tile A, B;
-  load_tile(base, active ? t0 : 0, lane, A);
-  load_tile(base, active && t0 + 1 < tl ? t0 + 1 : tl, lane, B);
+  load_next(base, zero, cur, lane, A);
+  load_next(base, zero, cur, lane, B);
 
   // interface values: P, or the next search direction z + beta p while the
   // interface iteration runs (formed here, not stored, see k_iter_a)
@@ -164,10 +204,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   if (active) {
     double2 *my = acc + wave * stride;
     SmvState st;
-    st.I = (int)((sqrt(8.0 * (double)t0 + 1.0) - 1.0) * 0.5);
-    while ((long long)(st.I + 1) * (st.I + 2) / 2 <= t0) st.I++;
-    while ((long long)st.I * (st.I + 1) / 2 > t0) st.I--;
-    st.J = (int)(t0 - (long long)st.I * (st.I + 1) / 2);
+    tile_ij(t0, st.I, st.J);
     st.q0 = cz();
     st.q1 = cz();
     st.pI0 = pv[st.I * 16 + (lane & 7)];
@@ -177,9 +214,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     // wait for one set while the other stays in flight
     for (long long t = t0; t < t1; t += 2) {
       tile_step(A, st, pv, my, nb, t == tl, lane);
-      load_tile(base, t + 2 < tl ? t + 2 : tl, lane, A);
+      load_next(base, zero, cur, lane, A);
       if (t + 1 < t1) tile_step(B, st, pv, my, nb, t + 1 == tl, lane);
-      load_tile(base, t + 3 < tl ? t + 3 : tl, lane, B);
+      load_next(base, zero, cur, lane, B);
     }
   }
   __syncthreads();

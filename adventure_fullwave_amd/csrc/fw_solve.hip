@@ -315,8 +315,9 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], L.v[a.gr_col[c]]);
       if (k == j) y = cscale(y, 0.5);
       const int I = k >> 4, ri = k & 15;
-      const long long t = (long long)I * (I + 1) / 2 + J;
-      dst[t * SMV_TILE + ((ri >> 3) * 2 + (cj >> 3)) * 64 + ((ri & 7) | ((cj & 7) << 3))] = y;
+      int sb = (ri >> 3) * 2 + (cj >> 3);
+      if (I == J) sb -= (sb > 0);  // diagonal tile: sub-block 1 is not stored
+      dst[tile_off(I, J) + sb * 64 + ((ri & 7) | ((cj & 7) << 3))] = y;
     }
   } else if (a.mode != MODE_RHS) {
     for (int k = lane; k < m.ng; k += WL) {
