@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--pml", default="sigma")
     ap.add_argument("--schur", choices=("explicit", "implicit"), default="explicit")
     ap.add_argument("--implicit-steps", type=int, default=1)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also time an adjoint permittivity gradient (forward solve, adjoint solve, gradient)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
@@ -152,6 +154,37 @@ def main():
                          f"({ns} subdomain solves, {tc:.1f}s) with the C oracle, subdomain Schur steps on "
                          f"{threads} OpenMP threads",
                "ldl_solves_per_s": round(ns / tc, 1), "one_core_iters_per_s": round(1.0 / t1, 4)}
+    adjoint = None
+    if args.adjoint:
+        import numpy as np
+        # J = sum |E_k|^2 over 16 interior edges; dJ/deps by one adjoint solve
+        off = np.concatenate([[0], np.cumsum(np.asarray(model.nmp, np.int64))])
+        cand = []
+        for s in range(model.nsub):
+            bad = np.zeros(model.nmp[s], bool)
+            bad[model.ib_nfid[model.ib_off[s]:model.ib_off[s + 1]]] = True
+            bad[model.ibc[model.bc_off[s]:model.bc_off[s + 1]]] = True
+            cand.append(off[s] + np.nonzero(~bad)[0])
+        probes = np.sort(np.random.default_rng(0).choice(np.concatenate(cand), 16, replace=False))
+        ta = time.perf_counter()
+        st_f = sol.solve(eps=args.eps, max_loop=args.max_loop, graph_iters=args.graph_iters, schur=expl)
+        E = sol.x()
+        tb = time.perf_counter()
+        f = np.zeros_like(E)
+        f[probes] = np.conj(E[probes])
+        sol.set_load(f)
+        st_a = sol.solve(eps=args.eps, max_loop=args.max_loop, graph_iters=args.graph_iters, schur=expl)
+        lam = sol.x()
+        sol.set_load(None)
+        tc_ = time.perf_counter()
+        grad = sol.grad_eps(E, lam)
+        td = time.perf_counter()
+        adjoint = {"objective": "sum |E|^2 over 16 interior probe edges", "forward_ms": round(1000 * (tb - ta), 2),
+                   "adjoint_ms": round(1000 * (tc_ - tb), 2), "gradient_ms": round(1000 * (td - tc_), 2),
+                   "iters_forward_adjoint": [st_f["iters"], st_a["iters"]], "n_elem": int(grad.size),
+                   "note": "host transfers of E and lambda included"}
+        log(f"[bench] adjoint: forward {adjoint['forward_ms']} ms, adjoint {adjoint['adjoint_ms']} ms, "
+            f"gradient {adjoint['gradient_ms']} ms")
     setup = {"assemble": round(info["ms_assemble"], 2), "factor": round(info["ms_factor"], 2)}
     if expl:
         setup["schur_form"] = round(info["ms_schur_form"], 2)
@@ -177,6 +210,7 @@ def main():
                      "solve_buckets": info["solve_buckets"], "solve_bucketed": info["solve_bucketed"],
                      "ms_solve_single_vs_bucketed": [round(info["ms_solve_single"], 4),
                                                      round(info["ms_solve_bucketed"], 4)]},
+        "adjoint": adjoint,
         "time_to_solution_ms": round(setup["factor"] + setup.get("schur_form", 0.0) + 1000 * dt / args.steps, 1),
     }
     print(json.dumps(out), flush=True)
