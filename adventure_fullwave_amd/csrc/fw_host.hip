@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -66,7 +67,13 @@ struct fwh_ctx {
   bool loaded = false, first_done = false, last_done = false, p_final = false;
 
   // sizes
-  int nsub = 0, npart = 0, ng = 0, nmax = 0, ngmax = 0, wmax = 0, G = 1, nblkmax = 0;
+  int nsub = 0, npart = 0, ng = 0, nmax = 0, ngmax = 0, wmax = 0, G = 1, nblkmax = 0, nblkmax_s = 0;
+  // prefetch depth of the LDL^T kernel (8 or 16 loads per lane, row blocks
+  // of 64*depth entries): the depth of the per-iteration sweeps is picked at
+  // load by timing; the Schur-complement formation (hundreds of waves per
+  // CU) uses 8, the depth with the higher occupancy, whenever every row fits
+  int pfk_iter = 16, pfk_unit = 16;
+  bool short_ok = false;
   long long ndof = 0, nslots = 0, nnz = 0, nlen = 0, nel = 0, ncpl = 0, ngrow = 0;
   std::vector<SubMeta> meta_h;
   std::vector<int> order_h;
@@ -116,7 +123,7 @@ struct fwh_ctx {
   // stream (fork/join on the library stream, so graphs capture them as
   // parallel branches); use_buckets is picked at load by timing
   struct Bucket {
-    int first, count, nmax, ngmax, nblkmax;
+    int first, count, nmax, ngmax, nblkmax, nblkmax_s;
   };
   std::vector<Bucket> buckets;
   int *border = nullptr;          // subdomains bucket by bucket, largest first
@@ -182,7 +189,7 @@ struct fwh_ctx {
       ab.base = B.first;
       ab.nmax = B.nmax;
       ab.ngmax = B.ngmax;
-      ab.nblkmax = B.nblkmax;
+      ab.nblkmax = a.pfk == 8 ? B.nblkmax_s : B.nblkmax;
       e = hipStreamWaitEvent(bst[b], ev_fork, 0);
       if (e == hipSuccess) e = fw::launch_solve(ab, B.count, bst[b]);
       if (e == hipSuccess) e = hipEventRecord(ev_join[b], bst[b]);
@@ -201,7 +208,8 @@ struct fwh_ctx {
     a.done = check_done ? &sc->done : nullptr;
     a.nmax = nmax;
     a.ngmax = ngmax;
-    a.nblkmax = nblkmax;
+    a.pfk = mode == fw::MODE_UNIT ? pfk_unit : pfk_iter;
+    a.nblkmax = a.pfk == 8 ? nblkmax_s : nblkmax;
     a.idx1 = idx1;
     a.rowbeg = rowbeg;
     a.fac = fac;
@@ -275,7 +283,7 @@ struct fwh_ctx {
       a.base = s;
       a.nmax = meta_h[s].n;
       a.ngmax = meta_h[s].ng;
-      a.nblkmax = meta_h[s].nblk;
+      a.nblkmax = a.pfk == 8 ? meta_h[s].nblk_s : meta_h[s].nblk;
       hipError_t e = fw::launch_solve(a, 1, st);
       if (e != hipSuccess) return e;
     }
@@ -686,6 +694,36 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   long long nlen_tot = 0;
   int wmax = 0;
   double flops = 0.0;
+  // widest profile row: the short partition (8 loads per lane) exists when
+  // every row fits 64*8 entries
+  {
+    int wide = 0;
+    for (int s = 0; s < nsub; s++)
+      for (int i = 0; i < meta[s].n; i++) wide = std::max(wide, i - idx1[meta[s].dof0 + i] + 1);
+    c->short_ok = wide <= fw::SOLVE_THREADS * 8;
+  }
+  // row blocks of subdomain M: <= BMAX rows and <= cap skyline entries each,
+  // padded to an even number of blocks with empty blocks [n, n) (the solve
+  // kernel sweeps block pairs without conditional loads)
+  auto partition = [&](const SubMeta &M, int cap, int &blk0, int &nblk) -> int {
+    blk0 = (int)blk_row.size();
+    int i0 = 0;
+    blk_row.push_back(0);
+    while (i0 < M.n) {
+      int i1 = i0;
+      while (i1 < M.n && i1 - i0 < fw::BMAX) {
+        const long long e1 = (i1 + 1 < M.n) ? rowbeg[M.dof0 + i1 + 1] : M.nlen;
+        if (e1 - rowbeg[M.dof0 + i0] > cap) break;
+        i1++;
+      }
+      if (i1 == i0) return -1;
+      blk_row.push_back(i1);
+      i0 = i1;
+    }
+    if (((int)blk_row.size() - blk0 - 1) % 2 == 1) blk_row.push_back(M.n);
+    nblk = (int)blk_row.size() - blk0 - 1;
+    return 0;
+  };
   for (int s = 0; s < nsub; s++) {
     SubMeta &M = meta[s];
     long long rb = 0;
@@ -700,26 +738,16 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
     M.nlen = rb;
     M.fac0 = nlen_tot;
     nlen_tot += rb;
-    // row blocks: <= BMAX rows and <= CAP skyline entries each
-    M.blk0 = (int)blk_row.size();
-    int i0 = 0;
-    blk_row.push_back(0);
-    while (i0 < M.n) {
-      int i1 = i0;
-      while (i1 < M.n && i1 - i0 < fw::BMAX) {
-        const long long e1 = (i1 + 1 < M.n) ? rowbeg[M.dof0 + i1 + 1] : rb;
-        if (e1 - rowbeg[M.dof0 + i0] > fw::CAP) break;
-        i1++;
-      }
-      if (i1 == i0) return fail(FWH_E_LIMIT, "subdomain %d row %d wider than %d entries", s, i0, fw::CAP);
-      blk_row.push_back(i1);
-      i0 = i1;
-    }
-    // pad to an even number of blocks with empty blocks [n, n): the solve
-    // kernel sweeps block pairs without conditional loads
-    if (((int)blk_row.size() - M.blk0 - 1) % 2 == 1) blk_row.push_back(M.n);
-    M.nblk = (int)blk_row.size() - M.blk0 - 1;
+    if (partition(M, fw::SOLVE_THREADS * fw::PFK, M.blk0, M.nblk))
+      return fail(FWH_E_LIMIT, "subdomain %d has a profile row wider than %d entries", s,
+                  fw::SOLVE_THREADS * fw::PFK);
     c->nblkmax = std::max(c->nblkmax, M.nblk);
+    M.nblk_s = 0;
+    M.blk0_s = M.blk0;
+    if (c->short_ok) {
+      if (partition(M, fw::SOLVE_THREADS * 8, M.blk0_s, M.nblk_s)) return fail(FWH_E_ARG, "bad profile");
+      c->nblkmax_s = std::max(c->nblkmax_s, M.nblk_s);
+    }
     // flops of the factorisation: 8 per complex multiply-add
     for (int i = 0; i < M.n; i++) {
       const int fi = idx1[M.dof0 + i];
@@ -729,8 +757,11 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   }
   c->nlen = nlen_tot;
   c->wmax = wmax;
-  if (fw::solve_lds_bytes(nmax, ngmax, c->nblkmax) > 160 * 1024)
+  if (fw::solve_lds_bytes(nmax, ngmax, c->nblkmax, fw::PFK) > 160 * 1024)
     return fail(FWH_E_LIMIT, "subdomain of %d DOFs / %d interface DOFs exceeds the LDS budget", nmax, ngmax);
+  if (c->short_ok && fw::solve_lds_bytes(nmax, ngmax, c->nblkmax_s, 8) > 160 * 1024) c->short_ok = false;
+  c->pfk_iter = fw::PFK;
+  c->pfk_unit = c->short_ok ? 8 : fw::PFK;
   if ((size_t)(wmax + 1 + nmax) * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "profile too wide for the factor kernel");
   HIPCHK(hipMemcpyAsync(c->meta, meta.data(), nsub * sizeof(SubMeta), hipMemcpyHostToDevice, st));
   if ((rc = c->upload(&c->rowbeg, rowbeg)) || (rc = c->upload(&c->blk_row, blk_row)) ||
@@ -747,7 +778,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   {
     std::vector<int> klass(nsub);
     for (int s = 0; s < nsub; s++) {
-      const size_t lds = fw::solve_lds_bytes(meta[s].n, meta[s].ng, meta[s].nblk);
+      const size_t lds = fw::solve_lds_bytes(meta[s].n, meta[s].ng, meta[s].nblk, fw::PFK);
       klass[s] = (int)std::min<size_t>(8, (160 * 1024) / std::max<size_t>(lds, 1));
     }
     std::vector<int> bo(order);
@@ -756,12 +787,13 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
     for (int i = 0; i < nsub; i++) {
       const int s = bo[i];
       if (c->buckets.empty() || klass[bo[c->buckets.back().first]] != klass[s])
-        c->buckets.push_back({i, 0, 0, 0, 0});
+        c->buckets.push_back({i, 0, 0, 0, 0, 0});
       fwh_ctx::Bucket &B = c->buckets.back();
       B.count++;
       B.nmax = std::max(B.nmax, meta[s].n);
       B.ngmax = std::max(B.ngmax, meta[s].ng);
       B.nblkmax = std::max(B.nblkmax, meta[s].nblk);
+      B.nblkmax_s = std::max(B.nblkmax_s, meta[s].nblk_s);
     }
     if ((rc = c->upload(&c->border, bo))) return rc;
     if (c->buckets.size() > 1) {
@@ -822,32 +854,45 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   I.bytes_iter = I.bytes_solve + 20.0 * (c->ncpl + c->ngrow) + 48.0 * nslots + 128.0 * ng;
   I.flops_factor = flops;
 
-  // the batched LDL^T kernel: one launch or one per size bucket, whichever
-  // is faster on this model (one warm-up, then the best of three)
+  // the batched LDL^T kernel of the iteration sweeps: prefetch depth 16 or 8
+  // (the short partition), one launch or one per size bucket, whichever is
+  // fastest on this model (one warm-up, then the best of three).  The depth
+  // sets the row blocks and with them the order of the sums, so 16 is kept
+  // unless 8 is more than 5 % faster: the choice does not flip between runs
+  // on timing noise.
   c->use_buckets = false;
-  if (c->buckets.size() > 1) {
+  {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    double best[2] = {1e30, 1e30};
-    for (int v = 0; v < 2; v++) {
-      fw::SolveArgs a = c->solve_args(fw::MODE_RHS, false);
-      HIPCHK(c->launch_solve_all(a, v == 1));
-      for (int r = 0; r < 3; r++) {
-        float ms = 0.f;
-        HIPCHK(hipEventRecord(e0, st));
+    double best[2][2] = {{1e30, 1e30}, {1e30, 1e30}};  // [depth 16, 8][single, bucketed]
+    const int depths = c->short_ok ? 2 : 1;
+    const int variants = c->buckets.size() > 1 ? 2 : 1;
+    for (int d = 0; d < depths; d++)
+      for (int v = 0; v < variants; v++) {
+        c->pfk_iter = d == 0 ? fw::PFK : 8;
+        fw::SolveArgs a = c->solve_args(fw::MODE_RHS, false);
         HIPCHK(c->launch_solve_all(a, v == 1));
-        HIPCHK(hipEventRecord(e1, st));
-        HIPCHK(hipEventSynchronize(e1));
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        best[v] = std::min(best[v], (double)ms);
+        for (int r = 0; r < 3; r++) {
+          float ms = 0.f;
+          HIPCHK(hipEventRecord(e0, st));
+          HIPCHK(c->launch_solve_all(a, v == 1));
+          HIPCHK(hipEventRecord(e1, st));
+          HIPCHK(hipEventSynchronize(e1));
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          best[d][v] = std::min(best[d][v], (double)ms);
+        }
       }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c->ms_single = best[0];
-    c->ms_bucketed = best[1];
-    c->use_buckets = best[1] < best[0];
+    const double t16 = std::min(best[0][0], best[0][1]), t8 = std::min(best[1][0], best[1][1]);
+    const int d = (depths == 2 && t8 < 0.95 * t16) ? 1 : 0;
+    c->pfk_iter = d == 0 ? fw::PFK : 8;
+    c->ms_single = best[d][0];
+    c->ms_bucketed = variants == 2 ? best[d][1] : 0.0;
+    c->use_buckets = variants == 2 && best[d][1] < best[d][0];
+    I.ms_solve_depth[0] = depths == 2 ? t8 : 0.0;
+    I.ms_solve_depth[1] = t16;
     HIPCHK(hipMemsetAsync(c->X, 0, ndof * sizeof(double2), st));
     HIPCHK(hipStreamSynchronize(st));
   }
@@ -855,6 +900,8 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   I.solve_bucketed = c->use_buckets ? 1 : 0;
   I.ms_solve_single = c->ms_single;
   I.ms_solve_bucketed = c->ms_bucketed;
+  I.solve_depth = c->pfk_iter;
+  I.unit_depth = c->pfk_unit;
   c->loaded = true;
   return 0;
 }
@@ -964,7 +1011,7 @@ int build_schur(fwh_ctx *c) {
       ab.base = (int)ufirst[b];
       ab.nmax = c->buckets[b].nmax;
       ab.ngmax = c->buckets[b].ngmax;
-      ab.nblkmax = c->buckets[b].nblkmax;
+      ab.nblkmax = a.pfk == 8 ? c->buckets[b].nblkmax_s : c->buckets[b].nblkmax;
       HIPCHK(hipStreamWaitEvent(c->bst[b], c->ev_fork, 0));
       HIPCHK(fw::launch_solve(ab, (int)(ufirst[b + 1] - ufirst[b]), c->bst[b]));
       HIPCHK(hipEventRecord(c->ev_join[b], c->bst[b]));

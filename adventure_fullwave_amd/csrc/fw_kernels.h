@@ -19,6 +19,8 @@ struct SubMeta {
   long long nlen; // skyline length
   int nnz0;       // offset into the lower-triangle CSR arrays (entries)
   int nnz;        // entries
+  int nblk_s;     // short row blocks (<= 64*8 entries; 0 when a row is wider)
+  int blk0_s;     // offset of the short blocks in blk_row
 };
 
 // MODE_UNIT: the Schur step applied to interface unit vector unit_k, output
@@ -36,8 +38,8 @@ __host__ __device__ __forceinline__ long long tile_off(int I, int J) {
 // solve kernel limits: rows per block, prefetch registers per thread
 constexpr int SOLVE_THREADS = 64;
 constexpr int BMAX = 16;
-constexpr int PFK = 16;
-constexpr int CAP = SOLVE_THREADS * PFK;  // skyline entries per row block
+constexpr int PFK = 16;                   // most prefetch loads per lane
+constexpr int CAP = SOLVE_THREADS * PFK;  // most skyline entries per row block
 constexpr int VT_HINT = 256;              // threads per block of the vector kernels
 
 struct SolveArgs {
@@ -48,6 +50,7 @@ struct SolveArgs {
   const int *done;       // iteration mode: skip when *done != 0 (may be null)
   int nmax, ngmax;       // LDS sizing
   int nblkmax;
+  int pfk;               // prefetch loads per lane (8 or 16): blocks of <= 64*pfk entries
   const int *idx1;       // per DOF, local first column of the profile
   const int *rowbeg;     // per DOF, start of row i within the subdomain's skyline
   const double2 *fac;
@@ -188,7 +191,7 @@ hipError_t launch_skydiag_bc(int ndof, const int *dof_sub, const SubMeta *meta,
                              hipStream_t st);
 hipError_t launch_factor(const FactorArgs &a, int nwork, hipStream_t st);
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st);
-size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax);
+size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk);
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st);
 size_t smv_lds_bytes(int nbmax, int waves);
 hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, const double2 *val,

@@ -10,8 +10,9 @@
 //   5. y_G = (A x)_G (- f_G)   (MxV_TMP_AIJ on the original matrix, rows of G)
 //
 // The skyline is streamed once per sweep in contiguous row blocks of at most
-// CAP entries, two blocks in flight per wavefront (register sets pa/pb, 16 KB
-// each), each staged in LDS when its turn comes.  A sweep is a chain of
+// 64*K entries (K = 8 or 16 loads per lane; the host keeps a row-block
+// partition for each K and picks K per launch kind), two blocks in flight per
+// wavefront (register sets pa/pb), each staged in LDS when its turn comes.  A sweep is a chain of
 // dependent rows, so the kernel is organised around that chain: no
 // workgroup barriers, block-row metadata in registers (broadcast with
 // v_readlane), the triangle of each block resolved by one lane per row with
@@ -30,19 +31,21 @@ __device__ __forceinline__ int readlane_i(int x, int lane) { return __builtin_am
 struct Pf2 {
   double x, y;
 };
+template <int K>
 struct PfSet {
-  Pf2 e[PFK];
+  Pf2 e[K];
 };
 
 // Branch-free block loads: addresses past the block are clamped to its last
 // entry (same cache line, no extra traffic), so the loads carry no control
 // flow and the compiler can keep the other register set's loads in flight
 // (a counted vmcnt wait instead of a full drain).
-__device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, int e0, int e1, PfSet &p) {
+template <int K>
+__device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, int e0, int e1, PfSet<K> &p) {
   const int lane = threadIdx.x;
   const int last = e1 - 1;
 #pragma unroll
-  for (int k = 0; k < PFK; k++) {
+  for (int k = 0; k < K; k++) {
     const double2 t = fac[min(e0 + lane + k * WL, last)];
     p.e[k].x = t.x;
     p.e[k].y = t.y;
@@ -50,10 +53,11 @@ __device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, int
 }
 
 // Unconditional staging: entries past the block land in unused LDS slots.
-__device__ __forceinline__ void stage_block(double2 *stg, const PfSet &p) {
+template <int K>
+__device__ __forceinline__ void stage_block(double2 *stg, const PfSet<K> &p) {
   const int lane = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < PFK; k++) stg[lane + k * WL] = make_double2(p.e[k].x, p.e[k].y);
+  for (int k = 0; k < K; k++) stg[lane + k * WL] = make_double2(p.e[k].x, p.e[k].y);
 }
 
 // The workgroup is a single wavefront: LDS traffic of one wave is processed
@@ -68,8 +72,9 @@ struct SolveLds {
 };
 
 // forward block b: rows [i0,i1) with skyline entries [e0,e1)
+template <int K>
 __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__restrict__ fac, int nblk, int b,
-                                          PfSet &p) {
+                                          PfSet<K> &p) {
   const int lane = threadIdx.x;
   const int i0 = L.blk[b], i1 = L.blk[b + 1];
   const int e0 = L.rb[i0], e1 = L.rb[i1];
@@ -102,16 +107,17 @@ __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__re
   acc.y += __shfl_xor(acc.y, 1, 4);
   acc.x += __shfl_xor(acc.x, 2, 4);
   acc.y += __shfl_xor(acc.y, 2, 4);
-  const double2 part = make_double2(__shfl(acc.x, 4 * (lane & 15)), __shfl(acc.y, 4 * (lane & 15)));
 
-  // B: the block's own triangle, lane r owns row r
-  const int r = lane;
+  // B: the block's own triangle, row r resolved by lane 4r (the quad that
+  // summed its columns before the block already holds the sum there)
+  const int r = ra;
+  const bool own = la == 0 && r < nb;
   double2 s = cz();
   int fr = i0, rbr = 0;
-  if (r < nb) {
+  if (own) {
     fr = L.idx1[i0 + r];
     rbr = L.rb[i0 + r] - e0 - fr;
-    s = csub(L.v[i0 + r], part);
+    s = csub(L.v[i0 + r], acc);
   }
   // operands of the chain first (independent LDS loads), then the chain of
   // readlane + multiply-subtract steps
@@ -119,7 +125,7 @@ __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__re
 #pragma unroll
   for (int q = 0; q < BMAX; q++) {
     const int j = i0 + q;
-    const bool use = q < r && r < nb && j >= fr;
+    const bool use = q < r && own && j >= fr;
     const double2 t = L.stg[use ? rbr + j : 0];
     lq[q].x = use ? t.x : 0.0;
     lq[q].y = use ? t.y : 0.0;
@@ -127,16 +133,17 @@ __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__re
 #pragma unroll
   for (int q = 0; q < BMAX; q++) {
     if (q < nb) {
-      const double2 xq = readlane_c(s, q);
+      const double2 xq = readlane_c(s, 4 * q);
       if (r > q) s = cfms(s, make_double2(lq[q].x, lq[q].y), xq);
     }
   }
-  if (r < nb) L.v[i0 + r] = s;
+  if (own) L.v[i0 + r] = s;
 }
 
 // backward block b (rows descending): triangle, then columns before the block
+template <int K>
 __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__restrict__ fac, int b,
-                                          PfSet &p) {
+                                          PfSet<K> &p) {
   const int lane = threadIdx.x;
   const int i0 = L.blk[b], i1 = L.blk[b + 1];
   const int e0 = L.rb[i0], e1 = L.rb[i1];
@@ -180,6 +187,8 @@ __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__re
   int jlo = fR;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) jlo = min(jlo, __shfl_xor(jlo, o, 64));
+  // (branch-free: a column outside row r's profile reads a valid entry and
+  // multiplies it by zero, so the 16 LDS reads pipeline)
   for (int j = jlo + lane; j < i0; j += WL) {
     double2 acc = cz();
 #pragma unroll
@@ -187,14 +196,17 @@ __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__re
       if (r < nb) {
         const int f_u = readlane_i(fR, r), rb_u = readlane_i(rbR, r);
         const double2 xr = readlane_c(x, r);
-        if (j >= f_u) acc = cfma(acc, L.stg[rb_u + j], xr);
+        const bool in = j >= f_u;
+        acc = cfma(acc, L.stg[in ? rb_u + j : 0], in ? xr : cz());
       }
     }
     L.v[j] = csub(L.v[j], acc);
   }
 }
 
-__global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_solve(SolveArgs a) {
+template <int K>  // prefetch loads per lane: row blocks of at most 64*K entries
+__global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 3 : 2))) void k_solve(SolveArgs a) {
+  constexpr int CAPK = WL * K;
   if (a.mode == MODE_ITER && a.done && *a.done) return;
   int w, uk = a.unit_k;
   if (a.mode == MODE_UNIT && a.unit_list) {
@@ -210,13 +222,16 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   extern __shared__ double2 lds[];
   SolveLds L;
   L.v = lds;                          // [nmax]
-  L.stg = L.v + a.nmax;               // [CAP]
-  L.idx1 = (int *)(L.stg + CAP);      // [nmax]
+  L.stg = L.v + a.nmax;               // [CAPK]
+  L.idx1 = (int *)(L.stg + CAPK);     // [nmax]
   L.rb = L.idx1 + a.nmax;             // [nmax+1]
   L.blk = L.rb + a.nmax + 1;          // [nblkmax+1]
   const int lane = threadIdx.x;
   const int n = m.n;
   const double2 *fac = a.fac + m.fac0;
+  // row blocks of this prefetch depth (the host keeps both partitions)
+  const int nblk = K == 8 ? m.nblk_s : m.nblk;
+  const int blk0 = K == 8 ? m.blk0_s : m.blk0;
 
   // ---- 0. metadata --------------------------------------------------------------
   for (int i = lane; i < n; i += WL) {
@@ -224,7 +239,7 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     L.rb[i] = a.rowbeg[m.dof0 + i];
   }
   if (lane == 0) L.rb[n] = (int)m.nlen;
-  for (int b = lane; b <= m.nblk; b += WL) L.blk[b] = a.blk_row[m.blk0 + b];
+  for (int b = lane; b <= nblk; b += WL) L.blk[b] = a.blk_row[blk0 + b];
   __syncthreads();
 
 
@@ -269,15 +284,15 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 
   // ---- 2. forward sweep: the first two blocks go in flight once the load
   // vector's own global reads are complete ----------------------------------
-  PfSet pa, pb;
+  PfSet<K> pa, pb;
   // (the host pads every subdomain to an even number >= 2 of row blocks, so
   // the sweeps run over block pairs without conditional loads: a branch
   // around a load would make the compiler drain every load in flight)
   issue_block(fac, L.rb[L.blk[0]], L.rb[L.blk[1]], pa);
   issue_block(fac, L.rb[L.blk[1]], L.rb[L.blk[2]], pb);
-  for (int b = 0; b < m.nblk; b += 2) {
-    fwd_block(L, fac, m.nblk, b, pa);
-    fwd_block(L, fac, m.nblk, b + 1, pb);
+  for (int b = 0; b < nblk; b += 2) {
+    fwd_block(L, fac, nblk, b, pa);
+    fwd_block(L, fac, nblk, b + 1, pb);
   }
   wave_sync();
 
@@ -289,7 +304,7 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   // the last two blocks go in flight for the backward sweep: the last block
   // in pa, the one before in pb, alternating from there (the register sets
   // are always named statically, so they stay in registers)
-  const int nbl = m.nblk - 1;
+  const int nbl = nblk - 1;
   issue_block(fac, L.rb[L.blk[nbl]], L.rb[L.blk[nbl + 1]], pa);
   issue_block(fac, L.rb[L.blk[nbl - 1]], L.rb[L.blk[nbl]], pb);
   wave_sync();
@@ -330,14 +345,18 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   }
 }
 
-size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax) {
+size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk) {
   (void)ngmax;
-  return sizeof(double2) * ((size_t)nmax + CAP) + sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
+  return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk) + sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
 }
 
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
   if (nwork <= 0) return hipSuccess;
-  k_solve<<<nwork, WL, solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax), st>>>(a);
+  const size_t lds = solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk);
+  if (a.pfk == 8)
+    k_solve<8><<<nwork, WL, lds, st>>>(a);
+  else
+    k_solve<16><<<nwork, WL, lds, st>>>(a);
   return hipGetLastError();
 }
 

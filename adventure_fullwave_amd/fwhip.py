@@ -55,7 +55,8 @@ class FwhInfo(ctypes.Structure):
                 ("bytes_smv", ctypes.c_double), ("solve_buckets", ctypes.c_int),
                 ("solve_bucketed", ctypes.c_int), ("ms_solve_single", ctypes.c_double),
                 ("ms_solve_bucketed", ctypes.c_double), ("smv_waves", ctypes.c_int),
-                ("ms_smv_tune", ctypes.c_double * 3)]
+                ("ms_smv_tune", ctypes.c_double * 3), ("solve_depth", ctypes.c_int),
+                ("unit_depth", ctypes.c_int), ("ms_solve_depth", ctypes.c_double * 2)]
 
     def as_dict(self):
         out = {}

@@ -209,7 +209,9 @@ def main():
         "autotune": {"smv_waves": info["smv_waves"], "ms_smv_by_waves_2_4_8": [round(v, 4) for v in info["ms_smv_tune"]],
                      "solve_buckets": info["solve_buckets"], "solve_bucketed": info["solve_bucketed"],
                      "ms_solve_single_vs_bucketed": [round(info["ms_solve_single"], 4),
-                                                     round(info["ms_solve_bucketed"], 4)]},
+                                                     round(info["ms_solve_bucketed"], 4)],
+                     "solve_depth": info["solve_depth"], "unit_depth": info["unit_depth"],
+                     "ms_solve_by_depth_8_16": [round(v, 4) for v in info["ms_solve_depth"]]},
         "adjoint": adjoint,
         "time_to_solution_ms": round(setup["factor"] + setup.get("schur_form", 0.0) + 1000 * dt / args.steps, 1),
     }

@@ -94,6 +94,12 @@ typedef struct fwh_info {
   int    smv_waves;     /* waves per workgroup of the explicit Schur matvec,
                            picked by timing when the S_s are formed        */
   double ms_smv_tune[3];/* its timings for 2, 4, 8 waves                   */
+  int    solve_depth;   /* loads per lane in flight (8 or 16) of the
+                           LDL^T kernel in the iteration sweeps, picked by
+                           timing at load                                  */
+  int    unit_depth;    /* the same for the Schur-complement formation     */
+  double ms_solve_depth[2]; /* best batched solve time at depth 8 (0 when
+                           a profile row is wider than 512), 16            */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */
