@@ -25,7 +25,9 @@ reference's full-wave rules:
   (read_Jodefine_file_pp, fileio.c:1226-1293; make_Jo_from_definition_pp,
   fileio.c:1556-1594) on the vertices of antenna-flagged elements, with the
   two correct-Jo passes of main.c:92-98 leaving Jor = the file's Jor and
-  Jo = the file's Joi, so Joe = -(Jo + i Jor) (make_elem_em.c:291-296).
+  Jo = the file's Joi, so Joe = -(Jo + i Jor) (make_elem_em.c:291-296);
+* a volt.dat with dg > 0 replaces the current source by the delta-gap
+  voltages of its boxes on the vertices inside them (make_elem_em.c:320-396).
 """
 from __future__ import annotations
 
@@ -182,10 +184,41 @@ def in_parallelpiped(x: np.ndarray, ap: np.ndarray) -> np.ndarray:
     return np.all((t >= -tol) & (t <= 1 + tol), axis=1)
 
 
+def read_volt(path: str) -> List[dict]:
+    """Delta-gap definition ``volt.dat`` (read per element by
+    tet_MakeElement_EM_E, lib/fem/make_elem_em.c:72-96,333-347): a first
+    line ``dg``, then per gap box four lines ``x0 x1`` / ``y0 y1`` /
+    ``z0 z1`` / ``dV``.  dg = 0 (no boxes) selects the current source."""
+    lines = open(path).read().splitlines()
+    dg = int(lines[0].split()[0])
+    boxes = []
+    for i in range(dg):
+        x0, x1 = (float(t) for t in lines[1 + 4 * i].split()[:2])
+        y0, y1 = (float(t) for t in lines[2 + 4 * i].split()[:2])
+        z0, z1 = (float(t) for t in lines[3 + 4 * i].split()[:2])
+        dv = float(lines[4 + 4 * i].split()[0])
+        boxes.append({"lo": np.array([x0, y0, z0]), "hi": np.array([x1, y1, z1]), "dV": dv})
+    return boxes
+
+
+def gap_voltage(x: np.ndarray, boxes: List[dict]) -> np.ndarray:
+    """Per point the dV of the last gap box containing it (bounds included,
+    make_elem_em.c:350-361), 0 outside every box."""
+    x = np.asarray(x, np.float64).reshape(-1, 3)
+    out = np.zeros(len(x))
+    for b in boxes:
+        inside = np.all((b["lo"] <= x) & (x <= b["hi"]), axis=1)
+        out[inside] = b["dV"]
+    return out
+
+
 def build_full_wave_model(parts: List[dict], mtrl: dict, antenna_pp: Dict[int, List[dict]],
-                          order: str = "reference", name: str = "sample") -> mesh.Model:
+                          order: str = "reference", name: str = "sample",
+                          volt: List[dict] = None) -> mesh.Model:
     """Solver data model of a partitioned quadratic-tet model (full-wave
-    E method).  order="reference" keeps the reference's local numbering."""
+    E method).  order="reference" keeps the reference's local numbering.
+    volt: gap boxes of a volt.dat with dg > 0 (delta-gap source), or
+    None/empty for the current source."""
     npart = len(parts)
     # global node data
     nglob = parts[0]["nnd_all"]
@@ -291,12 +324,14 @@ def build_full_wave_model(parts: List[dict], mtrl: dict, antenna_pp: Dict[int, L
         el_off.append(el_off[-1] + len(fl))
 
     cat = np.concatenate
+    crd_cat = cat(crd_all).reshape(-1)
+    dgv = gap_voltage(crd_cat, volt) if volt else None
     return mesh.Model(
-        npart=npart, nsub=nsub, omega=float(mtrl["omega"]),
+        npart=npart, nsub=nsub, omega=float(mtrl["omega"]), dgv=dgv,
         sub_part=np.array([s[0] for s in subs], np.int32), el_off=np.array(el_off, np.int32),
         nop=cat(nop_all).astype(np.int32), eps=cat(eps_all), sigma=cat(sig_all),
         src=cat(src_all).astype(np.int32), pml=None, ap_off=np.array(ap_off, np.int32),
-        crd=cat(crd_all).reshape(-1), joe=cat(joe_all).reshape(-1), nmp=np.array(nmp, np.int32),
+        crd=crd_cat, joe=cat(joe_all).reshape(-1), nmp=np.array(nmp, np.int32),
         bc_off=np.array(bc_off, np.int32), ibc=np.zeros(0, np.int32), ib_off=np.array(ib_off, np.int32),
         ib_nfid=cat(ib_nfid_all).astype(np.int32), ib_ln=cat(ib_ln_all).astype(np.int32),
         infree=infree.astype(np.int32), outfree=outfree.astype(np.int32),
@@ -311,15 +346,16 @@ def load_npz_model(path: str) -> mesh.Model:
     kw = {k: z[k] for k in ("sub_part", "el_off", "nop", "eps", "sigma", "src", "ap_off", "crd", "joe", "nmp",
                             "bc_off", "ibc", "ib_off", "ib_nfid", "ib_ln", "infree", "outfree", "op_off",
                             "op_ibid", "ledge_global")}
-    if "mp2sn" in z.files:
-        kw["mp2sn"] = z["mp2sn"]
+    for k in ("mp2sn", "dgv"):
+        if k in z.files:
+            kw[k] = z[k]
     return mesh.Model(npart=int(z["npart"]), nsub=int(z["nsub"]), omega=float(z["omega"]), pml=None,
                       n_edges=int(z["n_edges"]), n_tets=int(z["n_tets"]), name=os.path.basename(path), **kw)
 
 
 def load_reference_model(model_dir: str) -> mesh.Model:
     """The full-wave model of a reference MODEL directory (model/, mtrl.dat,
-    antenna definitions)."""
+    antenna definitions, volt.dat)."""
     files = sorted(f for f in os.listdir(os.path.join(model_dir, "model")) if f.startswith("advhddm_in_"))
     parts = [read_partition(os.path.join(model_dir, "model", f))
              for f in sorted(files, key=lambda s: int(s.split("_")[-1].split(".")[0]))]
@@ -327,7 +363,9 @@ def load_reference_model(model_dir: str) -> mesh.Model:
     pp = {}
     for flag, (mode, fname) in mtrl["antenna"].items():
         pp[flag] = read_parallelpiped(os.path.join(model_dir, fname))
-    return build_full_wave_model(parts, mtrl, pp)
+    vpath = os.path.join(model_dir, "volt.dat")
+    volt = read_volt(vpath) if os.path.exists(vpath) else None
+    return build_full_wave_model(parts, mtrl, pp, volt=volt)
 
 
 # ---------------------------------------------------------------------------

@@ -640,6 +640,11 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
     if ((rc = c->upload(&d_pml, pml))) return rc;
   }
   c->pml = d_pml;
+  double *d_dgv = nullptr;
+  if (m->dgv) {
+    std::vector<double> dgv(m->dgv, m->dgv + nap);
+    if ((rc = c->upload(&d_dgv, dgv))) return rc;
+  }
   if ((rc = c->alloc(&d_elem, 27 * (size_t)nel)) || (rc = c->alloc(&d_bad, 1))) return rc;
   HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(int), st));
   if ((rc = c->upload(&d_asm_ptr, asm_ptr)) || (rc = c->upload(&d_asm_src, asm_src)) ||
@@ -663,6 +668,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   aa.el_vtx = d_el_vtx;
   aa.crd = d_crd;
   aa.joe = d_joe;
+  aa.dgv = d_dgv;
   aa.eps = d_eps;
   aa.sigma = d_sig;
   aa.pml = d_pml;

@@ -111,6 +111,7 @@ struct AsmArgs {
   const int *el_vtx;     // [nel*4] global vertex index
   const double *crd;     // [nv*3]
   const double *joe;     // [nv*6]
+  const double *dgv;     // [nv] delta-gap voltage per vertex, or null (current source)
   const double *eps, *sigma, *pml;
   const int *src;
   double omega;

@@ -200,9 +200,28 @@ __global__ __launch_bounds__(128) void k_element(AsmArgs a) {
       out[p] = t;
     }
   }
-  // current source: exact integral of N_i . sum_j lambda_j J_j
-  // (tetNedelec_JoA_Complex, lib/fem/tetrahedral.c:794-858)
-  if (a.src[el]) {
+  if (a.dgv) {
+    // delta-gap source (make_elem_em.c:320-396): vertex potentials i dV_j,
+    // be_k = sigma omega sum_j (grad lambda_j, N_k) i dV_j with
+    // (grad lambda_j, N_k) = ls_k F_k . G_j / (6D) (tetNedelec_gradside,
+    // tetrahedral.c:645-661); the current source is not used in this mode
+    double G[4][3];
+    tet_grads(v, G);
+    double dv[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) dv[i] = a.dgv[a.el_vtx[4 * el + i]];
+    const double dd6 = 1.0 / (6.0 * D), so = sig * a.omega;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const double fx = ls[k] * F[k][0], fy = ls[k] * F[k][1], fz = ls[k] * F[k][2];
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) acc += dv[j] * (dd6 * (fx * G[j][0] + fy * G[j][1] + fz * G[j][2]));
+      out[21 + k] = make_double2(0.0, acc * so);
+    }
+  } else if (a.src[el]) {
+    // current source: exact integral of N_i . sum_j lambda_j J_j
+    // (tetNedelec_JoA_Complex, lib/fem/tetrahedral.c:794-858)
     double G[4][3];
     tet_grads(v, G);
     double2 J[4][3];

@@ -127,6 +127,7 @@ class Model:
     outfree: np.ndarray
     op_off: np.ndarray
     op_ibid: np.ndarray
+    dgv: Optional[np.ndarray] = None     # [nap] delta-gap voltage per local vertex, or None
     # bookkeeping (not part of the C struct)
     ledge_global: np.ndarray = None      # [sum nmp] global edge id of each local edge
     mp2sn: np.ndarray = None             # [sum nmp] subdomain node id of each mid-node
@@ -154,13 +155,13 @@ class FwhMeshDesc(ctypes.Structure):
         ("joe", ctypes.c_void_p), ("nmp", ctypes.c_void_p), ("bc_off", ctypes.c_void_p),
         ("ibc", ctypes.c_void_p), ("ib_off", ctypes.c_void_p), ("ib_nfid", ctypes.c_void_p),
         ("ib_ln", ctypes.c_void_p), ("infree", ctypes.c_void_p), ("outfree", ctypes.c_void_p),
-        ("op_off", ctypes.c_void_p), ("op_ibid", ctypes.c_void_p),
+        ("op_off", ctypes.c_void_p), ("op_ibid", ctypes.c_void_p), ("dgv", ctypes.c_void_p),
     ]
 
 
 _INT_FIELDS = ("sub_part", "el_off", "nop", "src", "ap_off", "nmp", "bc_off", "ibc", "ib_off",
                "ib_nfid", "ib_ln", "infree", "outfree", "op_off", "op_ibid")
-_DBL_FIELDS = ("eps", "sigma", "pml", "crd", "joe")
+_DBL_FIELDS = ("eps", "sigma", "pml", "crd", "joe", "dgv")
 
 
 def make_desc(m: Model) -> FwhMeshDesc:

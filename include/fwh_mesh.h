@@ -34,6 +34,11 @@
  * pml, when not NULL, holds per element the complex coordinate stretch
  * factors (sx, sy, sz) as 6 doubles (re, im interleaved); stretch 1 is the
  * reference element.
+ * dgv, when not NULL, selects the delta-gap source of a volt.dat with dg > 0
+ * (lib/fem/make_elem_em.c:320-396): per local vertex the gap voltage dV of
+ * the last gap box containing the vertex, 0 outside every box.  The element
+ * load is then sigma*omega * sum_j (grad lambda_j, N_i) * i dV_j and the
+ * current source (src/joe) is not used, as in the reference.
  */
 #ifndef FWH_MESH_H
 #define FWH_MESH_H
@@ -66,6 +71,7 @@ typedef struct fwh_mesh_desc {
   const int *outfree;     /* [npart] non-owned interface DOFs               */
   const int *op_off;      /* [npart*npart+1] offsets into op_ibid           */
   const int *op_ibid;     /* shared-DOF lists per part pair                 */
+  const double *dgv;      /* [nap] or NULL: delta-gap voltage per vertex    */
 } fwh_mesh_desc;
 
 #ifdef __cplusplus

@@ -96,18 +96,22 @@ static const int EDGE[6][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2},
 
 static const double NU = 7.957747e+05; /* fixed 1/mu0, make_elem_em.c:37 */
 
-int fwo_element(const double *v, double eps, double sigma, double omega,
-                const double *pml, int src, const double *joe, double *ae72,
-                double *be12) {
-  /* edge vectors from vertex 0 */
+/* Geometry shared by the element routines: G[i] = D grad(lambda_i) (the
+ * reference's simple_b/c/d, tetrahedral.c:122-184), D = 6V
+ * (tet_Volume6, :44-61), centroid g (tet_Center, :80-86), and per edge k
+ * the signed length ls_k = l_k si_k (tet_SideLength + tetNedelec_Direction,
+ * :27-40,225-263), e_k = v_d - v_c (tetNedelec_mk, :267-281) and
+ * w_k = v_c x v_d (tetNedelec_km_mk, :305-319).  Returns <0 for an
+ * inverted or degenerate element.                                         */
+static int tet_frame(const double *v, double G[4][3], double *Dout, double g[3], double ls[6],
+                     double e[6][3], double w[6][3]) {
   double r1[3], r2[3], r3[3];
   for (int a = 0; a < 3; a++) {
     r1[a] = v[3 + a] - v[a];
     r2[a] = v[6 + a] - v[a];
     r3[a] = v[9 + a] - v[a];
   }
-  /* G[i] = D * grad(lambda_i): columns of adj(J), J rows r1, r2, r3      */
-  double G[4][3];
+  /* columns of adj(J), J rows r1, r2, r3 */
   G[1][0] = r2[1] * r3[2] - r2[2] * r3[1];
   G[1][1] = r2[2] * r3[0] - r2[0] * r3[2];
   G[1][2] = r2[0] * r3[1] - r2[1] * r3[0];
@@ -118,33 +122,40 @@ int fwo_element(const double *v, double eps, double sigma, double omega,
   G[3][1] = r1[2] * r2[0] - r1[0] * r2[2];
   G[3][2] = r1[0] * r2[1] - r1[1] * r2[0];
   for (int a = 0; a < 3; a++) G[0][a] = -(G[1][a] + G[2][a] + G[3][a]);
-  /* six times the volume (tet_Volume6, tetrahedral.c:44-61)              */
   double D = r1[0] * G[1][0] + r1[1] * G[1][1] + r1[2] * G[1][2];
   if (!(D > 0.0)) return -1;
-
-  double g[3]; /* centroid, tet_Center tetrahedral.c:80-86 */
+  *Dout = D;
   for (int a = 0; a < 3; a++) g[a] = (v[a] + v[3 + a] + v[6 + a] + v[9 + a]) * 0.25;
-
-  double ls[6], e[6][3], F[6][3];
   for (int k = 0; k < 6; k++) {
     const double *va = v + 3 * EDGE[k][0], *vb = v + 3 * EDGE[k][1];
     const double *vc = v + 3 * EDGE[k][2], *vd = v + 3 * EDGE[k][3];
     double dl[3] = {vb[0] - va[0], vb[1] - va[1], vb[2] - va[2]};
     double l = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
-    /* orientation from coordinate order x, then y, then z
-     * (tetNedelec_Direction, tetrahedral.c:225-263)                        */
     int si = 0;
     for (int a = 0; a < 3 && si == 0; a++) si = dl[a] > 0.0 ? 1 : (dl[a] < 0.0 ? -1 : 0);
     if (si == 0) return -2;
     ls[k] = l * (double)si;
     for (int a = 0; a < 3; a++) e[k][a] = vd[a] - vc[a];
-    /* N_k(r) = (ls_k / D) (w_k + e_k x r) with w_k = v_c x v_d; F_k is the
-     * bracket at the centroid (tetNedelec_Fxyz, tetrahedral.c:323-340)     */
-    double w[3] = {vc[1] * vd[2] - vc[2] * vd[1], vc[2] * vd[0] - vc[0] * vd[2],
-                   vc[0] * vd[1] - vc[1] * vd[0]};
-    F[k][0] = w[0] + e[k][1] * g[2] - e[k][2] * g[1];
-    F[k][1] = w[1] + e[k][2] * g[0] - e[k][0] * g[2];
-    F[k][2] = w[2] + e[k][0] * g[1] - e[k][1] * g[0];
+    w[k][0] = vc[1] * vd[2] - vc[2] * vd[1];
+    w[k][1] = vc[2] * vd[0] - vc[0] * vd[2];
+    w[k][2] = vc[0] * vd[1] - vc[1] * vd[0];
+  }
+  return 0;
+}
+
+int fwo_element(const double *v, double eps, double sigma, double omega,
+                const double *pml, int src, const double *joe, double *ae72,
+                double *be12) {
+  double G[4][3], D, g[3], ls[6], e[6][3], w[6][3];
+  int rc = tet_frame(v, G, &D, g, ls, e, w);
+  if (rc != 0) return rc;
+  /* N_k(r) = (ls_k / D) (w_k + e_k x r); F_k is the bracket at the
+   * centroid (tetNedelec_Fxyz, tetrahedral.c:323-340, without ls)        */
+  double F[6][3];
+  for (int k = 0; k < 6; k++) {
+    F[k][0] = w[k][0] + e[k][1] * g[2] - e[k][2] * g[1];
+    F[k][1] = w[k][1] + e[k][2] * g[0] - e[k][0] * g[2];
+    F[k][2] = w[k][2] + e[k][0] * g[1] - e[k][1] * g[0];
   }
   /* second moments about the centroid: C = (1/V) int rho rho^T dV
    * = (1/20) sum_i (v_i - g)(v_i - g)^T                                    */
@@ -231,6 +242,35 @@ int fwo_element(const double *v, double eps, double sigma, double omega,
       /* be += (JoA - gradside * Ihe) with Ihe = 0 (make_elem_em.c:299-315) */
       be[i] = cadd(be[i], csub(acc, C0));
     }
+  }
+  return 0;
+}
+
+/* Delta-gap load of one element (make_elem_em.c:320-396, volt.dat dg > 0):
+ * DGe_j = i dV_j per vertex, gs_ij = (grad lambda_j, N_i) =
+ * (1/(6D)) (Fx_i b_j + Fy_i c_j + Fz_i d_j) with Fx_i = ls_i (yz_i - zi_i gy +
+ * yi_i gz) etc. (tetNedelec_gradside / _Fxyz, tetrahedral.c:323-340,
+ * 645-661), be_i = (sum_j DGe_j gs_ij) * (sigma omega) in the reference's
+ * order.  dv4: the 4 vertex voltages (0 outside every gap box).          */
+int fwo_dg_load(const double *v, double sigma, double omega, const double *dv4, double *be12) {
+  double G[4][3], D, g[3], ls[6], e[6][3], w[6][3];
+  int rc = tet_frame(v, G, &D, g, ls, e, w);
+  if (rc != 0) return rc;
+  const double dd = 1.0 / (6.0 * D);
+  const double so = sigma * omega;
+  fwo_c *be = (fwo_c *)be12;
+  for (int i = 0; i < 6; i++) {
+    const double Fx = ls[i] * (w[i][0] - e[i][2] * g[1] + e[i][1] * g[2]);
+    const double Fy = ls[i] * (w[i][1] - e[i][0] * g[2] + e[i][2] * g[0]);
+    const double Fz = ls[i] * (w[i][2] - e[i][1] * g[0] + e[i][0] * g[1]);
+    fwo_c acc = C0;
+    for (int j = 0; j < 4; j++) {
+      const double gs = dd * (Fx * G[j][0] + Fy * G[j][1] + Fz * G[j][2]);
+      acc.re += 0.0 * gs;
+      acc.im += dv4[j] * gs;
+    }
+    be[i].re = acc.re * so;
+    be[i].im = acc.im * so;
   }
   return 0;
 }
@@ -714,8 +754,15 @@ static int assemble_sub(const fwh_mesh_desc *m, int s, tmp_t *A) {
       memset(J, 0, sizeof(J));
     double ae[72], be[12];
     if (fwo_element(xyz, m->eps[el], m->sigma[el], m->omega,
-                    m->pml ? m->pml + 6 * (size_t)el : NULL, src, J, ae, be) != 0)
+                    m->pml ? m->pml + 6 * (size_t)el : NULL, m->dgv ? 0 : src, J, ae, be) != 0)
       return -1;
+    if (m->dgv) {
+      /* delta-gap mode: the gap load replaces the current source
+       * (make_elem_em.c:274-279 vs :320-396)                             */
+      const double *dv = m->dgv + (size_t)m->ap_off[s];
+      double d4[4] = {dv[ne[0]], dv[ne[1]], dv[ne[2]], dv[ne[3]]};
+      if (fwo_dg_load(xyz, m->sigma[el], m->omega, d4, be) != 0) return -1;
+    }
     int ind[6];
     for (int k = 0; k < 6; k++) ind[k] = ne[4 + k];
     tmp_set_ae(A, ind, (fwo_c *)ae);

@@ -35,6 +35,11 @@ int fwo_element(const double *xyz12, double eps, double sigma, double omega,
                 const double *pml, int src, const double *joe24,
                 double *ae72, double *be12);
 
+/* Delta-gap load (make_elem_em.c:320-396): be[6] of one element with the
+ * 4 vertex gap voltages dv4 (0 outside every gap box).                    */
+int fwo_dg_load(const double *xyz12, double sigma, double omega, const double *dv4,
+                double *be12);
+
 /* ---- element post-processing ------------------------------------------- */
 /* HDDM_post_em_E (main/fullwave/fullwave.c:640-763) for one element:
  * xe12 = solution at the 6 edges (re,im), nu = 1/mu of the element's

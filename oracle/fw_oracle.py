@@ -43,6 +43,8 @@ def lib():
         L.fwo_element.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, P,
                                   ctypes.c_int, P, P, P]
         L.fwo_post_element.argtypes = [P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P]
+        L.fwo_dg_load.restype = ctypes.c_int
+        L.fwo_dg_load.argtypes = [P, ctypes.c_double, ctypes.c_double, P, P]
         L.fwo_sky_factor.argtypes = [ctypes.c_int, P, P, P]
         L.fwo_sky_solve.argtypes = [ctypes.c_int, P, P, P, P]
         L.fwo_model_build.restype = P
@@ -78,6 +80,17 @@ def element(xyz, eps, sigma, omega, pml=None, src=0, joe=None):
     if rc != 0:
         raise ValueError(f"fwo_element failed ({rc})")
     return ae.view(np.complex128).reshape(6, 6), be.view(np.complex128)
+
+
+def dg_load(xyz, sigma, omega, dv):
+    """Delta-gap load (6 complex) of one tetrahedron with vertex gap voltages dv."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(12)
+    dv = np.ascontiguousarray(dv, dtype=np.float64).reshape(4)
+    be = np.zeros(12)
+    rc = lib().fwo_dg_load(_p(xyz), float(sigma), float(omega), _p(dv), _p(be))
+    if rc != 0:
+        raise ValueError(f"fwo_dg_load failed ({rc})")
+    return be.view(np.complex128)
 
 
 def post_element(xyz, xe, nu, sigma, omega):

@@ -46,3 +46,30 @@ def dense_lower(rowptr, col, val, n):
             if col[k] != i:
                 A[col[k], i] += val[k]
     return A
+
+
+def with_delta_gap(model, cells=8, sigma_gap=0.8):
+    """The model with a delta-gap source (volt.dat dg = 2) at the centre of
+    its unit box: box 1 holds the 3 vertices (c, c, c-h..c+h) at dV = 1,
+    box 2 the upper two at dV = 2 (the last box containing a vertex wins);
+    elements touching a gap vertex get conductivity sigma_gap (the gap load
+    is sigma*omega*...)."""
+    import dataclasses
+    from adventure_fullwave_amd import advio
+    h = 1.0 / cells
+    c = 0.5
+    boxes = [{"lo": np.array([c - 0.6 * h, c - 0.6 * h, c - 1.1 * h]),
+              "hi": np.array([c + 0.6 * h, c + 0.6 * h, c + 1.1 * h]), "dV": 1.0},
+             {"lo": np.array([c - 0.6 * h, c - 0.6 * h, c - 0.1 * h]),
+              "hi": np.array([c + 0.6 * h, c + 0.6 * h, c + 1.1 * h]), "dV": 2.0}]
+    crd = np.asarray(model.crd).reshape(-1, 3)
+    dgv = advio.gap_voltage(crd, boxes)
+    nop = np.asarray(model.nop).reshape(-1, 10)
+    vtx = np.zeros((nop.shape[0], 4), np.int64)
+    for s in range(model.nsub):
+        e0, e1 = model.el_off[s], model.el_off[s + 1]
+        vtx[e0:e1] = model.ap_off[s] + nop[e0:e1, :4]
+    touch = (dgv[vtx] != 0).any(axis=1)
+    sigma = np.array(model.sigma, np.float64)
+    sigma[touch] = sigma_gap
+    return dataclasses.replace(model, dgv=dgv, sigma=sigma, name=model.name + "+gap")
