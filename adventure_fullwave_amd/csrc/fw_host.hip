@@ -77,6 +77,12 @@ struct fwh_ctx {
   long long ndof = 0, nslots = 0, nnz = 0, nlen = 0, nel = 0, ncpl = 0, ngrow = 0;
   std::vector<SubMeta> meta_h;
   std::vector<int> order_h;
+  // systems of a batch (fwh_load_batch): shared structure, values
+  // concatenated system-major with strides ss; sel = the system the
+  // per-system calls address
+  int nsys = 1, sel = 0;
+  fw::SysStride ss{};
+  std::vector<double> omega_h;
 
   // device model
   SubMeta *meta = nullptr;
@@ -105,11 +111,10 @@ struct fwh_ctx {
   double *hist = nullptr;
   int hist_cap = 0;
 
-  // element data kept for post-processing
+  // element data kept for post-processing (sig: nsys*nel, pml: nsys*6*nel)
   int *el_vtx = nullptr, *el_dof = nullptr;
   double *crd = nullptr, *sig = nullptr, *post_nu = nullptr, *pml = nullptr, *grad = nullptr;
   double2 *post_out = nullptr, *f_asm = nullptr, *gx = nullptr;
-  double omega = 0.0;
 
   // explicit Schur complements (formed on first use)
   double2 *smat = nullptr;
@@ -138,7 +143,8 @@ struct fwh_ctx {
   int graph_n = 0;
   int graph_schur = 0;
 
-  fwh_stats stats{};
+  fwh_stats stats{};                // timings of the batch
+  std::vector<fwh_stats> sys_stats; // per system
   fwh_info info{};
 
   template <class T>
@@ -199,8 +205,12 @@ struct fwh_ctx {
   }
   // fused: the interface values are the next search direction z + beta p of
   // the running interface iteration (formed by the kernels from Z, P, beta)
-  fw::SolveArgs solve_args(int mode, bool check_done, bool fused = false) const {
+  // only >= 0: that system alone, else every system of the batch
+  fw::SolveArgs solve_args(int mode, bool check_done, bool fused = false, int only = -1) const {
     fw::SolveArgs a{};
+    a.sys0 = only >= 0 ? only : 0;
+    a.nsys = only >= 0 ? 1 : nsys;
+    a.ss = ss;
     a.meta = meta;
     a.order = order;
     a.base = 0;
@@ -237,8 +247,11 @@ struct fwh_ctx {
     a.sm_off = sm_off;
     return a;
   }
-  fw::SmvArgs smv_args(bool check_done, bool fused) const {
+  fw::SmvArgs smv_args(bool check_done, bool fused, int only = -1) const {
     fw::SmvArgs a{};
+    a.sys0 = only >= 0 ? only : 0;
+    a.nsys = only >= 0 ? 1 : nsys;
+    a.ss = ss;
     a.meta = meta;
     a.order = order;
     a.base = 0;
@@ -254,8 +267,11 @@ struct fwh_ctx {
     a.yslot = yslot;
     return a;
   }
-  fw::VecArgs vec_args() const {
+  fw::VecArgs vec_args(int only = -1) const {
     fw::VecArgs v{};
+    v.sys0 = only >= 0 ? only : 0;
+    v.nsys = only >= 0 ? 1 : nsys;
+    v.ss = ss;
     v.ng = ng;
     v.gq_ptr = gq_ptr;
     v.gq_slot = gq_slot;
@@ -275,8 +291,8 @@ struct fwh_ctx {
   }
   // all subdomains: one launch (batched, size-ordered) or one launch per
   // subdomain in the reference's loop order (plain)
-  hipError_t run_solve(int mode, bool plain, bool check_done, bool fused = false) {
-    fw::SolveArgs a = solve_args(mode, check_done, fused);
+  hipError_t run_solve(int mode, bool plain, bool check_done, bool fused = false, int only = -1) {
+    fw::SolveArgs a = solve_args(mode, check_done, fused, only);
     if (!plain) return launch_solve_all(a, use_buckets);
     a.order = nullptr;
     for (int s = 0; s < nsub; s++) {
@@ -291,9 +307,9 @@ struct fwh_ctx {
   }
   // the Schur step of an interface iteration (y_slot = S_s p_s): implicit
   // (Dirichlet solves through the factors) or explicit (formed S_s)
-  hipError_t run_schur(bool expl, bool plain, bool check_done, bool fused) {
-    if (!expl) return run_solve(fw::MODE_ITER, plain, check_done, fused);
-    fw::SmvArgs a = smv_args(check_done, fused);
+  hipError_t run_schur(bool expl, bool plain, bool check_done, bool fused, int only = -1) {
+    if (!expl) return run_solve(fw::MODE_ITER, plain, check_done, fused, only);
+    fw::SmvArgs a = smv_args(check_done, fused, only);
     if (!plain) return fw::launch_smv(a, nsub, st);
     a.order = nullptr;
     for (int s = 0; s < nsub; s++) {
@@ -344,10 +360,49 @@ struct Key {
   bool operator<(const Key &o) const { return key < o.key || (key == o.key && contrib < o.contrib); }
 };
 
-int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
+template <class T>
+bool same_array(const T *a, const T *b, size_t n) {
+  if (!a || !b) return a == b;
+  return n == 0 || memcmp(a, b, n * sizeof(T)) == 0;
+}
+
+// descs[0] gives the structure; descs[k] must share it (same mesh and
+// decomposition) and may differ in omega, materials, sources and PML
+int check_batch(const fwh_mesh_desc *d, int nsys) {
+  const fwh_mesh_desc &m = d[0];
+  if (m.nsub <= 0 || m.npart <= 0) return fail(FWH_E_ARG, "empty model (nsub=%d npart=%d)", m.nsub, m.npart);
+  const int nsub = m.nsub, npart = m.npart;
+  const size_t nel = (size_t)m.el_off[nsub], nap = (size_t)m.ap_off[nsub];
+  const size_t nbc = (size_t)m.bc_off[nsub], nib = (size_t)m.ib_off[nsub];
+  const size_t npp = (size_t)npart * npart;
+  for (int k = 0; k < nsys; k++) {
+    const fwh_mesh_desc &q = d[k];
+    if (!(q.omega > 0.0)) return fail(FWH_E_ARG, "system %d: omega must be positive", k);
+    if (!q.eps || !q.sigma) return fail(FWH_E_ARG, "system %d: eps and sigma are required", k);
+    if ((q.pml != nullptr) != (m.pml != nullptr)) return fail(FWH_E_ARG, "pml must be given for every system or none");
+    if (k == 0) continue;
+    if (q.nsub != nsub || q.npart != npart || !same_array(q.sub_part, m.sub_part, nsub) ||
+        !same_array(q.el_off, m.el_off, nsub + 1) || !same_array(q.nop, m.nop, 10 * nel) ||
+        !same_array(q.ap_off, m.ap_off, nsub + 1) || !same_array(q.crd, m.crd, 3 * nap) ||
+        !same_array(q.nmp, m.nmp, nsub) || !same_array(q.bc_off, m.bc_off, nsub + 1) ||
+        !same_array(q.ibc, m.ibc, nbc) || !same_array(q.ib_off, m.ib_off, nsub + 1) ||
+        !same_array(q.ib_nfid, m.ib_nfid, nib) || !same_array(q.ib_ln, m.ib_ln, nib) ||
+        !same_array(q.infree, m.infree, npart) || !same_array(q.outfree, m.outfree, npart) ||
+        !same_array(q.op_off, m.op_off, npp + 1) || !same_array(q.op_ibid, m.op_ibid, (size_t)m.op_off[npp]))
+      return fail(FWH_E_ARG, "system %d: mesh or decomposition differs from system 0", k);
+  }
+  return 0;
+}
+
+int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
+  int rc = check_batch(descs, nsys);
+  if (rc) return rc;
+  const fwh_mesh_desc *m = descs;  // structure
   const int nsub = m->nsub, npart = m->npart;
-  if (nsub <= 0 || npart <= 0) return fail(FWH_E_ARG, "empty model (nsub=%d npart=%d)", nsub, npart);
-  if (!(m->omega > 0.0)) return fail(FWH_E_ARG, "omega must be positive");
+  c->nsys = nsys;
+  c->sel = 0;
+  c->omega_h.resize(nsys);
+  for (int k = 0; k < nsys; k++) c->omega_h[k] = descs[k].omega;
   for (int s = 0; s < nsub; s++) {
     if (m->sub_part[s] < 0 || m->sub_part[s] >= npart) return fail(FWH_E_ARG, "subdomain %d: bad part", s);
     if (s > 0 && m->sub_part[s] < m->sub_part[s - 1]) return fail(FWH_E_ARG, "subdomains must be part-major");
@@ -617,85 +672,95 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
     if (bcflag[d] == 1) diag_slot[meta[dof_sub[d]].g0 + kidx[d]] = diag_ent[d];
 
   // ---- uploads + assembly on the device -------------------------------------------
+  // index arrays once; element values, loads and matrices per system
   hipStream_t st = c->st;
-  int rc = 0;
+  rc = 0;
   int *d_el_vtx, *d_src, *d_asm_ptr, *d_asm_src, *d_f_ptr, *d_f_src, *d_cp_ent, *d_gr_ent, *d_diag_slot, *d_bad;
-  double *d_crd, *d_joe, *d_eps, *d_sig, *d_pml = nullptr;
+  double *d_crd, *d_joe, *d_eps, *d_dgv;
   double2 *d_elem;
-  std::vector<double> crd(m->crd, m->crd + 3 * nap), joe(6 * nap, 0.0);
-  if (m->joe) std::copy(m->joe, m->joe + 6 * nap, joe.begin());
-  std::vector<double> eps(m->eps, m->eps + nel), sig(m->sigma, m->sigma + nel);
-  std::vector<int> src(nel, 0);
-  if (m->src) std::copy(m->src, m->src + nel, src.begin());
-  if ((rc = c->upload(&d_el_vtx, el_vtx)) || (rc = c->upload(&d_crd, crd)) || (rc = c->upload(&d_joe, joe)) ||
-      (rc = c->upload(&d_eps, eps)) || (rc = c->upload(&d_sig, sig)) || (rc = c->upload(&d_src, src)) ||
-      (rc = c->upload(&c->el_dof, el_dof)))
+  int *d_idx1k;
+  std::vector<double> crd(m->crd, m->crd + 3 * nap);
+  if ((rc = c->upload(&d_el_vtx, el_vtx)) || (rc = c->upload(&d_crd, crd)) || (rc = c->upload(&c->el_dof, el_dof)) ||
+      (rc = c->alloc(&d_joe, 6 * (size_t)nap)) || (rc = c->alloc(&d_eps, nel)) || (rc = c->alloc(&d_src, nel)) ||
+      (rc = c->alloc(&d_dgv, nap)) || (rc = c->alloc(&c->sig, (size_t)nsys * nel)))
     return rc;
   c->el_vtx = d_el_vtx;
   c->crd = d_crd;
-  c->sig = d_sig;
-  c->omega = m->omega;
-  if (m->pml) {
-    std::vector<double> pml(m->pml, m->pml + 6 * nel);
-    if ((rc = c->upload(&d_pml, pml))) return rc;
-  }
-  c->pml = d_pml;
-  double *d_dgv = nullptr;
-  if (m->dgv) {
-    std::vector<double> dgv(m->dgv, m->dgv + nap);
-    if ((rc = c->upload(&d_dgv, dgv))) return rc;
-  }
+  c->pml = nullptr;
+  if (m->pml && (rc = c->alloc(&c->pml, (size_t)nsys * 6 * nel))) return rc;
+  const long long ncpl = (long long)cp_k.size(), ngrow = (long long)gr_col.size();
   if ((rc = c->alloc(&d_elem, 27 * (size_t)nel)) || (rc = c->alloc(&d_bad, 1))) return rc;
   HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(int), st));
   if ((rc = c->upload(&d_asm_ptr, asm_ptr)) || (rc = c->upload(&d_asm_src, asm_src)) ||
       (rc = c->upload(&d_f_ptr, f_ptr)) || (rc = c->upload(&d_f_src, f_src)) ||
       (rc = c->upload(&c->rowptr, rowptr)) || (rc = c->upload(&c->col, col)) ||
-      (rc = c->upload(&c->ent_row, ent_row)) || (rc = c->alloc(&c->aval, nnz)) || (rc = c->alloc(&c->f, ndof)) ||
+      (rc = c->upload(&c->ent_row, ent_row)) || (rc = c->alloc(&c->aval, (size_t)nsys * nnz)) ||
+      (rc = c->alloc(&c->f, (size_t)nsys * ndof)) ||
       (rc = c->upload(&c->bcflag, bcflag)) || (rc = c->upload(&c->kidx, kidx)) ||
       (rc = c->upload(&c->dof_sub, dof_sub)) || (rc = c->upload(&c->cp_ptr, cp_ptr)) ||
       (rc = c->upload(&c->cp_k, cp_k)) || (rc = c->upload(&d_cp_ent, cp_ent)) ||
-      (rc = c->alloc(&c->cp_val, cp_k.size())) || (rc = c->upload(&c->gr_ptr, gr_ptr)) ||
+      (rc = c->alloc(&c->cp_val, (size_t)nsys * ncpl)) || (rc = c->upload(&c->gr_ptr, gr_ptr)) ||
       (rc = c->upload(&c->gr_col, gr_col)) || (rc = c->upload(&d_gr_ent, gr_ent)) ||
-      (rc = c->alloc(&c->gr_val, gr_col.size())) || (rc = c->upload(&c->g_nfid, g_nfid)) ||
+      (rc = c->alloc(&c->gr_val, (size_t)nsys * ngrow)) || (rc = c->upload(&c->g_nfid, g_nfid)) ||
       (rc = c->upload(&c->g_gid, g_gid)) || (rc = c->upload(&c->gq_ptr, gq_ptr)) ||
-      (rc = c->upload(&c->gq_slot, gq_slot)) || (rc = c->upload(&d_diag_slot, diag_slot)))
+      (rc = c->upload(&c->gq_slot, gq_slot)) || (rc = c->upload(&d_diag_slot, diag_slot)) ||
+      (rc = c->alloc(&c->pcslot, (size_t)nsys * nslots)) || (rc = c->alloc(&c->meta, nsub)) ||
+      (rc = c->alloc(&c->idx1, ndof)) || (rc = c->alloc(&d_idx1k, ndof)))
     return rc;
+  HIPCHK(hipMemcpyAsync(c->meta, meta.data(), nsub * sizeof(SubMeta), hipMemcpyHostToDevice, st));
 
   HIPCHK(hipStreamSynchronize(st));
   double t0 = now_ms();
-  fw::AsmArgs aa{};
-  aa.nel = (int)nel;
-  aa.el_vtx = d_el_vtx;
-  aa.crd = d_crd;
-  aa.joe = d_joe;
-  aa.dgv = d_dgv;
-  aa.eps = d_eps;
-  aa.sigma = d_sig;
-  aa.pml = d_pml;
-  aa.src = d_src;
-  aa.omega = m->omega;
-  aa.elem_out = d_elem;
-  aa.bad = d_bad;
-  HIPCHK(fw::launch_element(aa, st));
-  HIPCHK(fw::launch_gather_sum((int)nnz, d_asm_ptr, d_asm_src, d_elem, c->aval, st));
-  HIPCHK(fw::launch_gather_sum((int)ndof, d_f_ptr, d_f_src, d_elem, c->f, st));
-  HIPCHK(fw::launch_copy_idx((int)cp_k.size(), d_cp_ent, c->aval, c->cp_val, st));
-  HIPCHK(fw::launch_copy_idx((int)gr_col.size(), d_gr_ent, c->aval, c->gr_val, st));
-  if ((rc = c->alloc(&c->pcslot, nslots))) return rc;
-  HIPCHK(fw::launch_pcslot((int)nslots, d_diag_slot, nullptr, c->aval, c->pcslot, st));
+  // Dirichlet profile: the first column of each row that is nonzero in any
+  // system (with one system, the reference's profile; a column that is zero
+  // in one system only enters its profile as exact zeros, which leave its
+  // factor and solves unchanged)
+  std::vector<int> idx1(ndof), idx1k(ndof), rowbeg(ndof), blk_row;
+  for (int k = 0; k < nsys; k++) {
+    const fwh_mesh_desc &q = descs[k];
+    std::vector<double> joe(6 * nap, 0.0), dgv;
+    if (q.joe) std::copy(q.joe, q.joe + 6 * nap, joe.begin());
+    std::vector<int> src(nel, 0);
+    if (q.src) std::copy(q.src, q.src + nel, src.begin());
+    HIPCHK(hipMemcpyAsync(d_joe, joe.data(), joe.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_eps, q.eps, nel * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->sig + k * nel, q.sigma, nel * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_src, src.data(), nel * sizeof(int), hipMemcpyHostToDevice, st));
+    if (q.pml) HIPCHK(hipMemcpyAsync(c->pml + 6 * k * nel, q.pml, 6 * nel * sizeof(double), hipMemcpyHostToDevice, st));
+    if (q.dgv) HIPCHK(hipMemcpyAsync(d_dgv, q.dgv, nap * sizeof(double), hipMemcpyHostToDevice, st));
+    double2 *aval = c->aval + k * nnz;
+    fw::AsmArgs aa{};
+    aa.nel = (int)nel;
+    aa.el_vtx = d_el_vtx;
+    aa.crd = d_crd;
+    aa.joe = d_joe;
+    aa.dgv = q.dgv ? d_dgv : nullptr;
+    aa.eps = d_eps;
+    aa.sigma = c->sig + k * nel;
+    aa.pml = q.pml ? c->pml + 6 * k * nel : nullptr;
+    aa.src = d_src;
+    aa.omega = q.omega;
+    aa.elem_out = d_elem;
+    aa.bad = d_bad;
+    HIPCHK(fw::launch_element(aa, st));
+    HIPCHK(fw::launch_gather_sum((int)nnz, d_asm_ptr, d_asm_src, d_elem, aval, st));
+    HIPCHK(fw::launch_gather_sum((int)ndof, d_f_ptr, d_f_src, d_elem, c->f + k * ndof, st));
+    HIPCHK(fw::launch_copy_idx((int)ncpl, d_cp_ent, aval, c->cp_val + k * ncpl, st));
+    HIPCHK(fw::launch_copy_idx((int)ngrow, d_gr_ent, aval, c->gr_val + k * ngrow, st));
+    HIPCHK(fw::launch_pcslot((int)nslots, d_diag_slot, nullptr, aval, c->pcslot + k * nslots, st));
+    HIPCHK(fw::launch_profile((int)ndof, c->dof_sub, c->meta, c->rowptr, c->col, aval, c->bcflag, d_idx1k, st));
+    HIPCHK(hipMemcpyAsync(idx1k.data(), d_idx1k, ndof * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));  // host buffers of this system are reused
+    if (k == 0)
+      idx1 = idx1k;
+    else
+      for (long long d = 0; d < ndof; d++) idx1[d] = std::min(idx1[d], idx1k[d]);
+  }
   int bad = 0;
   HIPCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(c->idx1, idx1.data(), ndof * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
   if (bad) return fail(FWH_E_MESH, "degenerate element (non-positive volume or zero-length edge)");
-
-  // ---- Dirichlet profile + skyline layout -----------------------------------------
-  if ((rc = c->alloc(&c->meta, nsub))) return rc;
-  HIPCHK(hipMemcpyAsync(c->meta, meta.data(), nsub * sizeof(SubMeta), hipMemcpyHostToDevice, st));
-  if ((rc = c->alloc(&c->idx1, ndof))) return rc;
-  HIPCHK(fw::launch_profile((int)ndof, c->dof_sub, c->meta, c->rowptr, c->col, c->aval, c->bcflag, c->idx1, st));
-  std::vector<int> idx1(ndof), rowbeg(ndof), blk_row;
-  HIPCHK(hipMemcpyAsync(idx1.data(), c->idx1, ndof * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
   c->info.ms_assemble = now_ms() - t0;
   long long nlen_tot = 0;
   int wmax = 0;
@@ -771,7 +836,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   if ((size_t)(wmax + 1 + nmax) * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "profile too wide for the factor kernel");
   HIPCHK(hipMemcpyAsync(c->meta, meta.data(), nsub * sizeof(SubMeta), hipMemcpyHostToDevice, st));
   if ((rc = c->upload(&c->rowbeg, rowbeg)) || (rc = c->upload(&c->blk_row, blk_row)) ||
-      (rc = c->alloc(&c->fac, nlen_tot)) || (rc = c->alloc(&c->dinv, ndof)))
+      (rc = c->alloc(&c->fac, (size_t)nsys * nlen_tot)) || (rc = c->alloc(&c->dinv, (size_t)nsys * ndof)))
     return rc;
   // work order: largest skyline first
   std::vector<int> order(nsub);
@@ -814,33 +879,46 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
 
   HIPCHK(hipStreamSynchronize(st));
   t0 = now_ms();
-  HIPCHK(hipMemsetAsync(c->fac, 0, nlen_tot * sizeof(double2), st));
-  HIPCHK(fw::launch_skyfill((int)nnz, c->ent_row, c->dof_sub, c->meta, c->col, c->aval, c->bcflag, c->idx1,
-                            c->rowbeg, c->fac, st));
-  HIPCHK(fw::launch_skydiag_bc((int)ndof, c->dof_sub, c->meta, c->bcflag, c->rowbeg, c->idx1, c->fac, st));
-  fw::FactorArgs fa{};
-  fa.meta = c->meta;
-  fa.order = c->order;
-  fa.base = 0;
-  fa.idx1 = c->idx1;
-  fa.rowbeg = c->rowbeg;
-  fa.fac = c->fac;
-  fa.dinv = c->dinv;
-  fa.wmax = wmax;
-  fa.nmax = nmax;
-  HIPCHK(fw::launch_factor(fa, nsub, st));
+  HIPCHK(hipMemsetAsync(c->fac, 0, (size_t)nsys * nlen_tot * sizeof(double2), st));
+  for (int k = 0; k < nsys; k++) {  // one factor set per system
+    double2 *fac = c->fac + k * nlen_tot;
+    HIPCHK(fw::launch_skyfill((int)nnz, c->ent_row, c->dof_sub, c->meta, c->col, c->aval + k * nnz, c->bcflag,
+                              c->idx1, c->rowbeg, fac, st));
+    HIPCHK(fw::launch_skydiag_bc((int)ndof, c->dof_sub, c->meta, c->bcflag, c->rowbeg, c->idx1, fac, st));
+    fw::FactorArgs fa{};
+    fa.meta = c->meta;
+    fa.order = c->order;
+    fa.base = 0;
+    fa.idx1 = c->idx1;
+    fa.rowbeg = c->rowbeg;
+    fa.fac = fac;
+    fa.dinv = c->dinv + k * ndof;
+    fa.wmax = wmax;
+    fa.nmax = nmax;
+    HIPCHK(fw::launch_factor(fa, nsub, st));
+  }
   HIPCHK(hipStreamSynchronize(st));
   c->info.ms_factor = now_ms() - t0;
 
   // ---- interface vectors ------------------------------------------------------------
   c->G = std::max(1, std::min(512, (ng + fw::VT_HINT - 1) / fw::VT_HINT));
-  if ((rc = c->alloc(&c->P, ng)) || (rc = c->alloc(&c->Q, ng)) || (rc = c->alloc(&c->R, ng)) ||
-      (rc = c->alloc(&c->Z, ng)) || (rc = c->alloc(&c->U, ng)) || (rc = c->alloc(&c->PC, ng)) ||
-      (rc = c->alloc(&c->yslot, nslots)) || (rc = c->alloc(&c->X, ndof)) || (rc = c->alloc(&c->part, 3 * c->G)) ||
-      (rc = c->alloc(&c->sc, 1)))
+  const size_t ngs = (size_t)nsys * ng;
+  if ((rc = c->alloc(&c->P, ngs)) || (rc = c->alloc(&c->Q, ngs)) || (rc = c->alloc(&c->R, ngs)) ||
+      (rc = c->alloc(&c->Z, ngs)) || (rc = c->alloc(&c->U, ngs)) || (rc = c->alloc(&c->PC, ngs)) ||
+      (rc = c->alloc(&c->yslot, (size_t)nsys * nslots)) || (rc = c->alloc(&c->X, (size_t)nsys * ndof)) ||
+      (rc = c->alloc(&c->part, (size_t)nsys * 3 * c->G)) || (rc = c->alloc(&c->sc, nsys)))
     return rc;
-  HIPCHK(hipMemsetAsync(c->X, 0, ndof * sizeof(double2), st));
-  HIPCHK(hipMemsetAsync(c->sc, 0, sizeof(fw::Scal), st));
+  c->ss.dof = ndof;
+  c->ss.nlen = nlen_tot;
+  c->ss.cpl = ncpl;
+  c->ss.grow = ngrow;
+  c->ss.slot = nslots;
+  c->ss.g = ng;
+  c->ss.smat = 0;
+  c->ss.hist = 0;
+  c->sys_stats.assign(nsys, fwh_stats{});
+  HIPCHK(hipMemsetAsync(c->X, 0, (size_t)nsys * ndof * sizeof(double2), st));
+  HIPCHK(hipMemsetAsync(c->sc, 0, nsys * sizeof(fw::Scal), st));
   HIPCHK(hipStreamSynchronize(st));
 
   // ---- info ----------------------------------------------------------------------------
@@ -856,9 +934,10 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
   I.n_cpl = c->ncpl;
   I.n_grow = c->ngrow;
   I.max_n = nmax;
-  I.bytes_solve = 16.0 * nlen_tot + 40.0 * ndof;
-  I.bytes_iter = I.bytes_solve + 20.0 * (c->ncpl + c->ngrow) + 48.0 * nslots + 128.0 * ng;
-  I.flops_factor = flops;
+  I.nsys = nsys;
+  I.bytes_solve = nsys * (16.0 * nlen_tot + 40.0 * ndof);
+  I.bytes_iter = I.bytes_solve + nsys * (20.0 * (c->ncpl + c->ngrow) + 48.0 * nslots + 128.0 * ng);
+  I.flops_factor = nsys * flops;
 
   // the batched LDL^T kernel of the iteration sweeps: prefetch depth 16 or 8
   // (the short partition), one launch or one per size bucket, whichever is
@@ -899,7 +978,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
     c->use_buckets = variants == 2 && best[d][1] < best[d][0];
     I.ms_solve_depth[0] = depths == 2 ? t8 : 0.0;
     I.ms_solve_depth[1] = t16;
-    HIPCHK(hipMemsetAsync(c->X, 0, ndof * sizeof(double2), st));
+    HIPCHK(hipMemsetAsync(c->X, 0, (size_t)nsys * ndof * sizeof(double2), st));
     HIPCHK(hipStreamSynchronize(st));
   }
   I.solve_buckets = (int)c->buckets.size();
@@ -914,17 +993,28 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *m) {
 
 }  // namespace
 
-extern "C" int fwh_load(fwh_ctx *c, const fwh_mesh_desc *m) {
-  if (!c || !m) return fail(FWH_E_ARG, "null argument");
+extern "C" int fwh_load_batch(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
+  if (!c || !descs) return fail(FWH_E_ARG, "null argument");
+  if (nsys < 1 || nsys > 65535) return fail(FWH_E_ARG, "nsys %d out of range", nsys);
   if (c->loaded) return fail(FWH_E_STATE, "model already loaded");
   HIPCHK(hipSetDevice(c->device));
-  int rc = build_model(c, m);
+  int rc = build_model(c, descs, nsys);
   if (rc) {
     (void)hipStreamSynchronize(c->st);
     c->release();
     c->loaded = false;
   }
   return rc;
+}
+
+extern "C" int fwh_load(fwh_ctx *c, const fwh_mesh_desc *m) { return fwh_load_batch(c, m, 1); }
+
+extern "C" int fwh_select(fwh_ctx *c, int sys) {
+  if (!c) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  if (sys < 0 || sys >= c->nsys) return fail(FWH_E_ARG, "system %d out of range (%d systems)", sys, c->nsys);
+  c->sel = sys;
+  return 0;
 }
 
 extern "C" int fwh_get_info(const fwh_ctx *c, fwh_info *info) {
@@ -940,15 +1030,23 @@ extern "C" int fwh_get_info(const fwh_ctx *c, fwh_info *info) {
 namespace {
 int ensure_hist(fwh_ctx *c, int max_loop) {
   if (max_loop + 1 <= c->hist_cap) return 0;
-  int rc = c->alloc(&c->hist, (size_t)max_loop + 1);
+  int rc = c->alloc(&c->hist, (size_t)c->nsys * (max_loop + 1));
   if (rc) return rc;
   c->hist_cap = max_loop + 1;
+  c->ss.hist = c->hist_cap;
   c->release_graph();  // graphs hold the old pointer
   return 0;
 }
-int read_scal(fwh_ctx *c, fw::Scal *s) {
-  HIPCHK(hipMemcpyAsync(s, c->sc, sizeof(fw::Scal), hipMemcpyDeviceToHost, c->st));
+// the scalars of every system; returns 1 in *all_done when every system's
+// status is LAST
+int read_scal(fwh_ctx *c, std::vector<fw::Scal> &s, int *all_done = nullptr) {
+  s.resize(c->nsys);
+  HIPCHK(hipMemcpyAsync(s.data(), c->sc, c->nsys * sizeof(fw::Scal), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  if (all_done) {
+    *all_done = 1;
+    for (auto &x : s) *all_done &= x.done != 0;
+  }
   return 0;
 }
 // Form every S_s = A_GG - A_GI A_II^-1 A_IG once: column k of all subdomains
@@ -970,9 +1068,10 @@ int build_schur(fwh_ctx *c) {
   HIPCHK(hipStreamSynchronize(c->st));
   const double t0 = now_ms();
   int rc;
-  if ((rc = c->alloc(&c->smat, tot)) || (rc = c->upload(&c->sm_off, off))) return rc;
+  if ((rc = c->alloc(&c->smat, (size_t)c->nsys * tot)) || (rc = c->upload(&c->sm_off, off))) return rc;
   c->nbmax = nbmax;
-  HIPCHK(fw::launch_fill(c->smat, tot, make_double2(0.0, 0.0), c->st));
+  c->ss.smat = tot;  // every system: 64 zeros, then its tiles
+  HIPCHK(fw::launch_fill(c->smat, (long long)c->nsys * tot, make_double2(0.0, 0.0), c->st));
   // one launch over all (subdomain, unit vector) pairs.  Workgroups are
   // dealt to the XCDs round-robin (workgroup b on XCD b mod 8), so the pairs
   // of subdomain s are placed at b = 8 i + (s mod 8): each subdomain's
@@ -1026,7 +1125,7 @@ int build_schur(fwh_ctx *c) {
   }
   HIPCHK(hipStreamSynchronize(c->st));
   c->info.ms_schur_form = now_ms() - t0;
-  c->info.smat_bytes = 16.0 * (double)tot;
+  c->info.smat_bytes = 16.0 * (double)tot * c->nsys;
   // waves per workgroup of the matvec: 2, 4 or 8, whichever streams fastest
   // on this model (LDS permitting; one warm-up, then the best of three).  The
   // count sets how each subdomain's sum is split into partial sums, so 4 is
@@ -1074,7 +1173,7 @@ int build_schur(fwh_ctx *c) {
   }
   // packed lower triangle + p gather (value + index) + y out, per slot (the
   // kernel also reads z to form p = z + beta p: not counted)
-  c->info.bytes_smv = packed + 36.0 * (double)c->nslots;
+  c->info.bytes_smv = c->nsys * (packed + 36.0 * (double)c->nslots);
   c->schur_built = true;
   return 0;
 }
@@ -1098,15 +1197,19 @@ extern "C" int fwh_first(fwh_ctx *c, const fwh_opts *o) {
   memset(&c->stats, 0, sizeof(c->stats));
   HIPCHK(hipStreamSynchronize(c->st));
   const double t0 = now_ms();
-  HIPCHK(fw::launch_scal_init(c->sc, o->hddm_eps, o->hddm_max_loop, c->st));
+  HIPCHK(fw::launch_scal_init(c->sc, c->nsys, o->hddm_eps, o->hddm_max_loop, c->st));
   HIPCHK(c->run_solve(fw::MODE_FIRST, o->plain != 0, false));
   HIPCHK(fw::launch_first(c->vec_args(), c->st));
-  fw::Scal s;
-  if ((rc = read_scal(c, &s))) return rc;
+  std::vector<fw::Scal> s;
+  if ((rc = read_scal(c, s))) return rc;
   c->stats.ms_first = now_ms() - t0;
-  c->stats.r0 = 1.0 / s.r0inv;
-  c->stats.sub_solves = c->nsub;
-  c->stats.loops = s.loop;
+  for (int k = 0; k < c->nsys; k++) {
+    fwh_stats &t = c->sys_stats[k];
+    memset(&t, 0, sizeof(t));
+    t.r0 = 1.0 / s[k].r0inv;
+    t.sub_solves = c->nsub;
+    t.loops = s[k].loop;
+  }
   c->first_done = true;
   c->last_done = false;
   c->p_final = false;
@@ -1117,18 +1220,20 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   if (!c || !o) return fail(FWH_E_ARG, "null argument");
   if (!c->first_done) return fail(FWH_E_STATE, "fwh_first not called");
   HIPCHK(hipSetDevice(c->device));
-  fw::Scal s;
-  int rc;
-  if ((rc = read_scal(c, &s))) return rc;
+  std::vector<fw::Scal> s;
+  int rc, all = 0;
+  if ((rc = read_scal(c, s, &all))) return rc;
   const double t0 = now_ms();
   int it = 0;
   const bool expl = o->schur != 0;
   if (expl && !c->schur_built) return fail(FWH_E_STATE, "explicit Schur mode requested after fwh_first without it");
+  // every launch serves all systems; a system whose status is LAST skips
+  // its work inside the kernels, and the loop ends when all are LAST
   if (o->plain) {
-    while (it < n && !s.done) {
+    while (it < n && !all) {
       HIPCHK(c->run_schur(expl, true, true, true));
       HIPCHK(fw::launch_iter(c->vec_args(), c->st));
-      if ((rc = read_scal(c, &s))) return rc;
+      if ((rc = read_scal(c, s, &all))) return rc;
       it++;
     }
   } else if (o->graph_iters > 0) {
@@ -1150,28 +1255,31 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
       c->graph_n = o->graph_iters;
       c->graph_schur = o->schur;
     }
-    while (it < n && !s.done) {
+    while (it < n && !all) {
       HIPCHK(hipGraphLaunch(c->gexec, c->st));
-      if ((rc = read_scal(c, &s))) return rc;
+      if ((rc = read_scal(c, s, &all))) return rc;
       it += c->graph_n;
     }
   } else {
-    while (it < n && !s.done) {
+    while (it < n && !all) {
       HIPCHK(c->run_schur(expl, false, true, true));
       HIPCHK(fw::launch_iter(c->vec_args(), c->st));
       it++;
       if ((it & 15) == 0 || it >= n)
-        if ((rc = read_scal(c, &s))) return rc;
+        if ((rc = read_scal(c, s, &all))) return rc;
     }
   }
-  if ((rc = read_scal(c, &s))) return rc;
+  if ((rc = read_scal(c, s, &all))) return rc;
   c->stats.ms_loop += now_ms() - t0;
-  c->stats.iters = s.iters;
-  c->stats.loops = s.loop;
-  c->stats.converged = s.conv;
-  c->stats.final_resid = s.resi;
-  c->stats.sub_solves = (long)c->nsub * (1 + (expl ? 0 : s.iters));  // explicit: no solves per iteration
-  if (done) *done = s.done;
+  for (int k = 0; k < c->nsys; k++) {
+    fwh_stats &t = c->sys_stats[k];
+    t.iters = s[k].iters;
+    t.loops = s[k].loop;
+    t.converged = s[k].conv;
+    t.final_resid = s[k].resi;
+    t.sub_solves = (long)c->nsub * (1 + (expl ? 0 : s[k].iters));  // explicit: no solves per iteration
+  }
+  if (done) *done = all;
   return 0;
 }
 
@@ -1188,28 +1296,33 @@ extern "C" int fwh_last(fwh_ctx *c) {
   HIPCHK(c->run_solve(fw::MODE_LAST, false, false));
   HIPCHK(hipStreamSynchronize(c->st));
   c->stats.ms_last = now_ms() - t0;
-  c->stats.sub_solves += c->nsub;
+  for (auto &t : c->sys_stats) t.sub_solves += c->nsub;
   c->last_done = true;
   return 0;
 }
 
+// statistics of the selected system; the wall times are the batch's
 extern "C" int fwh_get_stats(const fwh_ctx *c, fwh_stats *st) {
   if (!c || !st) return fail(FWH_E_ARG, "null argument");
-  *st = c->stats;
+  fwh_stats t = c->sys_stats.empty() ? fwh_stats{} : c->sys_stats[c->sel];
+  t.ms_first = c->stats.ms_first;
+  t.ms_loop = c->stats.ms_loop;
+  t.ms_last = c->stats.ms_last;
+  *st = t;
   return 0;
 }
 
 extern "C" int fwh_get_x(const fwh_ctx *c, double *x) {
   if (!c || !x) return fail(FWH_E_ARG, "null argument");
   if (!c->last_done) return fail(FWH_E_STATE, "no result (fwh_last not called)");
-  HIPCHK(hipMemcpy(x, c->X, c->ndof * sizeof(double2), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(x, c->X + c->sel * c->ndof, c->ndof * sizeof(double2), hipMemcpyDeviceToHost));
   return 0;
 }
 
 extern "C" int fwh_get_u(const fwh_ctx *c, double *u) {
   if (!c || !u) return fail(FWH_E_ARG, "null argument");
   if (!c->first_done) return fail(FWH_E_STATE, "no solve");
-  HIPCHK(hipMemcpy(u, c->U, (size_t)c->ng * sizeof(double2), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(u, c->U + (size_t)c->sel * c->ng, (size_t)c->ng * sizeof(double2), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -1217,7 +1330,8 @@ extern "C" int fwh_get_resid(const fwh_ctx *c, double *r, int nr) {
   if (!c || !r) return fail(FWH_E_ARG, "null argument");
   if (!c->first_done) return fail(FWH_E_STATE, "no solve");
   if (nr > c->hist_cap) nr = c->hist_cap;
-  if (nr > 0) HIPCHK(hipMemcpy(r, c->hist, (size_t)nr * sizeof(double), hipMemcpyDeviceToHost));
+  if (nr > 0)
+    HIPCHK(hipMemcpy(r, c->hist + (size_t)c->sel * c->hist_cap, (size_t)nr * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -1239,10 +1353,10 @@ extern "C" int fwh_post(fwh_ctx *c, const double *nu, double *B, double *E, doub
   a.el_vtx = c->el_vtx;
   a.el_dof = c->el_dof;
   a.crd = c->crd;
-  a.sigma = c->sig;
+  a.sigma = c->sig + (size_t)c->sel * nel;
   a.nu = nu ? c->post_nu : nullptr;
-  a.omega = c->omega;
-  a.X = c->X;
+  a.omega = c->omega_h[c->sel];
+  a.X = c->X + (size_t)c->sel * c->ndof;
   a.B = c->post_out;
   a.E = c->post_out + 3 * nel;
   a.J = c->post_out + 6 * nel;
@@ -1266,13 +1380,14 @@ extern "C" int fwh_set_load(fwh_ctx *c, const double *f) {
   int rc;
   const size_t bytes = (size_t)c->ndof * sizeof(double2);
   if (!c->f_asm) {
-    if ((rc = c->alloc(&c->f_asm, c->ndof))) return rc;
-    HIPCHK(hipMemcpyAsync(c->f_asm, c->f, bytes, hipMemcpyDeviceToDevice, c->st));
+    if ((rc = c->alloc(&c->f_asm, (size_t)c->nsys * c->ndof))) return rc;
+    HIPCHK(hipMemcpyAsync(c->f_asm, c->f, c->nsys * bytes, hipMemcpyDeviceToDevice, c->st));
   }
+  const size_t o = (size_t)c->sel * c->ndof;
   if (f)
-    HIPCHK(hipMemcpyAsync(c->f, f, bytes, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->f + o, f, bytes, hipMemcpyHostToDevice, c->st));
   else
-    HIPCHK(hipMemcpyAsync(c->f, c->f_asm, bytes, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->f + o, c->f_asm + o, bytes, hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->first_done = c->last_done = false;
   return 0;
@@ -1295,8 +1410,8 @@ extern "C" int fwh_grad_eps(fwh_ctx *c, const double *xE, const double *xL, doub
   a.el_vtx = c->el_vtx;
   a.el_dof = c->el_dof;
   a.crd = c->crd;
-  a.pml = c->pml;
-  a.omega = c->omega;
+  a.pml = c->pml ? c->pml + (size_t)c->sel * 6 * c->nel : nullptr;
+  a.omega = c->omega_h[c->sel];
   a.xE = c->gx;
   a.xL = c->gx + c->ndof;
   a.grad = c->grad;
@@ -1317,10 +1432,11 @@ extern "C" int fwh_schur_apply(fwh_ctx *c, const double *p, double *q, int mode)
   const bool expl = mode >= 2;
   int rc;
   if (expl && (rc = build_schur(c))) return rc;
-  HIPCHK(hipMemcpyAsync(c->P, p, (size_t)c->ng * sizeof(double2), hipMemcpyHostToDevice, c->st));
-  HIPCHK(c->run_schur(expl, (mode & 1) != 0, false, false));
-  HIPCHK(fw::launch_gather_q(c->vec_args(), c->st));
-  HIPCHK(hipMemcpyAsync(q, c->Q, (size_t)c->ng * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+  const size_t o = (size_t)c->sel * c->ng;
+  HIPCHK(hipMemcpyAsync(c->P + o, p, (size_t)c->ng * sizeof(double2), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c->run_schur(expl, (mode & 1) != 0, false, false, c->sel));
+  HIPCHK(fw::launch_gather_q(c->vec_args(c->sel), c->st));
+  HIPCHK(hipMemcpyAsync(q, c->Q + o, (size_t)c->ng * sizeof(double2), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->first_done = false;
   return 0;
@@ -1330,9 +1446,10 @@ extern "C" int fwh_ldl_solve_all(fwh_ctx *c, double *rhs, int plain) {
   if (!c || !rhs) return fail(FWH_E_ARG, "null argument");
   if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
   HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipMemcpyAsync(c->X, rhs, c->ndof * sizeof(double2), hipMemcpyHostToDevice, c->st));
-  HIPCHK(c->run_solve(fw::MODE_RHS, plain != 0, false));
-  HIPCHK(hipMemcpyAsync(rhs, c->X, c->ndof * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+  double2 *X = c->X + (size_t)c->sel * c->ndof;
+  HIPCHK(hipMemcpyAsync(X, rhs, c->ndof * sizeof(double2), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c->run_solve(fw::MODE_RHS, plain != 0, false, false, c->sel));
+  HIPCHK(hipMemcpyAsync(rhs, X, c->ndof * sizeof(double2), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->last_done = false;
   return 0;
@@ -1387,7 +1504,8 @@ extern "C" int fwh_get_sub(const fwh_ctx *c, int s, int what, void *out) {
   const SubMeta &M = c->meta_h[s];
   switch (what) {
     case 0:
-      HIPCHK(hipMemcpy(out, c->aval + M.nnz0, (size_t)M.nnz * sizeof(double2), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(out, c->aval + (size_t)c->sel * c->nnz + M.nnz0, (size_t)M.nnz * sizeof(double2),
+                       hipMemcpyDeviceToHost));
       break;
     case 1: {
       std::vector<int> rp(M.n + 1);
@@ -1400,7 +1518,8 @@ extern "C" int fwh_get_sub(const fwh_ctx *c, int s, int what, void *out) {
       HIPCHK(hipMemcpy(out, c->col + M.nnz0, (size_t)M.nnz * sizeof(int), hipMemcpyDeviceToHost));
       break;
     case 3:
-      HIPCHK(hipMemcpy(out, c->f + M.dof0, (size_t)M.n * sizeof(double2), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(out, c->f + (size_t)c->sel * c->ndof + M.dof0, (size_t)M.n * sizeof(double2),
+                       hipMemcpyDeviceToHost));
       break;
     case 4:
       HIPCHK(hipMemcpy(out, c->idx1 + M.dof0, (size_t)M.n * sizeof(int), hipMemcpyDeviceToHost));
@@ -1414,7 +1533,8 @@ extern "C" int fwh_get_sub(const fwh_ctx *c, int s, int what, void *out) {
       break;
     }
     case 6:
-      HIPCHK(hipMemcpy(out, c->fac + M.fac0, (size_t)M.nlen * sizeof(double2), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(out, c->fac + (size_t)c->sel * c->nlen + M.fac0, (size_t)M.nlen * sizeof(double2),
+                       hipMemcpyDeviceToHost));
       break;
     case 7: {
       int *o = (int *)out;
@@ -1569,6 +1689,7 @@ extern "C" int fwh_cocg_icc_all(fwh_ctx *c, const double *b, double *x, int *sta
                                 double ic_shift) {
   if (!c || !b || !x || !status) return fail(FWH_E_ARG, "null argument");
   if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  if (c->nsys != 1) return fail(FWH_E_STATE, "the subdomain COCG runs on single-system contexts");
   if ((size_t)5 * c->nmax * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "subdomain too large for the COCG kernel");
   HIPCHK(hipSetDevice(c->device));
   int rc = build_dirichlet_csr(c);

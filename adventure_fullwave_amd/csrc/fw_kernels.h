@@ -42,7 +42,24 @@ constexpr int PFK = 16;                   // most prefetch loads per lane
 constexpr int CAP = SOLVE_THREADS * PFK;  // most skyline entries per row block
 constexpr int VT_HINT = 256;              // threads per block of the vector kernels
 
+// Systems of a batch (one per frequency of a sweep, say) share the mesh, the
+// decomposition and every index array; their values are concatenated
+// system-major with these strides (in entries).  A launch serves systems
+// sys0 .. sys0 + nsys - 1, one per grid row (blockIdx.y).
+struct SysStride {
+  long long dof;   // per-DOF arrays: f, dinv, X
+  long long nlen;  // skyline factor
+  long long cpl;   // interior-interface coupling values
+  long long grow;  // interface-row values
+  long long slot;  // per-slot arrays: yslot, pcslot
+  long long g;     // interface vectors
+  long long smat;  // Schur-complement tiles
+  long long hist;  // residual history
+};
+
 struct SolveArgs {
+  int sys0, nsys;        // systems of this launch (grid rows)
+  SysStride ss;
   const SubMeta *meta;
   const int *order;      // work order (may be null: identity)
   int base;              // first work item of this launch
@@ -79,6 +96,8 @@ struct SolveArgs {
 };
 
 struct SmvArgs {
+  int sys0, nsys;
+  SysStride ss;
   const SubMeta *meta;
   const int *order;
   int base;
@@ -161,6 +180,8 @@ struct Scal {
 };
 
 struct VecArgs {
+  int sys0, nsys;
+  SysStride ss;
   int ng;                 // N_Gamma
   const int *gq_ptr;      // gather CSR over interface unknowns
   const int *gq_slot;     // slot ids, ~slot marks the first slot of a part group
@@ -170,7 +191,7 @@ struct VecArgs {
   double2 *part;          // [3*G] partial sums
   Scal *sc;
   double *hist;
-  int G;                  // grid size of the vector kernels
+  int G;                  // grid size of the vector kernels (per system)
 };
 
 // launchers (fw_kernels.hip)
@@ -202,7 +223,7 @@ hipError_t launch_iter(const VecArgs &a, hipStream_t st);
 hipError_t launch_finalize_p(const VecArgs &a, hipStream_t st);
 hipError_t launch_gather_q(const VecArgs &a, hipStream_t st);
 hipError_t launch_fill(double2 *p, long long n, double2 v, hipStream_t st);
-hipError_t launch_scal_init(Scal *sc, double eps, int max_loop, hipStream_t st);
+hipError_t launch_scal_init(Scal *sc, int nsys, double eps, int max_loop, hipStream_t st);
 
 }  // namespace fw
 

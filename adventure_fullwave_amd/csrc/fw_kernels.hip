@@ -616,9 +616,28 @@ __device__ __forceinline__ double reduce_parts_max(const double2 *__restrict__ p
   return block_max(t, sh);
 }
 
+// the vectors of system sys0 + blockIdx.y (index arrays are shared)
+__device__ __forceinline__ void to_system(VecArgs &a) {
+  const long long s = a.sys0 + (long long)blockIdx.y;
+  if (s == 0) return;
+  a.yslot += s * a.ss.slot;
+  if (a.pcslot) a.pcslot += s * a.ss.slot;
+  a.P += s * a.ss.g;
+  a.Q += s * a.ss.g;
+  a.R += s * a.ss.g;
+  a.Z += s * a.ss.g;
+  a.U += s * a.ss.g;
+  a.PC += s * a.ss.g;
+  a.part += s * 3LL * a.G;
+  a.sc += s;
+  a.hist += s * a.ss.hist;
+}
+__host__ __forceinline__ dim3 vec_grid(const VecArgs &a, int gx) { return dim3(gx, a.nsys > 0 ? a.nsys : 1); }
+
 // first step: Q = gather(y), PC = 1/gather(diag) (hddm_pc.h:108-137),
 // R = Q_owned, Z = R PC, P = Z, U = 0 (hddm_solver_s.h:158-256)
 __global__ __launch_bounds__(VT) void k_first_a(VecArgs a) {
+  to_system(a);
   __shared__ double2 shc[VT / 64 + 1];
   __shared__ double shm[VT / 64 + 1];
   double2 rz = cz();
@@ -645,6 +664,7 @@ __global__ __launch_bounds__(VT) void k_first_a(VecArgs a) {
   }
 }
 __global__ __launch_bounds__(VT) void k_first_b(VecArgs a) {
+  to_system(a);
   __shared__ double2 shc[VT / 64 + 1];
   __shared__ double shm[VT / 64 + 1];
   const double2 rz = reduce_parts(a.part + a.G, a.G, shc);
@@ -663,10 +683,10 @@ __global__ __launch_bounds__(VT) void k_first_b(VecArgs a) {
   }
 }
 hipError_t launch_first(const VecArgs &a, hipStream_t st) {
-  k_first_a<<<a.G, VT, 0, st>>>(a);
+  k_first_a<<<vec_grid(a, a.G), VT, 0, st>>>(a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_first_b<<<1, VT, 0, st>>>(a);
+  k_first_b<<<vec_grid(a, 1), VT, 0, st>>>(a);
   return hipGetLastError();
 }
 
@@ -686,6 +706,7 @@ hipError_t launch_first(const VecArgs &a, hipStream_t st) {
 // same fused operations as k_iter_a); the final P (U when converged,
 // hddm_solver_s.h:370-376) is set by k_finalize_p before the last solve.
 __global__ __launch_bounds__(VT) void k_iter_a(VecArgs a) {
+  to_system(a);
   const Scal *s = a.sc;
   if (s->done) return;
   __shared__ double2 shc[VT / 64 + 1];
@@ -702,6 +723,7 @@ __global__ __launch_bounds__(VT) void k_iter_a(VecArgs a) {
   if (threadIdx.x == 0) a.part[blockIdx.x] = pq;
 }
 __global__ __launch_bounds__(VT) void k_iter_b(VecArgs a) {
+  to_system(a);
   const Scal *s = a.sc;
   if (s->done) return;
   __shared__ double2 shc[VT / 64 + 1];
@@ -730,6 +752,7 @@ __global__ __launch_bounds__(VT) void k_iter_b(VecArgs a) {
   }
 }
 __global__ __launch_bounds__(VT) void k_iter_c(VecArgs a) {
+  to_system(a);
   Scal *s = a.sc;
   if (s->done) return;
   __shared__ double2 shc[VT / 64 + 1];
@@ -755,15 +778,16 @@ __global__ __launch_bounds__(VT) void k_iter_c(VecArgs a) {
   }
 }
 hipError_t launch_iter(const VecArgs &a, hipStream_t st) {
-  k_iter_a<<<a.G, VT, 0, st>>>(a);
-  k_iter_b<<<a.G, VT, 0, st>>>(a);
-  k_iter_c<<<1, VT, 0, st>>>(a);
+  k_iter_a<<<vec_grid(a, a.G), VT, 0, st>>>(a);
+  k_iter_b<<<vec_grid(a, a.G), VT, 0, st>>>(a);
+  k_iter_c<<<vec_grid(a, 1), VT, 0, st>>>(a);
   return hipGetLastError();
 }
 
 // P for the last solve: U when the stop test fired, else the next search
 // direction z + beta p (max_loop reached, hddm_solver.h:125-143)
 __global__ __launch_bounds__(VT) void k_finalize_p(VecArgs a) {
+  to_system(a);
   const Scal *s = a.sc;
   const bool conv = s->conv != 0;
   const double2 beta = s->beta;
@@ -771,16 +795,17 @@ __global__ __launch_bounds__(VT) void k_finalize_p(VecArgs a) {
     a.P[g] = conv ? a.U[g] : zbp(a.Z[g], beta, a.P[g]);
 }
 hipError_t launch_finalize_p(const VecArgs &a, hipStream_t st) {
-  k_finalize_p<<<a.G, VT, 0, st>>>(a);
+  k_finalize_p<<<vec_grid(a, a.G), VT, 0, st>>>(a);
   return hipGetLastError();
 }
 
 __global__ __launch_bounds__(VT) void k_gather_q(VecArgs a) {
+  to_system(a);
   for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT)
     a.Q[g] = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
 }
 hipError_t launch_gather_q(const VecArgs &a, hipStream_t st) {
-  k_gather_q<<<a.G, VT, 0, st>>>(a);
+  k_gather_q<<<vec_grid(a, a.G), VT, 0, st>>>(a);
   return hipGetLastError();
 }
 
@@ -797,6 +822,7 @@ hipError_t launch_fill(double2 *p, long long n, double2 v, hipStream_t st) {
 }
 
 __global__ void k_scal_init(Scal *s, double eps, int max_loop) {
+  s += blockIdx.x;
   s->eps = eps;
   s->max_loop = max_loop;
   s->done = 0;
@@ -805,8 +831,8 @@ __global__ void k_scal_init(Scal *s, double eps, int max_loop) {
   s->iters = 0;
   s->beta = cz();
 }
-hipError_t launch_scal_init(Scal *sc, double eps, int max_loop, hipStream_t st) {
-  k_scal_init<<<1, 1, 0, st>>>(sc, eps, max_loop);
+hipError_t launch_scal_init(Scal *sc, int nsys, double eps, int max_loop, hipStream_t st) {
+  k_scal_init<<<nsys, 1, 0, st>>>(sc, eps, max_loop);
   return hipGetLastError();
 }
 

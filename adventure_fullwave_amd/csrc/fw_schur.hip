@@ -152,9 +152,22 @@ __device__ __forceinline__ void tile_step(const Tile &T, SmvState &st, const dou
 
 }  // namespace
 
+// the value arrays of system sys0 + blockIdx.y (index arrays are shared)
+__device__ __forceinline__ void to_system(SmvArgs &a) {
+  const long long s = a.sys0 + (long long)blockIdx.y;
+  if (s == 0) return;
+  a.smat += s * a.ss.smat;
+  a.P += s * a.ss.g;
+  if (a.Z) a.Z += s * a.ss.g;
+  a.sc += s;
+  if (a.done) a.done = &a.sc->done;
+  a.yslot += s * a.ss.slot;
+}
+
 template <int W>  // waves per workgroup
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_smv(SmvArgs a) {
   constexpr int SMV_WAVES = W, NT = 64 * W;
+  to_system(a);
   if (a.done && *a.done) return;
   const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
   const SubMeta m = a.meta[w];
@@ -232,10 +245,11 @@ size_t smv_lds_bytes(int nbmax, int waves) { return sizeof(double2) * (size_t)(1
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st) {
   if (nwork <= 0) return hipSuccess;
   const size_t lds = smv_lds_bytes(a.nbmax, a.waves);
+  const dim3 grid(nwork, a.nsys > 0 ? a.nsys : 1);
   switch (a.waves) {
-    case 2: k_smv<2><<<nwork, 128, lds, st>>>(a); break;
-    case 8: k_smv<8><<<nwork, 512, lds, st>>>(a); break;
-    default: k_smv<4><<<nwork, 256, lds, st>>>(a); break;
+    case 2: k_smv<2><<<grid, 128, lds, st>>>(a); break;
+    case 8: k_smv<8><<<grid, 512, lds, st>>>(a); break;
+    default: k_smv<4><<<grid, 256, lds, st>>>(a); break;
   }
   return hipGetLastError();
 }

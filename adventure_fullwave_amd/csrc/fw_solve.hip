@@ -204,9 +204,28 @@ __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__re
   }
 }
 
+// the value arrays of system sys0 + blockIdx.y (index arrays are shared)
+__device__ __forceinline__ void to_system(SolveArgs &a) {
+  const long long s = a.sys0 + (long long)blockIdx.y;
+  if (s == 0) return;
+  a.fac += s * a.ss.nlen;
+  a.dinv += s * a.ss.dof;
+  a.f += s * a.ss.dof;
+  a.cp_val += s * a.ss.cpl;
+  a.gr_val += s * a.ss.grow;
+  a.P += s * a.ss.g;
+  if (a.Z) a.Z += s * a.ss.g;
+  a.sc += s;
+  if (a.done) a.done = &a.sc->done;
+  a.yslot += s * a.ss.slot;
+  a.X += s * a.ss.dof;
+  if (a.smat) a.smat += s * a.ss.smat;
+}
+
 template <int K>  // prefetch loads per lane: row blocks of at most 64*K entries
 __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 3 : 2))) void k_solve(SolveArgs a) {
   constexpr int CAPK = WL * K;
+  to_system(a);
   if (a.mode == MODE_ITER && a.done && *a.done) return;
   int w, uk = a.unit_k;
   if (a.mode == MODE_UNIT && a.unit_list) {
@@ -353,10 +372,11 @@ size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk) {
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
   if (nwork <= 0) return hipSuccess;
   const size_t lds = solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk);
+  const dim3 grid(nwork, a.nsys > 0 ? a.nsys : 1);
   if (a.pfk == 8)
-    k_solve<8><<<nwork, WL, lds, st>>>(a);
+    k_solve<8><<<grid, WL, lds, st>>>(a);
   else
-    k_solve<16><<<nwork, WL, lds, st>>>(a);
+    k_solve<16><<<grid, WL, lds, st>>>(a);
   return hipGetLastError();
 }
 

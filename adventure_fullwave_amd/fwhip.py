@@ -27,7 +27,7 @@ EXPORTS = ("fwh_create", "fwh_destroy", "fwh_last_error", "fwh_load", "fwh_get_i
            "fwh_first", "fwh_iterate", "fwh_last", "fwh_get_stats", "fwh_get_x", "fwh_get_u",
            "fwh_get_resid", "fwh_schur_apply", "fwh_ldl_solve_all", "fwh_time_kernels", "fwh_get_sub",
            "fwh_element_batch", "fwh_cocg_icc_all", "fwh_schur_form", "fwh_post", "fwh_set_load",
-           "fwh_grad_eps")
+           "fwh_grad_eps", "fwh_load_batch", "fwh_select")
 
 
 class FwhOpts(ctypes.Structure):
@@ -56,7 +56,8 @@ class FwhInfo(ctypes.Structure):
                 ("solve_bucketed", ctypes.c_int), ("ms_solve_single", ctypes.c_double),
                 ("ms_solve_bucketed", ctypes.c_double), ("smv_waves", ctypes.c_int),
                 ("ms_smv_tune", ctypes.c_double * 3), ("solve_depth", ctypes.c_int),
-                ("unit_depth", ctypes.c_int), ("ms_solve_depth", ctypes.c_double * 2)]
+                ("unit_depth", ctypes.c_int), ("ms_solve_depth", ctypes.c_double * 2),
+                ("nsys", ctypes.c_int)]
 
     def as_dict(self):
         out = {}
@@ -85,6 +86,8 @@ def lib():
         L.fwh_destroy.argtypes = [P]
         L.fwh_destroy.restype = None
         L.fwh_load.argtypes = [P, P]
+        L.fwh_load_batch.argtypes = [P, P, I]
+        L.fwh_select.argtypes = [P, I]
         L.fwh_get_info.argtypes = [P, P]
         L.fwh_solve.argtypes = [P, P, P]
         L.fwh_first.argtypes = [P, P]
@@ -156,16 +159,41 @@ class Context:
 
 
 class Solver:
-    """A decomposed model resident on the GPU (fwh_load) and its HDDM solve."""
+    """A decomposed model resident on the GPU (fwh_load) and its HDDM solve.
+
+    model may also be a list of models on one mesh and decomposition (a
+    frequency sweep, say): they are loaded as one batch (fwh_load_batch) and
+    solved together; `select(k)` picks the system that the per-system
+    methods (stats, x, u, resid, post, set_load, grad_eps, schur_apply,
+    ldl_solve_all, sub) address."""
 
     def __init__(self, model, device: int = 0, ctx: Optional[Context] = None):
-        self.model = model
+        self.models = list(model) if isinstance(model, (list, tuple)) else [model]
+        self.model = self.models[0]
         self.ctx = ctx or Context(device)
-        self._desc = model.desc()
-        _check(lib().fwh_load(self.ctx.h, ctypes.byref(self._desc)), "fwh_load")
+        if len(self.models) == 1:
+            self._desc = self.model.desc()
+            _check(lib().fwh_load(self.ctx.h, ctypes.byref(self._desc)), "fwh_load")
+        else:
+            from .mesh import FwhMeshDesc
+            descs = [m.desc() for m in self.models]
+            self._desc = (FwhMeshDesc * len(descs))(*descs)
+            self._desc_keep = descs
+            _check(lib().fwh_load_batch(self.ctx.h, self._desc, len(descs)), "fwh_load_batch")
+        self.sel = 0
         inf = FwhInfo()
         _check(lib().fwh_get_info(self.ctx.h, ctypes.byref(inf)), "fwh_get_info")
         self.info = inf.as_dict()
+
+    @property
+    def nsys(self) -> int:
+        return len(self.models)
+
+    def select(self, k: int):
+        """the system that the per-system methods address"""
+        _check(lib().fwh_select(self.ctx.h, int(k)), "fwh_select")
+        self.sel = int(k)
+        return self
 
     @staticmethod
     def opts(eps=1e-8, max_loop=100000, plain=False, graph_iters=16, schur=False):
@@ -207,6 +235,13 @@ class Solver:
         st = FwhStats()
         _check(lib().fwh_get_stats(self.ctx.h, ctypes.byref(st)), "fwh_get_stats")
         return st.as_dict()
+
+    def stats_all(self):
+        """statistics of every system of the batch"""
+        keep = self.sel
+        out = [self.select(k).stats() for k in range(self.nsys)]
+        self.select(keep)
+        return out
 
     def x(self):
         out = np.zeros(int(self.info["n_dof"]), np.complex128)

@@ -48,6 +48,56 @@ def pmc_traffic(cfg):
         return None
 
 
+def frequency_sweep(args, model, fwhip, expl):
+    """n frequencies on the model's mesh: one batch (every launch serves all
+    systems) against n separate single-system solves, same options; each
+    timed after one warm-up solve.  Setup (assembly, factors, S_s formation)
+    is reported apart from the solve."""
+    import dataclasses
+    import numpy as np
+    n = args.sweep
+    fac = [0.9 + 0.2 * k / (n - 1) for k in range(n)]
+    models = [dataclasses.replace(model, omega=model.omega * f) for f in fac]
+    kw = dict(eps=args.eps, max_loop=args.max_loop, graph_iters=args.graph_iters, schur=expl)
+
+    def setup_and_solve(m):
+        t0 = time.perf_counter()
+        sol = fwhip.Solver(m)
+        if expl:
+            sol.schur_form()
+        t_setup = time.perf_counter() - t0
+        sol.solve(**kw)
+        t0 = time.perf_counter()
+        sol.solve(**kw)
+        return sol, t_setup, time.perf_counter() - t0
+
+    b, b_setup, b_dt = setup_and_solve(models)
+    b_stats = b.stats_all()
+    xb = [b.select(k).x() for k in range(n)]
+    b_info = dict(b.info)
+    b.close()
+    log(f"[bench] sweep batch of {n}: {[st['iters'] for st in b_stats]} iterations, {1000 * b_dt:.1f} ms")
+    s_iters, s_dt, s_setup, diff = [], 0.0, 0.0, 0.0
+    for k, m in enumerate(models):
+        s1, t_set, dt = setup_and_solve(m)
+        s_iters.append(s1.stats()["iters"])
+        s_dt += dt
+        s_setup += t_set
+        x1 = s1.x()
+        diff = max(diff, float(np.linalg.norm(xb[k] - x1) / np.linalg.norm(x1)))
+        s1.close()
+    log(f"[bench] sweep separately: {s_iters} iterations, {1000 * s_dt:.1f} ms")
+    tot_b = sum(st["iters"] for st in b_stats)
+    return {"n_freq": n, "omega_factors": [round(f, 4) for f in fac], "schur": "explicit" if expl else "implicit",
+            "batch": {"iters": [st["iters"] for st in b_stats], "solve_ms": round(1000 * b_dt, 2),
+                      "iters_per_s": round(tot_b / b_dt, 2), "setup_ms": round(1000 * b_setup, 1),
+                      "schur_form_ms": round(b_info["ms_schur_form"], 2), "factor_ms": round(b_info["ms_factor"], 2)},
+            "separate": {"iters": s_iters, "solve_ms": round(1000 * s_dt, 2),
+                         "iters_per_s": round(sum(s_iters) / s_dt, 2), "setup_ms": round(1000 * s_setup, 1)},
+            "solve_speedup": round(s_dt / b_dt, 3),
+            "max_rel_diff_batch_vs_separate": diff}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +116,9 @@ def main():
     ap.add_argument("--implicit-steps", type=int, default=1)
     ap.add_argument("--adjoint", action="store_true",
                     help="also time an adjoint permittivity gradient (forward solve, adjoint solve, gradient)")
+    ap.add_argument("--sweep", type=int, default=0,
+                    help="also solve a sweep of this many frequencies f0*(0.9 + 0.2k/(n-1)) as one batch "
+                         "(fwh_load_batch) and as separate solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
@@ -185,6 +238,9 @@ def main():
                    "note": "host transfers of E and lambda included"}
         log(f"[bench] adjoint: forward {adjoint['forward_ms']} ms, adjoint {adjoint['adjoint_ms']} ms, "
             f"gradient {adjoint['gradient_ms']} ms")
+    sweep = None
+    if args.sweep > 1:
+        sweep = frequency_sweep(args, model, fwhip, expl)
     setup = {"assemble": round(info["ms_assemble"], 2), "factor": round(info["ms_factor"], 2)}
     if expl:
         setup["schur_form"] = round(info["ms_schur_form"], 2)
@@ -213,6 +269,7 @@ def main():
                      "solve_depth": info["solve_depth"], "unit_depth": info["unit_depth"],
                      "ms_solve_by_depth_8_16": [round(v, 4) for v in info["ms_solve_depth"]]},
         "adjoint": adjoint,
+        "sweep": sweep,
         "time_to_solution_ms": round(setup["factor"] + setup.get("schur_form", 0.0) + 1000 * dt / args.steps, 1),
     }
     print(json.dumps(out), flush=True)

@@ -100,6 +100,8 @@ typedef struct fwh_info {
   int    unit_depth;    /* the same for the Schur-complement formation     */
   double ms_solve_depth[2]; /* best batched solve time at depth 8 (0 when
                            a profile row is wider than 512), 16            */
+  int    nsys;          /* systems of the batch (fwh_load_batch); the byte
+                           and flop counts above cover all of them          */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */
@@ -118,6 +120,24 @@ const char *fwh_last_error(void);
  * advlas_skyline_ldl_decomposite).  The model description is copied. */
 int  fwh_load(fwh_ctx *ctx, const fwh_mesh_desc *m);
 int  fwh_get_info(const fwh_ctx *ctx, fwh_info *info);
+
+/* ---- batches of systems on one mesh (no reference counterpart: the
+ * reference runs one hddm_solver_CS per model, main/fullwave/main.c:105) --
+ * fwh_load_batch loads nsys systems that share the mesh and the
+ * decomposition of descs[0] (every index array of descs[k] equal to
+ * descs[0]'s) and have their own omega, eps, sigma, source and PML: a
+ * frequency sweep is the typical batch.  Each system gets its own factors
+ * (and Schur complements), its own interface COCG, stop test and status;
+ * every batched launch of fwh_solve serves all systems (one grid row per
+ * system), and the solve ends once every system's status is LAST.  Results
+ * match nsys separate single-system solves bit for bit.  fwh_load is
+ * fwh_load_batch with nsys = 1.
+ * fwh_select chooses the system that the per-system calls address:
+ * fwh_get_stats, fwh_get_x, fwh_get_u, fwh_get_resid, fwh_post,
+ * fwh_set_load, fwh_grad_eps, fwh_schur_apply, fwh_ldl_solve_all and
+ * fwh_get_sub (0 after a load).                                           */
+int  fwh_load_batch(fwh_ctx *ctx, const fwh_mesh_desc *descs, int nsys);
+int  fwh_select(fwh_ctx *ctx, int sys);
 
 /* ---- the HDDM solve -----------------------------------------------------
  * fwh_solve replaces __HDDM_SOLVER_PARENT__ in s-mode
