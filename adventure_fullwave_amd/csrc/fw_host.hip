@@ -119,6 +119,7 @@ struct fwh_ctx {
   // explicit Schur complements (formed on first use)
   double2 *smat = nullptr;
   long long *sm_off = nullptr;
+  int *smv_order = nullptr;  // work order of the matvec (picked at formation)
   int nbmax = 0;
   int smv_waves = 4;
   bool schur_built = false;
@@ -253,7 +254,7 @@ struct fwh_ctx {
     a.nsys = only >= 0 ? 1 : nsys;
     a.ss = ss;
     a.meta = meta;
-    a.order = order;
+    a.order = smv_order ? smv_order : order;
     a.base = 0;
     a.done = check_done ? &sc->done : nullptr;
     a.nbmax = nbmax;
@@ -1163,6 +1164,51 @@ int build_schur(fwh_ctx *c) {
         c->info.smv_waves = cand[v];
       }
     c->smv_waves = c->info.smv_waves;
+    // work order of the matvec.  Every workgroup computes its subdomain's
+    // product alone, so the order changes neither results nor rounding,
+    // only how the subdomains' bytes spread over the CUs: 0 = largest
+    // skyline first (the LDL^T kernel's order), 1 = most tiles first,
+    // 2 = most tiles first with every other group of nCU subdomains
+    // reversed (workgroups are dealt round-robin, so CU c then gets a large
+    // and a small subdomain of each pair of groups).  Fastest is kept.
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    ncu = std::max(ncu, 1);
+    std::vector<int> by_tiles(c->order_h);
+    std::stable_sort(by_tiles.begin(), by_tiles.end(),
+                     [&](int x, int y) { return c->meta_h[x].ng > c->meta_h[y].ng; });
+    std::vector<int> snake(by_tiles);
+    for (int g0 = ncu; g0 < c->nsub; g0 += 2 * ncu)
+      std::reverse(snake.begin() + g0, snake.begin() + std::min(c->nsub, g0 + ncu));
+    int *d_orders = nullptr;
+    std::vector<int> all(c->order_h);
+    all.insert(all.end(), by_tiles.begin(), by_tiles.end());
+    all.insert(all.end(), snake.begin(), snake.end());
+    if ((rc = c->upload(&d_orders, all))) return rc;
+    double bo = 1e30;
+    int pick = 0;
+    for (int v = 0; v < 3; v++) {
+      c->smv_order = d_orders + (size_t)v * c->nsub;
+      fw::SmvArgs sa = c->smv_args(false, false);
+      HIPCHK(fw::launch_smv(sa, c->nsub, c->st));
+      double t = 1e30;
+      for (int r = 0; r < 3; r++) {
+        float ms = 0.f;
+        HIPCHK(hipEventRecord(e0, c->st));
+        HIPCHK(fw::launch_smv(sa, c->nsub, c->st));
+        HIPCHK(hipEventRecord(e1, c->st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        t = std::min(t, (double)ms);
+      }
+      c->info.ms_smv_order[v] = t;
+      if (t < bo) {
+        bo = t;
+        pick = v;
+      }
+    }
+    c->smv_order = d_orders + (size_t)pick * c->nsub;
+    c->info.smv_order = pick;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }

@@ -267,7 +267,9 @@ def main():
                      "ms_solve_single_vs_bucketed": [round(info["ms_solve_single"], 4),
                                                      round(info["ms_solve_bucketed"], 4)],
                      "solve_depth": info["solve_depth"], "unit_depth": info["unit_depth"],
-                     "ms_solve_by_depth_8_16": [round(v, 4) for v in info["ms_solve_depth"]]},
+                     "ms_solve_by_depth_8_16": [round(v, 4) for v in info["ms_solve_depth"]],
+                     "smv_order": info["smv_order"],
+                     "ms_smv_by_order": [round(v, 4) for v in info["ms_smv_order"]]},
         "adjoint": adjoint,
         "sweep": sweep,
         "time_to_solution_ms": round(setup["factor"] + setup.get("schur_form", 0.0) + 1000 * dt / args.steps, 1),

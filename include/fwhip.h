@@ -102,6 +102,10 @@ typedef struct fwh_info {
                            a profile row is wider than 512), 16            */
   int    nsys;          /* systems of the batch (fwh_load_batch); the byte
                            and flop counts above cover all of them          */
+  int    smv_order;     /* work order of the explicit Schur matvec picked
+                           by timing: 0 largest skyline first, 1 most tiles
+                           first, 2 most tiles first snaking over the CUs  */
+  double ms_smv_order[3]; /* its timings                                   */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */
