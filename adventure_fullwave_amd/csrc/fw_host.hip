@@ -89,6 +89,7 @@ struct fwh_ctx {
   int *order = nullptr, *dof_sub = nullptr;
   int *idx1 = nullptr, *rowbeg = nullptr, *blk_row = nullptr;
   double2 *fac = nullptr, *dinv = nullptr;
+  double2 *minv = nullptr;  // inverted diagonal triangles of the row blocks
   int8_t *bcflag = nullptr;
   int *kidx = nullptr;
   double2 *f = nullptr;
@@ -237,6 +238,7 @@ struct fwh_ctx {
     a.g_nfid = g_nfid;
     a.g_gid = g_gid;
     a.blk_row = blk_row;
+    a.minv = minv;
     a.P = P;
     a.Z = fused ? Z : nullptr;
     a.sc = sc;
@@ -837,7 +839,8 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   if ((size_t)(wmax + 1 + nmax) * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "profile too wide for the factor kernel");
   HIPCHK(hipMemcpyAsync(c->meta, meta.data(), nsub * sizeof(SubMeta), hipMemcpyHostToDevice, st));
   if ((rc = c->upload(&c->rowbeg, rowbeg)) || (rc = c->upload(&c->blk_row, blk_row)) ||
-      (rc = c->alloc(&c->fac, (size_t)nsys * nlen_tot)) || (rc = c->alloc(&c->dinv, (size_t)nsys * ndof)))
+      (rc = c->alloc(&c->fac, (size_t)nsys * nlen_tot)) || (rc = c->alloc(&c->dinv, (size_t)nsys * ndof)) ||
+      (rc = c->alloc(&c->minv, (size_t)nsys * blk_row.size() * fw::MINV)))
     return rc;
   // work order: largest skyline first
   std::vector<int> order(nsub);
@@ -897,6 +900,8 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     fa.wmax = wmax;
     fa.nmax = nmax;
     HIPCHK(fw::launch_factor(fa, nsub, st));
+    HIPCHK(fw::launch_blockinv(c->meta, nsub, c->idx1, c->rowbeg, c->blk_row, fac,
+                               c->minv + (size_t)k * blk_row.size() * fw::MINV, st));
   }
   HIPCHK(hipStreamSynchronize(st));
   c->info.ms_factor = now_ms() - t0;
@@ -917,6 +922,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   c->ss.g = ng;
   c->ss.smat = 0;
   c->ss.hist = 0;
+  c->ss.minv = (long long)blk_row.size() * fw::MINV;
   c->sys_stats.assign(nsys, fwh_stats{});
   HIPCHK(hipMemsetAsync(c->X, 0, (size_t)nsys * ndof * sizeof(double2), st));
   HIPCHK(hipMemsetAsync(c->sc, 0, nsys * sizeof(fw::Scal), st));

@@ -41,6 +41,8 @@ constexpr int BMAX = 16;
 constexpr int PFK = 16;                   // most prefetch loads per lane
 constexpr int CAP = SOLVE_THREADS * PFK;  // most skyline entries per row block
 constexpr int VT_HINT = 256;              // threads per block of the vector kernels
+constexpr int MINV = 128;                 // entries per row block of the inverted diagonal
+                                          // triangles (16*15/2 = 120 used, packed by rows)
 
 // Systems of a batch (one per frequency of a sweep, say) share the mesh, the
 // decomposition and every index array; their values are concatenated
@@ -55,6 +57,7 @@ struct SysStride {
   long long g;     // interface vectors
   long long smat;  // Schur-complement tiles
   long long hist;  // residual history
+  long long minv;  // inverted diagonal triangles of the row blocks
 };
 
 struct SolveArgs {
@@ -84,6 +87,7 @@ struct SolveArgs {
   const int *g_nfid;     // per slot: local DOF
   const int *g_gid;      // per slot: global interface index
   const int *blk_row;    // per subdomain nblk+1 block start rows
+  const double2 *minv;   // per block (index in blk_row) its inverted diagonal triangle
   const double2 *P;      // global interface values
   const double2 *Z;      // MODE_ITER: p = Z + beta P (null: p = P)
   const Scal *sc;        // beta of the interface iteration
@@ -212,6 +216,10 @@ hipError_t launch_skydiag_bc(int ndof, const int *dof_sub, const SubMeta *meta,
                              const int8_t *bcflag, const int *rowbeg, const int *idx1, double2 *fac,
                              hipStream_t st);
 hipError_t launch_factor(const FactorArgs &a, int nwork, hipStream_t st);
+// inverted unit lower triangles of every row block (both partitions) of
+// every subdomain, from the factors
+hipError_t launch_blockinv(const SubMeta *meta, int nsub, const int *idx1, const int *rowbeg, const int *blk_row,
+                           const double2 *fac, double2 *minv, hipStream_t st);
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st);
 size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk);
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st);

@@ -34,6 +34,7 @@ struct Pf2 {
 template <int K>
 struct PfSet {
   Pf2 e[K];
+  Pf2 m[MINV / 64];  // the block's inverted diagonal triangle
 };
 
 // Branch-free block loads: addresses past the block are clamped to its last
@@ -41,7 +42,8 @@ struct PfSet {
 // flow and the compiler can keep the other register set's loads in flight
 // (a counted vmcnt wait instead of a full drain).
 template <int K>
-__device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, int e0, int e1, PfSet<K> &p) {
+__device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, int e0, int e1,
+                                            const double2 *__restrict__ minv, PfSet<K> &p) {
   const int lane = threadIdx.x;
   const int last = e1 - 1;
 #pragma unroll
@@ -50,14 +52,22 @@ __device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, int
     p.e[k].x = t.x;
     p.e[k].y = t.y;
   }
+#pragma unroll
+  for (int k = 0; k < MINV / WL; k++) {
+    const double2 t = minv[lane + k * WL];
+    p.m[k].x = t.x;
+    p.m[k].y = t.y;
+  }
 }
 
 // Unconditional staging: entries past the block land in unused LDS slots.
 template <int K>
-__device__ __forceinline__ void stage_block(double2 *stg, const PfSet<K> &p) {
+__device__ __forceinline__ void stage_block(double2 *stg, double2 *mst, const PfSet<K> &p) {
   const int lane = threadIdx.x;
 #pragma unroll
   for (int k = 0; k < K; k++) stg[lane + k * WL] = make_double2(p.e[k].x, p.e[k].y);
+#pragma unroll
+  for (int k = 0; k < MINV / WL; k++) mst[lane + k * WL] = make_double2(p.m[k].x, p.m[k].y);
 }
 
 // The workgroup is a single wavefront: LDS traffic of one wave is processed
@@ -67,9 +77,12 @@ __device__ __forceinline__ void stage_block(double2 *stg, const PfSet<K> &p) {
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
 struct SolveLds {
-  double2 *v, *stg;
-  int *idx1, *rb, *blk;
+  double2 *v, *stg, *mst;       // LDS: vector, staged block, staged inverse triangle
+  int *idx1, *rb, *blk;         // LDS: row metadata, block rows
+  const double2 *minv;          // global: inverse triangles of this subdomain's blocks
 };
+// inverse triangle of block b (packed rows, MINV entries per block)
+__device__ __forceinline__ const double2 *minv_of(const SolveLds &L, int b) { return L.minv + (long long)b * MINV; }
 
 // forward block b: rows [i0,i1) with skyline entries [e0,e1)
 template <int K>
@@ -77,14 +90,14 @@ __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__re
                                           PfSet<K> &p) {
   const int lane = threadIdx.x;
   const int i0 = L.blk[b], i1 = L.blk[b + 1];
-  const int e0 = L.rb[i0], e1 = L.rb[i1];
+  const int e0 = L.rb[i0];
   const int nb = i1 - i0;
   wave_sync();
-  stage_block(L.stg, p);
+  stage_block(L.stg, L.mst, p);
   wave_sync();
   {
     const int bn = min(b + 2, nblk - 1);  // unconditional: see issue_block
-    issue_block(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], p);
+    issue_block(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
   }
 
   // A: columns before the block; rows r = lane/4, four lanes per row
@@ -108,36 +121,32 @@ __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__re
   acc.x += __shfl_xor(acc.x, 2, 4);
   acc.y += __shfl_xor(acc.y, 2, 4);
 
-  // B: the block's own triangle, row r resolved by lane 4r (the quad that
-  // summed its columns before the block already holds the sum there)
+  // B: the block's own triangle.  With s = v_B - (columns before the block)
+  // and M = L_BB^-1 (unit lower, precomputed at load), x_r = s_r +
+  // sum_{q<r} M_rq s_q: four lanes per row, one term of every fourth column
+  // each, then a quad sum (no chain of dependent steps)
   const int r = ra;
   const bool own = la == 0 && r < nb;
   double2 s = cz();
-  int fr = i0, rbr = 0;
   if (own) {
-    fr = L.idx1[i0 + r];
-    rbr = L.rb[i0 + r] - e0 - fr;
     s = csub(L.v[i0 + r], acc);
+    L.v[i0 + r] = s;
   }
-  // operands of the chain first (independent LDS loads), then the chain of
-  // readlane + multiply-subtract steps
-  Pf2 lq[BMAX];
+  wave_sync();
+  double2 t = cz();
 #pragma unroll
-  for (int q = 0; q < BMAX; q++) {
-    const int j = i0 + q;
-    const bool use = q < r && own && j >= fr;
-    const double2 t = L.stg[use ? rbr + j : 0];
-    lq[q].x = use ? t.x : 0.0;
-    lq[q].y = use ? t.y : 0.0;
+  for (int k = 0; k < 4; k++) {
+    const int q = la + 4 * k;
+    const bool use = q < r && r < nb;
+    const double2 mq = L.mst[use ? (r * (r - 1)) / 2 + q : 0];
+    const double2 sq = L.v[use ? i0 + q : 0];
+    t = cfma(t, use ? mq : cz(), sq);
   }
-#pragma unroll
-  for (int q = 0; q < BMAX; q++) {
-    if (q < nb) {
-      const double2 xq = readlane_c(s, 4 * q);
-      if (r > q) s = cfms(s, make_double2(lq[q].x, lq[q].y), xq);
-    }
-  }
-  if (own) L.v[i0 + r] = s;
+  t.x += __shfl_xor(t.x, 1, 4);
+  t.y += __shfl_xor(t.y, 1, 4);
+  t.x += __shfl_xor(t.x, 2, 4);
+  t.y += __shfl_xor(t.y, 2, 4);
+  if (own) L.v[i0 + r] = cadd(s, t);
 }
 
 // backward block b (rows descending): triangle, then columns before the block
@@ -146,14 +155,14 @@ __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__re
                                           PfSet<K> &p) {
   const int lane = threadIdx.x;
   const int i0 = L.blk[b], i1 = L.blk[b + 1];
-  const int e0 = L.rb[i0], e1 = L.rb[i1];
+  const int e0 = L.rb[i0];
   const int nb = i1 - i0;
   wave_sync();
-  stage_block(L.stg, p);
+  stage_block(L.stg, L.mst, p);
   wave_sync();
   {
     const int bn = max(b - 2, 0);
-    issue_block(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], p);
+    issue_block(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
   }
 
   // row metadata of the block: lane r holds row r's first column and base
@@ -162,28 +171,30 @@ __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__re
     fR = L.idx1[i0 + lane];
     rbR = L.rb[i0 + lane] - e0 - fR;
   }
-  // A: triangle, lane q owns column q
-  const int q = lane;
-  double2 x = q < nb ? L.v[i0 + q] : cz();
-  Pf2 lc[BMAX];
+  // A: triangle through M^T (L_BB^T x = v_B): x_q = v_q + sum_{r>q} M_rq v_r,
+  // column q on lanes 4q..4q+3, one term of every fourth row each
+  const int qc = lane >> 2, la = lane & 3;
+  double2 t = cz();
 #pragma unroll
-  for (int r = 0; r < BMAX; r++) {
-    const int f_u = readlane_i(fR, r), rb_u = readlane_i(rbR, r);
-    const bool use = r < nb && q < r && i0 + q >= f_u;
-    const double2 t = L.stg[use ? rb_u + i0 + q : 0];
-    lc[r].x = use ? t.x : 0.0;
-    lc[r].y = use ? t.y : 0.0;
+  for (int k = 0; k < 4; k++) {
+    const int r = qc + 1 + la + 4 * k;
+    const bool use = r < nb;
+    const double2 mr = L.mst[use ? (r * (r - 1)) / 2 + qc : 0];
+    const double2 vr = L.v[use ? i0 + r : 0];
+    t = cfma(t, use ? mr : cz(), vr);
   }
-#pragma unroll
-  for (int r = BMAX - 1; r >= 0; r--) {
-    if (r < nb) {
-      const double2 xr = readlane_c(x, r);
-      if (q < r) x = cfms(x, make_double2(lc[r].x, lc[r].y), xr);
-    }
-  }
-  if (q < nb) L.v[i0 + q] = x;
+  t.x += __shfl_xor(t.x, 1, 4);
+  t.y += __shfl_xor(t.y, 1, 4);
+  t.x += __shfl_xor(t.x, 2, 4);
+  t.y += __shfl_xor(t.y, 2, 4);
+  const bool own = la == 0 && qc < nb;
+  double2 x = cz();
+  if (own) x = cadd(L.v[i0 + qc], t);
+  wave_sync();
+  if (own) L.v[i0 + qc] = x;
 
-  // B: every column before the block receives all rows of the block
+  // B: every column before the block receives all rows of the block (row r's
+  // value on lane 4r)
   int jlo = fR;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) jlo = min(jlo, __shfl_xor(jlo, o, 64));
@@ -195,7 +206,7 @@ __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__re
     for (int r = 0; r < BMAX; r++) {
       if (r < nb) {
         const int f_u = readlane_i(fR, r), rb_u = readlane_i(rbR, r);
-        const double2 xr = readlane_c(x, r);
+        const double2 xr = readlane_c(x, 4 * r);
         const bool in = j >= f_u;
         acc = cfma(acc, L.stg[in ? rb_u + j : 0], in ? xr : cz());
       }
@@ -220,6 +231,7 @@ __device__ __forceinline__ void to_system(SolveArgs &a) {
   a.yslot += s * a.ss.slot;
   a.X += s * a.ss.dof;
   if (a.smat) a.smat += s * a.ss.smat;
+  a.minv += s * a.ss.minv;
 }
 
 template <int K>  // prefetch loads per lane: row blocks of at most 64*K entries
@@ -242,7 +254,8 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   SolveLds L;
   L.v = lds;                          // [nmax]
   L.stg = L.v + a.nmax;               // [CAPK]
-  L.idx1 = (int *)(L.stg + CAPK);     // [nmax]
+  L.mst = L.stg + CAPK;               // [MINV]
+  L.idx1 = (int *)(L.mst + MINV);     // [nmax]
   L.rb = L.idx1 + a.nmax;             // [nmax+1]
   L.blk = L.rb + a.nmax + 1;          // [nblkmax+1]
   const int lane = threadIdx.x;
@@ -251,6 +264,7 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   // row blocks of this prefetch depth (the host keeps both partitions)
   const int nblk = K == 8 ? m.nblk_s : m.nblk;
   const int blk0 = K == 8 ? m.blk0_s : m.blk0;
+  L.minv = a.minv + (long long)blk0 * MINV;
 
   // ---- 0. metadata --------------------------------------------------------------
   for (int i = lane; i < n; i += WL) {
@@ -307,8 +321,8 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   // (the host pads every subdomain to an even number >= 2 of row blocks, so
   // the sweeps run over block pairs without conditional loads: a branch
   // around a load would make the compiler drain every load in flight)
-  issue_block(fac, L.rb[L.blk[0]], L.rb[L.blk[1]], pa);
-  issue_block(fac, L.rb[L.blk[1]], L.rb[L.blk[2]], pb);
+  issue_block(fac, L.rb[L.blk[0]], L.rb[L.blk[1]], minv_of(L, 0), pa);
+  issue_block(fac, L.rb[L.blk[1]], L.rb[L.blk[2]], minv_of(L, 1), pb);
   for (int b = 0; b < nblk; b += 2) {
     fwd_block(L, fac, nblk, b, pa);
     fwd_block(L, fac, nblk, b + 1, pb);
@@ -324,8 +338,8 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   // in pa, the one before in pb, alternating from there (the register sets
   // are always named statically, so they stay in registers)
   const int nbl = nblk - 1;
-  issue_block(fac, L.rb[L.blk[nbl]], L.rb[L.blk[nbl + 1]], pa);
-  issue_block(fac, L.rb[L.blk[nbl - 1]], L.rb[L.blk[nbl]], pb);
+  issue_block(fac, L.rb[L.blk[nbl]], L.rb[L.blk[nbl + 1]], minv_of(L, nbl), pa);
+  issue_block(fac, L.rb[L.blk[nbl - 1]], L.rb[L.blk[nbl]], minv_of(L, nbl - 1), pb);
   wave_sync();
 
   // ---- 4. backward sweep ---------------------------------------------------------------------
@@ -364,9 +378,48 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   }
 }
 
+// Inverse of every row block's unit lower triangle L_BB (the factor's strict
+// lower entries inside the block, unit diagonal), packed by rows: entry
+// (r, q), q < r, at r(r-1)/2 + q, the rest of the MINV slots zero.
+// (L^-1)_rq = -(L_rq + sum_{q<j<r} L_rj (L^-1)_jq).  One workgroup per
+// subdomain, one thread per block, both row-block partitions.
+__global__ __launch_bounds__(64) void k_blockinv(const SubMeta *__restrict__ meta, const int *__restrict__ idx1,
+                                                 const int *__restrict__ rowbeg, const int *__restrict__ blk_row,
+                                                 const double2 *__restrict__ fac_all, double2 *__restrict__ minv) {
+  const SubMeta m = meta[blockIdx.x];
+  const double2 *fac = fac_all + m.fac0;
+  for (int part = 0; part < 2; part++) {
+    const int nblk = part ? m.nblk_s : m.nblk, blk0 = part ? m.blk0_s : m.blk0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+      const int i0 = blk_row[blk0 + b], nb = blk_row[blk0 + b + 1] - i0;
+      double2 M[MINV];
+      for (int t = 0; t < MINV; t++) M[t] = cz();
+      for (int r = 1; r < nb; r++) {
+        const int f = idx1[m.dof0 + i0 + r], rb = rowbeg[m.dof0 + i0 + r];
+        for (int q = 0; q < r; q++) {
+          double2 acc = (i0 + q >= f) ? fac[rb + (i0 + q - f)] : cz();
+          for (int j = q + 1; j < r; j++)
+            if (i0 + j >= f) acc = cfma(acc, fac[rb + (i0 + j - f)], M[(j * (j - 1)) / 2 + q]);
+          M[(r * (r - 1)) / 2 + q] = csub(cz(), acc);
+        }
+      }
+      double2 *out = minv + (long long)(blk0 + b) * MINV;
+      for (int t = 0; t < MINV; t++) out[t] = M[t];
+    }
+  }
+}
+
+hipError_t launch_blockinv(const SubMeta *meta, int nsub, const int *idx1, const int *rowbeg, const int *blk_row,
+                           const double2 *fac, double2 *minv, hipStream_t st) {
+  if (nsub <= 0) return hipSuccess;
+  k_blockinv<<<nsub, 64, 0, st>>>(meta, idx1, rowbeg, blk_row, fac, minv);
+  return hipGetLastError();
+}
+
 size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk) {
   (void)ngmax;
-  return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk) + sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
+  return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + MINV) +
+         sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
 }
 
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
