@@ -121,6 +121,7 @@ struct fwh_ctx {
   double2 *smat = nullptr;
   long long *sm_off = nullptr;
   int *smv_order = nullptr;  // work order of the matvec (picked at formation)
+  fw::SmvWork *smv_work = nullptr;  // per work item of that order: tiles, slots
   int nbmax = 0;
   int smv_waves = 4;
   bool schur_built = false;
@@ -257,6 +258,7 @@ struct fwh_ctx {
     a.ss = ss;
     a.meta = meta;
     a.order = smv_order ? smv_order : order;
+    a.work = smv_work;
     a.base = 0;
     a.done = check_done ? &sc->done : nullptr;
     a.nbmax = nbmax;
@@ -315,6 +317,7 @@ struct fwh_ctx {
     fw::SmvArgs a = smv_args(check_done, fused, only);
     if (!plain) return fw::launch_smv(a, nsub, st);
     a.order = nullptr;
+    a.work = nullptr;
     for (int s = 0; s < nsub; s++) {
       a.base = s;
       hipError_t e = fw::launch_smv(a, 1, st);
@@ -1187,14 +1190,23 @@ int build_schur(fwh_ctx *c) {
     for (int g0 = ncu; g0 < c->nsub; g0 += 2 * ncu)
       std::reverse(snake.begin() + g0, snake.begin() + std::min(c->nsub, g0 + ncu));
     int *d_orders = nullptr;
+    fw::SmvWork *d_work = nullptr;
     std::vector<int> all(c->order_h);
     all.insert(all.end(), by_tiles.begin(), by_tiles.end());
     all.insert(all.end(), snake.begin(), snake.end());
-    if ((rc = c->upload(&d_orders, all))) return rc;
+    std::vector<fw::SmvWork> work(all.size());
+    for (size_t i = 0; i < all.size(); i++) {
+      const int sd = all[i];
+      work[i].off = off[sd];
+      work[i].g0 = c->meta_h[sd].g0;
+      work[i].ng = c->meta_h[sd].ng;
+    }
+    if ((rc = c->upload(&d_orders, all)) || (rc = c->upload(&d_work, work))) return rc;
     double bo = 1e30;
     int pick = 0;
     for (int v = 0; v < 3; v++) {
       c->smv_order = d_orders + (size_t)v * c->nsub;
+      c->smv_work = d_work + (size_t)v * c->nsub;
       fw::SmvArgs sa = c->smv_args(false, false);
       HIPCHK(fw::launch_smv(sa, c->nsub, c->st));
       double t = 1e30;
@@ -1214,6 +1226,7 @@ int build_schur(fwh_ctx *c) {
       }
     }
     c->smv_order = d_orders + (size_t)pick * c->nsub;
+    c->smv_work = d_work + (size_t)pick * c->nsub;
     c->info.smv_order = pick;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -99,9 +99,19 @@ struct SolveArgs {
   const long long *sm_off;  // per subdomain first entry of its tiles
 };
 
+// one workgroup of the explicit Schur matvec: where its subdomain's tiles,
+// slots and interface count are (one scalar load instead of order -> meta ->
+// tile offset)
+struct SmvWork {
+  long long off;  // first tile entry
+  int g0;         // first slot
+  int ng;         // interface DOFs
+};
+
 struct SmvArgs {
   int sys0, nsys;
   SysStride ss;
+  const SmvWork *work;   // per work item (batched launches), or null: order/meta/sm_off
   const SubMeta *meta;
   const int *order;
   int base;

@@ -707,12 +707,27 @@ hipError_t launch_first(const VecArgs &a, hipStream_t st) {
 // hddm_solver_s.h:370-376) is set by k_finalize_p before the last solve.
 __global__ __launch_bounds__(VT) void k_iter_a(VecArgs a) {
   to_system(a);
+  // the operands of this thread's first unknown are loaded before the
+  // status, so the two latencies overlap
+  const int g0 = blockIdx.x * VT + threadIdx.x;
+  double2 q0 = cz(), z0 = cz(), p0 = cz();
+  if (g0 < a.ng) {
+    q0 = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g0);
+    z0 = a.Z[g0];
+    p0 = a.P[g0];
+  }
   const Scal *s = a.sc;
   if (s->done) return;
   __shared__ double2 shc[VT / 64 + 1];
   const double2 beta = s->beta;
   double2 pq = cz();
-  for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT) {
+  if (g0 < a.ng) {
+    const double2 p = zbp(z0, beta, p0);
+    a.Q[g0] = q0;
+    a.P[g0] = p;
+    pq = cmul(p, q0);
+  }
+  for (int g = g0 + a.G * VT; g < a.ng; g += a.G * VT) {
     const double2 q = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
     const double2 p = zbp(a.Z[g], beta, a.P[g]);
     a.Q[g] = q;
@@ -724,6 +739,16 @@ __global__ __launch_bounds__(VT) void k_iter_a(VecArgs a) {
 }
 __global__ __launch_bounds__(VT) void k_iter_b(VecArgs a) {
   to_system(a);
+  // operands of the first unknown before the status and the alpha reduction
+  const int g0 = blockIdx.x * VT + threadIdx.x;
+  double2 u0 = cz(), p0 = cz(), r0 = cz(), q0 = cz(), pc0 = cz();
+  if (g0 < a.ng) {
+    u0 = a.U[g0];
+    p0 = a.P[g0];
+    r0 = a.R[g0];
+    q0 = a.Q[g0];
+    pc0 = a.PC[g0];
+  }
   const Scal *s = a.sc;
   if (s->done) return;
   __shared__ double2 shc[VT / 64 + 1];
@@ -734,7 +759,17 @@ __global__ __launch_bounds__(VT) void k_iter_b(VecArgs a) {
   if (!(cabsv(rz) < NEARLY_ZERO && cabsv(pq) < NEARLY_ZERO)) malpha = csub(cz(), cdiv_naive(rz, pq));
   double2 rzn = cz();
   double mx = 0.0;
-  for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT) {
+  if (g0 < a.ng) {
+    const double2 u = cadd(u0, cmul(malpha, p0));
+    const double2 r = cadd(r0, cmul(malpha, q0));
+    const double2 z = cmul(r, pc0);
+    a.U[g0] = u;
+    a.R[g0] = r;
+    a.Z[g0] = z;
+    mx = cabsv(r);
+    rzn = cmul(r, z);
+  }
+  for (int g = g0 + a.G * VT; g < a.ng; g += a.G * VT) {
     const double2 u = cadd(a.U[g], cmul(malpha, a.P[g]));
     const double2 r = cadd(a.R[g], cmul(malpha, a.Q[g]));
     const double2 z = cmul(r, a.PC[g]);
@@ -754,11 +789,13 @@ __global__ __launch_bounds__(VT) void k_iter_b(VecArgs a) {
 __global__ __launch_bounds__(VT) void k_iter_c(VecArgs a) {
   to_system(a);
   Scal *s = a.sc;
-  if (s->done) return;
   __shared__ double2 shc[VT / 64 + 1];
   __shared__ double shm[VT / 64 + 1];
+  // (the reductions do not depend on the status: it is tested after them, a
+  // uniform branch, so both loads are in flight together)
   const double2 rzs = reduce_parts(a.part + a.G, a.G, shc);
   const double mxs = reduce_parts_max(a.part + 2 * a.G, a.G, shm);
+  if (s->done) return;
   if (threadIdx.x == 0) {
     const int loop = s->loop;
     const double resi = mxs * s->r0inv;

@@ -168,10 +168,16 @@ template <int W>  // waves per workgroup
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) void k_smv(SmvArgs a) {
   constexpr int SMV_WAVES = W, NT = 64 * W;
   to_system(a);
-  if (a.done && *a.done) return;
-  const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
-  const SubMeta m = a.meta[w];
-  const int ng = m.ng, nb = (ng + 15) >> 4, npad = nb * 16, stride = a.nbmax * 16;
+  SmvWork wk;
+  if (a.work) {
+    wk = a.work[a.base + blockIdx.x];
+  } else {
+    const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
+    wk.off = a.sm_off[w];
+    wk.g0 = a.meta[w].g0;
+    wk.ng = a.meta[w].ng;
+  }
+  const int ng = wk.ng, nb = (ng + 15) >> 4, npad = nb * 16, stride = a.nbmax * 16;
   extern __shared__ double2 lds[];
   double2 *pv = lds;          // [nbmax*16] interface values of the subdomain
   double2 *acc = pv + stride; // [SMV_WAVES][nbmax*16] per-wave accumulators
@@ -189,7 +195,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   const long long t0 = ntile * wave / SMV_WAVES, t1 = ntile * (wave + 1) / SMV_WAVES;
   const bool active = t0 < t1;
   const long long tl = active ? t1 - 1 : 0;
-  const double2 *base = a.smat + a.sm_off[w];
+  const double2 *base = a.smat + wk.off;
   Cursor cur;
   cur.t = active ? t0 : 0;
   cur.tl = tl;
@@ -198,6 +204,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   Tile A, B;
   load_next(base, zero, cur, lane, A);
   load_next(base, zero, cur, lane, B);
+  // the status is read once the first tiles are in flight (a finished
+  // system's workgroups discard them)
+  if (a.done && *a.done) return;
 
   // interface values: P, or the next search direction z + beta p while the
   // interface iteration runs (formed here, not stored, see k_iter_a)
@@ -205,7 +214,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   for (int k = tid; k < npad; k += NT) {
     double2 p = cz();
     if (k < ng) {
-      const int gid = a.g_gid[m.g0 + k];
+      const int gid = a.g_gid[wk.g0 + k];
       p = a.P[gid];
       if (a.Z) p = zbp(a.Z[gid], beta, p);
     }
@@ -236,7 +245,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   for (int k = tid; k < ng; k += NT) {
     double2 y = cadd(acc[k], acc[stride + k]);
     for (int v = 2; v < SMV_WAVES; v++) y = cadd(y, acc[v * stride + k]);
-    a.yslot[m.g0 + k] = y;
+    a.yslot[wk.g0 + k] = y;
   }
 }
 

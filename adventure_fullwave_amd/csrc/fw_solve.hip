@@ -238,7 +238,10 @@ template <int K>  // prefetch loads per lane: row blocks of at most 64*K entries
 __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 3 : 2))) void k_solve(SolveArgs a) {
   constexpr int CAPK = WL * K;
   to_system(a);
-  if (a.mode == MODE_ITER && a.done && *a.done) return;
+  // (the status of an iteration launch is read below, once the metadata
+  // loads are in flight)
+  const bool check_done = a.mode == MODE_ITER && a.done;
+  const int done0 = check_done ? *a.done : 0;
   int w, uk = a.unit_k;
   if (a.mode == MODE_UNIT && a.unit_list) {
     const int2 pr = a.unit_list[a.base + blockIdx.x];  // (subdomain, unit vector); -1: padding
@@ -274,6 +277,7 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   if (lane == 0) L.rb[n] = (int)m.nlen;
   for (int b = lane; b <= nblk; b += WL) L.blk[b] = a.blk_row[blk0 + b];
   __syncthreads();
+  if (done0) return;
 
 
   // ---- 1. load vector --------------------------------------------------------------
