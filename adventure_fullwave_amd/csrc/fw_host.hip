@@ -140,7 +140,10 @@ struct fwh_ctx {
   hipEvent_t ev_fork = nullptr, ev_join[8] = {};
   double ms_single = 0.0, ms_bucketed = 0.0;
 
-  // graph of batched iterations
+  // graph of batched iterations; status copies of two graphs in flight
+  // (pinned host slots, one event each)
+  fw::Scal *scal_pin = nullptr;  // [2 * nsys]
+  hipEvent_t ev_scal[2] = {};
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_n = 0;
@@ -186,6 +189,12 @@ struct fwh_ctx {
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     ev_fork = nullptr;
+    for (int k = 0; k < 2; k++) {
+      if (ev_scal[k]) (void)hipEventDestroy(ev_scal[k]);
+      ev_scal[k] = nullptr;
+    }
+    if (scal_pin) (void)hipHostFree(scal_pin);
+    scal_pin = nullptr;
   }
   // one batched k_solve launch over all subdomains, or one per size bucket
   hipError_t launch_solve_all(fw::SolveArgs a, bool bucketed) {
@@ -1320,10 +1329,33 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
       c->graph_n = o->graph_iters;
       c->graph_schur = o->schur;
     }
-    while (it < n && !all) {
+    // two graphs in flight: graph k+1 is queued before the status after
+    // graph k is read (from a pinned copy made in stream order), so the
+    // device does not wait for the host between graphs.  A graph queued
+    // after the status turned LAST runs its kernels as no-ops; no more
+    // graphs are launched than n allows.
+    if (!c->scal_pin) {
+      HIPCHK(hipHostMalloc((void **)&c->scal_pin, 2 * c->nsys * sizeof(fw::Scal), hipHostMallocDefault));
+      for (int k = 0; k < 2; k++) HIPCHK(hipEventCreateWithFlags(&c->ev_scal[k], hipEventDisableTiming));
+    }
+    int queued = 0, slot = 0, waited = 0;
+    auto launch_one = [&]() -> int {
       HIPCHK(hipGraphLaunch(c->gexec, c->st));
-      if ((rc = read_scal(c, s, &all))) return rc;
+      HIPCHK(hipMemcpyAsync(c->scal_pin + (queued & 1) * c->nsys, c->sc, c->nsys * sizeof(fw::Scal),
+                            hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipEventRecord(c->ev_scal[queued & 1], c->st));
+      queued++;
       it += c->graph_n;
+      return 0;
+    };
+    if (it < n && !all && (rc = launch_one())) return rc;
+    while (waited < queued) {
+      if (it < n && !all && (rc = launch_one())) return rc;  // the next graph before waiting
+      slot = waited & 1;
+      HIPCHK(hipEventSynchronize(c->ev_scal[slot]));
+      waited++;
+      all = 1;
+      for (int k = 0; k < c->nsys; k++) all &= c->scal_pin[slot * c->nsys + k].done != 0;
     }
   } else {
     while (it < n && !all) {
