@@ -847,7 +847,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     return fail(FWH_E_LIMIT, "subdomain of %d DOFs / %d interface DOFs exceeds the LDS budget", nmax, ngmax);
   if (c->short_ok && fw::solve_lds_bytes(nmax, ngmax, c->nblkmax_s, 8) > 160 * 1024) c->short_ok = false;
   c->pfk_iter = fw::PFK;
-  c->pfk_unit = c->short_ok ? 8 : fw::PFK;
+  c->pfk_unit = fw::PFK;  // both picked after the factorisation, by timing
   if ((size_t)(wmax + 1 + nmax) * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "profile too wide for the factor kernel");
   HIPCHK(hipMemcpyAsync(c->meta, meta.data(), nsub * sizeof(SubMeta), hipMemcpyHostToDevice, st));
   if ((rc = c->upload(&c->rowbeg, rowbeg)) || (rc = c->upload(&c->blk_row, blk_row)) ||
@@ -992,6 +992,10 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     const double t16 = std::min(best[0][0], best[0][1]), t8 = std::min(best[1][0], best[1][1]);
     const int d = (depths == 2 && t8 < 0.95 * t16) ? 1 : 0;
     c->pfk_iter = d == 0 ? fw::PFK : 8;
+    // the Schur formation runs hundreds of waves per CU, where the 8-deep
+    // kernel's higher occupancy pays unless its single solve is much slower
+    // (wide profiles cut into many short blocks: cfg 3's large subdomains)
+    c->pfk_unit = (depths == 2 && t8 <= 1.2 * t16) ? 8 : fw::PFK;
     c->ms_single = best[d][0];
     c->ms_bucketed = variants == 2 ? best[d][1] : 0.0;
     c->use_buckets = variants == 2 && best[d][1] < best[d][0];
