@@ -257,6 +257,7 @@ def main():
         "batched_ldl_kernel_solves_per_s": round(info["nsub"] / (ms_solve * 1e-3), 1),
         "implicit" if expl else "explicit": other,
         "plain": {"iters_per_s": round(plain_iters_per_s, 3), "window_iters": args.plain_iters,
+                  "ldl_solves_per_s": round(plain_iters_per_s * info["nsub"], 1),
                   "form": "implicit, one subdomain per launch"},
         "batched_over_plain": round(iters_per_s / plain_iters_per_s, 2),
         "roofline": roofline,
