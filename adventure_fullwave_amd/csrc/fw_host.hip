@@ -1083,7 +1083,7 @@ int build_schur(fwh_ctx *c) {
   for (int s = 0; s < c->nsub; s++) {
     const long long nb = (c->meta_h[s].ng + 15) / 16;
     off[s] = tot;
-    tot += fw::tile_off((int)nb, 0);
+    tot += fw::smv_entries(c->meta_h[s].ng);
     nbmax = std::max(nbmax, (int)nb);
   }
   if (fw::smv_lds_bytes(nbmax, 2) > 160 * 1024)

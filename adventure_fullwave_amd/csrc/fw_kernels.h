@@ -34,6 +34,18 @@ constexpr int SMV_TILE = 256;
 __host__ __device__ __forceinline__ long long tile_off(int I, int J) {
   return 128LL * I * (I - 1) + 192LL * I + 256LL * J;
 }
+// When the last row block holds 8 or fewer rows (ng mod 16 in 1..8), its
+// tiles keep only their top half: sub-blocks 0 and 1 (128 entries), the
+// diagonal tile sub-block 0 (64 entries); the bottom half would be padding.
+__host__ __device__ __forceinline__ bool smv_half(int ng) { return ng > 0 && ((ng - 1) & 15) < 8; }
+__host__ __device__ __forceinline__ long long tile_at(int I, int J, int nb, bool half) {
+  return (half && I == nb - 1) ? tile_off(I, 0) + 128LL * J : tile_off(I, J);
+}
+// tile entries of a subdomain with ng interface DOFs
+__host__ __device__ __forceinline__ long long smv_entries(int ng) {
+  const int nb = (ng + 15) >> 4;
+  return smv_half(ng) ? tile_off(nb - 1, 0) + 128LL * (nb - 1) + 64 : tile_off(nb, 0);
+}
 
 // solve kernel limits: rows per block, prefetch registers per thread
 constexpr int SOLVE_THREADS = 64;

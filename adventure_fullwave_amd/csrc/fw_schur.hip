@@ -19,8 +19,10 @@
 // contiguously and lane l holds rows l&7, (l&7)+8 and columns l>>3, (l>>3)+8.
 // Diagonal tiles hold the strict lower part and half the diagonal; their
 // sub-block 1 (rows 0-7, columns 8-15) is entirely upper and not stored, so
-// they take 192 entries (sub-blocks 0, 2, 3).  Every tile T is applied as
-// T + T^T, which restores S_s exactly.
+// they take 192 entries (sub-blocks 0, 2, 3).  When the last row block has 8
+// or fewer rows, its tiles keep only sub-blocks 0-1 (the diagonal one
+// sub-block 0): the rest would be padding (tile_at, smv_entries).  Every
+// tile T is applied as T + T^T, which restores S_s exactly.
 #include "fw_kernels.h"
 #include "fw_device.h"
 
@@ -55,12 +57,17 @@ __device__ __forceinline__ void tile_ij(long long t, int &I, int &J) {
 struct Cursor {
   long long t, tl;  // next tile, last tile of the wave
   int I, J;         // its row block and column block
+  int nb;           // row blocks of the subdomain
+  bool half;        // its last row block keeps the top half only
 };
 __device__ __forceinline__ void load_next(const double2 *__restrict__ base, const double2 *__restrict__ zero,
                                           Cursor &c, int lane, Tile &T) {
   const bool dg = c.I == c.J;
-  const double2 *p = base + tile_off(c.I, c.J) + lane;
-  const double2 *q1 = dg ? zero + lane : p + 64, *q2 = p + (dg ? 64 : 128), *q3 = p + (dg ? 128 : 192);
+  const bool hf = c.half && c.I == c.nb - 1;  // bottom sub-blocks not stored: read zeros
+  const double2 *p = base + tile_at(c.I, c.J, c.nb, c.half) + lane;
+  const double2 *q1 = dg ? zero + lane : p + 64;
+  const double2 *q2 = hf ? zero + lane : p + (dg ? 64 : 128);
+  const double2 *q3 = hf ? zero + lane : p + (dg ? 128 : 192);
   T.e[0].x = __builtin_nontemporal_load(&p->x);
   T.e[0].y = __builtin_nontemporal_load(&p->y);
   T.e[1].x = __builtin_nontemporal_load(&q1->x);
@@ -199,6 +206,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   Cursor cur;
   cur.t = active ? t0 : 0;
   cur.tl = tl;
+  cur.nb = nb;
+  cur.half = smv_half(ng);
   tile_ij(cur.t, cur.I, cur.J);
   const double2 *zero = a.smat;  // the 64 zero entries ahead of every subdomain's tiles
   Tile A, B;

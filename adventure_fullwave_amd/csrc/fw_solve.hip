@@ -360,6 +360,8 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
     // column unit_k of S_s: rows k >= unit_k into the lower-triangle tiles
     // (layout in fw_schur.hip), the diagonal halved
     const int j = uk, J = j >> 4, cj = j & 15;
+    const int nbk = (m.ng + 15) >> 4;
+    const bool hlf = smv_half(m.ng);
     double2 *dst = a.smat + a.sm_off[w];
     for (int k = j + lane; k < m.ng; k += WL) {
       const int sl = m.g0 + k;
@@ -369,7 +371,8 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
       const int I = k >> 4, ri = k & 15;
       int sb = (ri >> 3) * 2 + (cj >> 3);
       if (I == J) sb -= (sb > 0);  // diagonal tile: sub-block 1 is not stored
-      dst[tile_off(I, J) + sb * 64 + ((ri & 7) | ((cj & 7) << 3))] = y;
+      // (a half last row block has rows 0-7 only: sb stays 0 or 1)
+      dst[tile_at(I, J, nbk, hlf) + sb * 64 + ((ri & 7) | ((cj & 7) << 3))] = y;
     }
   } else if (a.mode != MODE_RHS) {
     for (int k = lane; k < m.ng; k += WL) {
