@@ -28,15 +28,19 @@ struct SubMeta {
 enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3, MODE_UNIT = 4 };
 
 // explicit Schur complements: 16x16 tiles of the lower triangle, row block
-// by row block; off-diagonal tiles 256 entries, diagonal tiles 192 (their
-// upper 8x8 sub-block is not stored)
+// by row block; off-diagonal tiles 256 entries (four 8x8 sub-blocks),
+// diagonal tiles 136: their two diagonal 8x8 sub-blocks as packed lower
+// triangles (36 each, entry (r, c), c <= r, at r(r+1)/2 + c) around the full
+// sub-block 2 (rows 8-15, columns 0-7, at 36); sub-block 1 is upper
 constexpr int SMV_TILE = 256;
+constexpr int SMV_DIAG = 136;
 __host__ __device__ __forceinline__ long long tile_off(int I, int J) {
-  return 128LL * I * (I - 1) + 192LL * I + 256LL * J;
+  return 128LL * I * (I - 1) + (long long)SMV_DIAG * I + 256LL * J;
 }
+__host__ __device__ __forceinline__ int tri8(int r, int c) { return (r * (r + 1)) / 2 + c; }
 // When the last row block holds 8 or fewer rows (ng mod 16 in 1..8), its
 // tiles keep only their top half: sub-blocks 0 and 1 (128 entries), the
-// diagonal tile sub-block 0 (64 entries); the bottom half would be padding.
+// diagonal tile its packed sub-block 0 (36 entries); the rest would be padding.
 __host__ __device__ __forceinline__ bool smv_half(int ng) { return ng > 0 && ((ng - 1) & 15) < 8; }
 __host__ __device__ __forceinline__ long long tile_at(int I, int J, int nb, bool half) {
   return (half && I == nb - 1) ? tile_off(I, 0) + 128LL * J : tile_off(I, J);
@@ -44,7 +48,7 @@ __host__ __device__ __forceinline__ long long tile_at(int I, int J, int nb, bool
 // tile entries of a subdomain with ng interface DOFs
 __host__ __device__ __forceinline__ long long smv_entries(int ng) {
   const int nb = (ng + 15) >> 4;
-  return smv_half(ng) ? tile_off(nb - 1, 0) + 128LL * (nb - 1) + 64 : tile_off(nb, 0);
+  return smv_half(ng) ? tile_off(nb - 1, 0) + 128LL * (nb - 1) + 36 : tile_off(nb, 0);
 }
 
 // solve kernel limits: rows per block, prefetch registers per thread

@@ -17,10 +17,11 @@
 // column c) is in 8x8 sub-block k = (r>>3)*2 + (c>>3) at k*64 +
 // ((r&7) | (c&7)<<3), so each of the four 64-lane loads of a tile reads 1 KB
 // contiguously and lane l holds rows l&7, (l&7)+8 and columns l>>3, (l>>3)+8.
-// Diagonal tiles hold the strict lower part and half the diagonal; their
-// sub-block 1 (rows 0-7, columns 8-15) is entirely upper and not stored, so
-// they take 192 entries (sub-blocks 0, 2, 3).  When the last row block has 8
-// or fewer rows, its tiles keep only sub-blocks 0-1 (the diagonal one
+// Diagonal tiles hold the strict lower part and half the diagonal: their
+// sub-block 1 (rows 0-7, columns 8-15) is entirely upper and not stored, and
+// sub-blocks 0 and 3 are packed lower triangles, so they take 136 entries
+// (lanes of an upper entry read zero).  When the last row block has 8 or
+// fewer rows, its tiles keep only their top half (the diagonal one its packed
 // sub-block 0): the rest would be padding (tile_at, smv_entries).  Every
 // tile T is applied as T + T^T, which restores S_s exactly.
 #include "fw_kernels.h"
@@ -64,12 +65,19 @@ __device__ __forceinline__ void load_next(const double2 *__restrict__ base, cons
                                           Cursor &c, int lane, Tile &T) {
   const bool dg = c.I == c.J;
   const bool hf = c.half && c.I == c.nb - 1;  // bottom sub-blocks not stored: read zeros
-  const double2 *p = base + tile_at(c.I, c.J, c.nb, c.half) + lane;
-  const double2 *q1 = dg ? zero + lane : p + 64;
-  const double2 *q2 = hf ? zero + lane : p + (dg ? 64 : 128);
-  const double2 *q3 = hf ? zero + lane : p + (dg ? 128 : 192);
-  T.e[0].x = __builtin_nontemporal_load(&p->x);
-  T.e[0].y = __builtin_nontemporal_load(&p->y);
+  const double2 *t = base + tile_at(c.I, c.J, c.nb, c.half);
+  const double2 *z = zero + lane;
+  // diagonal tile: packed triangles (lane (r, c) = (lane&7, lane>>3), upper
+  // lanes read zeros), full sub-block 2 at 36
+  const int r8 = lane & 7, c8 = lane >> 3;
+  const bool up = c8 > r8;
+  const int tr = tri8(r8, c8);
+  const double2 *q0 = dg ? (up ? z : t + tr) : t + lane;
+  const double2 *q1 = dg ? z : t + 64 + lane;
+  const double2 *q2 = hf ? z : (dg ? t + 36 + lane : t + 128 + lane);
+  const double2 *q3 = (hf || (dg && up)) ? z : (dg ? t + 100 + tr : t + 192 + lane);
+  T.e[0].x = __builtin_nontemporal_load(&q0->x);
+  T.e[0].y = __builtin_nontemporal_load(&q0->y);
   T.e[1].x = __builtin_nontemporal_load(&q1->x);
   T.e[1].y = __builtin_nontemporal_load(&q1->y);
   T.e[2].x = __builtin_nontemporal_load(&q2->x);

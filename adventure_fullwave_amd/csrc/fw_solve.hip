@@ -369,10 +369,20 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
       for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], L.v[a.gr_col[c]]);
       if (k == j) y = cscale(y, 0.5);
       const int I = k >> 4, ri = k & 15;
-      int sb = (ri >> 3) * 2 + (cj >> 3);
-      if (I == J) sb -= (sb > 0);  // diagonal tile: sub-block 1 is not stored
-      // (a half last row block has rows 0-7 only: sb stays 0 or 1)
-      dst[tile_at(I, J, nbk, hlf) + sb * 64 + ((ri & 7) | ((cj & 7) << 3))] = y;
+      // off-diagonal tile: sub-block (ri>>3, cj>>3) of 64; diagonal tile:
+      // packed triangles 0 (at 0) and 3 (at 100) around sub-block 2 (at 36)
+      // (a half last row block has rows 0-7 only)
+      const long long t = tile_at(I, J, nbk, hlf);
+      long long o;
+      if (I != J)
+        o = t + ((ri >> 3) * 2 + (cj >> 3)) * 64 + ((ri & 7) | ((cj & 7) << 3));
+      else if (ri < 8)
+        o = t + tri8(ri, cj);
+      else if (cj < 8)
+        o = t + 36 + ((ri & 7) | (cj << 3));
+      else
+        o = t + 100 + tri8(ri - 8, cj - 8);
+      dst[o] = y;
     }
   } else if (a.mode != MODE_RHS) {
     for (int k = lane; k < m.ng; k += WL) {
