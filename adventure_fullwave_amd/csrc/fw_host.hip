@@ -990,12 +990,21 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     const double t16 = std::min(best[0][0], best[0][1]), t8 = std::min(best[1][0], best[1][1]);
-    const int d = (depths == 2 && t8 < 0.95 * t16) ? 1 : 0;
+    int d = (depths == 2 && t8 < 0.95 * t16) ? 1 : 0;
     c->pfk_iter = d == 0 ? fw::PFK : 8;
     // the Schur formation runs hundreds of waves per CU, where the 8-deep
     // kernel's higher occupancy pays unless its single solve is much slower
     // (wide profiles cut into many short blocks: cfg 3's large subdomains)
     c->pfk_unit = (depths == 2 && t8 <= 1.2 * t16) ? 8 : fw::PFK;
+    // FWH_SOLVE_DEPTH (8 or 16) forces the depth of every launch kind (tests
+    // of each variant, experiments)
+    if (const char *e = getenv("FWH_SOLVE_DEPTH")) {
+      const int v = atoi(e);
+      if (v == fw::PFK || (v == 8 && c->short_ok)) {
+        c->pfk_iter = c->pfk_unit = v;
+        d = v == fw::PFK ? 0 : 1;
+      }
+    }
     c->ms_single = best[d][0];
     c->ms_bucketed = variants == 2 ? best[d][1] : 0.0;
     c->use_buckets = variants == 2 && best[d][1] < best[d][0];

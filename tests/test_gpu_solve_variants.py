@@ -1,0 +1,72 @@
+"""Both variants of the batched LDL^T kernel (run with -m gpu).
+
+The load-time tuner picks the prefetch depth (8 or 16 loads per lane, with
+a row-block partition for each) by timing, so a given run may exercise only
+one of them in the iteration sweeps.  Here each depth is forced
+(FWH_SOLVE_DEPTH, read at load) for every launch kind and checked against
+the oracle: the subdomain LDL^T solves (1e-10
+relative), the implicit Schur step and the explicit S_s formed with it
+(1e-10), and the full HDDM solve (iteration count within 2 %, solution rel.
+L2 1e-7)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import fw_oracle as fo
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def sigma_ref(small_sigma_model):
+    return fo.hddm_solve(small_sigma_model, eps=1e-8, max_loop=5000)
+
+
+@pytest.mark.parametrize("depth", [16, 8])
+def test_solve_variant(small_sigma_model, sigma_ref, depth):
+    from adventure_fullwave_amd import fwhip
+    model = small_sigma_model
+    say(f"[gpu] LDL^T variant depth={depth}")
+    os.environ["FWH_SOLVE_DEPTH"] = str(depth)
+    try:
+        sol = fwhip.Solver(model)
+    finally:
+        del os.environ["FWH_SOLVE_DEPTH"]
+    if sol.info["solve_depth"] != depth or sol.info["unit_depth"] != depth:
+        sol.close()
+        pytest.skip(f"depth {depth} not available on this model")
+    om = fo.OracleModel(model)
+    # subdomain LDL^T solves against the oracle's factors
+    rng = np.random.default_rng(depth)
+    nd = int(sol.info["n_dof"])
+    rhs = rng.standard_normal(nd) + 1j * rng.standard_normal(nd)
+    x = sol.ldl_solve_all(rhs.copy())
+    off = 0
+    for s in range(model.nsub):
+        g = om.get(s)
+        n = g["n"]
+        xo = fo.sky_solve(g["idx1"], g["idx2"], g["factor"], rhs[off:off + n])
+        assert np.abs(x[off:off + n] - xo).max() <= 1e-10 * np.abs(xo).max()
+        off += n
+    # implicit Schur step, then the explicit S_s formed with this variant
+    ng = int(sol.info["n_iface"])
+    p = rng.standard_normal(ng) + 1j * rng.standard_normal(ng)
+    qi = sol.schur_apply(p)
+    qe = sol.schur_apply(p, explicit=True)
+    assert np.abs(qe - qi).max() <= 1e-10 * np.abs(qi).max()
+    # full solve
+    st = sol.solve(eps=1e-8, max_loop=5000, graph_iters=8)
+    it_ref = sigma_ref["stats"]["iters"]
+    say(f"[gpu] iters gpu={st['iters']} oracle={it_ref}")
+    assert st["converged"] == 1
+    assert abs(st["iters"] - it_ref) <= max(1, 0.02 * it_ref)
+    xs = sol.x()
+    assert np.linalg.norm(xs - sigma_ref["x_flat"]) / np.linalg.norm(sigma_ref["x_flat"]) <= 1e-7
+    sol.close()
