@@ -230,7 +230,7 @@ struct fwh_ctx {
     a.done = check_done ? &sc->done : nullptr;
     a.nmax = nmax;
     a.ngmax = ngmax;
-    a.pfk = mode == fw::MODE_UNIT ? pfk_unit : pfk_iter;
+    a.pfk = (mode == fw::MODE_UNIT || mode == fw::MODE_FWD) ? pfk_unit : pfk_iter;
     a.nblkmax = a.pfk == 8 ? nblkmax_s : nblkmax;
     a.idx1 = idx1;
     a.rowbeg = rowbeg;
@@ -1136,12 +1136,14 @@ int build_schur(fwh_ctx *c) {
   ufirst.push_back(ulist.size());
   int2 *d_ulist = nullptr;
   if ((rc = c->upload(&d_ulist, ulist))) return rc;
-  fw::SolveArgs a = c->solve_args(fw::MODE_UNIT, false);
-  a.unit_list = d_ulist;
-  a.order = nullptr;
-  if (!bk) {
-    HIPCHK(fw::launch_solve(a, (int)ulist.size(), c->st));
-  } else {
+  // the unit-vector launches: one over all pairs, or one per size bucket
+  auto unit_launch = [&](fw::SolveArgs a) -> int {
+    a.unit_list = d_ulist;
+    a.order = nullptr;
+    if (!bk) {
+      HIPCHK(fw::launch_solve(a, (int)ulist.size(), c->st));
+      return 0;
+    }
     HIPCHK(hipEventRecord(c->ev_fork, c->st));
     for (size_t b = 0; b < c->buckets.size(); b++) {
       fw::SolveArgs ab = a;
@@ -1154,8 +1156,92 @@ int build_schur(fwh_ctx *c) {
       HIPCHK(hipEventRecord(c->ev_join[b], c->bst[b]));
       HIPCHK(hipStreamWaitEvent(c->st, c->ev_join[b], 0));
     }
+    return 0;
+  };
+  // GEMM formation: W = L^-1 (-A_IG) by forward sweeps only (MODE_FWD, the
+  // same unit-vector launches), then S_s = A_GG - W^T D^-1 W on the matrix
+  // cores (k_schur_gemm, one wave per lower 16x16 tile, the tiles of
+  // subdomain s dealt to XCD s mod 8 so that its W is read from one L2) and
+  // A_GG added (k_schur_agg).  W is temporary: n_s x ng_s per subdomain.  The
+  // unit-vector Schur step (MODE_UNIT, forward and backward sweeps per
+  // column) remains for FWH_SCHUR_FORM=solve or when W would take more than
+  // half the free device memory.
+  std::vector<long long> woff(c->nsub);
+  long long wtot = 0;
+  for (int s = 0; s < c->nsub; s++) {
+    woff[s] = wtot;
+    wtot += (long long)c->meta_h[s].n * c->meta_h[s].ng;
+  }
+  size_t mem_free = 0, mem_tot = 0;
+  HIPCHK(hipMemGetInfo(&mem_free, &mem_tot));
+  const char *fe = getenv("FWH_SCHUR_FORM");
+  bool gemm = !(fe && strcmp(fe, "solve") == 0) && 16.0 * (double)wtot < 0.5 * (double)mem_free;
+  if (gemm) {
+    std::vector<int4> tiles;
+    {
+      std::vector<std::vector<int4>> lane(NXCD);
+      for (int s : c->order_h) {
+        const int nb = (c->meta_h[s].ng + 15) / 16;
+        for (int I = 0; I < nb; I++)
+          for (int J = 0; J <= I; J++) lane[s % NXCD].push_back(make_int4(s, I, J, 0));
+      }
+      size_t len = 0;
+      for (auto &v : lane) len = std::max(len, v.size());
+      tiles.assign(len * NXCD, make_int4(-1, 0, 0, 0));
+      for (int x = 0; x < NXCD; x++)
+        for (size_t i = 0; i < lane[x].size(); i++) tiles[i * NXCD + x] = lane[x][i];
+    }
+    double2 *W = nullptr;
+    long long *d_woff = nullptr;
+    int4 *d_tiles = nullptr;
+    hipError_t e = hipMalloc(&W, std::max<long long>(wtot, 1) * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc(&d_woff, c->nsub * sizeof(long long));
+    if (e == hipSuccess) e = hipMalloc(&d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(int4));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_woff, woff.data(), c->nsub * sizeof(long long), hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess && !tiles.empty())
+      e = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, c->st);
+    for (int k = 0; k < c->nsys && e == hipSuccess; k++) {
+      fw::SolveArgs a = c->solve_args(fw::MODE_FWD, false, false, k);
+      a.wmat = W;
+      a.w_off = d_woff;
+      if ((rc = unit_launch(a))) {
+        e = hipErrorUnknown;
+        break;
+      }
+      fw::GemmArgs g{};
+      g.meta = c->meta;
+      g.tiles = d_tiles;
+      g.ntile = (int)tiles.size();
+      g.wmat = W;
+      g.w_off = d_woff;
+      g.dinv = c->dinv + (size_t)k * c->ndof;
+      g.smat = c->smat + (size_t)k * tot;
+      g.sm_off = c->sm_off;
+      e = fw::launch_schur_gemm(g, c->st);
+      fw::AggArgs q{};
+      q.meta = c->meta;
+      q.gr_ptr = c->gr_ptr;
+      q.gr_col = c->gr_col;
+      q.gr_val = c->gr_val + (size_t)k * c->ngrow;
+      q.bcflag = c->bcflag;
+      q.kidx = c->kidx;
+      q.smat = c->smat + (size_t)k * tot;
+      q.sm_off = c->sm_off;
+      if (e == hipSuccess) e = fw::launch_schur_agg(q, c->nsub, c->st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipStreamSynchronize(c->st);
+    if (W) (void)hipFree(W);
+    if (d_woff) (void)hipFree(d_woff);
+    if (d_tiles) (void)hipFree(d_tiles);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(FWH_E_HIP, "Schur formation (GEMM): %s", hipGetErrorString(e));
+  } else {
+    if ((rc = unit_launch(c->solve_args(fw::MODE_UNIT, false)))) return rc;
   }
   HIPCHK(hipStreamSynchronize(c->st));
+  c->info.schur_gemm = gemm ? 1 : 0;
   c->info.ms_schur_form = now_ms() - t0;
   c->info.smat_bytes = 16.0 * (double)tot * c->nsys;
   // waves per workgroup of the matvec: 2, 4 or 8, whichever streams fastest

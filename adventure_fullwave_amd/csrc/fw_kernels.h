@@ -25,7 +25,9 @@ struct SubMeta {
 
 // MODE_UNIT: the Schur step applied to interface unit vector unit_k, output
 // written into the explicit Schur-complement tiles (fw_schur.hip)
-enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3, MODE_UNIT = 4 };
+// MODE_FWD: the forward sweep L^-1 b of the same unit-vector load (interface
+// rows zero), written to a column of W (the GEMM formation of S_s)
+enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3, MODE_UNIT = 4, MODE_FWD = 5 };
 
 // explicit Schur complements: 16x16 tiles of the lower triangle, row block
 // by row block; off-diagonal tiles 256 entries (four 8x8 sub-blocks),
@@ -44,6 +46,17 @@ __host__ __device__ __forceinline__ int tri8(int r, int c) { return (r * (r + 1)
 __host__ __device__ __forceinline__ bool smv_half(int ng) { return ng > 0 && ((ng - 1) & 15) < 8; }
 __host__ __device__ __forceinline__ long long tile_at(int I, int J, int nb, bool half) {
   return (half && I == nb - 1) ? tile_off(I, 0) + 128LL * J : tile_off(I, J);
+}
+// position of S entry (row 16 I + ri, column 16 J + cj), J <= I, in the tiles
+// of a subdomain with nb row blocks: off-diagonal tiles sub-block
+// (ri>>3, cj>>3) of 64 entries; diagonal tiles packed triangles 0 (at 0) and
+// 3 (at 100) around sub-block 2 (at 36); a half last row block has rows 0-7
+__host__ __device__ __forceinline__ long long tile_pos(int I, int J, int ri, int cj, int nb, bool half) {
+  const long long t = tile_at(I, J, nb, half);
+  if (I != J) return t + ((ri >> 3) * 2 + (cj >> 3)) * 64 + ((ri & 7) | ((cj & 7) << 3));
+  if (ri < 8) return t + tri8(ri, cj);
+  if (cj < 8) return t + 36 + ((ri & 7) | (cj << 3));
+  return t + 100 + tri8(ri - 8, cj - 8);
 }
 // tile entries of a subdomain with ng interface DOFs
 __host__ __device__ __forceinline__ long long smv_entries(int ng) {
@@ -113,6 +126,31 @@ struct SolveArgs {
   const int2 *unit_list; // MODE_UNIT, if set: (subdomain, unit vector) per work item
   double2 *smat;         // MODE_UNIT: Schur-complement tiles
   const long long *sm_off;  // per subdomain first entry of its tiles
+  double2 *wmat;         // MODE_FWD: W, per subdomain column-major (n x ng)
+  const long long *w_off;   // per subdomain first entry of its W
+};
+
+// S_s = A_GG - W^T D^-1 W with W = L^-1 A_IG (forward sweeps, MODE_FWD):
+// the product on the FP64 matrix cores, one wavefront per 16x16 lower tile
+struct GemmArgs {
+  const SubMeta *meta;
+  const int4 *tiles;     // (subdomain, row block, column block, -) per wave; subdomain -1: padding
+  int ntile;
+  const double2 *wmat;
+  const long long *w_off;
+  const double2 *dinv;   // per DOF 1/d_i
+  double2 *smat;
+  const long long *sm_off;
+};
+// A_GG added to the tiles: one workgroup per subdomain
+struct AggArgs {
+  const SubMeta *meta;
+  const int *gr_ptr, *gr_col;
+  const double2 *gr_val;
+  const int8_t *bcflag;
+  const int *kidx;
+  double2 *smat;
+  const long long *sm_off;
 };
 
 // one workgroup of the explicit Schur matvec: where its subdomain's tiles,
@@ -249,6 +287,8 @@ hipError_t launch_blockinv(const SubMeta *meta, int nsub, const int *idx1, const
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st);
 size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk);
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st);
+hipError_t launch_schur_gemm(const GemmArgs &a, hipStream_t st);
+hipError_t launch_schur_agg(const AggArgs &a, int nsub, hipStream_t st);
 size_t smv_lds_bytes(int nbmax, int waves);
 hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, const double2 *val,
                          double2 *pcslot, hipStream_t st);

@@ -266,6 +266,95 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   }
 }
 
+// ---------------------------------------------------------------------------
+// GEMM formation of S_s: with the Dirichlet matrix factored as L D L^T and
+// W = L^-1 (-A_IG) (forward sweeps, MODE_FWD of k_solve; interface rows
+// zero), S_s = A_GG - W^T D^-1 W.  One wavefront per 16x16 lower tile
+// (I, J) accumulates C = W_I^T D^-1 W_J over the subdomain's n rows with the
+// FP64 matrix instruction v_mfma_f64_16x16x4f64 (A[i][k] in lane i + 16 k,
+// B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i mod 4), register
+// i / 4): four real products per complex step,
+// C_re += W_r V_r - W_i V_i, C_im += W_r V_i + W_i V_r with V = D^-1 W_J.
+// The tile then holds -C (diagonal halved); k_schur_agg adds A_GG.
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_schur_gemm(GemmArgs a) {
+  const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wid >= a.ntile) return;
+  const int4 t = a.tiles[wid];
+  if (t.x < 0) return;
+  const SubMeta m = a.meta[t.x];
+  const int n = m.n, ng = m.ng, I = t.y, J = t.z;
+  const double2 *W = a.wmat + a.w_off[t.x];  // W[c][k] at c n + k
+  const double2 *dinv = a.dinv + m.dof0;
+  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+  const int ci = 16 * I + li, cj = 16 * J + li;  // this lane's column of W for A and for B
+  const bool vi = ci < ng, vj = cj < ng;
+  const double2 *wi = W + (long long)(vi ? ci : 0) * n;
+  const double2 *wj = W + (long long)(vj ? cj : 0) * n;
+  mfma_d4 cre = {0.0, 0.0, 0.0, 0.0}, cim = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < n; k0 += 4) {
+    const int k = k0 + lk;
+    const bool in = k < n;
+    const int kc = in ? k : 0;
+    const double2 x = wi[kc], y = wj[kc], d = dinv[kc];
+    const double ar = (vi && in) ? x.x : 0.0, ai = (vi && in) ? x.y : 0.0;
+    const double2 v = cmul(d, y);
+    const double br = (vj && in) ? v.x : 0.0, bi = (vj && in) ? v.y : 0.0;
+    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cre, 0, 0, 0);
+    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, cre, 0, 0, 0);
+    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, cim, 0, 0, 0);
+    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, cim, 0, 0, 0);
+  }
+  const int nb = (ng + 15) >> 4;
+  const bool half = smv_half(ng);
+  double2 *dst = a.smat + a.sm_off[t.x];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int ri = 4 * v + lk, c = li;  // C[ri][c] of this tile
+    const int gi = 16 * I + ri, gj = 16 * J + c;
+    if (gi < ng && gj < ng && (I > J || ri >= c)) {
+      const double sc = gi == gj ? -0.5 : -1.0;
+      dst[tile_pos(I, J, ri, c, nb, half)] = make_double2(sc * cre[v], sc * cim[v]);
+    }
+  }
+}
+
+// A_GG into the tiles: row i's interface-row entries at interface columns
+// j <= i (the diagonal halved), added to -W^T D^-1 W; one workgroup per
+// subdomain, one thread per interface row (each tile entry has one writer)
+__global__ __launch_bounds__(256) void k_schur_agg(AggArgs a) {
+  const SubMeta m = a.meta[blockIdx.x];
+  const int ng = m.ng, nb = (ng + 15) >> 4;
+  const bool half = smv_half(ng);
+  double2 *dst = a.smat + a.sm_off[blockIdx.x];
+  for (int i = threadIdx.x; i < ng; i += blockDim.x) {
+    const int sl = m.g0 + i;
+    for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) {
+      const int d = m.dof0 + a.gr_col[c];
+      if (a.bcflag[d] != 1) continue;
+      const int j = a.kidx[d];
+      if (j > i) continue;
+      double2 g = a.gr_val[c];
+      if (j == i) g = cscale(g, 0.5);
+      double2 *e = dst + tile_pos(i >> 4, j >> 4, i & 15, j & 15, nb, half);
+      *e = cadd(*e, g);
+    }
+  }
+}
+
+hipError_t launch_schur_gemm(const GemmArgs &a, hipStream_t st) {
+  if (a.ntile <= 0) return hipSuccess;
+  k_schur_gemm<<<(a.ntile + 3) / 4, 256, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_schur_agg(const AggArgs &a, int nsub, hipStream_t st) {
+  if (nsub <= 0) return hipSuccess;
+  k_schur_agg<<<nsub, 256, 0, st>>>(a);
+  return hipGetLastError();
+}
+
 size_t smv_lds_bytes(int nbmax, int waves) { return sizeof(double2) * (size_t)(1 + waves) * nbmax * 16; }
 
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st) {

@@ -248,6 +248,7 @@ def main():
     setup = {"assemble": round(info["ms_assemble"], 2), "factor": round(info["ms_factor"], 2)}
     if expl:
         setup["schur_form"] = round(info["ms_schur_form"], 2)
+        setup["schur_form_method"] = "W^T D^-1 W on FP64 MFMA" if info["schur_gemm"] else "unit-vector Schur steps"
     out = {
         "metric": METRIC, "value": round(iters_per_s, 2), "unit": "iters/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000 * dt / args.steps, 2),

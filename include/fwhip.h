@@ -106,6 +106,9 @@ typedef struct fwh_info {
                            by timing: 0 largest skyline first, 1 most tiles
                            first, 2 most tiles first snaking over the CUs  */
   double ms_smv_order[3]; /* its timings                                   */
+  int    schur_gemm;    /* 1: S_s formed as A_GG - W^T D^-1 W on the matrix
+                           cores (W by forward sweeps); 0: by the Schur
+                           step on every interface unit vector             */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */

