@@ -74,6 +74,63 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
 }
+constexpr double NEARLY_ZERO = 1.0e-60;  // lib/std/defs.h:105
+
+// The status step of the interface COCG after an iteration: beta from the new
+// and old (r,z) (hddm_solver_s.h:340-350, both near zero: 0), the max-norm
+// residual ratio and stop test (:311-318), and the status machine's end of
+// the loop on convergence or at max_loop (hddm_solver.h:125-143).
+struct StatusStep {
+  double2 beta;
+  double resi;
+  int conv, done;
+};
+__device__ __forceinline__ StatusStep status_step(double2 rzs, double mxs, double2 rzo, double r0inv, double eps,
+                                                  int loop, int max_loop) {
+  StatusStep r;
+  r.resi = mxs * r0inv;
+  r.beta = cz();
+  if (!(cabsv(rzs) < NEARLY_ZERO && cabsv(rzo) < NEARLY_ZERO)) r.beta = cdiv_naive(rzs, rzo);
+  r.conv = r.resi < eps ? 1 : 0;
+  r.done = (r.conv || loop == max_loop) ? 1 : 0;
+  return r;
+}
+
+// Fixed-order reductions of G per-block partials, as a 256-thread block
+// computes them (thread t sums entries t, t+256, ... in order, each 64-lane
+// wave sums its threads, then the four wave sums are added in order), with
+// any number NT of threads: wave w takes the virtual waves w, w+NT/64, ... < 4.
+// Every kernel that needs the same scalar thus obtains the same bits.
+// sh: 4 entries of LDS; contains barriers (call uniformly).
+__device__ __forceinline__ double2 parts_csum256(const double2 *__restrict__ p, int G, double2 *sh, int NT) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int vw = wave; vw < 4; vw += NT >> 6) {
+    double2 t = cz();
+    for (int k = vw * 64 + lane; k < G; k += 256) t = cadd(t, p[k]);
+    t = wave_csum(t);
+    if (lane == 0) sh[vw] = t;
+  }
+  __syncthreads();
+  double2 t = cz();
+#pragma unroll
+  for (int k = 0; k < 4; k++) t = cadd(t, sh[k]);
+  return t;
+}
+__device__ __forceinline__ double parts_max256(const double2 *__restrict__ p, int G, double *sh, int NT) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int vw = wave; vw < 4; vw += NT >> 6) {
+    double t = 0.0;
+    for (int k = vw * 64 + lane; k < G; k += 256) t = fmax(t, p[k].x);
+    t = wave_max(t);
+    if (lane == 0) sh[vw] = t;
+  }
+  __syncthreads();
+  double t = sh[0];
+#pragma unroll
+  for (int k = 1; k < 4; k++) t = fmax(t, sh[k]);
+  return t;
+}
+
 // sum over aligned groups of G lanes (G power of two <= 64)
 template <int G>
 __device__ __forceinline__ double2 group_csum(double2 v) {

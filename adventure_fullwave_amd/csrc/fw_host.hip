@@ -278,6 +278,8 @@ struct fwh_ctx {
     a.P = P;
     a.Z = fused ? Z : nullptr;
     a.sc = sc;
+    a.part = part;
+    a.G = G;
     a.yslot = yslot;
     return a;
   }
@@ -1413,9 +1415,11 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
     if (!c->gexec || c->graph_n != o->graph_iters || c->graph_schur != o->schur) {
       c->release_graph();
       HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+      // explicit: the Schur matvec takes each iteration's status step, so
+      // k_iter_c runs only after the last iteration of the graph
       for (int k = 0; k < o->graph_iters; k++) {
         hipError_t e = c->run_schur(expl, false, true, true);
-        if (e == hipSuccess) e = fw::launch_iter(c->vec_args(), c->st);
+        if (e == hipSuccess) e = fw::launch_iter(c->vec_args(), c->st, !expl || k == o->graph_iters - 1);
         if (e != hipSuccess) {
           hipGraph_t g;
           (void)hipStreamEndCapture(c->st, &g);

@@ -177,7 +177,9 @@ struct SmvArgs {
   const int *g_gid;
   const double2 *P;
   const double2 *Z;      // p = Z + beta P (null: p = P)
-  const Scal *sc;
+  Scal *sc;              // status of the interface iteration (with Z)
+  const double2 *part;   // (r,z) and max|r| partials of k_iter_b (with Z)
+  int G;                 // their count per system
   double2 *yslot;
 };
 
@@ -244,7 +246,14 @@ struct Scal {
   int conv;
   int iters;
   int max_loop;
-  int pad;
+  // the status step of an iteration ended without k_iter_c: k_iter_b sets
+  // pending; the next Schur matvec takes the step from k_iter_b's partials
+  // and stages its result (st_*, st_fresh); the next k_iter_a commits it.
+  // With nothing staged, st_beta and st_done equal beta and done.
+  int pending;
+  double2 st_rz, st_beta;
+  double st_resi;
+  int st_conv, st_done, st_fresh, pad;
 };
 
 struct VecArgs {
@@ -293,7 +302,8 @@ size_t smv_lds_bytes(int nbmax, int waves);
 hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, const double2 *val,
                          double2 *pcslot, hipStream_t st);
 hipError_t launch_first(const VecArgs &a, hipStream_t st);
-hipError_t launch_iter(const VecArgs &a, hipStream_t st);
+// k_iter_a, k_iter_b and (status_step) k_iter_c
+hipError_t launch_iter(const VecArgs &a, hipStream_t st, bool status_step = true);
 hipError_t launch_finalize_p(const VecArgs &a, hipStream_t st);
 hipError_t launch_gather_q(const VecArgs &a, hipStream_t st);
 hipError_t launch_fill(double2 *p, long long n, double2 v, hipStream_t st);

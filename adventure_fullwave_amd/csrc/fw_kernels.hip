@@ -12,7 +12,6 @@
 
 namespace fw {
 
-constexpr double NEARLY_ZERO = 1.0e-60;  // lib/std/defs.h:105
 constexpr double NU = 7.957747e+05;      // lib/fem/make_elem_em.c:37
 
 // edge k: vertices EDGE[k][0] -> EDGE[k][1]; opposite edge EDGE[k][2] -> EDGE[k][3]
@@ -679,6 +678,10 @@ __global__ __launch_bounds__(VT) void k_first_b(VecArgs a) {
     s->iters = 0;
     s->conv = 0;
     s->resi = 1.0;
+    s->pending = 0;
+    s->st_beta = cz();
+    s->st_done = s->done;
+    s->st_fresh = 0;
     a.hist[0] = 1.0;
   }
 }
@@ -705,6 +708,11 @@ hipError_t launch_first(const VecArgs &a, hipStream_t st) {
 // The Schur step of the next iteration forms p = z + beta p itself (zbp,
 // same fused operations as k_iter_a); the final P (U when converged,
 // hddm_solver_s.h:370-376) is set by k_finalize_p before the last solve.
+// Inside a graph of explicit iterations k_iter_c runs only after the last
+// one: the explicit Schur matvec takes the status step from k_iter_b's
+// partials itself (every workgroup, the same reduction order) and stages it,
+// and k_iter_a's block 0 commits it (Scal::pending, st_*), so no kernel
+// reads a status field that another block of the same kernel writes.
 __global__ __launch_bounds__(VT) void k_iter_a(VecArgs a) {
   to_system(a);
   // the operands of this thread's first unknown are loaded before the
@@ -716,10 +724,27 @@ __global__ __launch_bounds__(VT) void k_iter_a(VecArgs a) {
     z0 = a.Z[g0];
     p0 = a.P[g0];
   }
-  const Scal *s = a.sc;
-  if (s->done) return;
+  // the staged status and beta are requested together (one round trip);
+  // block 0 commits a status step staged by the Schur matvec
+  Scal *s = a.sc;
+  const int done = s->st_done;
+  const double2 beta = s->st_beta;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && s->st_fresh) {
+    const int loop = s->loop;
+    const double resi = s->st_resi;
+    a.hist[loop] = resi;
+    s->resi = resi;
+    s->conv = s->st_conv;
+    s->rz = s->st_rz;
+    s->beta = s->st_beta;
+    s->iters = s->iters + 1;
+    s->done = s->st_done;
+    s->loop = loop + 1;
+    s->st_fresh = 0;
+    s->pending = 0;
+  }
+  if (done) return;
   __shared__ double2 shc[VT / 64 + 1];
-  const double2 beta = s->beta;
   double2 pq = cz();
   if (g0 < a.ng) {
     const double2 p = zbp(z0, beta, p0);
@@ -749,12 +774,18 @@ __global__ __launch_bounds__(VT) void k_iter_b(VecArgs a) {
     q0 = a.Q[g0];
     pc0 = a.PC[g0];
   }
-  const Scal *s = a.sc;
-  if (s->done) return;
+  // the status, rz and the (p,q) partials are requested together; the
+  // status is tested after the reduction (a uniform branch)
+  Scal *s = a.sc;
+  const int done = s->done;
+  const double2 rz = s->rz;
   __shared__ double2 shc[VT / 64 + 1];
   __shared__ double shm[VT / 64 + 1];
   const double2 pq = reduce_parts(a.part, a.G, shc);
-  const double2 rz = s->rz;
+  if (done) return;
+  // this iteration's status step is pending until k_iter_c or the next
+  // Schur matvec takes it (no other block of this kernel reads the flag)
+  if (blockIdx.x == 0 && threadIdx.x == 0) s->pending = 1;
   double2 malpha = cz();
   if (!(cabsv(rz) < NEARLY_ZERO && cabsv(pq) < NEARLY_ZERO)) malpha = csub(cz(), cdiv_naive(rz, pq));
   double2 rzn = cz();
@@ -791,33 +822,35 @@ __global__ __launch_bounds__(VT) void k_iter_c(VecArgs a) {
   Scal *s = a.sc;
   __shared__ double2 shc[VT / 64 + 1];
   __shared__ double shm[VT / 64 + 1];
-  // (the reductions do not depend on the status: it is tested after them, a
-  // uniform branch, so both loads are in flight together)
+  // every scalar is requested with the partials (one round trip); the
+  // reductions do not depend on the status, which is tested after them (a
+  // uniform branch)
+  const int done = s->done, pending = s->pending, loop = s->loop, iters = s->iters, max_loop = s->max_loop;
+  const double r0inv = s->r0inv, eps = s->eps;
+  const double2 rzo = s->rz;
   const double2 rzs = reduce_parts(a.part + a.G, a.G, shc);
   const double mxs = reduce_parts_max(a.part + 2 * a.G, a.G, shm);
-  if (s->done) return;
+  if (done || !pending) return;
   if (threadIdx.x == 0) {
-    const int loop = s->loop;
-    const double resi = mxs * s->r0inv;
-    const double2 rzo = s->rz;
-    double2 beta = cz();
-    if (!(cabsv(rzs) < NEARLY_ZERO && cabsv(rzo) < NEARLY_ZERO)) beta = cdiv_naive(rzs, rzo);
-    const bool conv = resi < s->eps;
-    a.hist[loop] = resi;
-    s->resi = resi;
-    s->conv = conv ? 1 : 0;
+    const StatusStep r = status_step(rzs, mxs, rzo, r0inv, eps, loop, max_loop);
+    a.hist[loop] = r.resi;
+    s->resi = r.resi;
+    s->conv = r.conv;
     s->rz = rzs;
-    s->beta = beta;
-    s->iters += 1;
-    if (conv) s->done = 1;
-    if (loop == s->max_loop) s->done = 1;
+    s->beta = r.beta;
+    s->iters = iters + 1;
+    s->done = r.done;
     s->loop = loop + 1;
+    s->pending = 0;
+    s->st_beta = r.beta;
+    s->st_done = r.done;
+    s->st_fresh = 0;
   }
 }
-hipError_t launch_iter(const VecArgs &a, hipStream_t st) {
+hipError_t launch_iter(const VecArgs &a, hipStream_t st, bool status_step) {
   k_iter_a<<<vec_grid(a, a.G), VT, 0, st>>>(a);
   k_iter_b<<<vec_grid(a, a.G), VT, 0, st>>>(a);
-  k_iter_c<<<vec_grid(a, 1), VT, 0, st>>>(a);
+  if (status_step) k_iter_c<<<vec_grid(a, 1), VT, 0, st>>>(a);
   return hipGetLastError();
 }
 
@@ -867,6 +900,10 @@ __global__ void k_scal_init(Scal *s, double eps, int max_loop) {
   s->loop = 0;
   s->iters = 0;
   s->beta = cz();
+  s->pending = 0;
+  s->st_beta = cz();
+  s->st_done = 0;
+  s->st_fresh = 0;
 }
 hipError_t launch_scal_init(Scal *sc, int nsys, double eps, int max_loop, hipStream_t st) {
   k_scal_init<<<nsys, 1, 0, st>>>(sc, eps, max_loop);

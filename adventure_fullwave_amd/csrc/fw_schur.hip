@@ -7,10 +7,12 @@
 // iteration, y_G = (A x)_G with A_II x_I = -A_IG p_G, x_G = p_G; that is
 // y = S_s p_s with the complex-symmetric Schur complement
 // S_s = A_GG - A_GI A_II^-1 A_IG.  In explicit mode the S_s are formed once
-// from the skyline factors (the same Schur step applied to the unit vectors
-// of the interface, MODE_UNIT of k_solve) and each iteration streams them
-// once through k_smv: a batched, HBM-bound symmetric matvec instead of two
-// dependent triangular sweeps per subdomain.
+// from the skyline factors (S_s = A_GG - W^T D^-1 W with W = L^-1 (-A_IG):
+// forward sweeps of k_solve, MODE_FWD, then k_schur_gemm on the matrix cores
+// and k_schur_agg; or the Schur step of k_solve on every interface unit
+// vector, MODE_UNIT) and each iteration streams them once through k_smv: a
+// batched, HBM-bound symmetric matvec instead of two dependent triangular
+// sweeps per subdomain.
 //
 // Storage of S_s: the lower triangle in 16x16 tiles, row block by row block;
 // tile (I,J), J <= I, starts at tile_off(I,J).  Inside a tile, entry (row r,
@@ -176,7 +178,42 @@ __device__ __forceinline__ void to_system(SmvArgs &a) {
   if (a.Z) a.Z += s * a.ss.g;
   a.sc += s;
   if (a.done) a.done = &a.sc->done;
+  if (a.part) a.part += s * 3LL * a.G;
   a.yslot += s * a.ss.slot;
+}
+
+// The status of the interface iteration for the Schur matvec (false: the
+// system is finished and the workgroup's work is discarded; beta: the
+// coefficient of the next search direction).  When the previous iteration's
+// status step is still pending (k_iter_b ran, k_iter_c did not), every
+// workgroup takes it from k_iter_b's partials in k_iter_c's order, and
+// workgroup 0 stages it for the next k_iter_a to commit; no workgroup of
+// this kernel reads the staged fields.  Contains barriers: call uniformly.
+__device__ __forceinline__ bool iter_status(const SmvArgs &a, int NT, double2 &beta) {
+  Scal *s = a.sc;
+  const int done = s->done, pending = s->pending, loop = s->loop, max_loop = s->max_loop;
+  const double r0inv = s->r0inv, eps = s->eps;
+  const double2 rzo = s->rz;
+  beta = s->beta;
+  // the partials are requested with the status fields (one round trip) and
+  // reduced whether or not the step is pending
+  __shared__ double2 shz[4];
+  __shared__ double shx[4];
+  const double2 rzs = parts_csum256(a.part + a.G, a.G, shz, NT);
+  const double mxs = parts_max256(a.part + 2 * a.G, a.G, shx, NT);
+  if (done) return false;
+  if (!pending) return true;
+  const StatusStep r = status_step(rzs, mxs, rzo, r0inv, eps, loop, max_loop);
+  if (threadIdx.x == 0 && blockIdx.x == 0 && a.base == 0) {
+    s->st_rz = rzs;
+    s->st_beta = r.beta;
+    s->st_resi = r.resi;
+    s->st_conv = r.conv;
+    s->st_done = r.done;
+    s->st_fresh = 1;
+  }
+  beta = r.beta;
+  return !r.done;
 }
 
 template <int W>  // waves per workgroup
@@ -200,7 +237,11 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   // tile ranges and the load cursor in scalar registers
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
 
-  if (ng == 0) return;
+  if (ng == 0) {
+    double2 beta;
+    if (a.done && a.Z && blockIdx.x == 0 && a.base == 0) (void)iter_status(a, NT, beta);  // it may have to stage
+    return;
+  }
 
   // contiguous, equal shares of the tile sequence (row-block major); the
   // first two tiles of every wave go in flight before the interface values
@@ -223,11 +264,19 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   load_next(base, zero, cur, lane, B);
   // the status is read once the first tiles are in flight (a finished
   // system's workgroups discard them)
-  if (a.done && *a.done) return;
+  double2 beta = cz();
+  if (a.done) {
+    if (a.Z) {
+      if (!iter_status(a, NT, beta)) return;
+    } else if (*a.done) {
+      return;
+    }
+  } else if (a.Z) {
+    beta = a.sc->beta;  // a timing run: the current status as it stands
+  }
 
   // interface values: P, or the next search direction z + beta p while the
   // interface iteration runs (formed here, not stored, see k_iter_a)
-  const double2 beta = a.Z ? a.sc->beta : cz();
   for (int k = tid; k < npad; k += NT) {
     double2 p = cz();
     if (k < ng) {
