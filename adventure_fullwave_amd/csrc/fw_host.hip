@@ -230,7 +230,7 @@ struct fwh_ctx {
     a.done = check_done ? &sc->done : nullptr;
     a.nmax = nmax;
     a.ngmax = ngmax;
-    a.pfk = (mode == fw::MODE_UNIT || mode == fw::MODE_FWD) ? pfk_unit : pfk_iter;
+    a.pfk = mode == fw::MODE_UNIT ? pfk_unit : pfk_iter;
     a.nblkmax = a.pfk == 8 ? nblkmax_s : nblkmax;
     a.idx1 = idx1;
     a.rowbeg = rowbeg;
@@ -1160,67 +1160,123 @@ int build_schur(fwh_ctx *c) {
     }
     return 0;
   };
-  // GEMM formation: W = L^-1 (-A_IG) by forward sweeps only (MODE_FWD, the
-  // same unit-vector launches), then S_s = A_GG - W^T D^-1 W on the matrix
-  // cores (k_schur_gemm, one wave per lower 16x16 tile, the tiles of
-  // subdomain s dealt to XCD s mod 8 so that its W is read from one L2) and
-  // A_GG added (k_schur_agg).  W is temporary: n_s x ng_s per subdomain.  The
+  // GEMM formation: W = L^-1 (-A_IG) by blocked forward substitution on the
+  // matrix cores (k_blockinv16, then k_schur_fwd: one wave per (subdomain,
+  // 16-column chunk), the chunks of subdomain s dealt to XCD s mod 8 so that
+  // its factor is read from one L2), then S_s = A_GG - W^T D^-1 W
+  // (k_schur_gemm, one wave per lower 16x16 tile, dealt the same way) and
+  // A_GG added (k_schur_agg).  W and the block inverses are temporary: W
+  // covers each subdomain's rows up to its last interior one (nsw).  The
   // unit-vector Schur step (MODE_UNIT, forward and backward sweeps per
-  // column) remains for FWH_SCHUR_FORM=solve or when W would take more than
-  // half the free device memory.
-  std::vector<long long> woff(c->nsub);
+  // column) remains for FWH_SCHUR_FORM=solve or when they would take more
+  // than half the free device memory.
+  std::vector<int8_t> bcf(c->ndof);
+  HIPCHK(hipMemcpy(bcf.data(), c->bcflag, c->ndof, hipMemcpyDeviceToHost));
+  std::vector<int> nsw(c->nsub, 0);
+  std::vector<long long> woff(c->nsub), moff(c->nsub);
+  std::vector<int2> blist;
   long long wtot = 0;
   for (int s = 0; s < c->nsub; s++) {
+    const SubMeta &M = c->meta_h[s];
+    for (int i = 0; i < M.n; i++)
+      if (bcf[M.dof0 + i] == 0) nsw[s] = i + 1;
     woff[s] = wtot;
-    wtot += (long long)c->meta_h[s].n * c->meta_h[s].ng;
+    wtot += (long long)((M.ng + 15) / 16) * nsw[s] * 16;
+    moff[s] = (long long)blist.size();
+    for (int b = 0; b < (nsw[s] + 15) / 16; b++) blist.push_back(make_int2(s, b));
   }
+  const long long mtot = (long long)blist.size() * 256;
   size_t mem_free = 0, mem_tot = 0;
   HIPCHK(hipMemGetInfo(&mem_free, &mem_tot));
   const char *fe = getenv("FWH_SCHUR_FORM");
-  bool gemm = !(fe && strcmp(fe, "solve") == 0) && 16.0 * (double)wtot < 0.5 * (double)mem_free;
+  bool gemm = !(fe && strcmp(fe, "solve") == 0) && 16.0 * (double)(wtot + mtot) < 0.5 * (double)mem_free;
   if (gemm) {
-    std::vector<int4> tiles;
-    {
-      std::vector<std::vector<int4>> lane(NXCD);
-      for (int s : c->order_h) {
-        const int nb = (c->meta_h[s].ng + 15) / 16;
-        for (int I = 0; I < nb; I++)
-          for (int J = 0; J <= I; J++) lane[s % NXCD].push_back(make_int4(s, I, J, 0));
-      }
+    // per XCD lanes of work, subdomains largest first
+    auto deal = [&](auto item_count, auto make) {
+      using T = decltype(make(0, 0));
+      std::vector<std::vector<T>> lane(NXCD);
+      for (int s : c->order_h)
+        for (int k = 0; k < item_count(s); k++) lane[s % NXCD].push_back(make(s, k));
       size_t len = 0;
       for (auto &v : lane) len = std::max(len, v.size());
-      tiles.assign(len * NXCD, make_int4(-1, 0, 0, 0));
+      std::vector<T> out(len * NXCD, make(-1, 0));
       for (int x = 0; x < NXCD; x++)
-        for (size_t i = 0; i < lane[x].size(); i++) tiles[i * NXCD + x] = lane[x][i];
-    }
-    double2 *W = nullptr;
-    long long *d_woff = nullptr;
+        for (size_t i = 0; i < lane[x].size(); i++) out[i * NXCD + x] = lane[x][i];
+      return out;
+    };
+    const std::vector<int2> fwork = deal([&](int s) { return nsw[s] > 0 ? (c->meta_h[s].ng + 15) / 16 : 0; },
+                                         [](int s, int k) { return make_int2(s, k); });
+    std::vector<int4> tiles = deal(
+        [&](int s) {
+          const int nb = (c->meta_h[s].ng + 15) / 16;
+          return nb * (nb + 1) / 2;
+        },
+        [&](int s, int k) {
+          if (s < 0) return make_int4(-1, 0, 0, 0);
+          int I = 0;
+          while ((I + 1) * (I + 2) / 2 <= k) I++;
+          return make_int4(s, I, k - I * (I + 1) / 2, 0);
+        });
+    double2 *W = nullptr, *M16 = nullptr;
+    long long *d_woff = nullptr, *d_moff = nullptr;
+    int *d_nsw = nullptr;
+    int2 *d_blist = nullptr, *d_fwork = nullptr;
     int4 *d_tiles = nullptr;
     hipError_t e = hipMalloc(&W, std::max<long long>(wtot, 1) * sizeof(double2));
-    if (e == hipSuccess) e = hipMalloc(&d_woff, c->nsub * sizeof(long long));
-    if (e == hipSuccess) e = hipMalloc(&d_tiles, std::max<size_t>(tiles.size(), 1) * sizeof(int4));
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(d_woff, woff.data(), c->nsub * sizeof(long long), hipMemcpyHostToDevice, c->st);
-    if (e == hipSuccess && !tiles.empty())
-      e = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice, c->st);
+    if (e == hipSuccess) e = hipMalloc(&M16, std::max<long long>(mtot, 1) * sizeof(double2));
+    auto put = [&](auto **dst, const auto &v) {
+      if (e != hipSuccess) return;
+      using T = typename std::decay_t<decltype(v)>::value_type;
+      e = hipMalloc(dst, std::max<size_t>(v.size(), 1) * sizeof(T));
+      if (e == hipSuccess && !v.empty())
+        e = hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->st);
+    };
+    put(&d_woff, woff);
+    put(&d_moff, moff);
+    put(&d_nsw, nsw);
+    put(&d_blist, blist);
+    put(&d_fwork, fwork);
+    put(&d_tiles, tiles);
     for (int k = 0; k < c->nsys && e == hipSuccess; k++) {
-      fw::SolveArgs a = c->solve_args(fw::MODE_FWD, false, false, k);
-      a.wmat = W;
-      a.w_off = d_woff;
-      if ((rc = unit_launch(a))) {
-        e = hipErrorUnknown;
-        break;
-      }
+      const double2 *fac = c->fac + (size_t)k * c->ss.nlen;
+      fw::BlockInv16Args bi{};
+      bi.blist = d_blist;
+      bi.nblk = (int)blist.size();
+      bi.meta = c->meta;
+      bi.nsw = d_nsw;
+      bi.idx1 = c->idx1;
+      bi.rowbeg = c->rowbeg;
+      bi.fac = fac;
+      bi.m16 = M16;
+      e = fw::launch_blockinv16(bi, c->st);
+      fw::FwdArgs fa{};
+      fa.work = d_fwork;
+      fa.nwork = (int)fwork.size();
+      fa.meta = c->meta;
+      fa.nsw = d_nsw;
+      fa.idx1 = c->idx1;
+      fa.rowbeg = c->rowbeg;
+      fa.fac = fac;
+      fa.bcflag = c->bcflag;
+      fa.cp_ptr = c->cp_ptr;
+      fa.cp_k = c->cp_k;
+      fa.cp_val = c->cp_val + (size_t)k * c->ss.cpl;
+      fa.m16 = M16;
+      fa.m16_off = d_moff;
+      fa.wmat = W;
+      fa.w_off = d_woff;
+      if (e == hipSuccess) e = fw::launch_schur_fwd(fa, c->st);
       fw::GemmArgs g{};
       g.meta = c->meta;
       g.tiles = d_tiles;
       g.ntile = (int)tiles.size();
+      g.nsw = d_nsw;
       g.wmat = W;
       g.w_off = d_woff;
       g.dinv = c->dinv + (size_t)k * c->ndof;
       g.smat = c->smat + (size_t)k * tot;
       g.sm_off = c->sm_off;
-      e = fw::launch_schur_gemm(g, c->st);
+      if (e == hipSuccess) e = fw::launch_schur_gemm(g, c->st);
       fw::AggArgs q{};
       q.meta = c->meta;
       q.gr_ptr = c->gr_ptr;
@@ -1234,10 +1290,9 @@ int build_schur(fwh_ctx *c) {
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     (void)hipStreamSynchronize(c->st);
-    if (W) (void)hipFree(W);
-    if (d_woff) (void)hipFree(d_woff);
-    if (d_tiles) (void)hipFree(d_tiles);
-    if (rc) return rc;
+    for (void *ptr : {(void *)W, (void *)M16, (void *)d_woff, (void *)d_moff, (void *)d_nsw, (void *)d_blist,
+                      (void *)d_fwork, (void *)d_tiles})
+      if (ptr) (void)hipFree(ptr);
     if (e != hipSuccess) return fail(FWH_E_HIP, "Schur formation (GEMM): %s", hipGetErrorString(e));
   } else {
     if ((rc = unit_launch(c->solve_args(fw::MODE_UNIT, false)))) return rc;

@@ -25,9 +25,7 @@ struct SubMeta {
 
 // MODE_UNIT: the Schur step applied to interface unit vector unit_k, output
 // written into the explicit Schur-complement tiles (fw_schur.hip)
-// MODE_FWD: the forward sweep L^-1 b of the same unit-vector load (interface
-// rows zero), written to a column of W (the GEMM formation of S_s)
-enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3, MODE_UNIT = 4, MODE_FWD = 5 };
+enum SolveMode { MODE_FIRST = 0, MODE_ITER = 1, MODE_LAST = 2, MODE_RHS = 3, MODE_UNIT = 4 };
 
 // explicit Schur complements: 16x16 tiles of the lower triangle, row block
 // by row block; off-diagonal tiles 256 entries (four 8x8 sub-blocks),
@@ -126,16 +124,45 @@ struct SolveArgs {
   const int2 *unit_list; // MODE_UNIT, if set: (subdomain, unit vector) per work item
   double2 *smat;         // MODE_UNIT: Schur-complement tiles
   const long long *sm_off;  // per subdomain first entry of its tiles
-  double2 *wmat;         // MODE_FWD: W, per subdomain column-major (n x ng)
-  const long long *w_off;   // per subdomain first entry of its W
 };
 
-// S_s = A_GG - W^T D^-1 W with W = L^-1 A_IG (forward sweeps, MODE_FWD):
-// the product on the FP64 matrix cores, one wavefront per 16x16 lower tile
+// S_s = A_GG - W^T D^-1 W with W = L^-1 (-A_IG), all on the FP64 matrix
+// cores.  W is formed over a subdomain's rows up to its last interior one
+// (nsw; the rest of W is zero) and stored in chunks of 16 interface columns,
+// each chunk row-major: row i of chunk c at (c nsw + i) 16.
+// L_BB^-1 of the uniform 16-row blocks of the swept rows, 16x16 row-major
+// (one launch per system)
+struct BlockInv16Args {
+  const int2 *blist;     // (subdomain, block) per block, subdomain-major
+  int nblk;
+  const SubMeta *meta;
+  const int *nsw;        // per subdomain: rows swept
+  const int *idx1, *rowbeg;
+  const double2 *fac;    // this system's factors
+  double2 *m16;          // [nblk][256]
+};
+// W by blocked forward substitution: one wavefront per (subdomain, chunk)
+struct FwdArgs {
+  const int2 *work;      // (subdomain, chunk) per wave; subdomain -1: padding
+  int nwork;
+  const SubMeta *meta;
+  const int *nsw;
+  const int *idx1, *rowbeg;
+  const double2 *fac;
+  const int8_t *bcflag;
+  const int *cp_ptr, *cp_k;
+  const double2 *cp_val; // this system's couplings
+  const double2 *m16;
+  const long long *m16_off;  // per subdomain its first block in m16
+  double2 *wmat;
+  const long long *w_off;
+};
+// one wavefront per 16x16 lower tile of -W^T D^-1 W
 struct GemmArgs {
   const SubMeta *meta;
   const int4 *tiles;     // (subdomain, row block, column block, -) per wave; subdomain -1: padding
   int ntile;
+  const int *nsw;
   const double2 *wmat;
   const long long *w_off;
   const double2 *dinv;   // per DOF 1/d_i
@@ -296,6 +323,8 @@ hipError_t launch_blockinv(const SubMeta *meta, int nsub, const int *idx1, const
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st);
 size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk);
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st);
+hipError_t launch_blockinv16(const BlockInv16Args &a, hipStream_t st);
+hipError_t launch_schur_fwd(const FwdArgs &a, hipStream_t st);
 hipError_t launch_schur_gemm(const GemmArgs &a, hipStream_t st);
 hipError_t launch_schur_agg(const AggArgs &a, int nsub, hipStream_t st);
 size_t smv_lds_bytes(int nbmax, int waves);

@@ -316,44 +316,180 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
 }
 
 // ---------------------------------------------------------------------------
-// GEMM formation of S_s: with the Dirichlet matrix factored as L D L^T and
-// W = L^-1 (-A_IG) (forward sweeps, MODE_FWD of k_solve; interface rows
-// zero), S_s = A_GG - W^T D^-1 W.  One wavefront per 16x16 lower tile
-// (I, J) accumulates C = W_I^T D^-1 W_J over the subdomain's n rows with the
-// FP64 matrix instruction v_mfma_f64_16x16x4f64 (A[i][k] in lane i + 16 k,
-// B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i mod 4), register
-// i / 4): four real products per complex step,
-// C_re += W_r V_r - W_i V_i, C_im += W_r V_i + W_i V_r with V = D^-1 W_J.
-// The tile then holds -C (diagonal halved); k_schur_agg adds A_GG.
+// GEMM formation of S_s.  With the Dirichlet matrix factored as L D L^T and
+// W = L^-1 (-A_IG), S_s = A_GG - W^T D^-1 W.  The interface and essential
+// rows of W are zero (their rows of L are trivial, their loads zero), so W is
+// formed over the rows up to the last interior one (nsw) only, in chunks of
+// 16 interface columns (row i of chunk c at (c nsw + i) 16, the 16 columns of
+// a row 256 contiguous bytes; columns past ng are zero).
+// All three steps run on the FP64 matrix instruction v_mfma_f64_16x16x4f64:
+// A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane
+// j + 16 (i mod 4), register i / 4.  Register v of a result therefore holds
+// rows 4v..4v+3, which is the B operand of the k-step over those rows: a
+// block just solved feeds the next block's products from registers.
+// A complex product takes four real ones: re += Ar Br - Ai Bi,
+// im += Ar Bi + Ai Br.
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
+__device__ __forceinline__ mfma_d4 mfma64(double a, double b, mfma_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// L_BB^-1 of every uniform 16-row block of the swept rows (unit lower
+// triangle; rows past nsw are identity rows), 16x16 row-major:
+// (L^-1)_rq = -(L_rq + sum_{q<j<r} L_rj (L^-1)_jq), one lane per column q,
+// four blocks per wavefront.
+__global__ __launch_bounds__(64) void k_blockinv16(BlockInv16Args a) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 4);
+  if (g >= a.nblk) return;
+  const int2 sb = a.blist[g];
+  const SubMeta m = a.meta[sb.x];
+  const int i0 = 16 * sb.y, nb = min(16, a.nsw[sb.x] - i0);
+  const int q = threadIdx.x & 15;
+  const int *idx1 = a.idx1 + m.dof0, *rowbeg = a.rowbeg + m.dof0;
+  const double2 *fac = a.fac + m.fac0;
+  double2 x[16];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const bool rv = r < nb;
+    const int f = rv ? idx1[i0 + r] : 0, rb = rv ? rowbeg[i0 + r] : 0;
+    double2 acc = cz();
+    if (r > q) {
+      if (rv && i0 + q >= f) acc = fac[rb + (i0 + q - f)];
+#pragma unroll
+      for (int j = 1; j < r; j++)
+        if (j > q && rv && i0 + j >= f) acc = cfma(acc, fac[rb + (i0 + j - f)], x[j]);
+    }
+    x[r] = r < q ? cz() : (r == q ? make_double2(1.0, 0.0) : csub(cz(), acc));
+  }
+  double2 *out = a.m16 + (long long)g * 256;
+#pragma unroll
+  for (int r = 0; r < 16; r++) out[r * 16 + q] = x[r];
+}
+
+// W = L^-1 (-A_IG), 16 columns per wavefront, by blocked forward
+// substitution over uniform 16-row blocks: X = B_R - L[R, <R] W[<R] (the
+// off-block products, k-steps of 4 columns: the previous block's columns
+// from registers, older ones from this chunk of W as stored), W_R = M_R X.
+// B_R is the unit-vector load of the Schur step (hddm_solver.h:559-622 with
+// renewal_bc_sp): for an interior row, minus its coupling to the column's
+// interface DOF.  Rows of W are read only after being stored by the same
+// wavefront one block earlier or more; its vector memory operations
+// complete in issue order, and the row loads wait on loads issued after
+// those stores.
+__global__ __launch_bounds__(64) void k_schur_fwd(FwdArgs a) {
+  const int2 pr = a.work[blockIdx.x];
+  if (pr.x < 0) return;
+  const int s = pr.x, ch = pr.y;
+  const SubMeta m = a.meta[s];
+  const int nsw = a.nsw[s];
+  const int l = threadIdx.x, li = l & 15, lk = l >> 4;
+  const int col = 16 * ch + li;  // interface column of this lane's B and D entries
+  const int *idx1 = a.idx1 + m.dof0, *rowbeg = a.rowbeg + m.dof0;
+  const double2 *fac = a.fac + m.fac0;
+  double2 *W = a.wmat + a.w_off[s] + (long long)ch * nsw * 16;  // row i at W + 16 i
+  const double2 *M = a.m16 + a.m16_off[s] * 256;
+  mfma_d4 wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};  // the previous block's rows of W
+  for (int r0 = 0; r0 < nsw; r0 += 16) {
+    const int nb = min(16, nsw - r0);
+    // this lane's row as the A operand: row li of the block
+    const bool rv = li < nb;
+    const int row = r0 + (rv ? li : 0);
+    const int f = rv ? idx1[row] : r0, rb = rowbeg[row];
+    int cmin = f;  // first column any row of the block reaches
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) cmin = min(cmin, __shfl_xor(cmin, o, 64));
+    // the load rows (D layout: register v holds row 4v + lk, column li)
+    mfma_d4 xr, xi;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = 4 * v + lk;
+      double2 b = cz();
+      if (i < nb) {
+        const int d = m.dof0 + r0 + i;
+        if (a.bcflag[d] == 0)
+          for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++)
+            if (a.cp_k[c] == col) b = csub(b, a.cp_val[c]);
+      }
+      xr[v] = b.x;
+      xi[v] = b.y;
+    }
+    // off-block products L[R, <R] W[<R] in four accumulators
+    mfma_d4 arr = {0.0, 0.0, 0.0, 0.0}, aii = arr, ari = arr, air = arr;
+    if (r0 > 0) {
+      const int p0 = r0 - 16;  // the previous block (always 16 rows)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int c = p0 + 4 * v + lk;
+        const double2 L = (rv && c >= f) ? fac[rb + (c - f)] : cz();
+        arr = mfma64(L.x, wr[v], arr);
+        aii = mfma64(L.y, wi[v], aii);
+        ari = mfma64(L.x, wi[v], ari);
+        air = mfma64(L.y, wr[v], air);
+      }
+      for (int c0 = p0 - 4; c0 + 3 >= cmin; c0 -= 4) {
+        const int c = c0 + lk;
+        const bool cv = c >= 0;
+        const double2 L = (rv && cv && c >= f) ? fac[rb + (c - f)] : cz();
+        const double2 w = cv ? W[16 * c + li] : cz();
+        arr = mfma64(L.x, w.x, arr);
+        aii = mfma64(L.y, w.y, aii);
+        ari = mfma64(L.x, w.y, ari);
+        air = mfma64(L.y, w.x, air);
+      }
+    }
+    xr = xr - arr + aii;
+    xi = xi - ari - air;
+    // W_R = M_R X
+    const double2 *Mb = M + (r0 >> 4) * 256 + li * 16 + lk;
+    mfma_d4 orr = {0.0, 0.0, 0.0, 0.0}, oii = orr, ori = orr, oir = orr;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const double2 mm = Mb[4 * v];
+      orr = mfma64(mm.x, xr[v], orr);
+      oii = mfma64(mm.y, xi[v], oii);
+      ori = mfma64(mm.x, xi[v], ori);
+      oir = mfma64(mm.y, xr[v], oir);
+    }
+    wr = orr - oii;
+    wi = ori + oir;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = 4 * v + lk;
+      if (i < nb) W[16 * (r0 + i) + li] = make_double2(wr[v], wi[v]);
+    }
+  }
+}
+
+// S_s tiles: one wavefront per 16x16 lower tile (I, J) accumulates
+// C = W_I^T D^-1 W_J over the swept rows, D^-1 applied to the B operand as
+// it is loaded; the tile then holds -C (diagonal halved) and k_schur_agg
+// adds A_GG.
 __global__ __launch_bounds__(256) void k_schur_gemm(GemmArgs a) {
   const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (wid >= a.ntile) return;
   const int4 t = a.tiles[wid];
   if (t.x < 0) return;
   const SubMeta m = a.meta[t.x];
-  const int n = m.n, ng = m.ng, I = t.y, J = t.z;
-  const double2 *W = a.wmat + a.w_off[t.x];  // W[c][k] at c n + k
+  const int ng = m.ng, nsw = a.nsw[t.x], I = t.y, J = t.z;
+  const double2 *W = a.wmat + a.w_off[t.x];
   const double2 *dinv = a.dinv + m.dof0;
   const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
-  const int ci = 16 * I + li, cj = 16 * J + li;  // this lane's column of W for A and for B
-  const bool vi = ci < ng, vj = cj < ng;
-  const double2 *wi = W + (long long)(vi ? ci : 0) * n;
-  const double2 *wj = W + (long long)(vj ? cj : 0) * n;
+  const double2 *wi = W + (long long)I * nsw * 16 + li;  // row k of chunk I at wi[16 k]
+  const double2 *wj = W + (long long)J * nsw * 16 + li;
   mfma_d4 cre = {0.0, 0.0, 0.0, 0.0}, cim = {0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < n; k0 += 4) {
+  for (int k0 = 0; k0 < nsw; k0 += 4) {
     const int k = k0 + lk;
-    const bool in = k < n;
+    const bool in = k < nsw;
     const int kc = in ? k : 0;
-    const double2 x = wi[kc], y = wj[kc], d = dinv[kc];
-    const double ar = (vi && in) ? x.x : 0.0, ai = (vi && in) ? x.y : 0.0;
+    const double2 x = wi[16 * kc], y = wj[16 * kc], d = dinv[kc];
+    const double ar = in ? x.x : 0.0, ai = in ? x.y : 0.0;
     const double2 v = cmul(d, y);
-    const double br = (vj && in) ? v.x : 0.0, bi = (vj && in) ? v.y : 0.0;
-    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cre, 0, 0, 0);
-    cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, cre, 0, 0, 0);
-    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, cim, 0, 0, 0);
-    cim = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, cim, 0, 0, 0);
+    const double br = in ? v.x : 0.0, bi = in ? v.y : 0.0;
+    cre = mfma64(ar, br, cre);
+    cre = mfma64(-ai, bi, cre);
+    cim = mfma64(ar, bi, cim);
+    cim = mfma64(ai, br, cim);
   }
   const int nb = (ng + 15) >> 4;
   const bool half = smv_half(ng);
@@ -390,6 +526,18 @@ __global__ __launch_bounds__(256) void k_schur_agg(AggArgs a) {
       *e = cadd(*e, g);
     }
   }
+}
+
+hipError_t launch_blockinv16(const BlockInv16Args &a, hipStream_t st) {
+  if (a.nblk <= 0) return hipSuccess;
+  k_blockinv16<<<(a.nblk + 3) / 4, 64, 0, st>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_schur_fwd(const FwdArgs &a, hipStream_t st) {
+  if (a.nwork <= 0) return hipSuccess;
+  k_schur_fwd<<<a.nwork, 64, 0, st>>>(a);
+  return hipGetLastError();
 }
 
 hipError_t launch_schur_gemm(const GemmArgs &a, hipStream_t st) {

@@ -243,7 +243,7 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   const bool check_done = a.mode == MODE_ITER && a.done;
   const int done0 = check_done ? *a.done : 0;
   int w, uk = a.unit_k;
-  if ((a.mode == MODE_UNIT || a.mode == MODE_FWD) && a.unit_list) {
+  if (a.mode == MODE_UNIT && a.unit_list) {
     const int2 pr = a.unit_list[a.base + blockIdx.x];  // (subdomain, unit vector); -1: padding
     if (pr.x < 0) return;
     w = pr.x;
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
     w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
   }
   const SubMeta m = a.meta[w];
-  if ((a.mode == MODE_UNIT || a.mode == MODE_FWD) && uk >= m.ng) return;
+  if (a.mode == MODE_UNIT && uk >= m.ng) return;
   extern __shared__ double2 lds[];
   SolveLds L;
   L.v = lds;                          // [nmax]
@@ -294,11 +294,11 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
     double2 b;
     if (a.mode == MODE_RHS) {
       b = a.X[d];
-    } else if (a.mode == MODE_UNIT || a.mode == MODE_FWD) {
+    } else if (a.mode == MODE_UNIT) {
       const int8_t fl = a.bcflag[d];
       b = cz();
       if (fl == 1) {
-        if (a.mode == MODE_UNIT && a.kidx[d] == uk) b.x = 1.0;
+        if (a.kidx[d] == uk) b.x = 1.0;
       } else if (fl == 0) {
         for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++)
           if (a.cp_k[c] == uk) b = csub(b, a.cp_val[c]);
@@ -332,13 +332,6 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
     fwd_block(L, fac, nblk, b + 1, pb);
   }
   wave_sync();
-  if (a.mode == MODE_FWD) {
-    // column unit_k of W = L^-1 (-A_IG e_k): interface and essential rows are
-    // zero (their rows of L are trivial)
-    double2 *dst = a.wmat + a.w_off[w] + (long long)uk * n;
-    for (int i = lane; i < n; i += WL) dst[i] = L.v[i];
-    return;
-  }
 
   // ---- 3. diagonal (before the backward loads are issued: global loads
   // complete in issue order, so a later dinv load would drain them) ---------
