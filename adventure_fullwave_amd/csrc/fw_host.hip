@@ -76,6 +76,7 @@ struct fwh_ctx {
   bool short_ok = false;
   long long ndof = 0, nslots = 0, nnz = 0, nlen = 0, nel = 0, ncpl = 0, ngrow = 0;
   std::vector<SubMeta> meta_h;
+  std::vector<int8_t> bcflag_h;  // per DOF: 0 interior, 1 interface, 2 essential
   std::vector<int> order_h;
   // systems of a batch (fwh_load_batch): shared structure, values
   // concatenated system-major with strides ss; sel = the system the
@@ -708,6 +709,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   const long long ncpl = (long long)cp_k.size(), ngrow = (long long)gr_col.size();
   if ((rc = c->alloc(&d_elem, 27 * (size_t)nel)) || (rc = c->alloc(&d_bad, 1))) return rc;
   HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(int), st));
+  c->bcflag_h = bcflag;
   if ((rc = c->upload(&d_asm_ptr, asm_ptr)) || (rc = c->upload(&d_asm_src, asm_src)) ||
       (rc = c->upload(&d_f_ptr, f_ptr)) || (rc = c->upload(&d_f_src, f_src)) ||
       (rc = c->upload(&c->rowptr, rowptr)) || (rc = c->upload(&c->col, col)) ||
@@ -1106,60 +1108,7 @@ int build_schur(fwh_ctx *c) {
   c->nbmax = nbmax;
   c->ss.smat = tot;  // every system: 64 zeros, then its tiles
   HIPCHK(fw::launch_fill(c->smat, (long long)c->nsys * tot, make_double2(0.0, 0.0), c->st));
-  // one launch over all (subdomain, unit vector) pairs.  Workgroups are
-  // dealt to the XCDs round-robin (workgroup b on XCD b mod 8), so the pairs
-  // of subdomain s are placed at b = 8 i + (s mod 8): each subdomain's
-  // factor is read by one XCD and reused from its L2.
-  // With size buckets in use, every bucket gets its own list and launch
-  // (on the bucket streams, with that bucket's LDS size).
   constexpr int NXCD = 8;
-  std::vector<int2> ulist;
-  std::vector<size_t> ufirst;
-  const bool bk = c->use_buckets && c->buckets.size() > 1;
-  std::vector<std::vector<int>> groups;
-  if (bk) {
-    std::vector<int> bo(c->nsub);
-    HIPCHK(hipMemcpy(bo.data(), c->border, c->nsub * sizeof(int), hipMemcpyDeviceToHost));
-    for (auto &B : c->buckets) groups.emplace_back(bo.begin() + B.first, bo.begin() + B.first + B.count);
-  } else {
-    groups.push_back(c->order_h);  // largest skyline first
-  }
-  for (auto &grp : groups) {
-    std::vector<std::vector<int2>> lane(NXCD);
-    for (int s : grp)
-      for (int k = 0; k < c->meta_h[s].ng; k++) lane[s % NXCD].push_back(make_int2(s, k));
-    size_t len = 0;
-    for (auto &v : lane) len = std::max(len, v.size());
-    ufirst.push_back(ulist.size());
-    ulist.resize(ulist.size() + len * NXCD, make_int2(-1, 0));
-    for (int x = 0; x < NXCD; x++)
-      for (size_t i = 0; i < lane[x].size(); i++) ulist[ufirst.back() + i * NXCD + x] = lane[x][i];
-  }
-  ufirst.push_back(ulist.size());
-  int2 *d_ulist = nullptr;
-  if ((rc = c->upload(&d_ulist, ulist))) return rc;
-  // the unit-vector launches: one over all pairs, or one per size bucket
-  auto unit_launch = [&](fw::SolveArgs a) -> int {
-    a.unit_list = d_ulist;
-    a.order = nullptr;
-    if (!bk) {
-      HIPCHK(fw::launch_solve(a, (int)ulist.size(), c->st));
-      return 0;
-    }
-    HIPCHK(hipEventRecord(c->ev_fork, c->st));
-    for (size_t b = 0; b < c->buckets.size(); b++) {
-      fw::SolveArgs ab = a;
-      ab.base = (int)ufirst[b];
-      ab.nmax = c->buckets[b].nmax;
-      ab.ngmax = c->buckets[b].ngmax;
-      ab.nblkmax = a.pfk == 8 ? c->buckets[b].nblkmax_s : c->buckets[b].nblkmax;
-      HIPCHK(hipStreamWaitEvent(c->bst[b], c->ev_fork, 0));
-      HIPCHK(fw::launch_solve(ab, (int)(ufirst[b + 1] - ufirst[b]), c->bst[b]));
-      HIPCHK(hipEventRecord(c->ev_join[b], c->bst[b]));
-      HIPCHK(hipStreamWaitEvent(c->st, c->ev_join[b], 0));
-    }
-    return 0;
-  };
   // GEMM formation: W = L^-1 (-A_IG) by blocked forward substitution on the
   // matrix cores (k_blockinv16, then k_schur_fwd: one wave per (subdomain,
   // 16-column chunk), the chunks of subdomain s dealt to XCD s mod 8 so that
@@ -1170,8 +1119,7 @@ int build_schur(fwh_ctx *c) {
   // unit-vector Schur step (MODE_UNIT, forward and backward sweeps per
   // column) remains for FWH_SCHUR_FORM=solve or when they would take more
   // than half the free device memory.
-  std::vector<int8_t> bcf(c->ndof);
-  HIPCHK(hipMemcpy(bcf.data(), c->bcflag, c->ndof, hipMemcpyDeviceToHost));
+  const std::vector<int8_t> &bcf = c->bcflag_h;
   std::vector<int> nsw(c->nsub, 0);
   std::vector<long long> woff(c->nsub), moff(c->nsub);
   std::vector<int2> blist;
@@ -1295,6 +1243,59 @@ int build_schur(fwh_ctx *c) {
       if (ptr) (void)hipFree(ptr);
     if (e != hipSuccess) return fail(FWH_E_HIP, "Schur formation (GEMM): %s", hipGetErrorString(e));
   } else {
+    // one launch over all (subdomain, unit vector) pairs.  Workgroups are
+    // dealt to the XCDs round-robin (workgroup b on XCD b mod 8), so the pairs
+    // of subdomain s are placed at b = 8 i + (s mod 8): each subdomain's
+    // factor is read by one XCD and reused from its L2.
+    // With size buckets in use, every bucket gets its own list and launch
+    // (on the bucket streams, with that bucket's LDS size).
+    std::vector<int2> ulist;
+    std::vector<size_t> ufirst;
+    const bool bk = c->use_buckets && c->buckets.size() > 1;
+    std::vector<std::vector<int>> groups;
+    if (bk) {
+      std::vector<int> bo(c->nsub);
+      HIPCHK(hipMemcpy(bo.data(), c->border, c->nsub * sizeof(int), hipMemcpyDeviceToHost));
+      for (auto &B : c->buckets) groups.emplace_back(bo.begin() + B.first, bo.begin() + B.first + B.count);
+    } else {
+      groups.push_back(c->order_h);  // largest skyline first
+    }
+    for (auto &grp : groups) {
+      std::vector<std::vector<int2>> lane(NXCD);
+      for (int s : grp)
+        for (int k = 0; k < c->meta_h[s].ng; k++) lane[s % NXCD].push_back(make_int2(s, k));
+      size_t len = 0;
+      for (auto &v : lane) len = std::max(len, v.size());
+      ufirst.push_back(ulist.size());
+      ulist.resize(ulist.size() + len * NXCD, make_int2(-1, 0));
+      for (int x = 0; x < NXCD; x++)
+        for (size_t i = 0; i < lane[x].size(); i++) ulist[ufirst.back() + i * NXCD + x] = lane[x][i];
+    }
+    ufirst.push_back(ulist.size());
+    int2 *d_ulist = nullptr;
+    if ((rc = c->upload(&d_ulist, ulist))) return rc;
+    // the unit-vector launches: one over all pairs, or one per size bucket
+    auto unit_launch = [&](fw::SolveArgs a) -> int {
+      a.unit_list = d_ulist;
+      a.order = nullptr;
+      if (!bk) {
+        HIPCHK(fw::launch_solve(a, (int)ulist.size(), c->st));
+        return 0;
+      }
+      HIPCHK(hipEventRecord(c->ev_fork, c->st));
+      for (size_t b = 0; b < c->buckets.size(); b++) {
+        fw::SolveArgs ab = a;
+        ab.base = (int)ufirst[b];
+        ab.nmax = c->buckets[b].nmax;
+        ab.ngmax = c->buckets[b].ngmax;
+        ab.nblkmax = a.pfk == 8 ? c->buckets[b].nblkmax_s : c->buckets[b].nblkmax;
+        HIPCHK(hipStreamWaitEvent(c->bst[b], c->ev_fork, 0));
+        HIPCHK(fw::launch_solve(ab, (int)(ufirst[b + 1] - ufirst[b]), c->bst[b]));
+        HIPCHK(hipEventRecord(c->ev_join[b], c->bst[b]));
+        HIPCHK(hipStreamWaitEvent(c->st, c->ev_join[b], 0));
+      }
+      return 0;
+    };
     if ((rc = unit_launch(c->solve_args(fw::MODE_UNIT, false)))) return rc;
   }
   HIPCHK(hipStreamSynchronize(c->st));
