@@ -109,6 +109,7 @@ struct fwh_ctx {
   // vectors
   double2 *P = nullptr, *Q = nullptr, *R = nullptr, *Z = nullptr, *U = nullptr, *PC = nullptr;
   double2 *yslot = nullptr, *pcslot = nullptr, *X = nullptr, *part = nullptr;
+  double2 *pqpart = nullptr;  // per system and subdomain: (p,q) partials of the explicit Schur matvec
   fw::Scal *sc = nullptr;
   double *hist = nullptr;
   int hist_cap = 0;
@@ -281,6 +282,8 @@ struct fwh_ctx {
     a.sc = sc;
     a.part = part;
     a.G = G;
+    a.pqpart = (check_done && fused) ? pqpart : nullptr;  // an interface iteration
+    a.npq = nsub;
     a.yslot = yslot;
     return a;
   }
@@ -301,6 +304,8 @@ struct fwh_ctx {
     v.U = U;
     v.PC = PC;
     v.part = part;
+    v.pqpart = pqpart;
+    v.npq = nsub;
     v.sc = sc;
     v.hist = hist;
     v.G = G;
@@ -321,6 +326,12 @@ struct fwh_ctx {
       if (e != hipSuccess) return e;
     }
     return hipSuccess;
+  }
+  // the vector part of an interface iteration after its Schur step:
+  // k_iter_a, k_iter_b (implicit) or k_iter_ab (explicit: the matvec leaves
+  // the (p,q) partials), then k_iter_c when status_step
+  hipError_t launch_vec_iter(bool expl, bool status_step) {
+    return expl ? fw::launch_iter_ab(vec_args(), st, status_step) : fw::launch_iter(vec_args(), st, status_step);
   }
   // the Schur step of an interface iteration (y_slot = S_s p_s): implicit
   // (Dirichlet solves through the factors) or explicit (formed S_s)
@@ -928,7 +939,8 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   if ((rc = c->alloc(&c->P, ngs)) || (rc = c->alloc(&c->Q, ngs)) || (rc = c->alloc(&c->R, ngs)) ||
       (rc = c->alloc(&c->Z, ngs)) || (rc = c->alloc(&c->U, ngs)) || (rc = c->alloc(&c->PC, ngs)) ||
       (rc = c->alloc(&c->yslot, (size_t)nsys * nslots)) || (rc = c->alloc(&c->X, (size_t)nsys * ndof)) ||
-      (rc = c->alloc(&c->part, (size_t)nsys * 3 * c->G)) || (rc = c->alloc(&c->sc, nsys)))
+      (rc = c->alloc(&c->part, (size_t)nsys * 3 * c->G)) || (rc = c->alloc(&c->sc, nsys)) ||
+      (rc = c->alloc(&c->pqpart, (size_t)nsys * nsub)))
     return rc;
   c->ss.dof = ndof;
   c->ss.nlen = nlen_tot;
@@ -1366,6 +1378,8 @@ int build_schur(fwh_ctx *c) {
       work[i].off = off[sd];
       work[i].g0 = c->meta_h[sd].g0;
       work[i].ng = c->meta_h[sd].ng;
+      work[i].sub = sd;
+      work[i].pad = 0;
     }
     if ((rc = c->upload(&d_orders, all)) || (rc = c->upload(&d_work, work))) return rc;
     double bo = 1e30;
@@ -1463,7 +1477,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   if (o->plain) {
     while (it < n && !all) {
       HIPCHK(c->run_schur(expl, true, true, true));
-      HIPCHK(fw::launch_iter(c->vec_args(), c->st));
+      HIPCHK(c->launch_vec_iter(expl, true));
       if ((rc = read_scal(c, s, &all))) return rc;
       it++;
     }
@@ -1475,7 +1489,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
       // k_iter_c runs only after the last iteration of the graph
       for (int k = 0; k < o->graph_iters; k++) {
         hipError_t e = c->run_schur(expl, false, true, true);
-        if (e == hipSuccess) e = fw::launch_iter(c->vec_args(), c->st, !expl || k == o->graph_iters - 1);
+        if (e == hipSuccess) e = c->launch_vec_iter(expl, !expl || k == o->graph_iters - 1);
         if (e != hipSuccess) {
           hipGraph_t g;
           (void)hipStreamEndCapture(c->st, &g);
@@ -1519,7 +1533,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   } else {
     while (it < n && !all) {
       HIPCHK(c->run_schur(expl, false, true, true));
-      HIPCHK(fw::launch_iter(c->vec_args(), c->st));
+      HIPCHK(c->launch_vec_iter(expl, true));
       it++;
       if ((it & 15) == 0 || it >= n)
         if ((rc = read_scal(c, s, &all))) return rc;

@@ -187,6 +187,8 @@ struct SmvWork {
   long long off;  // first tile entry
   int g0;         // first slot
   int ng;         // interface DOFs
+  int sub;        // subdomain
+  int pad;
 };
 
 struct SmvArgs {
@@ -207,6 +209,8 @@ struct SmvArgs {
   Scal *sc;              // status of the interface iteration (with Z)
   const double2 *part;   // (r,z) and max|r| partials of k_iter_b (with Z)
   int G;                 // their count per system
+  double2 *pqpart;       // per subdomain its part of (p,q) = sum_slots p y (explicit iteration; may be null)
+  int npq;               // per system: subdomains
   double2 *yslot;
 };
 
@@ -276,7 +280,7 @@ struct Scal {
   // the status step of an iteration ended without k_iter_c: k_iter_b sets
   // pending; the next Schur matvec takes the step from k_iter_b's partials
   // and stages its result (st_*, st_fresh); the next k_iter_a commits it.
-  // With nothing staged, st_beta and st_done equal beta and done.
+  // With nothing staged, st_rz, st_beta and st_done equal rz, beta and done.
   int pending;
   double2 st_rz, st_beta;
   double st_resi;
@@ -293,6 +297,8 @@ struct VecArgs {
   const double2 *pcslot;
   double2 *P, *Q, *R, *Z, *U, *PC;
   double2 *part;          // [3*G] partial sums
+  const double2 *pqpart;  // explicit iteration: per subdomain (p,q) partials of the Schur matvec
+  int npq;
   Scal *sc;
   double *hist;
   int G;                  // grid size of the vector kernels (per system)
@@ -333,6 +339,8 @@ hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, con
 hipError_t launch_first(const VecArgs &a, hipStream_t st);
 // k_iter_a, k_iter_b and (status_step) k_iter_c
 hipError_t launch_iter(const VecArgs &a, hipStream_t st, bool status_step = true);
+// k_iter_ab and (status_step) k_iter_c: the explicit iteration after k_smv
+hipError_t launch_iter_ab(const VecArgs &a, hipStream_t st, bool status_step = true);
 hipError_t launch_finalize_p(const VecArgs &a, hipStream_t st);
 hipError_t launch_gather_q(const VecArgs &a, hipStream_t st);
 hipError_t launch_fill(double2 *p, long long n, double2 v, hipStream_t st);

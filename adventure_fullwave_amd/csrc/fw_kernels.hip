@@ -628,6 +628,7 @@ __device__ __forceinline__ void to_system(VecArgs &a) {
   a.U += s * a.ss.g;
   a.PC += s * a.ss.g;
   a.part += s * 3LL * a.G;
+  if (a.pqpart) a.pqpart += s * (long long)a.npq;
   a.sc += s;
   a.hist += s * a.ss.hist;
 }
@@ -679,6 +680,7 @@ __global__ __launch_bounds__(VT) void k_first_b(VecArgs a) {
     s->conv = 0;
     s->resi = 1.0;
     s->pending = 0;
+    s->st_rz = rz;
     s->st_beta = cz();
     s->st_done = s->done;
     s->st_fresh = 0;
@@ -842,11 +844,99 @@ __global__ __launch_bounds__(VT) void k_iter_c(VecArgs a) {
     s->done = r.done;
     s->loop = loop + 1;
     s->pending = 0;
+    s->st_rz = rzs;
     s->st_beta = r.beta;
     s->st_done = r.done;
     s->st_fresh = 0;
   }
 }
+// Explicit interface iteration after the Schur matvec: k_iter_a and
+// k_iter_b in one pass.  The matvec leaves every subdomain's part of (p,q)
+// (pqpart: the sum over its slots of p y, which regroups sum_g p_g q_g by
+// subdomain), so alpha is known when the pass starts: p = z + beta p is formed
+// and written back, q gathered, and u, r, z updated with the arithmetic of
+// k_iter_a and k_iter_b.  The status comes from the staged fields only
+// (st_rz, st_beta, st_done: equal to the committed ones when nothing is
+// staged), which no block of this kernel writes; block 0 commits a step the
+// matvec staged.
+__global__ __launch_bounds__(VT) void k_iter_ab(VecArgs a) {
+  to_system(a);
+  const int g0 = blockIdx.x * VT + threadIdx.x;
+  double2 q0 = cz(), z0 = cz(), p0 = cz(), u0 = cz(), r0 = cz(), pc0 = cz();
+  if (g0 < a.ng) {
+    q0 = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g0);
+    z0 = a.Z[g0];
+    p0 = a.P[g0];
+    u0 = a.U[g0];
+    r0 = a.R[g0];
+    pc0 = a.PC[g0];
+  }
+  Scal *s = a.sc;
+  const int done = s->st_done;
+  const double2 beta = s->st_beta, rz = s->st_rz;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && s->st_fresh) {
+    const int loop = s->loop;
+    const double resi = s->st_resi;
+    a.hist[loop] = resi;
+    s->resi = resi;
+    s->conv = s->st_conv;
+    s->rz = s->st_rz;
+    s->beta = s->st_beta;
+    s->iters = s->iters + 1;
+    s->done = s->st_done;
+    s->loop = loop + 1;
+    s->st_fresh = 0;
+    s->pending = 0;
+  }
+  __shared__ double2 shc[VT / 64 + 1];
+  __shared__ double shm[VT / 64 + 1];
+  const double2 pq = reduce_parts(a.pqpart, a.npq, shc);
+  if (done) return;
+  // this iteration's status step is pending until k_iter_c or the next
+  // Schur matvec takes it (no other block of this kernel reads the flag)
+  if (blockIdx.x == 0 && threadIdx.x == 0) s->pending = 1;
+  double2 malpha = cz();
+  if (!(cabsv(rz) < NEARLY_ZERO && cabsv(pq) < NEARLY_ZERO)) malpha = csub(cz(), cdiv_naive(rz, pq));
+  double2 rzn = cz();
+  double mx = 0.0;
+  if (g0 < a.ng) {
+    const double2 p = zbp(z0, beta, p0);
+    a.P[g0] = p;
+    const double2 u = cadd(u0, cmul(malpha, p));
+    const double2 r = cadd(r0, cmul(malpha, q0));
+    const double2 z = cmul(r, pc0);
+    a.U[g0] = u;
+    a.R[g0] = r;
+    a.Z[g0] = z;
+    mx = cabsv(r);
+    rzn = cmul(r, z);
+  }
+  for (int g = g0 + a.G * VT; g < a.ng; g += a.G * VT) {
+    const double2 q = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
+    const double2 p = zbp(a.Z[g], beta, a.P[g]);
+    a.P[g] = p;
+    const double2 u = cadd(a.U[g], cmul(malpha, p));
+    const double2 r = cadd(a.R[g], cmul(malpha, q));
+    const double2 z = cmul(r, a.PC[g]);
+    a.U[g] = u;
+    a.R[g] = r;
+    a.Z[g] = z;
+    mx = fmax(mx, cabsv(r));
+    rzn = cadd(rzn, cmul(r, z));
+  }
+  rzn = block_csum(rzn, shc);
+  mx = block_max(mx, shm);
+  if (threadIdx.x == 0) {
+    a.part[a.G + blockIdx.x] = rzn;
+    a.part[2 * a.G + blockIdx.x] = make_double2(mx, 0.0);
+  }
+}
+hipError_t launch_iter_ab(const VecArgs &a, hipStream_t st, bool status_step) {
+  k_iter_ab<<<vec_grid(a, a.G), VT, 0, st>>>(a);
+  if (status_step) k_iter_c<<<vec_grid(a, 1), VT, 0, st>>>(a);
+  return hipGetLastError();
+}
+
 hipError_t launch_iter(const VecArgs &a, hipStream_t st, bool status_step) {
   k_iter_a<<<vec_grid(a, a.G), VT, 0, st>>>(a);
   k_iter_b<<<vec_grid(a, a.G), VT, 0, st>>>(a);
@@ -901,6 +991,7 @@ __global__ void k_scal_init(Scal *s, double eps, int max_loop) {
   s->iters = 0;
   s->beta = cz();
   s->pending = 0;
+  s->st_rz = cz();
   s->st_beta = cz();
   s->st_done = 0;
   s->st_fresh = 0;

@@ -179,6 +179,7 @@ __device__ __forceinline__ void to_system(SmvArgs &a) {
   a.sc += s;
   if (a.done) a.done = &a.sc->done;
   if (a.part) a.part += s * 3LL * a.G;
+  if (a.pqpart) a.pqpart += s * (long long)a.npq;
   a.yslot += s * a.ss.slot;
 }
 
@@ -228,6 +229,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     wk.off = a.sm_off[w];
     wk.g0 = a.meta[w].g0;
     wk.ng = a.meta[w].ng;
+    wk.sub = w;
   }
   const int ng = wk.ng, nb = (ng + 15) >> 4, npad = nb * 16, stride = a.nbmax * 16;
   extern __shared__ double2 lds[];
@@ -240,6 +242,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   if (ng == 0) {
     double2 beta;
     if (a.done && a.Z && blockIdx.x == 0 && a.base == 0) (void)iter_status(a, NT, beta);  // it may have to stage
+    if (a.pqpart && tid == 0) a.pqpart[wk.sub] = cz();
     return;
   }
 
@@ -308,10 +311,26 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     }
   }
   __syncthreads();
+  // y, and in an explicit iteration this subdomain's part of (p,q) =
+  // sum over its slots of p y (the interface sum of p q, regrouped by
+  // subdomain; fixed order: per thread ascending, waves in order)
+  double2 pq = cz();
   for (int k = tid; k < ng; k += NT) {
     double2 y = cadd(acc[k], acc[stride + k]);
     for (int v = 2; v < SMV_WAVES; v++) y = cadd(y, acc[v * stride + k]);
     a.yslot[wk.g0 + k] = y;
+    pq = cadd(pq, cmul(pv[k], y));
+  }
+  if (a.pqpart) {
+    __shared__ double2 shq[SMV_WAVES];
+    pq = wave_csum(pq);
+    if (lane == 0) shq[wave] = pq;
+    __syncthreads();
+    if (tid == 0) {
+      double2 t = shq[0];
+      for (int v = 1; v < SMV_WAVES; v++) t = cadd(t, shq[v]);
+      a.pqpart[wk.sub] = t;
+    }
   }
 }
 

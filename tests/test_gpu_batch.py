@@ -6,9 +6,9 @@ at a third frequency.  The batch solve (every launch serving all three) must
 reproduce three separate single-system solves: bit for bit when the
 load-time kernel choices agree (matvec waves, LDL^T prefetch depth, size
 buckets: they set summation orders), else to rel. L2 1e-9 with counts within
-2 %; and each system must match the oracle (count within 2 %, solution rel.
-L2 1e-7).  Per-system accessors (select) and the structure check are
-covered too."""
+2 %; and each system must match the oracle (count and solution bars per
+system from measured spreads, ORACLE_BARS).  Per-system accessors (select)
+and the structure check are covered too."""
 import dataclasses
 import sys
 
@@ -63,6 +63,16 @@ def test_batch_matches_single_solves(systems, schur):
     b.close()
 
 
+# Per-system bars against the oracle (count, solution rel. L2).  The
+# delta-gap system at 1.05 omega has a residual plateau near 5e-8 before the
+# 1e-8 stop, so its count is bimodal: over 12 few-ulp permittivity
+# perturbations both GPU paths stop at 124, 125 or 129 iterations (the
+# oracle's own count moves 124-125 over 18), and its 1e-8-stopped solution
+# lies 1.5e-7 from the fully converged one (the other two: 7e-9 and 2.7e-8),
+# so two correct stops can differ by twice that.
+ORACLE_BARS = [(0.02, 1e-7), (0.02, 1e-7), (0.05, 3e-7)]
+
+
 def test_batch_systems_match_oracle(systems):
     from adventure_fullwave_amd import fwhip
     say("[gpu] batch systems vs oracle")
@@ -73,9 +83,10 @@ def test_batch_systems_match_oracle(systems):
         st = b.select(k).stats()
         it_ref = ref["stats"]["iters"]
         say(f"[gpu] system {k}: iters gpu={st['iters']} oracle={it_ref}")
-        assert abs(st["iters"] - it_ref) <= max(1, 0.02 * it_ref)
+        count_bar, x_bar = ORACLE_BARS[k]
+        assert abs(st["iters"] - it_ref) <= max(1, count_bar * it_ref)
         x = b.x()
-        assert np.linalg.norm(x - ref["x_flat"]) / np.linalg.norm(ref["x_flat"]) <= 1e-7
+        assert np.linalg.norm(x - ref["x_flat"]) / np.linalg.norm(ref["x_flat"]) <= x_bar
     b.close()
 
 
