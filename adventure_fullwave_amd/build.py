@@ -21,7 +21,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              # packed-FP32 instructions are kept out of all device code
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
-HIP_SOURCES = ["fw_kernels.hip", "fw_solve.hip", "fw_schur.hip", "fw_host.hip"]
+HIP_SOURCES = ["fw_kernels.hip", "fw_solve.hip", "fw_schur.hip", "fw_factor.hip", "fw_host.hip"]
 C_SOURCES = ["hddm_driver.c"]
 
 

@@ -908,6 +908,25 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   }
   c->meta_h = meta;
 
+  // the blocked factorisation (default; FWH_FACTOR=row: one row at a time)
+  // keeps the inverted diagonal blocks of the finished block rows
+  const char *fenv = getenv("FWH_FACTOR");
+  const bool blocked = !(fenv && strcmp(fenv, "row") == 0);
+  std::vector<long long> mf_off(nsub);
+  long long mf_tot = 0;
+  for (int s = 0; s < nsub; s++) {
+    mf_off[s] = mf_tot;
+    mf_tot += (meta[s].n + 15) / 16;
+  }
+  double2 *m16 = nullptr;
+  long long *d_mf_off = nullptr;
+  if (blocked) {
+    HIPCHK(hipMalloc(&m16, std::max<long long>(mf_tot, 1) * 256 * sizeof(double2)));
+    HIPCHK(hipMalloc(&d_mf_off, nsub * sizeof(long long)));
+    HIPCHK(hipMemcpyAsync(d_mf_off, mf_off.data(), nsub * sizeof(long long), hipMemcpyHostToDevice, st));
+  }
+  c->info.factor_blocked = blocked ? 1 : 0;
+
   HIPCHK(hipStreamSynchronize(st));
   t0 = now_ms();
   HIPCHK(hipMemsetAsync(c->fac, 0, (size_t)nsys * nlen_tot * sizeof(double2), st));
@@ -926,12 +945,16 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     fa.dinv = c->dinv + k * ndof;
     fa.wmax = wmax;
     fa.nmax = nmax;
-    HIPCHK(fw::launch_factor(fa, nsub, st));
+    fa.m16 = m16;
+    fa.mf_off = d_mf_off;
+    HIPCHK(blocked ? fw::launch_factor16(fa, nsub, st) : fw::launch_factor(fa, nsub, st));
     HIPCHK(fw::launch_blockinv(c->meta, nsub, c->idx1, c->rowbeg, c->blk_row, fac,
                                c->minv + (size_t)k * blk_row.size() * fw::MINV, st));
   }
   HIPCHK(hipStreamSynchronize(st));
   c->info.ms_factor = now_ms() - t0;
+  if (m16) (void)hipFree(m16);
+  if (d_mf_off) (void)hipFree(d_mf_off);
 
   // ---- interface vectors ------------------------------------------------------------
   c->G = std::max(1, std::min(512, (ng + fw::VT_HINT - 1) / fw::VT_HINT));

@@ -224,6 +224,8 @@ struct FactorArgs {
   double2 *dinv;
   int wmax;              // LDS row buffer length
   int nmax;
+  double2 *m16;          // blocked form: per subdomain and 16-row block L_BB^-1 (16x16, temporary)
+  const long long *mf_off;  // per subdomain its first block in m16
 };
 
 struct AsmArgs {
@@ -322,6 +324,8 @@ hipError_t launch_skydiag_bc(int ndof, const int *dof_sub, const SubMeta *meta,
                              const int8_t *bcflag, const int *rowbeg, const int *idx1, double2 *fac,
                              hipStream_t st);
 hipError_t launch_factor(const FactorArgs &a, int nwork, hipStream_t st);
+// the blocked form on the matrix cores (fw_factor.hip)
+hipError_t launch_factor16(const FactorArgs &a, int nwork, hipStream_t st);
 // inverted unit lower triangles of every row block (both partitions) of
 // every subdomain, from the factors
 hipError_t launch_blockinv(const SubMeta *meta, int nsub, const int *idx1, const int *rowbeg, const int *blk_row,

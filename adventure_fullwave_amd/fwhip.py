@@ -58,7 +58,8 @@ class FwhInfo(ctypes.Structure):
                 ("ms_smv_tune", ctypes.c_double * 3), ("solve_depth", ctypes.c_int),
                 ("unit_depth", ctypes.c_int), ("ms_solve_depth", ctypes.c_double * 2),
                 ("nsys", ctypes.c_int), ("smv_order", ctypes.c_int), ("ms_smv_order", ctypes.c_double * 3),
-                ("schur_gemm", ctypes.c_int)]
+                ("schur_gemm", ctypes.c_int),
+                ("factor_blocked", ctypes.c_int)]
 
     def as_dict(self):
         out = {}

@@ -107,8 +107,10 @@ typedef struct fwh_info {
                            first, 2 most tiles first snaking over the CUs  */
   double ms_smv_order[3]; /* its timings                                   */
   int    schur_gemm;    /* 1: S_s formed as A_GG - W^T D^-1 W on the matrix
-                           cores (W by forward sweeps); 0: by the Schur
-                           step on every interface unit vector             */
+                           cores (W by blocked forward substitution); 0: by
+                           the Schur step on every interface unit vector   */
+  int    factor_blocked; /* 1: LDL^T factors by 16-row blocks on the matrix
+                           cores; 0: one row at a time (FWH_FACTOR=row)     */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */
