@@ -106,7 +106,14 @@ __device__ __forceinline__ double2 parts_csum256(const double2 *__restrict__ p, 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int vw = wave; vw < 4; vw += NT >> 6) {
     double2 t = cz();
-    for (int k = vw * 64 + lane; k < G; k += 256) t = cadd(t, p[k]);
+    for (int k = vw * 64 + lane; k < G; k += 1024) {  // four loads in flight, added in order
+      double2 v[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) v[j] = k + 256 * j < G ? p[k + 256 * j] : cz();
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (k + 256 * j < G) t = cadd(t, v[j]);
+    }
     t = wave_csum(t);
     if (lane == 0) sh[vw] = t;
   }
@@ -120,7 +127,14 @@ __device__ __forceinline__ double parts_max256(const double2 *__restrict__ p, in
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int vw = wave; vw < 4; vw += NT >> 6) {
     double t = 0.0;
-    for (int k = vw * 64 + lane; k < G; k += 256) t = fmax(t, p[k].x);
+    for (int k = vw * 64 + lane; k < G; k += 1024) {
+      double v[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) v[j] = k + 256 * j < G ? p[k + 256 * j].x : 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (k + 256 * j < G) t = fmax(t, v[j]);
+    }
     t = wave_max(t);
     if (lane == 0) sh[vw] = t;
   }

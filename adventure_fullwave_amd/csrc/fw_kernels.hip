@@ -558,6 +558,29 @@ __device__ __forceinline__ double2 gather_slots(const int *__restrict__ ptr, con
                                                 const double2 *__restrict__ y, int g) {
   double2 acc = cz(), part = cz();
   const int c0 = ptr[g], c1 = ptr[g + 1];
+  if (c1 - c0 <= 4) {
+    // the usual case (an interface edge in 2-4 subdomains): the slot ids
+    // and values are requested together, then summed in list order
+    const int cnt = c1 - c0;
+    int sl[4];
+    double2 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) sl[j] = j < cnt ? slot[c0 + j] : 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = j < cnt ? y[sl[j] < 0 ? ~sl[j] : sl[j]] : cz();
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (j < cnt) {
+        if (sl[j] < 0) {
+          if (j != 0) acc = cadd(acc, part);
+          part = v[j];
+        } else {
+          part = cadd(part, v[j]);
+        }
+      }
+    }
+    return cnt > 0 ? cadd(acc, part) : acc;
+  }
   for (int c = c0; c < c1; c++) {
     int sl = slot[c];
     if (sl < 0) {
@@ -604,14 +627,29 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
   return sh[VT / 64];
 }
 // identical fixed-order reduction of G partials in every block
+// (four loads per thread in flight, added in the same order)
 __device__ __forceinline__ double2 reduce_parts(const double2 *__restrict__ p, int G, double2 *sh) {
   double2 t = cz();
-  for (int k = threadIdx.x; k < G; k += VT) t = cadd(t, p[k]);
+  for (int k = threadIdx.x; k < G; k += 4 * VT) {
+    double2 v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = k + j * VT < G ? p[k + j * VT] : cz();
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (k + j * VT < G) t = cadd(t, v[j]);
+  }
   return block_csum(t, sh);
 }
 __device__ __forceinline__ double reduce_parts_max(const double2 *__restrict__ p, int G, double *sh) {
   double t = 0.0;
-  for (int k = threadIdx.x; k < G; k += VT) t = fmax(t, p[k].x);
+  for (int k = threadIdx.x; k < G; k += 4 * VT) {
+    double v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = k + j * VT < G ? p[k + j * VT].x : 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (k + j * VT < G) t = fmax(t, v[j]);
+  }
   return block_max(t, sh);
 }
 

@@ -190,18 +190,66 @@ __device__ __forceinline__ void to_system(SmvArgs &a) {
 // workgroup takes it from k_iter_b's partials in k_iter_c's order, and
 // workgroup 0 stages it for the next k_iter_a to commit; no workgroup of
 // this kernel reads the staged fields.  Contains barriers: call uniformly.
-__device__ __forceinline__ bool iter_status(const SmvArgs &a, int NT, double2 &beta) {
+// With 256 threads each thread is its own virtual thread of that order, and
+// its first four partials of each kind can be loaded before the workgroup's
+// first tiles: vector loads complete in issue order, so waiting for them
+// then does not wait for the tiles.
+struct StatusPre {
+  double2 rz[4];
+  double mx[4];
+};
+__device__ __forceinline__ void status_prefetch(const SmvArgs &a, StatusPre &q) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int k = threadIdx.x + 256 * j;
+    q.rz[j] = k < a.G ? a.part[a.G + k] : cz();
+    q.mx[j] = k < a.G ? a.part[2 * a.G + k].x : 0.0;
+  }
+}
+// parts_csum256 / parts_max256 of 256 threads with the first four entries
+// of every thread already loaded (same order: t, t+256, ...)
+__device__ __forceinline__ double2 csum256_pre(const double2 *__restrict__ p, int G, double2 *sh, const StatusPre &q) {
+  const int t = threadIdx.x;
+  double2 v = cz();
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (t + 256 * j < G) v = cadd(v, q.rz[j]);
+  for (int k = t + 1024; k < G; k += 256) v = cadd(v, p[k]);
+  v = wave_csum(v);
+  if ((t & 63) == 0) sh[t >> 6] = v;
+  __syncthreads();
+  double2 r = cz();
+#pragma unroll
+  for (int k = 0; k < 4; k++) r = cadd(r, sh[k]);
+  return r;
+}
+__device__ __forceinline__ double max256_pre(const double2 *__restrict__ p, int G, double *sh, const StatusPre &q) {
+  const int t = threadIdx.x;
+  double v = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    if (t + 256 * j < G) v = fmax(v, q.mx[j]);
+  for (int k = t + 1024; k < G; k += 256) v = fmax(v, p[k].x);
+  v = wave_max(v);
+  if ((t & 63) == 0) sh[t >> 6] = v;
+  __syncthreads();
+  double r = sh[0];
+#pragma unroll
+  for (int k = 1; k < 4; k++) r = fmax(r, sh[k]);
+  return r;
+}
+
+__device__ __forceinline__ bool iter_status(const SmvArgs &a, int NT, double2 &beta, const StatusPre *pre) {
   Scal *s = a.sc;
   const int done = s->done, pending = s->pending, loop = s->loop, max_loop = s->max_loop;
   const double r0inv = s->r0inv, eps = s->eps;
   const double2 rzo = s->rz;
   beta = s->beta;
-  // the partials are requested with the status fields (one round trip) and
-  // reduced whether or not the step is pending
+  // the partials are reduced whether or not the step is pending
   __shared__ double2 shz[4];
   __shared__ double shx[4];
-  const double2 rzs = parts_csum256(a.part + a.G, a.G, shz, NT);
-  const double mxs = parts_max256(a.part + 2 * a.G, a.G, shx, NT);
+  const double2 rzs = pre ? csum256_pre(a.part + a.G, a.G, shz, *pre) : parts_csum256(a.part + a.G, a.G, shz, NT);
+  const double mxs = pre ? max256_pre(a.part + 2 * a.G, a.G, shx, *pre) : parts_max256(a.part + 2 * a.G, a.G, shx, NT);
   if (done) return false;
   if (!pending) return true;
   const StatusStep r = status_step(rzs, mxs, rzo, r0inv, eps, loop, max_loop);
@@ -241,7 +289,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
 
   if (ng == 0) {
     double2 beta;
-    if (a.done && a.Z && blockIdx.x == 0 && a.base == 0) (void)iter_status(a, NT, beta);  // it may have to stage
+    if (a.done && a.Z && blockIdx.x == 0 && a.base == 0) (void)iter_status(a, NT, beta, nullptr);  // it may have to stage
     if (a.pqpart && tid == 0) a.pqpart[wk.sub] = cz();
     return;
   }
@@ -262,6 +310,15 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   cur.half = smv_half(ng);
   tile_ij(cur.t, cur.I, cur.J);
   const double2 *zero = a.smat;  // the 64 zero entries ahead of every subdomain's tiles
+  // requested before the tiles (loads complete in issue order): the status
+  // partials of an interface iteration (256 threads) and the ids of this
+  // thread's first two slots
+  StatusPre sp;
+  const bool pre = W == 4 && a.done && a.Z;
+  if (pre) status_prefetch(a, sp);
+  const int k0 = tid, k1 = tid + NT;
+  const int gid0 = k0 < ng ? a.g_gid[wk.g0 + k0] : 0;
+  const int gid1 = k1 < ng ? a.g_gid[wk.g0 + k1] : 0;
   Tile A, B;
   load_next(base, zero, cur, lane, A);
   load_next(base, zero, cur, lane, B);
@@ -270,7 +327,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   double2 beta = cz();
   if (a.done) {
     if (a.Z) {
-      if (!iter_status(a, NT, beta)) return;
+      if (!iter_status(a, NT, beta, pre ? &sp : nullptr)) return;
     } else if (*a.done) {
       return;
     }
@@ -283,7 +340,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   for (int k = tid; k < npad; k += NT) {
     double2 p = cz();
     if (k < ng) {
-      const int gid = a.g_gid[wk.g0 + k];
+      const int gid = k == k0 ? gid0 : (k == k1 ? gid1 : a.g_gid[wk.g0 + k]);
       p = a.P[gid];
       if (a.Z) p = zbp(a.Z[gid], beta, p);
     }
