@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -24,6 +25,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fwhip.h"
@@ -52,6 +54,25 @@ int fail(int code, const char *fmt, ...) {
       return fail(FWH_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),   \
                   __FILE__, __LINE__);                                                \
   } while (0)
+
+// f(s) for s in [0, n) on up to 16 host threads (the host share of one
+// GPU); every s is handled by exactly one thread
+template <class F>
+void parallel_for(int n, F f) {
+  const int nt = std::max(1, std::min<int>({16, (int)std::thread::hardware_concurrency(), n}));
+  if (nt <= 1) {
+    for (int s = 0; s < n; s++) f(s);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&]() {
+      for (int s; (s = next.fetch_add(1)) < n;) f(s);
+    });
+  for (auto &x : th) x.join();
+}
 
 double now_ms() {
   using namespace std::chrono;
@@ -424,6 +445,7 @@ int check_batch(const fwh_mesh_desc *d, int nsys) {
 }
 
 int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
+  const double t_host0 = now_ms();
   int rc = check_batch(descs, nsys);
   if (rc) return rc;
   const fwh_mesh_desc *m = descs;  // structure
@@ -564,47 +586,105 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   static const int PA[21] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 5};
   static const int PB[21] = {0, 1, 2, 3, 4, 5, 1, 2, 3, 4, 5, 2, 3, 4, 5, 3, 4, 5, 4, 5, 5};
   {
-    std::vector<Key> keys, fkeys;
-    for (int s = 0; s < nsub; s++) {
+    // per subdomain (host threads): the sorted, merged (row, column) keys of
+    // its element matrices' lower triangles and of its loads, with their
+    // contributions in key order; then concatenated in subdomain order, so
+    // the lists are those of one pass over the subdomains
+    struct SubPattern {
+      std::vector<int> col, row, cnt, src, fsrc, fcnt, diag;  // diag: per local row its entry or -1
+      long long bad_el = -1;
+      int bad = 0;  // 1: edge id out of range, 2: repeated edge
+    };
+    std::vector<SubPattern> pat(nsub);
+    parallel_for(nsub, [&](int s) {
       const SubMeta &M = meta[s];
       const long long n = M.n;
-      keys.clear();
-      fkeys.clear();
+      SubPattern &P = pat[s];
+      std::vector<Key> keys, fkeys;
+      keys.reserve(21 * (size_t)(m->el_off[s + 1] - m->el_off[s]));
+      fkeys.reserve(6 * (size_t)(m->el_off[s + 1] - m->el_off[s]));
       for (long long el = m->el_off[s]; el < m->el_off[s + 1]; el++) {
         int id[6];
         for (int k = 0; k < 6; k++) {
           id[k] = m->nop[10 * el + 4 + k];
-          if (id[k] < 0 || id[k] >= M.n) return fail(FWH_E_ARG, "element %lld: edge id out of range", el);
+          if (id[k] < 0 || id[k] >= M.n) {
+            P.bad = 1;
+            P.bad_el = el;
+            return;
+          }
         }
         for (int p = 0; p < 21; p++) {
           const int ia = id[PA[p]], ib = id[PB[p]];
           const int r = std::max(ia, ib), cc = std::min(ia, ib);
-          if (PA[p] != PB[p] && ia == ib) return fail(FWH_E_ARG, "element %lld repeats an edge", el);
+          if (PA[p] != PB[p] && ia == ib) {
+            P.bad = 2;
+            P.bad_el = el;
+            return;
+          }
           keys.push_back({(long long)r * n + cc, (int)(27 * el + p)});
         }
         for (int k = 0; k < 6; k++) fkeys.push_back({(long long)id[k], (int)(27 * el + 21 + k)});
       }
       std::sort(keys.begin(), keys.end());
       std::sort(fkeys.begin(), fkeys.end());
+      P.diag.assign(M.n, -1);
+      P.fcnt.assign(M.n, 0);
+      P.src.reserve(keys.size());
       for (size_t t = 0; t < keys.size(); t++) {
         if (t == 0 || keys[t].key != keys[t - 1].key) {
           const int r = (int)(keys[t].key / n), cc = (int)(keys[t].key % n);
-          col.push_back(cc);
-          ent_row.push_back(M.dof0 + r);
-          rowptr[M.dof0 + r + 1]++;
-          asm_ptr.push_back(asm_ptr.back());
-          if (r == cc) diag_ent[M.dof0 + r] = (int)col.size() - 1;
+          P.col.push_back(cc);
+          P.row.push_back(r);
+          P.cnt.push_back(0);
+          if (r == cc) P.diag[r] = (int)P.col.size() - 1;
         }
-        asm_src.push_back(keys[t].contrib);
-        asm_ptr.back()++;
+        P.src.push_back(keys[t].contrib);
+        P.cnt.back()++;
       }
+      P.fsrc.reserve(fkeys.size());
       for (auto &k : fkeys) {
-        f_src.push_back(k.contrib);
-        f_ptr[M.dof0 + k.key + 1]++;
+        P.fsrc.push_back(k.contrib);
+        P.fcnt[k.key]++;
       }
-      if ((long long)col.size() > (1LL << 31) - 64 || (long long)asm_src.size() > (1LL << 31) - 64)
-        return fail(FWH_E_LIMIT, "assembly lists exceed 32-bit indexing");
+    });
+    for (int s = 0; s < nsub; s++) {
+      if (pat[s].bad == 1) return fail(FWH_E_ARG, "element %lld: edge id out of range", pat[s].bad_el);
+      if (pat[s].bad == 2) return fail(FWH_E_ARG, "element %lld repeats an edge", pat[s].bad_el);
     }
+    // offsets of every subdomain's entries and contributions
+    std::vector<long long> e_off(nsub + 1, 0), a_off(nsub + 1, 0), f_off(nsub + 1, 0);
+    for (int s = 0; s < nsub; s++) {
+      e_off[s + 1] = e_off[s] + (long long)pat[s].col.size();
+      a_off[s + 1] = a_off[s] + (long long)pat[s].src.size();
+      f_off[s + 1] = f_off[s] + (long long)pat[s].fsrc.size();
+    }
+    if (e_off[nsub] > (1LL << 31) - 64 || a_off[nsub] > (1LL << 31) - 64)
+      return fail(FWH_E_LIMIT, "assembly lists exceed 32-bit indexing");
+    col.resize(e_off[nsub]);
+    ent_row.resize(e_off[nsub]);
+    asm_ptr.resize(e_off[nsub] + 1);
+    asm_src.resize(a_off[nsub]);
+    f_src.resize(f_off[nsub]);
+    parallel_for(nsub, [&](int s) {
+      const SubPattern &P = pat[s];
+      const int dof0 = meta[s].dof0;
+      const long long e0 = e_off[s];
+      long long a = a_off[s];
+      for (size_t e = 0; e < P.col.size(); e++) {
+        col[e0 + e] = P.col[e];
+        ent_row[e0 + e] = dof0 + P.row[e];
+        rowptr[dof0 + P.row[e] + 1]++;
+        asm_ptr[e0 + e] = (int)a;
+        a += P.cnt[e];
+      }
+      std::copy(P.src.begin(), P.src.end(), asm_src.begin() + a_off[s]);
+      std::copy(P.fsrc.begin(), P.fsrc.end(), f_src.begin() + f_off[s]);
+      for (int r = 0; r < meta[s].n; r++) {
+        f_ptr[dof0 + r + 1] = P.fcnt[r];
+        if (P.diag[r] >= 0) diag_ent[dof0 + r] = (int)(e0 + P.diag[r]);
+      }
+    });
+    asm_ptr[e_off[nsub]] = (int)a_off[nsub];
     for (long long d = 0; d < ndof; d++) {
       rowptr[d + 1] += rowptr[d];
       f_ptr[d + 1] += f_ptr[d];
@@ -637,7 +717,6 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         if (j != d) up_ent[fp[j]++] = (int)e;
       }
     }
-    std::vector<std::vector<std::pair<int, int>>> grow(nslots);
     for (long long d = 0; d < ndof; d++) {
       const int s = dof_sub[d];
       const int dof0 = meta[s].dof0;
@@ -700,6 +779,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   for (long long d = 0; d < ndof; d++)
     if (bcflag[d] == 1) diag_slot[meta[dof_sub[d]].g0 + kidx[d]] = diag_ent[d];
 
+  c->info.ms_host_index = now_ms() - t_host0;
   // ---- uploads + assembly on the device -------------------------------------------
   // index arrays once; element values, loads and matrices per system
   hipStream_t st = c->st;
@@ -1069,7 +1149,9 @@ extern "C" int fwh_load_batch(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) 
   if (nsys < 1 || nsys > 65535) return fail(FWH_E_ARG, "nsys %d out of range", nsys);
   if (c->loaded) return fail(FWH_E_STATE, "model already loaded");
   HIPCHK(hipSetDevice(c->device));
+  const double t0 = now_ms();
   int rc = build_model(c, descs, nsys);
+  c->info.ms_load = now_ms() - t0;
   if (rc) {
     (void)hipStreamSynchronize(c->st);
     c->release();

@@ -59,7 +59,8 @@ class FwhInfo(ctypes.Structure):
                 ("unit_depth", ctypes.c_int), ("ms_solve_depth", ctypes.c_double * 2),
                 ("nsys", ctypes.c_int), ("smv_order", ctypes.c_int), ("ms_smv_order", ctypes.c_double * 3),
                 ("schur_gemm", ctypes.c_int),
-                ("factor_blocked", ctypes.c_int)]
+                ("factor_blocked", ctypes.c_int),
+                ("ms_host_index", ctypes.c_double), ("ms_load", ctypes.c_double)]
 
     def as_dict(self):
         out = {}

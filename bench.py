@@ -250,6 +250,10 @@ def main():
         setup["schur_form"] = round(info["ms_schur_form"], 2)
         setup["schur_form_method"] = "W^T D^-1 W on FP64 MFMA" if info["schur_gemm"] else "unit-vector Schur steps"
     setup["factor_method"] = "16-row blocks on FP64 MFMA" if info["factor_blocked"] else "row by row"
+    # the whole load (host index work, uploads, assembly, factorisation,
+    # load-time tuning), of which the host index work
+    setup["load_wall"] = round(info["ms_load"], 1)
+    setup["host_index"] = round(info["ms_host_index"], 1)
     out = {
         "metric": METRIC, "value": round(iters_per_s, 2), "unit": "iters/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1000 * dt / args.steps, 2),

@@ -111,6 +111,11 @@ typedef struct fwh_info {
                            the Schur step on every interface unit vector   */
   int    factor_blocked; /* 1: LDL^T factors by 16-row blocks on the matrix
                            cores; 0: one row at a time (FWH_FACTOR=row)     */
+  double ms_host_index; /* host work of the load: patterns, assembly and
+                           gather lists, couplings, profiles               */
+  double ms_load;       /* the whole load (fwh_load / fwh_load_batch):
+                           host work, uploads, assembly, factorisation and
+                           the load-time kernel tuning                     */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */
