@@ -625,7 +625,27 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         }
         for (int k = 0; k < 6; k++) fkeys.push_back({(long long)id[k], (int)(27 * el + 21 + k)});
       }
-      std::sort(keys.begin(), keys.end());
+      // (key, contribution) order, key = row n + column: a counting sort by
+      // row, then each row's few keys by (column, contribution)
+      {
+        std::vector<int> rc(M.n + 1, 0);
+        for (const Key &k : keys) rc[k.key / n + 1]++;
+        for (int r = 0; r < M.n; r++) rc[r + 1] += rc[r];
+        std::vector<Key> by(keys.size());
+        std::vector<int> fill(rc.begin(), rc.end() - 1);
+        for (const Key &k : keys) by[fill[k.key / n]++] = k;
+        for (int r = 0; r < M.n; r++)
+          for (int a = rc[r] + 1; a < rc[r + 1]; a++) {
+            const Key v = by[a];
+            int b = a - 1;
+            while (b >= rc[r] && v < by[b]) {
+              by[b + 1] = by[b];
+              b--;
+            }
+            by[b + 1] = v;
+          }
+        keys.swap(by);
+      }
       std::sort(fkeys.begin(), fkeys.end());
       P.diag.assign(M.n, -1);
       P.fcnt.assign(M.n, 0);
@@ -698,80 +718,110 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   }
 
   // ---- couplings A_IG and interface rows A_G. (entry ids into aval) --------
+  // Entries couple DOFs of one subdomain only, so every subdomain builds its
+  // lists on a host thread: for an interior DOF its couplings to interface
+  // DOFs (row entries, then the upper-part entries of its column, each in
+  // entry order), for an interface DOF (slot) its row (the same two parts);
+  // the lists are concatenated in subdomain order (DOF order, slot order).
   std::vector<int> cp_ptr(ndof + 1, 0), cp_k, cp_ent, gr_ptr(nslots + 1, 0), gr_col, gr_ent;
   {
-    // upper-part access: for every row d, entries (r, d) with r > d
-    std::vector<int> up_ptr(ndof + 1, 0), up_ent;
-    for (long long e = 0; e < nnz; e++) {
-      const int d = ent_row[e];
-      const int j = meta[dof_sub[d]].dof0 + col[e];
-      if (j != d) up_ptr[j + 1]++;
-    }
-    for (long long d = 0; d < ndof; d++) up_ptr[d + 1] += up_ptr[d];
-    up_ent.resize(up_ptr[ndof]);
-    {
-      std::vector<int> fp(up_ptr.begin(), up_ptr.end() - 1);
-      for (long long e = 0; e < nnz; e++) {
-        const int d = ent_row[e];
-        const int j = meta[dof_sub[d]].dof0 + col[e];
-        if (j != d) up_ent[fp[j]++] = (int)e;
+    struct SubCpl {
+      std::vector<int> cp_k, cp_ent, gr_col, gr_ent;
+    };
+    std::vector<SubCpl> cpl(nsub);
+    parallel_for(nsub, [&](int s) {
+      const int dof0 = meta[s].dof0, n = meta[s].n, g0 = meta[s].g0, ngs = meta[s].ng;
+      const int e0 = rowptr[dof0], e1 = rowptr[dof0 + n];
+      // upper-part access: for every local column j, entries (r, j), r > j
+      std::vector<int> up_ptr(n + 1, 0), up_ent;
+      for (int e = e0; e < e1; e++)
+        if (ent_row[e] != dof0 + col[e]) up_ptr[col[e] + 1]++;
+      for (int i = 0; i < n; i++) up_ptr[i + 1] += up_ptr[i];
+      up_ent.resize(up_ptr[n]);
+      {
+        std::vector<int> fp(up_ptr.begin(), up_ptr.end() - 1);
+        for (int e = e0; e < e1; e++)
+          if (ent_row[e] != dof0 + col[e]) up_ent[fp[col[e]]++] = e;
       }
-    }
-    for (long long d = 0; d < ndof; d++) {
-      const int s = dof_sub[d];
-      const int dof0 = meta[s].dof0;
-      if (bcflag[d] == 0) {
-        for (int e = rowptr[d]; e < rowptr[d + 1]; e++) {
-          const int j = dof0 + col[e];
-          if (j != d && bcflag[j] == 1) {
-            cp_k.push_back(kidx[j]);
-            cp_ent.push_back(e);
+      SubCpl &P = cpl[s];
+      // interface rows, appended in DOF order, regrouped by slot below
+      std::vector<int> gcnt(ngs, 0), gcol, gent;
+      for (int i = 0; i < n; i++) {
+        const int d = dof0 + i;
+        if (bcflag[d] == 0) {
+          for (int e = rowptr[d]; e < rowptr[d + 1]; e++) {
+            const int j = dof0 + col[e];
+            if (j != d && bcflag[j] == 1) {
+              P.cp_k.push_back(kidx[j]);
+              P.cp_ent.push_back(e);
+            }
+          }
+          for (int t = up_ptr[i]; t < up_ptr[i + 1]; t++) {
+            const int e = up_ent[t];
+            const int r = ent_row[e];
+            if (bcflag[r] == 1) {
+              P.cp_k.push_back(kidx[r]);
+              P.cp_ent.push_back(e);
+            }
+          }
+          cp_ptr[d + 1] = (int)P.cp_k.size();  // local count so far, made global below
+        } else {
+          cp_ptr[d + 1] = (int)P.cp_k.size();
+          if (bcflag[d] == 1) {
+            const int k = kidx[d];
+            for (int e = rowptr[d]; e < rowptr[d + 1]; e++) {
+              gcol.push_back(col[e]);
+              gent.push_back(e);
+              gcnt[k]++;
+            }
+            for (int t = up_ptr[i]; t < up_ptr[i + 1]; t++) {
+              const int e = up_ent[t];
+              gcol.push_back(ent_row[e] - dof0);
+              gent.push_back(e);
+              gcnt[k]++;
+            }
           }
         }
-        for (int t = up_ptr[d]; t < up_ptr[d + 1]; t++) {
-          const int e = up_ent[t];
-          const int r = ent_row[e];
-          if (bcflag[r] == 1) {
-            cp_k.push_back(kidx[r]);
-            cp_ent.push_back(e);
-          }
-        }
-      } else if (bcflag[d] == 1) {
-        const int sl = meta[s].g0 + kidx[d];
-        for (int e = rowptr[d]; e < rowptr[d + 1]; e++) {
-          gr_col.push_back(col[e]);
-          gr_ent.push_back(e);
-          gr_ptr[sl + 1]++;
-        }
-        for (int t = up_ptr[d]; t < up_ptr[d + 1]; t++) {
-          const int e = up_ent[t];
-          gr_col.push_back(ent_row[e] - dof0);
-          gr_ent.push_back(e);
-          gr_ptr[sl + 1]++;
-        }
       }
-      cp_ptr[d + 1] = (int)cp_k.size();
-    }
-    // gr entries were appended in DOF order; slots of one subdomain are in
-    // kidx order which need not follow DOF order: regroup by slot
-    std::vector<int> gp(nslots + 1, 0);
-    for (long long sl = 0; sl < nslots; sl++) gp[sl + 1] = gp[sl] + gr_ptr[sl + 1];
-    std::vector<int> gcol2(gr_col.size()), gent2(gr_ent.size());
-    {
+      // regroup by slot (slots follow kidx, which need not follow DOF order)
+      std::vector<int> gp(ngs + 1, 0);
+      for (int k = 0; k < ngs; k++) gp[k + 1] = gp[k] + gcnt[k];
+      P.gr_col.resize(gcol.size());
+      P.gr_ent.resize(gent.size());
       size_t t = 0;
-      for (long long d = 0; d < ndof; d++) {
+      for (int i = 0; i < n; i++) {
+        const int d = dof0 + i;
         if (bcflag[d] != 1) continue;
-        const int sl = meta[dof_sub[d]].g0 + kidx[d];
-        const int cnt = gr_ptr[sl + 1];
-        for (int q = 0; q < cnt; q++, t++) {
-          gcol2[gp[sl] + q] = gr_col[t];
-          gent2[gp[sl] + q] = gr_ent[t];
+        const int k = kidx[d];
+        for (int q = 0; q < gcnt[k]; q++, t++) {
+          P.gr_col[gp[k] + q] = gcol[t];
+          P.gr_ent[gp[k] + q] = gent[t];
         }
       }
+      for (int k = 0; k < ngs; k++) gr_ptr[g0 + k + 1] = gcnt[k];
+    });
+    // concatenate in subdomain order
+    std::vector<long long> c_off(nsub + 1, 0), r_off(nsub + 1, 0);
+    for (int s = 0; s < nsub; s++) {
+      c_off[s + 1] = c_off[s] + (long long)cpl[s].cp_k.size();
+      r_off[s + 1] = r_off[s] + (long long)cpl[s].gr_col.size();
     }
-    gr_ptr = gp;
-    gr_col.swap(gcol2);
-    gr_ent.swap(gent2);
+    if (c_off[nsub] > (1LL << 31) - 64 || r_off[nsub] > (1LL << 31) - 64)
+      return fail(FWH_E_LIMIT, "coupling lists exceed 32-bit indexing");
+    cp_k.resize(c_off[nsub]);
+    cp_ent.resize(c_off[nsub]);
+    gr_col.resize(r_off[nsub]);
+    gr_ent.resize(r_off[nsub]);
+    parallel_for(nsub, [&](int s) {
+      const SubCpl &P = cpl[s];
+      std::copy(P.cp_k.begin(), P.cp_k.end(), cp_k.begin() + c_off[s]);
+      std::copy(P.cp_ent.begin(), P.cp_ent.end(), cp_ent.begin() + c_off[s]);
+      std::copy(P.gr_col.begin(), P.gr_col.end(), gr_col.begin() + r_off[s]);
+      std::copy(P.gr_ent.begin(), P.gr_ent.end(), gr_ent.begin() + r_off[s]);
+      const int dof0 = meta[s].dof0;
+      for (int i = 0; i < meta[s].n; i++) cp_ptr[dof0 + i + 1] += (int)c_off[s];
+    });
+    for (long long sl = 0; sl < nslots; sl++) gr_ptr[sl + 1] += gr_ptr[sl];
   }
   c->ncpl = (long long)cp_k.size();
   c->ngrow = (long long)gr_col.size();
