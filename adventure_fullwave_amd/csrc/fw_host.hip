@@ -935,11 +935,11 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   }
   // row blocks of subdomain M: <= BMAX rows and <= cap skyline entries each,
   // padded to an even number of blocks with empty blocks [n, n) (the solve
-  // kernel sweeps block pairs without conditional loads)
-  auto partition = [&](const SubMeta &M, int cap, int &blk0, int &nblk) -> int {
-    blk0 = (int)blk_row.size();
+  // kernel sweeps block pairs without conditional loads); the block start
+  // rows, 0 first and n last
+  auto partition = [&](const SubMeta &M, int cap, std::vector<int> &out) -> int {
     int i0 = 0;
-    blk_row.push_back(0);
+    out.assign(1, 0);
     while (i0 < M.n) {
       int i1 = i0;
       while (i1 < M.n && i1 - i0 < fw::BMAX) {
@@ -948,42 +948,80 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         i1++;
       }
       if (i1 == i0) return -1;
-      blk_row.push_back(i1);
+      out.push_back(i1);
       i0 = i1;
     }
-    if (((int)blk_row.size() - blk0 - 1) % 2 == 1) blk_row.push_back(M.n);
-    nblk = (int)blk_row.size() - blk0 - 1;
+    if (((int)out.size() - 1) % 2 == 1) out.push_back(M.n);
     return 0;
   };
-  for (int s = 0; s < nsub; s++) {
-    SubMeta &M = meta[s];
-    long long rb = 0;
-    for (int i = 0; i < M.n; i++) {
-      const int fi = idx1[M.dof0 + i];
-      if (fi < 0 || fi > i) return fail(FWH_E_ARG, "bad profile");
-      rowbeg[M.dof0 + i] = (int)rb;
-      rb += i - fi + 1;
-      wmax = std::max(wmax, i - fi);
-    }
-    if (rb > (1LL << 31) - 1) return fail(FWH_E_LIMIT, "subdomain %d skyline too long", s);
-    M.nlen = rb;
-    M.fac0 = nlen_tot;
-    nlen_tot += rb;
-    if (partition(M, fw::SOLVE_THREADS * fw::PFK, M.blk0, M.nblk))
-      return fail(FWH_E_LIMIT, "subdomain %d has a profile row wider than %d entries", s,
-                  fw::SOLVE_THREADS * fw::PFK);
-    c->nblkmax = std::max(c->nblkmax, M.nblk);
-    M.nblk_s = 0;
-    M.blk0_s = M.blk0;
-    if (c->short_ok) {
-      if (partition(M, fw::SOLVE_THREADS * 8, M.blk0_s, M.nblk_s)) return fail(FWH_E_ARG, "bad profile");
-      c->nblkmax_s = std::max(c->nblkmax_s, M.nblk_s);
-    }
-    // flops of the factorisation: 8 per complex multiply-add
-    for (int i = 0; i < M.n; i++) {
-      const int fi = idx1[M.dof0 + i];
-      for (int j = fi; j < i; j++) flops += 8.0 * (j - std::max(fi, idx1[M.dof0 + j]));
-      flops += 14.0 * (i - fi);
+  // per subdomain on host threads: row starts, skyline length, both
+  // partitions and the factorisation flops (8 per complex multiply-add;
+  // integers, so the sum over subdomains is exact in any order)
+  {
+    struct SubProfile {
+      std::vector<int> blk, blk_s;
+      double flops = 0.0;
+      int wmax = 0, bad = 0;
+    };
+    std::vector<SubProfile> sp(nsub);
+    parallel_for(nsub, [&](int s) {
+      SubMeta &M = meta[s];
+      SubProfile &P = sp[s];
+      long long rb = 0;
+      for (int i = 0; i < M.n; i++) {
+        const int fi = idx1[M.dof0 + i];
+        if (fi < 0 || fi > i) {
+          P.bad = 1;
+          return;
+        }
+        rowbeg[M.dof0 + i] = (int)rb;
+        rb += i - fi + 1;
+        P.wmax = std::max(P.wmax, i - fi);
+      }
+      if (rb > (1LL << 31) - 1) {
+        P.bad = 2;
+        return;
+      }
+      M.nlen = rb;
+      if (partition(M, fw::SOLVE_THREADS * fw::PFK, P.blk)) {
+        P.bad = 3;
+        return;
+      }
+      if (c->short_ok && partition(M, fw::SOLVE_THREADS * 8, P.blk_s)) {
+        P.bad = 1;
+        return;
+      }
+      for (int i = 0; i < M.n; i++) {
+        const int fi = idx1[M.dof0 + i];
+        long long f = 0;
+        for (int j = fi; j < i; j++) f += j - std::max(fi, idx1[M.dof0 + j]);
+        P.flops += 8.0 * (double)f + 14.0 * (i - fi);
+      }
+    });
+    for (int s = 0; s < nsub; s++) {
+      SubMeta &M = meta[s];
+      const SubProfile &P = sp[s];
+      if (P.bad == 1) return fail(FWH_E_ARG, "bad profile");
+      if (P.bad == 2) return fail(FWH_E_LIMIT, "subdomain %d skyline too long", s);
+      if (P.bad == 3)
+        return fail(FWH_E_LIMIT, "subdomain %d has a profile row wider than %d entries", s,
+                    fw::SOLVE_THREADS * fw::PFK);
+      wmax = std::max(wmax, P.wmax);
+      M.fac0 = nlen_tot;
+      nlen_tot += M.nlen;
+      M.blk0 = (int)blk_row.size();
+      blk_row.insert(blk_row.end(), P.blk.begin(), P.blk.end());
+      M.nblk = (int)P.blk.size() - 1;
+      c->nblkmax = std::max(c->nblkmax, M.nblk);
+      M.nblk_s = 0;
+      M.blk0_s = M.blk0;
+      if (c->short_ok) {
+        M.blk0_s = (int)blk_row.size();
+        blk_row.insert(blk_row.end(), P.blk_s.begin(), P.blk_s.end());
+        M.nblk_s = (int)P.blk_s.size() - 1;
+        c->nblkmax_s = std::max(c->nblkmax_s, M.nblk_s);
+      }
+      flops += P.flops;
     }
   }
   c->nlen = nlen_tot;
