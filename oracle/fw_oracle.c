@@ -1162,9 +1162,38 @@ int fwo_hddm_solve(const fwh_mesh_desc *m, const fwo_opts *o, double *resid,
   S.r0 = 1.0 / H.hv[0].r0;
   loop++;
   double t0 = now_s();
+  /* with nthreads > 1 the main loop's subdomain Schur steps run in parallel
+   * into per-subdomain buffers and are accumulated afterwards in the
+   * reference's order: the same sums as s_solve(H, s, 1) in turn          */
+  const int par = o->nthreads > 1 && o->dom_solver == 0;
+  fwo_c **ybuf = NULL;
+  if (par) {
+    ybuf = (fwo_c **)calloc(m->nsub, sizeof(fwo_c *));
+    for (int s = 0; s < m->nsub; s++) ybuf[s] = (fwo_c *)malloc(md->s[s].A.nf * sizeof(fwo_c));
+#ifdef _OPENMP
+    omp_set_num_threads(o->nthreads);
+#endif
+  }
   while (1) { /* hddm_solver.h:125-143 */
     if (status == 2) break;
-    for (int s = 0; s < m->nsub; s++) s_solve(&H, s, 1);
+    if (par) {
+      const int ns = m->nsub;
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int s = 0; s < ns; s++) {
+        sub_t *u = &md->s[s];
+        hvec_t *hv = &H.hv[u->part];
+        for (int i = 0, j = u->nbc - u->ninbc; i < u->ninbc; i++, j++) u->vbc[j] = hv->P[u->ln[i]];
+        sub_solve_dom(u, 1, 1, ybuf[s]);
+      }
+      for (int s = 0; s < ns; s++) {
+        sub_t *u = &md->s[s];
+        hvec_t *hv = &H.hv[u->part];
+        for (int i = 0; i < u->ninbc; i++) hv->Q[u->ln[i]] = cadd(hv->Q[u->ln[i]], ybuf[s][u->nfid[i]]);
+        H.sub_solves++;
+      }
+    } else {
+      for (int s = 0; s < m->nsub; s++) s_solve(&H, s, 1);
+    }
     status = s_iteration(&H, loop, resid);
     S.iters++;
     if (status == 2) S.converged = 1;
@@ -1174,6 +1203,10 @@ int fwo_hddm_solve(const fwh_mesh_desc *m, const fwo_opts *o, double *resid,
   }
   S.seconds_loop = now_s() - t0;
   S.loops = loop;
+  if (ybuf) {
+    for (int s = 0; s < m->nsub; s++) free(ybuf[s]);
+    free(ybuf);
+  }
   for (int s = 0; s < m->nsub; s++) s_solve(&H, s, 2); /* LOOP_LAST */
   S.sub_solves = H.sub_solves;
   if (x_out) {

@@ -97,6 +97,9 @@ typedef struct {
   int    perturb;       /* 0: reference summation order; 1: interface dot
                            products accumulated in reverse unknown order
                            (measures rounding sensitivity only)            */
+  int    nthreads;      /* > 1: the main loop's subdomain Schur steps on
+                           that many OpenMP threads, accumulated afterwards
+                           in the reference's order (the same sums)        */
 } fwo_opts;
 
 typedef struct {

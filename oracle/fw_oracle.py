@@ -18,7 +18,7 @@ class FwoOpts(ctypes.Structure):
     _fields_ = [("hddm_eps", ctypes.c_double), ("hddm_max_loop", ctypes.c_int),
                 ("dom_solver", ctypes.c_int), ("sol_eps", ctypes.c_double),
                 ("ic_shift", ctypes.c_double), ("max_sub", ctypes.c_int),
-                ("perturb", ctypes.c_int)]
+                ("perturb", ctypes.c_int), ("nthreads", ctypes.c_int)]
 
 
 class FwoStats(ctypes.Structure):
@@ -181,11 +181,13 @@ class OracleModel:
         return x, y
 
 
-def hddm_solve(model, eps=1e-8, max_loop=100000, dom_solver=0, sol_eps=1e-9, ic_shift=1.2, perturb=0):
+def hddm_solve(model, eps=1e-8, max_loop=100000, dom_solver=0, sol_eps=1e-9, ic_shift=1.2, perturb=0,
+               threads=1):
     """Full s-mode HDDM solve; returns dict(resid, x (per-subdomain list),
-    u (owned interface values, part-major), stats)."""
+    u (owned interface values, part-major), stats).  threads > 1: the main
+    loop's subdomain Schur steps on OpenMP threads (same sums)."""
     d = model.desc()
-    o = FwoOpts(eps, max_loop, dom_solver, sol_eps, ic_shift, 0, perturb)
+    o = FwoOpts(eps, max_loop, dom_solver, sol_eps, ic_shift, 0, perturb, int(threads))
     resid = np.zeros(max_loop + 1)
     nmp = np.asarray(model.nmp, np.int64)
     x = np.zeros(int(nmp.sum()), np.complex128)
