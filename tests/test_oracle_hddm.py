@@ -128,3 +128,14 @@ def test_iteration_count_rounding_spread(small_model):
     counts = [fo.hddm_solve(small_model, eps=1e-8, max_loop=5000, perturb=p)["stats"]["iters"] for p in range(4)]
     assert min(counts) > 0
     assert (max(counts) - min(counts)) <= 0.05 * counts[0]
+
+
+def test_threaded_schur_steps_same_sums(small_sigma_model):
+    """The oracle's main-loop Schur steps on host threads (accumulated
+    afterwards in the reference's order) give the sequential loop's bits:
+    residual history, count and solution."""
+    a = fo.hddm_solve(small_sigma_model, eps=1e-8, max_loop=5000)
+    b = fo.hddm_solve(small_sigma_model, eps=1e-8, max_loop=5000, threads=4)
+    assert a["stats"]["iters"] == b["stats"]["iters"]
+    assert np.array_equal(a["resid"], b["resid"])
+    assert np.array_equal(a["x_flat"], b["x_flat"])
