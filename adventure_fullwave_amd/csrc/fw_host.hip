@@ -58,10 +58,10 @@ int fail(int code, const char *fmt, ...) {
 // f(s) for s in [0, n) on up to 16 host threads (the host share of one
 // GPU); every s is handled by exactly one thread
 template <class F>
-void parallel_for(int n, F f) {
+void parallel_for(int n, F f, const int *order = nullptr) {
   const int nt = std::max(1, std::min<int>({16, (int)std::thread::hardware_concurrency(), n}));
   if (nt <= 1) {
-    for (int s = 0; s < n; s++) f(s);
+    for (int k = 0; k < n; k++) f(order ? order[k] : k);
     return;
   }
   std::atomic<int> next(0);
@@ -69,7 +69,7 @@ void parallel_for(int n, F f) {
   th.reserve(nt);
   for (int t = 0; t < nt; t++)
     th.emplace_back([&]() {
-      for (int s; (s = next.fetch_add(1)) < n;) f(s);
+      for (int k; (k = next.fetch_add(1)) < n;) f(order ? order[k] : k);
     });
   for (auto &x : th) x.join();
 }
@@ -580,6 +580,11 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     }
 
   // ---- symbolic lower-triangle pattern + assembly lists --------------------
+  // the threaded load phases take their subdomains largest first (a large
+  // subdomain left for last would run alone)
+  std::vector<int> big_first(nsub);
+  std::iota(big_first.begin(), big_first.end(), 0);
+  std::stable_sort(big_first.begin(), big_first.end(), [&](int x, int y) { return meta[x].n > meta[y].n; });
   std::vector<int> rowptr(ndof + 1, 0), col, ent_row, asm_ptr(1, 0), asm_src;
   std::vector<int> f_ptr(ndof + 1, 0), f_src;
   std::vector<int> diag_ent(ndof, -1);
@@ -666,7 +671,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         P.fsrc.push_back(k.contrib);
         P.fcnt[k.key]++;
       }
-    });
+    }, big_first.data());
     for (int s = 0; s < nsub; s++) {
       if (pat[s].bad == 1) return fail(FWH_E_ARG, "element %lld: edge id out of range", pat[s].bad_el);
       if (pat[s].bad == 2) return fail(FWH_E_ARG, "element %lld repeats an edge", pat[s].bad_el);
@@ -703,7 +708,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         f_ptr[dof0 + r + 1] = P.fcnt[r];
         if (P.diag[r] >= 0) diag_ent[dof0 + r] = (int)(e0 + P.diag[r]);
       }
-    });
+    }, big_first.data());
     asm_ptr[e_off[nsub]] = (int)a_off[nsub];
     for (long long d = 0; d < ndof; d++) {
       rowptr[d + 1] += rowptr[d];
@@ -799,7 +804,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         }
       }
       for (int k = 0; k < ngs; k++) gr_ptr[g0 + k + 1] = gcnt[k];
-    });
+    }, big_first.data());
     // concatenate in subdomain order
     std::vector<long long> c_off(nsub + 1, 0), r_off(nsub + 1, 0);
     for (int s = 0; s < nsub; s++) {
@@ -820,7 +825,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
       std::copy(P.gr_ent.begin(), P.gr_ent.end(), gr_ent.begin() + r_off[s]);
       const int dof0 = meta[s].dof0;
       for (int i = 0; i < meta[s].n; i++) cp_ptr[dof0 + i + 1] += (int)c_off[s];
-    });
+    }, big_first.data());
     for (long long sl = 0; sl < nslots; sl++) gr_ptr[sl + 1] += gr_ptr[sl];
   }
   c->ncpl = (long long)cp_k.size();
@@ -997,7 +1002,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         for (int j = fi; j < i; j++) f += j - std::max(fi, idx1[M.dof0 + j]);
         P.flops += 8.0 * (double)f + 14.0 * (i - fi);
       }
-    });
+    }, big_first.data());
     for (int s = 0; s < nsub; s++) {
       SubMeta &M = meta[s];
       const SubProfile &P = sp[s];
@@ -1183,8 +1188,16 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
       for (int v = 0; v < variants; v++) {
         c->pfk_iter = d == 0 ? fw::PFK : 8;
         fw::SolveArgs a = c->solve_args(fw::MODE_RHS, false);
+        // a timed warm-up; three more repetitions only for short launches
+        // (one suffices when a launch takes milliseconds)
+        float ms0 = 0.f;
+        HIPCHK(hipEventRecord(e0, st));
         HIPCHK(c->launch_solve_all(a, v == 1));
-        for (int r = 0; r < 3; r++) {
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms0, e0, e1));
+        const int reps = ms0 > 2.0f ? 1 : 3;
+        for (int r = 0; r < reps; r++) {
           float ms = 0.f;
           HIPCHK(hipEventRecord(e0, st));
           HIPCHK(c->launch_solve_all(a, v == 1));
