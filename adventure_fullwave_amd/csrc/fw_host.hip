@@ -281,6 +281,8 @@ struct fwh_ctx {
     a.unit_list = nullptr;
     a.smat = smat;
     a.sm_off = sm_off;
+    a.pqpart = (mode == fw::MODE_ITER && check_done && fused) ? pqpart : nullptr;  // an interface iteration
+    a.npq = nsub;
     return a;
   }
   fw::SmvArgs smv_args(bool check_done, bool fused, int only = -1) const {
@@ -348,12 +350,10 @@ struct fwh_ctx {
     }
     return hipSuccess;
   }
-  // the vector part of an interface iteration after its Schur step:
-  // k_iter_a, k_iter_b (implicit) or k_iter_ab (explicit: the matvec leaves
-  // the (p,q) partials), then k_iter_c when status_step
-  hipError_t launch_vec_iter(bool expl, bool status_step) {
-    return expl ? fw::launch_iter_ab(vec_args(), st, status_step) : fw::launch_iter(vec_args(), st, status_step);
-  }
+  // the vector part of an interface iteration after its Schur step (which
+  // leaves every subdomain's (p,q) part): k_iter_ab, then k_iter_c when
+  // status_step
+  hipError_t launch_vec_iter(bool status_step) { return fw::launch_iter_ab(vec_args(), st, status_step); }
   // the Schur step of an interface iteration (y_slot = S_s p_s): implicit
   // (Dirichlet solves through the factors) or explicit (formed S_s)
   hipError_t run_schur(bool expl, bool plain, bool check_done, bool fused, int only = -1) {
@@ -1683,7 +1683,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   if (o->plain) {
     while (it < n && !all) {
       HIPCHK(c->run_schur(expl, true, true, true));
-      HIPCHK(c->launch_vec_iter(expl, true));
+      HIPCHK(c->launch_vec_iter(true));
       if ((rc = read_scal(c, s, &all))) return rc;
       it++;
     }
@@ -1695,7 +1695,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
       // k_iter_c runs only after the last iteration of the graph
       for (int k = 0; k < o->graph_iters; k++) {
         hipError_t e = c->run_schur(expl, false, true, true);
-        if (e == hipSuccess) e = c->launch_vec_iter(expl, !expl || k == o->graph_iters - 1);
+        if (e == hipSuccess) e = c->launch_vec_iter(!expl || k == o->graph_iters - 1);
         if (e != hipSuccess) {
           hipGraph_t g;
           (void)hipStreamEndCapture(c->st, &g);
@@ -1739,7 +1739,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   } else {
     while (it < n && !all) {
       HIPCHK(c->run_schur(expl, false, true, true));
-      HIPCHK(c->launch_vec_iter(expl, true));
+      HIPCHK(c->launch_vec_iter(true));
       it++;
       if ((it & 15) == 0 || it >= n)
         if ((rc = read_scal(c, s, &all))) return rc;

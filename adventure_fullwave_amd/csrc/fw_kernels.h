@@ -124,6 +124,8 @@ struct SolveArgs {
   const int2 *unit_list; // MODE_UNIT, if set: (subdomain, unit vector) per work item
   double2 *smat;         // MODE_UNIT: Schur-complement tiles
   const long long *sm_off;  // per subdomain first entry of its tiles
+  double2 *pqpart;       // MODE_ITER of an interface iteration: per subdomain its (p,q) part (may be null)
+  int npq;               // per system: subdomains
 };
 
 // S_s = A_GG - W^T D^-1 W with W = L^-1 (-A_IG), all on the FP64 matrix
@@ -207,7 +209,7 @@ struct SmvArgs {
   const double2 *P;
   const double2 *Z;      // p = Z + beta P (null: p = P)
   Scal *sc;              // status of the interface iteration (with Z)
-  const double2 *part;   // (r,z) and max|r| partials of k_iter_b (with Z)
+  const double2 *part;   // (r,z) and max|r| partials of k_iter_ab (with Z)
   int G;                 // their count per system
   double2 *pqpart;       // per subdomain its part of (p,q) = sum_slots p y (explicit iteration; may be null)
   int npq;               // per system: subdomains
@@ -279,9 +281,10 @@ struct Scal {
   int conv;
   int iters;
   int max_loop;
-  // the status step of an iteration ended without k_iter_c: k_iter_b sets
-  // pending; the next Schur matvec takes the step from k_iter_b's partials
-  // and stages its result (st_*, st_fresh); the next k_iter_a commits it.
+  // the status step of an iteration ended without k_iter_c: k_iter_ab sets
+  // pending; the next explicit Schur matvec takes the step from k_iter_ab's
+  // partials and stages its result (st_*, st_fresh); the next k_iter_ab
+  // commits it.
   // With nothing staged, st_rz, st_beta and st_done equal rz, beta and done.
   int pending;
   double2 st_rz, st_beta;
@@ -341,8 +344,6 @@ size_t smv_lds_bytes(int nbmax, int waves);
 hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, const double2 *val,
                          double2 *pcslot, hipStream_t st);
 hipError_t launch_first(const VecArgs &a, hipStream_t st);
-// k_iter_a, k_iter_b and (status_step) k_iter_c
-hipError_t launch_iter(const VecArgs &a, hipStream_t st, bool status_step = true);
 // k_iter_ab and (status_step) k_iter_c: the explicit iteration after k_smv
 hipError_t launch_iter_ab(const VecArgs &a, hipStream_t st, bool status_step = true);
 hipError_t launch_finalize_p(const VecArgs &a, hipStream_t st);

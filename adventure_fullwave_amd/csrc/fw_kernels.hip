@@ -733,130 +733,25 @@ hipError_t launch_first(const VecArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Interface iteration (hddm_solver_s.h:260-400): two passes over the
-// unknowns and one single-block scalar step.  Every reduction sums the
-// per-block partials in one fixed order, so results do not depend on
-// scheduling; no kernel writes a scalar that another block of the same
-// kernel reads.
-//   k_iter_a: p_k = z_k + beta p_{k-1} (written back to P), q = gather(y),
-//             partials of (p,q)
-//   k_iter_b: alpha from the (p,q) partials (every block, same order,
-//             :286-295), u += alpha p, r -= alpha q, z = PC r, partials of
-//             (r,z) and max|r|
-//   k_iter_c: one block: beta (:340-350), the stop test (:311-318, :370-376)
-//             and the status machine (hddm_solver.h:125-143)
+// Interface iteration (hddm_solver_s.h:260-400) after the Schur step (which
+// leaves y per slot and every subdomain's part of (p,q)): one pass over the
+// unknowns and the scalar status step.  Every reduction sums per-block or
+// per-subdomain partials in one fixed order, so results do not depend on
+// scheduling; no kernel writes a scalar that another block of the same kernel
+// reads.
+//   k_iter_ab: alpha from the (p,q) parts (every block, same order,
+//              :286-295), p_k = z_k + beta p_{k-1} (written back to P),
+//              q = gather(y), u += alpha p, r -= alpha q, z = PC r, partials
+//              of (r,z) and max|r|
+//   k_iter_c:  one block: beta (:340-350), the stop test (:311-318,
+//              :370-376) and the status machine (hddm_solver.h:125-143)
 // The Schur step of the next iteration forms p = z + beta p itself (zbp,
-// same fused operations as k_iter_a); the final P (U when converged,
+// same fused operations as k_iter_ab); the final P (U when converged,
 // hddm_solver_s.h:370-376) is set by k_finalize_p before the last solve.
 // Inside a graph of explicit iterations k_iter_c runs only after the last
-// one: the explicit Schur matvec takes the status step from k_iter_b's
+// one: the explicit Schur matvec takes the status step from k_iter_ab's
 // partials itself (every workgroup, the same reduction order) and stages it,
-// and k_iter_a's block 0 commits it (Scal::pending, st_*), so no kernel
-// reads a status field that another block of the same kernel writes.
-__global__ __launch_bounds__(VT) void k_iter_a(VecArgs a) {
-  to_system(a);
-  // the operands of this thread's first unknown are loaded before the
-  // status, so the two latencies overlap
-  const int g0 = blockIdx.x * VT + threadIdx.x;
-  double2 q0 = cz(), z0 = cz(), p0 = cz();
-  if (g0 < a.ng) {
-    q0 = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g0);
-    z0 = a.Z[g0];
-    p0 = a.P[g0];
-  }
-  // the staged status and beta are requested together (one round trip);
-  // block 0 commits a status step staged by the Schur matvec
-  Scal *s = a.sc;
-  const int done = s->st_done;
-  const double2 beta = s->st_beta;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && s->st_fresh) {
-    const int loop = s->loop;
-    const double resi = s->st_resi;
-    a.hist[loop] = resi;
-    s->resi = resi;
-    s->conv = s->st_conv;
-    s->rz = s->st_rz;
-    s->beta = s->st_beta;
-    s->iters = s->iters + 1;
-    s->done = s->st_done;
-    s->loop = loop + 1;
-    s->st_fresh = 0;
-    s->pending = 0;
-  }
-  if (done) return;
-  __shared__ double2 shc[VT / 64 + 1];
-  double2 pq = cz();
-  if (g0 < a.ng) {
-    const double2 p = zbp(z0, beta, p0);
-    a.Q[g0] = q0;
-    a.P[g0] = p;
-    pq = cmul(p, q0);
-  }
-  for (int g = g0 + a.G * VT; g < a.ng; g += a.G * VT) {
-    const double2 q = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
-    const double2 p = zbp(a.Z[g], beta, a.P[g]);
-    a.Q[g] = q;
-    a.P[g] = p;
-    pq = cadd(pq, cmul(p, q));
-  }
-  pq = block_csum(pq, shc);
-  if (threadIdx.x == 0) a.part[blockIdx.x] = pq;
-}
-__global__ __launch_bounds__(VT) void k_iter_b(VecArgs a) {
-  to_system(a);
-  // operands of the first unknown before the status and the alpha reduction
-  const int g0 = blockIdx.x * VT + threadIdx.x;
-  double2 u0 = cz(), p0 = cz(), r0 = cz(), q0 = cz(), pc0 = cz();
-  if (g0 < a.ng) {
-    u0 = a.U[g0];
-    p0 = a.P[g0];
-    r0 = a.R[g0];
-    q0 = a.Q[g0];
-    pc0 = a.PC[g0];
-  }
-  // the status, rz and the (p,q) partials are requested together; the
-  // status is tested after the reduction (a uniform branch)
-  Scal *s = a.sc;
-  const int done = s->done;
-  const double2 rz = s->rz;
-  __shared__ double2 shc[VT / 64 + 1];
-  __shared__ double shm[VT / 64 + 1];
-  const double2 pq = reduce_parts(a.part, a.G, shc);
-  if (done) return;
-  // this iteration's status step is pending until k_iter_c or the next
-  // Schur matvec takes it (no other block of this kernel reads the flag)
-  if (blockIdx.x == 0 && threadIdx.x == 0) s->pending = 1;
-  double2 malpha = cz();
-  if (!(cabsv(rz) < NEARLY_ZERO && cabsv(pq) < NEARLY_ZERO)) malpha = csub(cz(), cdiv_naive(rz, pq));
-  double2 rzn = cz();
-  double mx = 0.0;
-  if (g0 < a.ng) {
-    const double2 u = cadd(u0, cmul(malpha, p0));
-    const double2 r = cadd(r0, cmul(malpha, q0));
-    const double2 z = cmul(r, pc0);
-    a.U[g0] = u;
-    a.R[g0] = r;
-    a.Z[g0] = z;
-    mx = cabsv(r);
-    rzn = cmul(r, z);
-  }
-  for (int g = g0 + a.G * VT; g < a.ng; g += a.G * VT) {
-    const double2 u = cadd(a.U[g], cmul(malpha, a.P[g]));
-    const double2 r = cadd(a.R[g], cmul(malpha, a.Q[g]));
-    const double2 z = cmul(r, a.PC[g]);
-    a.U[g] = u;
-    a.R[g] = r;
-    a.Z[g] = z;
-    mx = fmax(mx, cabsv(r));
-    rzn = cadd(rzn, cmul(r, z));
-  }
-  rzn = block_csum(rzn, shc);
-  mx = block_max(mx, shm);
-  if (threadIdx.x == 0) {
-    a.part[a.G + blockIdx.x] = rzn;
-    a.part[2 * a.G + blockIdx.x] = make_double2(mx, 0.0);
-  }
-}
+// and the next k_iter_ab's block 0 commits it (Scal::pending, st_*).
 __global__ __launch_bounds__(VT) void k_iter_c(VecArgs a) {
   to_system(a);
   Scal *s = a.sc;
@@ -888,12 +783,10 @@ __global__ __launch_bounds__(VT) void k_iter_c(VecArgs a) {
     s->st_fresh = 0;
   }
 }
-// Explicit interface iteration after the Schur matvec: k_iter_a and
-// k_iter_b in one pass.  The matvec leaves every subdomain's part of (p,q)
-// (pqpart: the sum over its slots of p y, which regroups sum_g p_g q_g by
-// subdomain), so alpha is known when the pass starts: p = z + beta p is formed
-// and written back, q gathered, and u, r, z updated with the arithmetic of
-// k_iter_a and k_iter_b.  The status comes from the staged fields only
+// The vector pass of an interface iteration.  The Schur step leaves every
+// subdomain's part of (p,q) (pqpart: the sum over its slots of p y, which
+// regroups sum_g p_g q_g by subdomain), so alpha is known when the pass
+// starts.  The status comes from the staged fields only
 // (st_rz, st_beta, st_done: equal to the committed ones when nothing is
 // staged), which no block of this kernel writes; block 0 commits a step the
 // matvec staged.
@@ -975,12 +868,6 @@ hipError_t launch_iter_ab(const VecArgs &a, hipStream_t st, bool status_step) {
   return hipGetLastError();
 }
 
-hipError_t launch_iter(const VecArgs &a, hipStream_t st, bool status_step) {
-  k_iter_a<<<vec_grid(a, a.G), VT, 0, st>>>(a);
-  k_iter_b<<<vec_grid(a, a.G), VT, 0, st>>>(a);
-  if (status_step) k_iter_c<<<vec_grid(a, 1), VT, 0, st>>>(a);
-  return hipGetLastError();
-}
 
 // P for the last solve: U when the stop test fired, else the next search
 // direction z + beta p (max_loop reached, hddm_solver.h:125-143)

@@ -186,9 +186,9 @@ __device__ __forceinline__ void to_system(SmvArgs &a) {
 // The status of the interface iteration for the Schur matvec (false: the
 // system is finished and the workgroup's work is discarded; beta: the
 // coefficient of the next search direction).  When the previous iteration's
-// status step is still pending (k_iter_b ran, k_iter_c did not), every
-// workgroup takes it from k_iter_b's partials in k_iter_c's order, and
-// workgroup 0 stages it for the next k_iter_a to commit; no workgroup of
+// status step is still pending (k_iter_ab ran, k_iter_c did not), every
+// workgroup takes it from k_iter_ab's partials in k_iter_c's order, and
+// workgroup 0 stages it for the next k_iter_ab to commit; no workgroup of
 // this kernel reads the staged fields.  Contains barriers: call uniformly.
 // With 256 threads each thread is its own virtual thread of that order, and
 // its first four partials of each kind can be loaded before the workgroup's
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   }
 
   // interface values: P, or the next search direction z + beta p while the
-  // interface iteration runs (formed here, not stored, see k_iter_a)
+  // interface iteration runs (formed here, not stored, see k_iter_ab)
   for (int k = tid; k < npad; k += NT) {
     double2 p = cz();
     if (k < ng) {

@@ -231,6 +231,7 @@ __device__ __forceinline__ void to_system(SolveArgs &a) {
   a.yslot += s * a.ss.slot;
   a.X += s * a.ss.dof;
   if (a.smat) a.smat += s * a.ss.smat;
+  if (a.pqpart) a.pqpart += s * (long long)a.npq;
   a.minv += s * a.ss.minv;
 }
 
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   // ---- 1. load vector --------------------------------------------------------------
   const bool zero_g = (a.mode == MODE_FIRST || a.mode == MODE_RHS);
   // interface values: P, or the next search direction z + beta p while the
-  // interface iteration runs (formed here, not stored, see k_iter_a)
+  // interface iteration runs (formed here, not stored, see k_iter_ab)
   const double2 beta = (a.mode == MODE_ITER && a.Z) ? a.sc->beta : cz();
   auto pval = [&](int gid) -> double2 {
     const double2 p = a.P[gid];
@@ -371,12 +372,21 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
       dst[tile_pos(k >> 4, J, k & 15, cj, nbk, hlf)] = y;
     }
   } else if (a.mode != MODE_RHS) {
+    // in an interface iteration also this subdomain's part of (p,q) =
+    // sum over its slots of p y: an interface row's solution value is p
+    // itself (its factor row is the identity, its column in L zero)
+    double2 pq = cz();
     for (int k = lane; k < m.ng; k += WL) {
       const int sl = m.g0 + k;
       double2 y = cz();
       for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], L.v[a.gr_col[c]]);
       if (a.mode != MODE_ITER) y = csub(y, a.f[m.dof0 + a.g_nfid[sl]]);
       a.yslot[sl] = y;
+      pq = cadd(pq, cmul(L.v[a.g_nfid[sl]], y));
+    }
+    if (a.mode == MODE_ITER && a.pqpart) {
+      pq = wave_csum(pq);
+      if (lane == 0) a.pqpart[w] = pq;
     }
   }
 }
