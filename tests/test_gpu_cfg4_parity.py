@@ -1,0 +1,73 @@
+"""End-to-end parity at the largest configuration (run with -m gpu and FWH_LARGE_PARITY=1).
+
+cfg 4 (64 x 64 x 48 cells, 1,407,152 edge DOFs, 3,584 subdomains, 8 layers of
+lossy dielectric, PEC strip, reference-mode absorber) solved to
+||r||_inf / ||r0||_inf < 1e-8 on the GPU in both Schur forms and by the oracle
+with its subdomain Schur steps on host threads (the same sums as its
+sequential loop).  The oracle run takes a few minutes on 16 host threads, so
+the test runs only when FWH_LARGE_PARITY=1.  Bars from measured spreads
+(below): iteration count within COUNT_BAR, solution rel. L2 X_BAR.  Measured:
+GPU explicit 2,692 and implicit 2,689 against the oracle's 2,637, residual
+histories within 1.9e-11 of the oracle's over the first 100 iterations
+(departing after about 570), solutions 2.6e-7 from the oracle's."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import fw_oracle as fo
+
+# The count is rounding-chaotic on this case: the oracle's own count moves
+# 2,637-2,695 over five runs (nominal, reversed dot-product order, three
+# seeded few-ulp load perturbations) and the GPU's 2,675-2,698 over eight
+# few-ulp permittivity perturbations in either Schur form: 2.3 % of the
+# oracle's nominal count.
+COUNT_BAR = 0.03
+# the GPU's 1e-8-stopped solution lies 2.0e-7 (rel. L2) from its
+# 1e-12-stopped one (4,433 iterations); two correct 1e-8 stops may differ by
+# about twice that
+X_BAR = 5e-7
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(os.environ.get("FWH_LARGE_PARITY") != "1",
+                                 reason="cfg 4 oracle solve takes minutes; set FWH_LARGE_PARITY=1")]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cfg4():
+    from adventure_fullwave_amd import mesh
+    return mesh.make_config(4, pml_mode="sigma")
+
+
+@pytest.fixture(scope="module")
+def cfg4_ref(cfg4):
+    threads = max(1, min(16, os.cpu_count() or 1))
+    say(f"[gpu] cfg4 oracle solve on {threads} host threads (a few minutes)")
+    return fo.hddm_solve(cfg4, eps=1e-8, max_loop=6000, threads=threads)
+
+
+@pytest.mark.parametrize("schur", [True, False])
+def test_cfg4_matches_oracle(cfg4, cfg4_ref, schur):
+    from adventure_fullwave_amd import fwhip
+    say(f"[gpu] cfg4 solve explicit={schur}")
+    sol = fwhip.Solver(cfg4)
+    st = sol.solve(eps=1e-8, max_loop=6000, graph_iters=16, schur=schur)
+    ref = cfg4_ref
+    it_ref = ref["stats"]["iters"]
+    say(f"[gpu] cfg4 iters gpu={st['iters']} oracle={it_ref}")
+    assert ref["stats"]["converged"] == 1 and st["converged"] == 1
+    assert abs(st["iters"] - it_ref) <= COUNT_BAR * it_ref
+    r = sol.resid(st["loops"])
+    k = min(30, len(r), len(ref["resid"]))
+    assert np.allclose(r[:k], ref["resid"][:k], rtol=1e-6)
+    x = sol.x()
+    rel = np.linalg.norm(x - ref["x_flat"]) / np.linalg.norm(ref["x_flat"])
+    say(f"[gpu] cfg4 solution rel. L2 to the oracle {rel:.3e}")
+    assert rel <= X_BAR
+    sol.close()
