@@ -627,27 +627,27 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
   return sh[VT / 64];
 }
 // identical fixed-order reduction of G partials in every block
-// (four loads per thread in flight, added in the same order)
+// (eight loads per thread in flight, added in the same order)
 __device__ __forceinline__ double2 reduce_parts(const double2 *__restrict__ p, int G, double2 *sh) {
   double2 t = cz();
-  for (int k = threadIdx.x; k < G; k += 4 * VT) {
-    double2 v[4];
+  for (int k = threadIdx.x; k < G; k += 8 * VT) {
+    double2 v[8];
 #pragma unroll
-    for (int j = 0; j < 4; j++) v[j] = k + j * VT < G ? p[k + j * VT] : cz();
+    for (int j = 0; j < 8; j++) v[j] = k + j * VT < G ? p[k + j * VT] : cz();
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int j = 0; j < 8; j++)
       if (k + j * VT < G) t = cadd(t, v[j]);
   }
   return block_csum(t, sh);
 }
 __device__ __forceinline__ double reduce_parts_max(const double2 *__restrict__ p, int G, double *sh) {
   double t = 0.0;
-  for (int k = threadIdx.x; k < G; k += 4 * VT) {
-    double v[4];
+  for (int k = threadIdx.x; k < G; k += 8 * VT) {
+    double v[8];
 #pragma unroll
-    for (int j = 0; j < 4; j++) v[j] = k + j * VT < G ? p[k + j * VT].x : 0.0;
+    for (int j = 0; j < 8; j++) v[j] = k + j * VT < G ? p[k + j * VT].x : 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int j = 0; j < 8; j++)
       if (k + j * VT < G) t = fmax(t, v[j]);
   }
   return block_max(t, sh);

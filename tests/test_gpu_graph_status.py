@@ -1,12 +1,13 @@
 """The status step of explicit interface iterations inside graphs (run with
 -m gpu).
 
-Launched one kernel at a time, an iteration is the Schur matvec, k_iter_a,
-k_iter_b and k_iter_c (beta, stop test, status machine; hddm_solver_s.h:
-311-350, hddm_solver.h:125-143).  In a graph of explicit iterations k_iter_c
-runs only after the last one: the Schur matvec of the next iteration takes
-the status step from k_iter_b's partials (every workgroup, k_iter_c's
-reduction order) and k_iter_a commits it.  Both forms must give the same
+Launched one kernel at a time, an iteration is the Schur matvec, k_iter_ab
+(alpha and the vector updates) and k_iter_c (beta, stop test, status
+machine; hddm_solver_s.h:311-350, hddm_solver.h:125-143).  In a graph of
+explicit iterations k_iter_c runs only after the last one: the Schur matvec
+of the next iteration takes the status step from k_iter_ab's partials (every
+workgroup, k_iter_c's reduction order) and the next k_iter_ab commits it.
+Both forms must give the same
 bits: solution, iteration count, loop count and residual history, for graphs
 whose length divides the count or not, for single systems and batches, and
 at max_loop."""
