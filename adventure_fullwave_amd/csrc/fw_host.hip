@@ -1556,6 +1556,12 @@ int build_schur(fwh_ctx *c) {
         best = c->info.ms_smv_tune[v];
         c->info.smv_waves = cand[v];
       }
+    // FWH_SMV_WAVES=2, 4 or 8 fixes the count (for profiling runs, whose
+    // counter collection changes the timings), where LDS allows it
+    if (const char *e = getenv("FWH_SMV_WAVES")) {
+      const int v = atoi(e);
+      if ((v == 2 || v == 4 || v == 8) && fw::smv_lds_bytes(nbmax, v) <= 160 * 1024) c->info.smv_waves = v;
+    }
     c->smv_waves = c->info.smv_waves;
     // work order of the matvec.  Every workgroup computes its subdomain's
     // product alone, so the order changes neither results nor rounding,

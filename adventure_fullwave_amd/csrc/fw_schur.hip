@@ -8,8 +8,10 @@
 // y = S_s p_s with the complex-symmetric Schur complement
 // S_s = A_GG - A_GI A_II^-1 A_IG.  In explicit mode the S_s are formed once
 // from the skyline factors (S_s = A_GG - W^T D^-1 W with W = L^-1 (-A_IG):
-// forward sweeps of k_solve, MODE_FWD, then k_schur_gemm on the matrix cores
-// and k_schur_agg; or the Schur step of k_solve on every interface unit
+// the inverted 16-row diagonal blocks of L (k_blockinv16), blocked forward
+// substitution of 16-column chunks of -A_IG (k_schur_fwd), then
+// k_schur_gemm on the matrix cores and k_schur_agg; or the Schur step of
+// k_solve on every interface unit
 // vector, MODE_UNIT) and each iteration streams them once through k_smv: a
 // batched, HBM-bound symmetric matvec instead of two dependent triangular
 // sweeps per subdomain.

@@ -32,16 +32,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(cfg):
+def pmc_traffic(cfg, waves):
     """HBM bytes per k_smv launch from the committed rocprofv3 --pmc summary
     (FETCH_SIZE, WRITE_SIZE, scaled by the copy-kernel calibration), used only
-    while the kernel source is the one that was profiled; else None."""
+    while the kernel source is the one that was profiled and the run uses the
+    profiled waves per workgroup; else None."""
     import hashlib
     path = os.path.join(ROOT, "profiles", f"k_smv_cfg{cfg}_pmc.json")
     src = os.path.join(ROOT, "adventure_fullwave_amd", "csrc", "fw_schur.hip")
     try:
         d = json.load(open(path))
         if d.get("source_sha256") != hashlib.sha256(open(src, "rb").read()).hexdigest():
+            return None
+        if d.get("smv_waves") != waves:
             return None
         return d["traffic_bytes_per_launch"]
     except (OSError, ValueError, KeyError):
@@ -188,7 +191,7 @@ def main():
     if expl and ms_smv > 0:
         achieved = info["bytes_smv"] / (ms_smv * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.cfg),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.cfg, info["smv_waves"]),
                     "kernel": "k_smv (explicit Schur matvec, all subdomains)", "ms_per_launch": round(ms_smv, 4),
                     "bytes_per_launch": info["bytes_smv"], "implicit_kernel": k_solve}
     else:
