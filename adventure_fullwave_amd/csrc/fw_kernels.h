@@ -292,12 +292,17 @@ struct Scal {
   int st_conv, st_done, st_fresh, pad;
 };
 
+// entries of VecArgs::gq_slot4: no slot, and (in .y, with SLOT_NONE in .x)
+// an unknown shared by more than four subdomains
+constexpr int SLOT_NONE = -2147483647 - 1;
+constexpr int SLOT_MORE = -2147483647;
 struct VecArgs {
   int sys0, nsys;
   SysStride ss;
   int ng;                 // N_Gamma
   const int *gq_ptr;      // gather CSR over interface unknowns
   const int *gq_slot;     // slot ids, ~slot marks the first slot of a part group
+  const int4 *gq_slot4;   // the same ids, four per unknown (SLOT_NONE pads; SLOT_MORE in .y: use the CSR)
   const double2 *yslot;
   const double2 *pcslot;
   double2 *P, *Q, *R, *Z, *U, *PC;

@@ -555,17 +555,19 @@ hipError_t launch_pcslot(int nslots, const int *diag_ent, const int *unused, con
 // hddm_solver_sp.h:41-54); parts summed owner-first, ascending
 // (hddm_solver_s.h:404-424)
 __device__ __forceinline__ double2 gather_slots(const int *__restrict__ ptr, const int *__restrict__ slot,
-                                                const double2 *__restrict__ y, int g) {
+                                                const int4 *__restrict__ slot4, const double2 *__restrict__ y,
+                                                int g) {
   double2 acc = cz(), part = cz();
-  const int c0 = ptr[g], c1 = ptr[g + 1];
-  if (c1 - c0 <= 4) {
+  const int4 s4 = slot4[g];
+  if (!(s4.x == SLOT_NONE && s4.y == SLOT_MORE)) {
     // the usual case (an interface edge in 2-4 subdomains): the slot ids
-    // and values are requested together, then summed in list order
-    const int cnt = c1 - c0;
-    int sl[4];
-    double2 v[4];
+    // come in one load, the values are requested together, then summed in
+    // list order
+    int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+    int cnt = 0;
 #pragma unroll
-    for (int j = 0; j < 4; j++) sl[j] = j < cnt ? slot[c0 + j] : 0;
+    for (int j = 0; j < 4; j++) cnt += sl[j] != SLOT_NONE;
+    double2 v[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) v[j] = j < cnt ? y[sl[j] < 0 ? ~sl[j] : sl[j]] : cz();
 #pragma unroll
@@ -581,6 +583,7 @@ __device__ __forceinline__ double2 gather_slots(const int *__restrict__ ptr, con
     }
     return cnt > 0 ? cadd(acc, part) : acc;
   }
+  const int c0 = ptr[g], c1 = ptr[g + 1];
   for (int c = c0; c < c1; c++) {
     int sl = slot[c];
     if (sl < 0) {
@@ -681,8 +684,8 @@ __global__ __launch_bounds__(VT) void k_first_a(VecArgs a) {
   double2 rz = cz();
   double mx = 0.0;
   for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT) {
-    const double2 q = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
-    const double2 ps = gather_slots(a.gq_ptr, a.gq_slot, a.pcslot, g);
+    const double2 q = gather_slots(a.gq_ptr, a.gq_slot, a.gq_slot4, a.yslot, g);
+    const double2 ps = gather_slots(a.gq_ptr, a.gq_slot, a.gq_slot4, a.pcslot, g);
     const double2 pc = ciszero(ps) ? make_double2(1.0, 0.0) : cdiv_naive(make_double2(1.0, 0.0), ps);
     const double2 z = cmul(q, pc);
     a.Q[g] = q;
@@ -795,7 +798,7 @@ __global__ __launch_bounds__(VT) void k_iter_ab(VecArgs a) {
   const int g0 = blockIdx.x * VT + threadIdx.x;
   double2 q0 = cz(), z0 = cz(), p0 = cz(), u0 = cz(), r0 = cz(), pc0 = cz();
   if (g0 < a.ng) {
-    q0 = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g0);
+    q0 = gather_slots(a.gq_ptr, a.gq_slot, a.gq_slot4, a.yslot, g0);
     z0 = a.Z[g0];
     p0 = a.P[g0];
     u0 = a.U[g0];
@@ -843,7 +846,7 @@ __global__ __launch_bounds__(VT) void k_iter_ab(VecArgs a) {
     rzn = cmul(r, z);
   }
   for (int g = g0 + a.G * VT; g < a.ng; g += a.G * VT) {
-    const double2 q = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
+    const double2 q = gather_slots(a.gq_ptr, a.gq_slot, a.gq_slot4, a.yslot, g);
     const double2 p = zbp(a.Z[g], beta, a.P[g]);
     a.P[g] = p;
     const double2 u = cadd(a.U[g], cmul(malpha, p));
@@ -887,7 +890,7 @@ hipError_t launch_finalize_p(const VecArgs &a, hipStream_t st) {
 __global__ __launch_bounds__(VT) void k_gather_q(VecArgs a) {
   to_system(a);
   for (int g = blockIdx.x * VT + threadIdx.x; g < a.ng; g += a.G * VT)
-    a.Q[g] = gather_slots(a.gq_ptr, a.gq_slot, a.yslot, g);
+    a.Q[g] = gather_slots(a.gq_ptr, a.gq_slot, a.gq_slot4, a.yslot, g);
 }
 hipError_t launch_gather_q(const VecArgs &a, hipStream_t st) {
   k_gather_q<<<vec_grid(a, a.G), VT, 0, st>>>(a);
