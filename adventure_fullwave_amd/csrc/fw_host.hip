@@ -564,12 +564,15 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     }
   }
   // the same lists four per unknown, so a gather reads its slot ids in one
-  // load (interface edges lie in 2-4 subdomains)
+  // load (interface edges lie in 2-4 subdomains); FWH_GATHER=csr sends every
+  // unknown through the CSR, the path of unknowns in more than four
   std::vector<int4> gq_slot4(ng);
+  const char *genv = getenv("FWH_GATHER");
+  const bool csr_only = genv && strcmp(genv, "csr") == 0;
   for (int g = 0; g < ng; g++) {
     const int c0 = gq_ptr[g], cnt = gq_ptr[g + 1] - c0;
     int e[4] = {fw::SLOT_NONE, fw::SLOT_NONE, fw::SLOT_NONE, fw::SLOT_NONE};
-    if (cnt <= 4)
+    if (cnt <= 4 && !csr_only)
       for (int j = 0; j < cnt; j++) e[j] = gq_slot[c0 + j];
     else
       e[1] = fw::SLOT_MORE;

@@ -133,3 +133,31 @@ def test_factor_forms(request, model_name):
     say(f"[gpu] iterations blocked={ib} row={ir}")
     assert abs(ib - ir) <= max(1, 0.02 * ir)
     assert np.linalg.norm(xb - xr) <= 1e-7 * np.linalg.norm(xr)
+
+
+@pytest.mark.parametrize("schur", [True, False])
+def test_gather_forms(small_sigma_model, schur):
+    """An interface unknown's slot values are summed in the reference's order
+    (hddm_solver_sp.h:41-54, hddm_solver_s.h:404-424) from a four-wide table
+    of slot ids when it lies in at most four subdomains, else from the CSR.
+    The box partitions here put every interface edge in 2-4 subdomains, so
+    FWH_GATHER=csr sends all of them through the CSR loop: both must give
+    the same bits (iterations, residual history, solution), in the explicit
+    and the implicit form."""
+    from adventure_fullwave_amd import fwhip
+    out = {}
+    for form in ("table", "csr"):
+        if form == "csr":
+            os.environ["FWH_GATHER"] = "csr"
+        try:
+            sol = fwhip.Solver(small_sigma_model)
+        finally:
+            os.environ.pop("FWH_GATHER", None)
+        st = sol.solve(eps=1e-8, max_loop=20000, graph_iters=16, schur=schur)
+        out[form] = (st["iters"], st["loops"], st["converged"], sol.x().copy(), sol.resid(st["loops"]).copy())
+        sol.close()
+        say(f"[gpu] gather {form} explicit={schur}: {st['iters']} iterations")
+    a, b = out["table"], out["csr"]
+    assert a[:3] == b[:3] and a[2] == 1
+    assert np.array_equal(a[3], b[3])
+    assert np.array_equal(a[4], b[4])
