@@ -324,6 +324,17 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   Tile A, B;
   load_next(base, zero, cur, lane, A);
   load_next(base, zero, cur, lane, B);
+  // the values of those two slots (they do not depend on the status) follow
+  // the first tiles, ahead of the status fields
+  double2 pa = cz(), za = cz(), pb = cz(), zb = cz();
+  if (k0 < ng) {
+    pa = a.P[gid0];
+    if (a.Z) za = a.Z[gid0];
+  }
+  if (k1 < ng) {
+    pb = a.P[gid1];
+    if (a.Z) zb = a.Z[gid1];
+  }
   // the status is read once the first tiles are in flight (a finished
   // system's workgroups discard them)
   double2 beta = cz();
@@ -342,9 +353,19 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   for (int k = tid; k < npad; k += NT) {
     double2 p = cz();
     if (k < ng) {
-      const int gid = k == k0 ? gid0 : (k == k1 ? gid1 : a.g_gid[wk.g0 + k]);
-      p = a.P[gid];
-      if (a.Z) p = zbp(a.Z[gid], beta, p);
+      double2 z;
+      if (k == k0) {
+        p = pa;
+        z = za;
+      } else if (k == k1) {
+        p = pb;
+        z = zb;
+      } else {
+        const int gid = a.g_gid[wk.g0 + k];
+        p = a.P[gid];
+        z = a.Z ? a.Z[gid] : cz();
+      }
+      if (a.Z) p = zbp(z, beta, p);
     }
     pv[k] = p;
   }
