@@ -79,6 +79,42 @@ double now_ms() {
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// temporary device buffers of one call, freed on every exit path
+struct TempDev {
+  std::vector<void *> p;
+  TempDev() = default;
+  TempDev(const TempDev &) = delete;
+  TempDev &operator=(const TempDev &) = delete;
+  ~TempDev() {
+    if (!p.empty()) (void)hipDeviceSynchronize();  // no kernel may still use them
+    for (void *q : p) (void)hipFree(q);
+  }
+  template <class T>
+  hipError_t alloc(T **q, size_t count) {
+    void *v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) p.push_back(v);
+    *q = (T *)v;
+    return e;
+  }
+};
+
+// events of one timing, destroyed on every exit path
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  EventPair() = default;
+  EventPair(const EventPair &) = delete;
+  EventPair &operator=(const EventPair &) = delete;
+  hipError_t create() {
+    hipError_t e = hipEventCreate(&a);
+    return e == hipSuccess ? hipEventCreate(&b) : e;
+  }
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
 }  // namespace
 
 struct fwh_ctx {
@@ -186,6 +222,15 @@ struct fwh_ctx {
     allocs.push_back(q);
     *p = (T *)q;
     return 0;
+  }
+  // frees one allocation made by alloc (waits for the device first)
+  void free_one(void *q) {
+    if (!q) return;
+    auto it = std::find(allocs.begin(), allocs.end(), q);
+    if (it == allocs.end()) return;
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(q);
+    allocs.erase(it);
   }
   template <class T>
   int upload(T **p, const std::vector<T> &h) {
@@ -1110,15 +1155,20 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   }
   double2 *m16 = nullptr;
   long long *d_mf_off = nullptr;
+  TempDev ftmp;  // the inverted diagonal blocks of the blocked factorisation
   if (blocked) {
-    HIPCHK(hipMalloc(&m16, std::max<long long>(mf_tot, 1) * 256 * sizeof(double2)));
-    HIPCHK(hipMalloc(&d_mf_off, nsub * sizeof(long long)));
+    HIPCHK(ftmp.alloc(&m16, (size_t)std::max<long long>(mf_tot, 1) * 256));
+    HIPCHK(ftmp.alloc(&d_mf_off, (size_t)nsub));
     HIPCHK(hipMemcpyAsync(d_mf_off, mf_off.data(), nsub * sizeof(long long), hipMemcpyHostToDevice, st));
   }
   c->info.factor_blocked = blocked ? 1 : 0;
 
   HIPCHK(hipStreamSynchronize(st));
   t0 = now_ms();
+  // the factor kernel alone is timed with events on the library stream
+  EventPair fev;
+  HIPCHK(fev.create());
+  double ms_fk = 0.0;
   HIPCHK(hipMemsetAsync(c->fac, 0, (size_t)nsys * nlen_tot * sizeof(double2), st));
   for (int k = 0; k < nsys; k++) {  // one factor set per system
     double2 *fac = c->fac + k * nlen_tot;
@@ -1137,14 +1187,19 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     fa.nmax = nmax;
     fa.m16 = m16;
     fa.mf_off = d_mf_off;
+    HIPCHK(hipEventRecord(fev.a, st));
     HIPCHK(blocked ? fw::launch_factor16(fa, nsub, st) : fw::launch_factor(fa, nsub, st));
+    HIPCHK(hipEventRecord(fev.b, st));
     HIPCHK(fw::launch_blockinv(c->meta, nsub, c->idx1, c->rowbeg, c->blk_row, fac,
                                c->minv + (size_t)k * blk_row.size() * fw::MINV, st));
+    HIPCHK(hipEventSynchronize(fev.b));
+    float fms = 0.f;
+    HIPCHK(hipEventElapsedTime(&fms, fev.a, fev.b));
+    ms_fk += fms;
   }
   HIPCHK(hipStreamSynchronize(st));
   c->info.ms_factor = now_ms() - t0;
-  if (m16) (void)hipFree(m16);
-  if (d_mf_off) (void)hipFree(d_mf_off);
+  c->info.ms_factor_kernel = ms_fk;
 
   // ---- interface vectors ------------------------------------------------------------
   c->G = std::max(1, std::min(512, (ng + fw::VT_HINT - 1) / fw::VT_HINT));
@@ -1301,11 +1356,14 @@ extern "C" int fwh_get_info(const fwh_ctx *c, fwh_info *info) {
 namespace {
 int ensure_hist(fwh_ctx *c, int max_loop) {
   if (max_loop + 1 <= c->hist_cap) return 0;
+  c->release_graph();  // graphs hold the old pointer
+  c->free_one(c->hist);
+  c->hist = nullptr;
+  c->hist_cap = 0;
   int rc = c->alloc(&c->hist, (size_t)c->nsys * (max_loop + 1));
   if (rc) return rc;
   c->hist_cap = max_loop + 1;
   c->ss.hist = c->hist_cap;
-  c->release_graph();  // graphs hold the old pointer
   return 0;
 }
 // the scalars of every system; returns 1 in *all_done when every system's
@@ -1368,6 +1426,21 @@ int build_schur(fwh_ctx *c) {
     moff[s] = (long long)blist.size();
     for (int b = 0; b < (nsw[s] + 15) / 16; b++) blist.push_back(make_int2(s, b));
   }
+  // algorithmic flops of the formation: the substitution touches every
+  // off-diagonal profile entry once per interface column, the tile products
+  // one complex multiply-add per (interior row, lower-triangle entry)
+  {
+    double ff = 0.0, fg = 0.0;
+    for (int s = 0; s < c->nsub; s++) {
+      const SubMeta &M = c->meta_h[s];
+      long long ni = 0;
+      for (int i = 0; i < M.n; i++) ni += bcf[M.dof0 + i] == 0;
+      ff += 8.0 * M.ng * (double)(M.nlen - M.n);
+      fg += 8.0 * (double)ni * M.ng * (M.ng + 1) / 2.0 + 6.0 * (double)ni * M.ng;
+    }
+    c->info.flops_schur_fwd = c->nsys * ff;
+    c->info.flops_schur_gemm = c->nsys * fg;
+  }
   const long long mtot = (long long)blist.size() * 256;
   size_t mem_free = 0, mem_tot = 0;
   HIPCHK(hipMemGetInfo(&mem_free, &mem_tot));
@@ -1420,6 +1493,11 @@ int build_schur(fwh_ctx *c) {
     put(&d_blist, blist);
     put(&d_fwork, fwork);
     put(&d_tiles, tiles);
+    // the substitution and the tile products are timed with events
+    EventPair ev0, ev1;
+    if (e == hipSuccess) e = ev0.create();
+    if (e == hipSuccess) e = ev1.create();
+    double ms_fwd = 0.0, ms_gemm = 0.0;
     for (int k = 0; k < c->nsys && e == hipSuccess; k++) {
       const double2 *fac = c->fac + (size_t)k * c->ss.nlen;
       fw::BlockInv16Args bi{};
@@ -1448,7 +1526,9 @@ int build_schur(fwh_ctx *c) {
       fa.m16_off = d_moff;
       fa.wmat = W;
       fa.w_off = d_woff;
+      if (e == hipSuccess) e = hipEventRecord(ev0.a, c->st);
       if (e == hipSuccess) e = fw::launch_schur_fwd(fa, c->st);
+      if (e == hipSuccess) e = hipEventRecord(ev0.b, c->st);
       fw::GemmArgs g{};
       g.meta = c->meta;
       g.tiles = d_tiles;
@@ -1459,7 +1539,9 @@ int build_schur(fwh_ctx *c) {
       g.dinv = c->dinv + (size_t)k * c->ndof;
       g.smat = c->smat + (size_t)k * tot;
       g.sm_off = c->sm_off;
+      if (e == hipSuccess) e = hipEventRecord(ev1.a, c->st);
       if (e == hipSuccess) e = fw::launch_schur_gemm(g, c->st);
+      if (e == hipSuccess) e = hipEventRecord(ev1.b, c->st);
       fw::AggArgs q{};
       q.meta = c->meta;
       q.gr_ptr = c->gr_ptr;
@@ -1470,7 +1552,15 @@ int build_schur(fwh_ctx *c) {
       q.smat = c->smat + (size_t)k * tot;
       q.sm_off = c->sm_off;
       if (e == hipSuccess) e = fw::launch_schur_agg(q, c->nsub, c->st);
+      float t_fwd = 0.f, t_gemm = 0.f;
+      if (e == hipSuccess) e = hipEventSynchronize(ev1.b);
+      if (e == hipSuccess) e = hipEventElapsedTime(&t_fwd, ev0.a, ev0.b);
+      if (e == hipSuccess) e = hipEventElapsedTime(&t_gemm, ev1.a, ev1.b);
+      ms_fwd += t_fwd;
+      ms_gemm += t_gemm;
     }
+    c->info.ms_schur_fwd = ms_fwd;
+    c->info.ms_schur_gemm = ms_gemm;
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     (void)hipStreamSynchronize(c->st);
     for (void *ptr : {(void *)W, (void *)M16, (void *)d_woff, (void *)d_moff, (void *)d_nsw, (void *)d_blist,
@@ -1699,7 +1789,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   std::vector<fw::Scal> s;
   int rc, all = 0;
   if ((rc = read_scal(c, s, &all))) return rc;
-  const double t0 = now_ms();
+  double t0 = now_ms();
   int it = 0;
   const bool expl = o->schur != 0;
   if (expl && !c->schur_built) return fail(FWH_E_STATE, "explicit Schur mode requested after fwh_first without it");
@@ -1712,9 +1802,13 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
       if ((rc = read_scal(c, s, &all))) return rc;
       it++;
     }
-  } else if (o->graph_iters > 0) {
+  } else if (o->graph_iters > 0 && n >= o->graph_iters && !all) {
+    // whole graphs of graph_iters iterations while they fit in n, then the
+    // remainder one batched iteration at a time: exactly n iterations are
+    // launched (those after the status turned LAST are no-ops)
     if (!c->gexec || c->graph_n != o->graph_iters || c->graph_schur != o->schur) {
       c->release_graph();
+      const double tc = now_ms();
       HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
       // explicit: the Schur matvec takes each iteration's status step, so
       // k_iter_c runs only after the last iteration of the graph
@@ -1722,7 +1816,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
         hipError_t e = c->run_schur(expl, false, true, true);
         if (e == hipSuccess) e = c->launch_vec_iter(!expl || k == o->graph_iters - 1);
         if (e != hipSuccess) {
-          hipGraph_t g;
+          hipGraph_t g = nullptr;
           (void)hipStreamEndCapture(c->st, &g);
           if (g) (void)hipGraphDestroy(g);
           return fail(FWH_E_HIP, "graph capture: %s", hipGetErrorString(e));
@@ -1732,17 +1826,17 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
       HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
       c->graph_n = o->graph_iters;
       c->graph_schur = o->schur;
+      t0 += now_ms() - tc;  // capture and instantiation are not loop time
     }
     // two graphs in flight: graph k+1 is queued before the status after
     // graph k is read (from a pinned copy made in stream order), so the
     // device does not wait for the host between graphs.  A graph queued
-    // after the status turned LAST runs its kernels as no-ops; no more
-    // graphs are launched than n allows.
+    // after the status turned LAST runs its kernels as no-ops.
     if (!c->scal_pin) {
       HIPCHK(hipHostMalloc((void **)&c->scal_pin, 2 * c->nsys * sizeof(fw::Scal), hipHostMallocDefault));
       for (int k = 0; k < 2; k++) HIPCHK(hipEventCreateWithFlags(&c->ev_scal[k], hipEventDisableTiming));
     }
-    int queued = 0, slot = 0, waited = 0;
+    int queued = 0, waited = 0;
     auto launch_one = [&]() -> int {
       HIPCHK(hipGraphLaunch(c->gexec, c->st));
       HIPCHK(hipMemcpyAsync(c->scal_pin + (queued & 1) * c->nsys, c->sc, c->nsys * sizeof(fw::Scal),
@@ -1752,14 +1846,21 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
       it += c->graph_n;
       return 0;
     };
-    if (it < n && !all && (rc = launch_one())) return rc;
+    if ((rc = launch_one())) return rc;
     while (waited < queued) {
-      if (it < n && !all && (rc = launch_one())) return rc;  // the next graph before waiting
-      slot = waited & 1;
+      // the next graph before waiting: slot (queued & 1) was read already
+      if (n - it >= c->graph_n && !all && (rc = launch_one())) return rc;
+      const int slot = waited & 1;
       HIPCHK(hipEventSynchronize(c->ev_scal[slot]));
       waited++;
       all = 1;
       for (int k = 0; k < c->nsys; k++) all &= c->scal_pin[slot * c->nsys + k].done != 0;
+    }
+    while (it < n && !all) {
+      HIPCHK(c->run_schur(expl, false, true, true));
+      HIPCHK(c->launch_vec_iter(true));
+      it++;
+      if (it >= n && (rc = read_scal(c, s, &all))) return rc;
     }
   } else {
     while (it < n && !all) {
@@ -1960,38 +2061,35 @@ extern "C" int fwh_time_kernels(fwh_ctx *c, int reps, double *ms_solve, double *
   if (!c || reps <= 0) return fail(FWH_E_ARG, "bad argument");
   if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
   HIPCHK(hipSetDevice(c->device));
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  EventPair ev;
+  HIPCHK(ev.create());
   float ms = 0.f;
   // warm up
   HIPCHK(c->run_solve(fw::MODE_RHS, false, false));
-  HIPCHK(hipEventRecord(e0, c->st));
+  HIPCHK(hipEventRecord(ev.a, c->st));
   for (int r = 0; r < reps; r++) HIPCHK(c->run_solve(fw::MODE_RHS, false, false));
-  HIPCHK(hipEventRecord(e1, c->st));
-  HIPCHK(hipEventSynchronize(e1));
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  HIPCHK(hipEventRecord(ev.b, c->st));
+  HIPCHK(hipEventSynchronize(ev.b));
+  HIPCHK(hipEventElapsedTime(&ms, ev.a, ev.b));
   if (ms_solve) *ms_solve = ms / reps;
-  HIPCHK(hipEventRecord(e0, c->st));
+  HIPCHK(hipEventRecord(ev.a, c->st));
   for (int r = 0; r < reps; r++) HIPCHK(c->run_solve(fw::MODE_ITER, false, false, true));
-  HIPCHK(hipEventRecord(e1, c->st));
-  HIPCHK(hipEventSynchronize(e1));
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  HIPCHK(hipEventRecord(ev.b, c->st));
+  HIPCHK(hipEventSynchronize(ev.b));
+  HIPCHK(hipEventElapsedTime(&ms, ev.a, ev.b));
   if (ms_schur) *ms_schur = ms / reps;
   if (ms_smv) {
     *ms_smv = -1.0;
     if (c->schur_built) {
       HIPCHK(c->run_schur(true, false, false, true));
-      HIPCHK(hipEventRecord(e0, c->st));
+      HIPCHK(hipEventRecord(ev.a, c->st));
       for (int r = 0; r < reps; r++) HIPCHK(c->run_schur(true, false, false, true));
-      HIPCHK(hipEventRecord(e1, c->st));
-      HIPCHK(hipEventSynchronize(e1));
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      HIPCHK(hipEventRecord(ev.b, c->st));
+      HIPCHK(hipEventSynchronize(ev.b));
+      HIPCHK(hipEventElapsedTime(&ms, ev.a, ev.b));
       *ms_smv = ms / reps;
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   c->first_done = false;
   c->last_done = false;
   return 0;

@@ -60,7 +60,10 @@ class FwhInfo(ctypes.Structure):
                 ("nsys", ctypes.c_int), ("smv_order", ctypes.c_int), ("ms_smv_order", ctypes.c_double * 3),
                 ("schur_gemm", ctypes.c_int),
                 ("factor_blocked", ctypes.c_int),
-                ("ms_host_index", ctypes.c_double), ("ms_load", ctypes.c_double)]
+                ("ms_host_index", ctypes.c_double), ("ms_load", ctypes.c_double),
+                ("ms_factor_kernel", ctypes.c_double), ("ms_schur_fwd", ctypes.c_double),
+                ("ms_schur_gemm", ctypes.c_double), ("flops_schur_fwd", ctypes.c_double),
+                ("flops_schur_gemm", ctypes.c_double)]
 
     def as_dict(self):
         out = {}
@@ -305,6 +308,8 @@ class Solver:
 
     def schur_apply(self, p, plain=False, explicit=False):
         p = np.ascontiguousarray(p, np.complex128)
+        if p.shape != (int(self.info["n_iface"]),):
+            raise FwhError(f"p must have shape ({int(self.info['n_iface'])},)")
         q = np.zeros_like(p)
         mode = int(bool(plain)) + 2 * int(bool(explicit))
         _check(lib().fwh_schur_apply(self.ctx.h, _ptr(p), _ptr(q), mode), "fwh_schur_apply")
@@ -314,6 +319,8 @@ class Solver:
 
     def ldl_solve_all(self, rhs, plain=False):
         v = np.ascontiguousarray(rhs, np.complex128).copy()
+        if v.shape != (int(self.info["n_dof"]),):
+            raise FwhError(f"rhs must have shape ({int(self.info['n_dof'])},)")
         _check(lib().fwh_ldl_solve_all(self.ctx.h, _ptr(v), int(bool(plain))), "fwh_ldl_solve_all")
         return v
 

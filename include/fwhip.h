@@ -116,6 +116,16 @@ typedef struct fwh_info {
   double ms_load;       /* the whole load (fwh_load / fwh_load_batch):
                            host work, uploads, assembly, factorisation and
                            the load-time kernel tuning                     */
+  double ms_factor_kernel; /* device time (HIP events) of the LDL^T factor
+                           kernel alone, all systems                       */
+  double ms_schur_fwd;  /* device time of the forward substitution
+                           W = L^-1 (-A_IG) of the explicit formation      */
+  double ms_schur_gemm; /* device time of its tile products
+                           S_s -= W^T D^-1 W                               */
+  double flops_schur_fwd;  /* complex-FP64 flops of that substitution:
+                           sum 8 ng (nlen - n)                             */
+  double flops_schur_gemm; /* and of the tile products: sum over subdomains
+                           8 nI ng (ng+1)/2 + 6 nI ng (nI interior rows)   */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */

@@ -1126,10 +1126,30 @@ static fwo_model *model_for_hddm(const fwh_mesh_desc *m) {
   return md;
 }
 
+static int hddm_solve_impl(const fwh_mesh_desc *m, const fwo_opts *o, const double *load,
+                           double *resid, double *x_out, double *u_out, fwo_stats *st);
+
 int fwo_hddm_solve(const fwh_mesh_desc *m, const fwo_opts *o, double *resid,
                    double *x_out, double *u_out, fwo_stats *st) {
+  return hddm_solve_impl(m, o, NULL, resid, x_out, u_out, st);
+}
+
+int fwo_hddm_solve_load(const fwh_mesh_desc *m, const fwo_opts *o, const double *load,
+                        double *resid, double *x_out, double *u_out, fwo_stats *st) {
+  return hddm_solve_impl(m, o, load, resid, x_out, u_out, st);
+}
+
+static int hddm_solve_impl(const fwh_mesh_desc *m, const fwo_opts *o, const double *load,
+                           double *resid, double *x_out, double *u_out, fwo_stats *st) {
   fwo_model *md = model_for_hddm(m);
   if (!md) return -1;
+  if (load) { /* the per-DOF load replaces every subdomain's assembled one */
+    size_t off = 0;
+    for (int s = 0; s < md->nsub; s++) {
+      memcpy(md->s[s].A.b, load + off, md->s[s].A.nf * sizeof(fwo_c));
+      off += 2 * (size_t)md->s[s].A.nf;
+    }
+  }
   if (o->perturb >= 2) {
     /* rounding-level perturbation of the load (a few ulps, seeded): used
      * only to measure how the iteration count reacts to rounding          */

@@ -118,6 +118,12 @@ typedef struct {
 int fwo_hddm_solve(const fwh_mesh_desc *m, const fwo_opts *o, double *resid,
                    double *x_out, double *u_out, fwo_stats *st);
 
+/* fwo_hddm_solve with the load replaced: load holds every subdomain's
+ * per-DOF load (2*n_s doubles each, concatenated in subdomain order, the
+ * layout of x_out); an adjoint solve A lambda = g uses it (A^T = A).      */
+int fwo_hddm_solve_load(const fwh_mesh_desc *m, const fwo_opts *o, const double *load,
+                        double *resid, double *x_out, double *u_out, fwo_stats *st);
+
 /* Time `n_iter` main-loop interface iterations on a built model (the
  * CPU baseline) with `nthreads` OpenMP threads over the subdomain Schur
  * steps (accumulated in the reference's order); returns seconds.         */

@@ -55,6 +55,8 @@ def lib():
         L.fwo_model_schur_sub.argtypes = [P, ctypes.c_int, P, P, P]
         L.fwo_hddm_solve.restype = ctypes.c_int
         L.fwo_hddm_solve.argtypes = [P, P, P, P, P, P]
+        L.fwo_hddm_solve_load.restype = ctypes.c_int
+        L.fwo_hddm_solve_load.argtypes = [P, P, P, P, P, P, P]
         L.fwo_hddm_time_iterations.restype = ctypes.c_double
         L.fwo_hddm_time_iterations.argtypes = [P, ctypes.c_int, P, ctypes.c_int]
         L.fwo_cocg_icc.restype = ctypes.c_int
@@ -182,10 +184,11 @@ class OracleModel:
 
 
 def hddm_solve(model, eps=1e-8, max_loop=100000, dom_solver=0, sol_eps=1e-9, ic_shift=1.2, perturb=0,
-               threads=1):
+               threads=1, load=None):
     """Full s-mode HDDM solve; returns dict(resid, x (per-subdomain list),
     u (owned interface values, part-major), stats).  threads > 1: the main
-    loop's subdomain Schur steps on OpenMP threads (same sums)."""
+    loop's subdomain Schur steps on OpenMP threads (same sums).  load: a
+    per-DOF load (complex, the layout of x_flat) replacing the assembled one."""
     d = model.desc()
     o = FwoOpts(eps, max_loop, dom_solver, sol_eps, ic_shift, 0, perturb, int(threads))
     resid = np.zeros(max_loop + 1)
@@ -193,8 +196,15 @@ def hddm_solve(model, eps=1e-8, max_loop=100000, dom_solver=0, sol_eps=1e-9, ic_
     x = np.zeros(int(nmp.sum()), np.complex128)
     u = np.zeros(int(np.asarray(model.infree).sum()), np.complex128)
     st = FwoStats()
-    rc = lib().fwo_hddm_solve(ctypes.byref(d), ctypes.byref(o), _p(resid), _p(x), _p(u),
-                              ctypes.byref(st))
+    if load is None:
+        rc = lib().fwo_hddm_solve(ctypes.byref(d), ctypes.byref(o), _p(resid), _p(x), _p(u),
+                                  ctypes.byref(st))
+    else:
+        f = np.ascontiguousarray(load, np.complex128)
+        if f.shape != x.shape:
+            raise ValueError("load must have one entry per subdomain DOF")
+        rc = lib().fwo_hddm_solve_load(ctypes.byref(d), ctypes.byref(o), _p(f), _p(resid), _p(x), _p(u),
+                                       ctypes.byref(st))
     if rc != 0:
         raise RuntimeError(f"fwo_hddm_solve failed ({rc})")
     offs = np.concatenate([[0], np.cumsum(nmp)])

@@ -73,3 +73,25 @@ def with_delta_gap(model, cells=8, sigma_gap=0.8):
     sigma = np.array(model.sigma, np.float64)
     sigma[touch] = sigma_gap
     return dataclasses.replace(model, dgv=dgv, sigma=sigma, name=model.name + "+gap")
+
+
+def iface_gids(model):
+    """per subdomain: the global interface unknown (part-major, owned DOFs in
+    ln_part order) of each of its interface DOFs, in ib_nfid order"""
+    goff = np.concatenate([[0], np.cumsum(model.infree)])
+    npart = model.npart
+    lnmap = []
+    for pp in range(npart):
+        tf = model.infree[pp] + model.outfree[pp]
+        m = -np.ones(tf, np.int64)
+        m[model.outfree[pp]:] = goff[pp] + np.arange(model.infree[pp])
+        lnmap.append(m)
+    for pp in range(npart):
+        for o in range(pp):
+            a = model.op_ibid[model.op_off[pp * npart + o]:model.op_off[pp * npart + o + 1]]
+            b = model.op_ibid[model.op_off[o * npart + pp]:model.op_off[o * npart + pp + 1]]
+            for lp, lo in zip(a, b):
+                if lnmap[pp][lp] < 0 and lo >= model.outfree[o]:
+                    lnmap[pp][lp] = lnmap[o][lo]
+    return [lnmap[model.sub_part[s]][model.ib_ln[model.ib_off[s]:model.ib_off[s + 1]]]
+            for s in range(model.nsub)]
