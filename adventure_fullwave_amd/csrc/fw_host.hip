@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -32,6 +33,33 @@
 #include "fw_kernels.h"
 
 using fw::SubMeta;
+
+// ---------------------------------------------------------------------------
+// Device scratch of the allocation-free element batch
+// (fwh_element_batch_static): inputs staged from the host, the packed element
+// output, its unpacked copy and the degenerate-element flag, for at most
+// FWH_STATIC_MAX_ELEM elements.  Byte offsets below, each a multiple of 256.
+// Host code reaches the array only through the per-device address resolved
+// by hipGetSymbolAddress in fwh_create.
+// ---------------------------------------------------------------------------
+namespace scr {
+constexpr size_t up(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t M = FWH_STATIC_MAX_ELEM;
+constexpr size_t VTX = 0;                                  // int   [4 M]
+constexpr size_t CRD = VTX + up(4 * M * sizeof(int));      // double[12 M]
+constexpr size_t JOE = CRD + up(12 * M * sizeof(double));  // double[24 M]
+constexpr size_t EPS = JOE + up(24 * M * sizeof(double));  // double[M]
+constexpr size_t SIG = EPS + up(M * sizeof(double));       // double[M]
+constexpr size_t PML = SIG + up(M * sizeof(double));       // double[6 M]
+constexpr size_t SRC = PML + up(6 * M * sizeof(double));   // int   [M]
+constexpr size_t OUT = SRC + up(M * sizeof(int));          // double2[27 M]
+constexpr size_t AE = OUT + up(27 * M * sizeof(double2));  // double[72 M]
+constexpr size_t BE = AE + up(72 * M * sizeof(double));    // double[12 M]
+constexpr size_t BAD = BE + up(12 * M * sizeof(double));   // int
+constexpr size_t BYTES = BAD + 256;
+}  // namespace scr
+
+static __device__ __align__(256) unsigned char scratch[scr::BYTES];
 
 namespace {
 
@@ -420,12 +448,40 @@ struct fwh_ctx {
 
 extern "C" const char *fwh_last_error(void) { return g_err.c_str(); }
 
+namespace {
+// per device: the address of that device's copy of `scratch` (each GPU holds
+// its own copy at its own address), resolved once with the device current
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int resolve_scratch(int device) {
+  if (device >= kMaxDevices) return fail(FWH_E_LIMIT, "device %d beyond the %d supported", device, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(FWH_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+  g_scratch_ptr[device] = ptr;
+  return 0;
+}
+
+void *scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 extern "C" int fwh_create(fwh_ctx **out, int device) {
   if (!out) return fail(FWH_E_ARG, "null ctx pointer");
   int ndev = 0;
   HIPCHK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(FWH_E_ARG, "device %d out of range (%d devices)", device, ndev);
   HIPCHK(hipSetDevice(device));
+  int rc = resolve_scratch(device);
+  if (rc) return rc;
   fwh_ctx *c = new fwh_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
@@ -2330,5 +2386,100 @@ extern "C" int fwh_cocg_icc_all(fwh_ctx *c, const double *b, double *x, int *sta
   tmp.st = nullptr;
   c->last_done = false;
   if (e != hipSuccess) return fail(FWH_E_HIP, "cocg/icc: %s", hipGetErrorString(e));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// fwh_element_batch without allocation: every device buffer is a region of
+// the file-scope `scratch` array (at the address resolved per device in
+// fwh_create); host arrays are copied straight in and out, and nothing on the
+// host or the device is allocated.
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void k_static_prep(int nel, int *vtx, double *joe, int *src, int *bad, int have_joe, int have_src) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *bad = 0;
+  if (i < 4 * nel) vtx[i] = i;  // element e uses rows 4e..4e+3 of the staged coordinates
+  if (!have_joe)
+    for (int k = i; k < 24 * nel; k += gridDim.x * blockDim.x) joe[k] = 0.0;
+  if (!have_src && i < nel) src[i] = 0;
+}
+
+// packed element output (21 entries of the upper triangle, row by row, then
+// the 6 loads) -> 6x6 complex row-major matrix and 6 complex loads
+__global__ void k_static_unpack(int nel, const double2 *out, double *ae, double *be) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int el = t / 36, q = t % 36;
+  if (el >= nel) return;
+  const int i = q / 6, j = q % 6;
+  const int a = i < j ? i : j, b = i < j ? j : i;
+  const int p = a * 6 - a * (a - 1) / 2 + (b - a);  // index of (a, b) in the packed rows
+  const double2 v = out[27 * (size_t)el + p];
+  ae[72 * (size_t)el + 2 * q] = v.x;
+  ae[72 * (size_t)el + 2 * q + 1] = v.y;
+  if (q < 6) {
+    const double2 l = out[27 * (size_t)el + 21 + q];
+    be[12 * (size_t)el + 2 * q] = l.x;
+    be[12 * (size_t)el + 2 * q + 1] = l.y;
+  }
+}
+}  // namespace
+
+extern "C" int fwh_element_batch_static(fwh_ctx *c, int nel, const double *xyz, const double *eps,
+                                        const double *sigma, double omega, const double *pml, const int *src,
+                                        const double *joe, double *ae, double *be) {
+  if (!c || nel < 0 || !xyz || !eps || !sigma || !ae || !be) return fail(FWH_E_ARG, "bad argument");
+  if (nel > FWH_STATIC_MAX_ELEM)
+    return fail(FWH_E_LIMIT, "%d elements exceed the %d of the allocation-free element batch", nel,
+                FWH_STATIC_MAX_ELEM);
+  if (nel == 0) return 0;
+  HIPCHK(hipSetDevice(c->device));
+  unsigned char *base = (unsigned char *)scratch_ptr(c->device);
+  if (!base) return fail(FWH_E_STATE, "scratch of device %d not resolved", c->device);
+  int *d_vtx = (int *)(base + scr::VTX);
+  double *d_crd = (double *)(base + scr::CRD);
+  double *d_joe = (double *)(base + scr::JOE);
+  double *d_eps = (double *)(base + scr::EPS);
+  double *d_sig = (double *)(base + scr::SIG);
+  double *d_pml = (double *)(base + scr::PML);
+  int *d_src = (int *)(base + scr::SRC);
+  double2 *d_out = (double2 *)(base + scr::OUT);
+  double *d_ae = (double *)(base + scr::AE);
+  double *d_be = (double *)(base + scr::BE);
+  int *d_bad = (int *)(base + scr::BAD);
+  const size_t n = (size_t)nel;
+  hipStream_t st = c->st;
+  HIPCHK(hipMemcpyAsync(d_crd, xyz, 12 * n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_eps, eps, n * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_sig, sigma, n * sizeof(double), hipMemcpyHostToDevice, st));
+  if (pml) HIPCHK(hipMemcpyAsync(d_pml, pml, 6 * n * sizeof(double), hipMemcpyHostToDevice, st));
+  if (joe) HIPCHK(hipMemcpyAsync(d_joe, joe, 24 * n * sizeof(double), hipMemcpyHostToDevice, st));
+  if (src) HIPCHK(hipMemcpyAsync(d_src, src, n * sizeof(int), hipMemcpyHostToDevice, st));
+  // the flag, the vertex index and any absent source arrays are set by a
+  // kernel (no memset node if a caller captures the stream)
+  k_static_prep<<<(int)((4 * n + 255) / 256), 256, 0, st>>>(nel, d_vtx, d_joe, d_src, d_bad, joe != nullptr,
+                                                           src != nullptr);
+  HIPCHK(hipGetLastError());
+  fw::AsmArgs aa{};
+  aa.nel = nel;
+  aa.el_vtx = d_vtx;
+  aa.crd = d_crd;
+  aa.joe = d_joe;
+  aa.eps = d_eps;
+  aa.sigma = d_sig;
+  aa.pml = pml ? d_pml : nullptr;
+  aa.src = d_src;
+  aa.omega = omega;
+  aa.elem_out = d_out;
+  aa.bad = d_bad;
+  HIPCHK(fw::launch_element(aa, st));
+  k_static_unpack<<<(int)((36 * n + 255) / 256), 256, 0, st>>>(nel, d_out, d_ae, d_be);
+  HIPCHK(hipGetLastError());
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(ae, d_ae, 72 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(be, d_be, 12 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (bad) return fail(FWH_E_MESH, "degenerate element in batch");
   return 0;
 }

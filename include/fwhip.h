@@ -251,6 +251,20 @@ int  fwh_element_batch(fwh_ctx *ctx, int nel, const double *xyz,
                        const double *pml, const int *src, const double *joe,
                        double *ae, double *be);
 
+/* fwh_element_batch without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as fwh_element_batch, at
+ * most FWH_STATIC_MAX_ELEM elements per call (more: FWH_E_LIMIT).  Its
+ * device buffers are one fixed array per GPU (about 6 MB, held until the
+ * process ends), shared by every call on that GPU: calls that overlap on one
+ * GPU (from several threads or contexts) race and must be serialised by the
+ * caller.  Synchronous: the results are in ae/be on return.               */
+#define FWH_STATIC_MAX_ELEM 4096
+int  fwh_element_batch_static(fwh_ctx *ctx, int nel, const double *xyz,
+                              const double *eps, const double *sigma,
+                              double omega, const double *pml, const int *src,
+                              const double *joe, double *ae, double *be);
+
 /* Subdomain COCG with IC(0) (mymt_CS_AIJ_Solver_COCG,
  * lib/mymt_aij_solver/aij_solver_cg.h:9-44,113-355 with
  * mymt_CS_AIJ_PC_ICC_make/set, aij_pc_func.h:36-167) on every subdomain's
