@@ -22,7 +22,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 HIP_SOURCES = ["fw_kernels.hip", "fw_solve.hip", "fw_schur.hip", "fw_factor.hip", "fw_host.hip"]
-C_SOURCES = ["hddm_driver.c"]
+C_SOURCES = ["hddm_driver.c", "fw_refapi.c"]
 
 
 def _run(cmd):
@@ -44,7 +44,8 @@ def build(verbose: bool = False) -> str:
         objs.append(o)
     for src in C_SOURCES:
         o = os.path.join(OBJ, src.replace(".c", ".o"))
-        _run(["gcc", "-O2", "-std=c99", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", o])
+        _run(["gcc", "-O2", "-std=c99", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", "-c",
+              os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])

@@ -107,6 +107,29 @@ double now_ms() {
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// row blocks of one skyline (rows 0..n-1, row i starting at rowbeg[i],
+// nlen entries in all): <= BMAX rows and <= cap entries each, padded to an
+// even number of blocks with empty blocks [n, n) (the solve kernel sweeps
+// block pairs without conditional loads); the block start rows, 0 first and
+// n last.  -1 when a row alone exceeds cap.
+int partition_rows(int n, const int *rowbeg, long long nlen, int cap, std::vector<int> &out) {
+  int i0 = 0;
+  out.assign(1, 0);
+  while (i0 < n) {
+    int i1 = i0;
+    while (i1 < n && i1 - i0 < fw::BMAX) {
+      const long long e1 = (i1 + 1 < n) ? rowbeg[i1 + 1] : nlen;
+      if (e1 - rowbeg[i0] > cap) break;
+      i1++;
+    }
+    if (i1 == i0) return -1;
+    out.push_back(i1);
+    i0 = i1;
+  }
+  if (((int)out.size() - 1) % 2 == 1) out.push_back(n);
+  return 0;
+}
+
 // temporary device buffers of one call, freed on every exit path
 struct TempDev {
   std::vector<void *> p;
@@ -447,6 +470,11 @@ struct fwh_ctx {
 };
 
 extern "C" const char *fwh_last_error(void) { return g_err.c_str(); }
+
+// the C part of the library (fw_refapi.c) records its failures here too
+extern "C" __attribute__((visibility("hidden"))) int fwhi_fail(int code, const char *msg) {
+  return fail(code, "%s", msg);
+}
 
 namespace {
 // per device: the address of that device's copy of `scratch` (each GPU holds
@@ -1061,21 +1089,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   // kernel sweeps block pairs without conditional loads); the block start
   // rows, 0 first and n last
   auto partition = [&](const SubMeta &M, int cap, std::vector<int> &out) -> int {
-    int i0 = 0;
-    out.assign(1, 0);
-    while (i0 < M.n) {
-      int i1 = i0;
-      while (i1 < M.n && i1 - i0 < fw::BMAX) {
-        const long long e1 = (i1 + 1 < M.n) ? rowbeg[M.dof0 + i1 + 1] : M.nlen;
-        if (e1 - rowbeg[M.dof0 + i0] > cap) break;
-        i1++;
-      }
-      if (i1 == i0) return -1;
-      out.push_back(i1);
-      i0 = i1;
-    }
-    if (((int)out.size() - 1) % 2 == 1) out.push_back(M.n);
-    return 0;
+    return partition_rows(M.n, rowbeg.data() + M.dof0, M.nlen, cap, out);
   };
   // per subdomain on host threads: row starts, skyline length, both
   // partitions and the factorisation flops (8 per complex multiply-add;
@@ -2387,6 +2401,117 @@ extern "C" int fwh_cocg_icc_all(fwh_ctx *c, const double *b, double *x, int *sta
   c->last_done = false;
   if (e != hipSuccess) return fail(FWH_E_HIP, "cocg/icc: %s", hipGetErrorString(e));
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// One skyline matrix in the reference's DMatrix layout (advlas_skyline.c):
+// row i holds columns idx1[i]..i at mat[idx2[i] + j], rows contiguous
+// (idx2[i] + idx1[i] = sum over k < i of (k - idx1[k] + 1)).  The matrix is
+// staged as a one-subdomain model and goes through the batched kernels:
+// factor k_factor16, solve k_blockinv + k_solve MODE_RHS.
+// ---------------------------------------------------------------------------
+namespace {
+int sky_one(fwh_ctx *c, int n, const int *idx1, const int *idx2, double *mat, double *v, double *dinv_out,
+            bool factor) {
+  if (!c || n <= 0 || !idx1 || !idx2 || !mat || (!factor && !v)) return fail(FWH_E_ARG, "bad argument");
+  std::vector<int> rb(n), blk;
+  long long run = 0;
+  for (int i = 0; i < n; i++) {
+    if (idx1[i] < 0 || idx1[i] > i) return fail(FWH_E_ARG, "idx1[%d] = %d outside [0, %d]", i, idx1[i], i);
+    if ((long long)idx2[i] + idx1[i] != run)
+      return fail(FWH_E_ARG, "row %d of the skyline does not follow row %d (idx2[%d] = %d)", i, i - 1, i, idx2[i]);
+    rb[i] = (int)run;
+    run += i - idx1[i] + 1;
+    if (run > (1LL << 31) - 1) return fail(FWH_E_LIMIT, "skyline too long");
+  }
+  const long long nlen = run;
+  if (partition_rows(n, rb.data(), nlen, fw::SOLVE_THREADS * fw::PFK, blk))
+    return fail(FWH_E_LIMIT, "a profile row is wider than %d entries", fw::SOLVE_THREADS * fw::PFK);
+  const int nblk = (int)blk.size() - 1;
+  if (!factor && fw::solve_lds_bytes(n, 0, nblk, fw::PFK) > 160 * 1024)
+    return fail(FWH_E_LIMIT, "a skyline of %d rows exceeds the LDS budget of the solve kernel", n);
+  HIPCHK(hipSetDevice(c->device));
+  SubMeta m{};
+  m.n = n;
+  m.nblk = nblk;
+  m.nlen = nlen;
+  m.blk0_s = 0;
+  TempDev t;
+  SubMeta *d_meta;
+  int *d_idx1, *d_rb, *d_blk;
+  double2 *d_fac, *d_dinv;
+  HIPCHK(t.alloc(&d_meta, 1));
+  HIPCHK(t.alloc(&d_idx1, (size_t)n));
+  HIPCHK(t.alloc(&d_rb, (size_t)n));
+  HIPCHK(t.alloc(&d_blk, blk.size()));
+  HIPCHK(t.alloc(&d_fac, (size_t)nlen));
+  HIPCHK(t.alloc(&d_dinv, (size_t)n));
+  hipStream_t st = c->st;
+  HIPCHK(hipMemcpyAsync(d_meta, &m, sizeof(SubMeta), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_idx1, idx1, n * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_rb, rb.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_blk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_fac, mat, nlen * sizeof(double2), hipMemcpyHostToDevice, st));
+  if (factor) {
+    double2 *d_m16;
+    long long *d_mfoff;
+    const long long zero = 0;
+    HIPCHK(t.alloc(&d_m16, (size_t)((n + 15) / 16) * 256));
+    HIPCHK(t.alloc(&d_mfoff, 1));
+    HIPCHK(hipMemcpyAsync(d_mfoff, &zero, sizeof(long long), hipMemcpyHostToDevice, st));
+    fw::FactorArgs fa{};
+    fa.meta = d_meta;
+    fa.idx1 = d_idx1;
+    fa.rowbeg = d_rb;
+    fa.fac = d_fac;
+    fa.dinv = d_dinv;
+    fa.nmax = n;
+    fa.m16 = d_m16;
+    fa.mf_off = d_mfoff;
+    HIPCHK(fw::launch_factor16(fa, 1, st));
+    HIPCHK(hipMemcpyAsync(mat, d_fac, nlen * sizeof(double2), hipMemcpyDeviceToHost, st));
+    if (dinv_out) HIPCHK(hipMemcpyAsync(dinv_out, d_dinv, n * sizeof(double2), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  }
+  // solve: 1/d from the factor's diagonal, the inverted diagonal triangles
+  // of the row blocks, then the forward / diagonal / backward sweeps
+  std::vector<double2> dg(n);
+  for (int i = 0; i < n; i++) dg[i] = make_double2(mat[2 * ((size_t)idx2[i] + i)], mat[2 * ((size_t)idx2[i] + i) + 1]);
+  double2 *d_minv, *d_x;
+  HIPCHK(t.alloc(&d_minv, (size_t)nblk * fw::MINV));
+  HIPCHK(t.alloc(&d_x, (size_t)n));
+  HIPCHK(hipMemcpyAsync(d_dinv, dg.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_x, v, n * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIPCHK(fw::launch_blockinv(d_meta, 1, d_idx1, d_rb, d_blk, d_fac, d_minv, st));
+  fw::SolveArgs a{};
+  a.sys0 = 0;
+  a.nsys = 1;
+  a.meta = d_meta;
+  a.mode = fw::MODE_RHS;
+  a.nmax = n;
+  a.nblkmax = nblk;
+  a.pfk = fw::PFK;
+  a.idx1 = d_idx1;
+  a.rowbeg = d_rb;
+  a.fac = d_fac;
+  a.dinv = d_dinv;
+  a.blk_row = d_blk;
+  a.minv = d_minv;
+  a.X = d_x;
+  HIPCHK(fw::launch_solve(a, 1, st));
+  HIPCHK(hipMemcpyAsync(v, d_x, n * sizeof(double2), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+}  // namespace
+
+extern "C" int fwh_sky_factor(fwh_ctx *c, int n, const int *idx1, const int *idx2, double *mat, double *dinv) {
+  return sky_one(c, n, idx1, idx2, mat, nullptr, dinv, true);
+}
+
+extern "C" int fwh_sky_solve(fwh_ctx *c, int n, const int *idx1, const int *idx2, const double *mat, double *v) {
+  return sky_one(c, n, idx1, idx2, const_cast<double *>(mat), v, nullptr, false);
 }
 
 // ---------------------------------------------------------------------------

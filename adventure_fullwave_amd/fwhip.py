@@ -27,7 +27,11 @@ EXPORTS = ("fwh_create", "fwh_destroy", "fwh_last_error", "fwh_load", "fwh_get_i
            "fwh_first", "fwh_iterate", "fwh_last", "fwh_get_stats", "fwh_get_x", "fwh_get_u",
            "fwh_get_resid", "fwh_schur_apply", "fwh_ldl_solve_all", "fwh_time_kernels", "fwh_get_sub",
            "fwh_element_batch", "fwh_cocg_icc_all", "fwh_schur_form", "fwh_post", "fwh_set_load",
-           "fwh_grad_eps", "fwh_load_batch", "fwh_select", "fwh_element_batch_static")
+           "fwh_grad_eps", "fwh_load_batch", "fwh_select", "fwh_element_batch_static", "fwh_sky_factor",
+           "fwh_sky_solve")
+# the reference-shaped call surface (include/fwh_ref.h)
+REF_EXPORTS = ("hddm_solver_CS_hip", "hddm_solver_CS_hip_status", "fwh_ref_desc_from_partmesh",
+               "fwh_ref_desc_free", "advlas_skyline_ldl_decomposite", "advlas_skyline_ldl_solve")
 
 
 class FwhOpts(ctypes.Structure):
@@ -113,6 +117,8 @@ def lib():
         L.fwh_get_sub.argtypes = [P, I, I, P]
         L.fwh_element_batch.argtypes = [P, I, P, P, P, D, P, P, P, P, P]
         L.fwh_element_batch_static.argtypes = [P, I, P, P, P, D, P, P, P, P, P]
+        L.fwh_sky_factor.argtypes = [P, I, P, P, P, P]
+        L.fwh_sky_solve.argtypes = [P, I, P, P, P, P]
         L.fwh_cocg_icc_all.argtypes = [P, P, P, P, D, D]
         for name in EXPORTS:
             if name not in ("fwh_destroy", "fwh_last_error"):

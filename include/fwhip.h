@@ -227,6 +227,19 @@ int  fwh_schur_form(fwh_ctx *ctx);
 /* advlas_skyline_ldl_solve (lib/hddm/advlas_skyline.c:134-167) applied to
  * every subdomain at once: rhs/x are 2*n_dof doubles (in place).          */
 int  fwh_ldl_solve_all(fwh_ctx *ctx, double *rhs, int plain);
+/* One skyline matrix in the reference's DMatrix layout (lib/std/types.h:
+ * 143-157): n rows, row i holding columns idx1[i]..i at mat[idx2[i] + j]
+ * (2 doubles each), rows contiguous.  fwh_sky_factor replaces
+ * advlas_skyline_ldl_decomposite (lib/hddm/advlas_skyline.c:79-128): mat is
+ * overwritten by L (strictly lower) with 1/d on the diagonal; dinv (2n
+ * doubles, may be NULL) receives 1/d.  fwh_sky_solve replaces
+ * advlas_skyline_ldl_solve (:134-167): v (2n doubles) in place, mat the
+ * factor.  Both stage the matrix on the device and run the batched kernels
+ * on it (pivot-free complex-symmetric LDL^T, Smith reciprocals).         */
+int  fwh_sky_factor(fwh_ctx *ctx, int n, const int *idx1, const int *idx2,
+                    double *mat, double *dinv);
+int  fwh_sky_solve(fwh_ctx *ctx, int n, const int *idx1, const int *idx2,
+                   const double *mat, double *v);
 /* Device time (HIP events on the library stream) of one batched LDL^T solve
  * of all subdomains, of one batched implicit Schur-step kernel (Dirichlet
  * load, solve, interface rows of A x) and of one explicit Schur matvec

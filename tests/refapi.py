@@ -1,0 +1,125 @@
+"""ctypes front end of the test driver tests/refdrv.c (reference-shaped
+structs -> hddm_solver_CS_hip / fwh_ref_desc_from_partmesh / advlas_*).
+TEST INFRASTRUCTURE."""
+import ctypes
+import os
+
+import numpy as np
+
+from adventure_fullwave_amd.mesh import FwhMeshDesc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class RefdrvMtrl(ctypes.Structure):
+    _fields_ = [("nflag", ctypes.c_int), ("flag_i", ctypes.c_void_p), ("flag_nu", ctypes.c_void_p),
+                ("nR", ctypes.c_int), ("R_i", ctypes.c_void_p), ("R_sigma", ctypes.c_void_p),
+                ("nant", ctypes.c_int), ("ant_flag", ctypes.c_void_p), ("omega", ctypes.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        from adventure_fullwave_amd import fwhip
+        fwhip.lib()  # the same libfwhip.so instance first
+        L = ctypes.CDLL(os.path.join(HERE, "librefdrv.so"))
+        P = ctypes.c_void_p
+        L.refdrv_desc.argtypes = [P, P, P, P]
+        L.refdrv_desc.restype = ctypes.c_int
+        L.refdrv_desc_free.argtypes = [P]
+        L.refdrv_desc_free.restype = None
+        L.refdrv_solve.argtypes = [P, P, P, ctypes.c_double, ctypes.c_int, ctypes.c_char_p, P, P, P, P, P]
+        L.refdrv_solve.restype = ctypes.c_int
+        L.refdrv_advlas.argtypes = [ctypes.c_int, P, P, P, P, P]
+        L.refdrv_advlas.restype = None
+        _LIB = L
+    return _LIB
+
+
+class Materials:
+    """element flags and MTRL tables for a flat model: one flag per distinct
+    (eps, sigma, source) triple; Permittivity and Conductor entries for every
+    flag, an Antenna entry for the source flags"""
+
+    def __init__(self, model):
+        key = np.stack([np.asarray(model.eps, np.float64), np.asarray(model.sigma, np.float64),
+                        np.asarray(model.src, np.float64)], axis=1)
+        uniq, inv = np.unique(key, axis=0, return_inverse=True)
+        self.flag = np.ascontiguousarray(inv.ravel() + 1, np.int32)
+        ids = np.arange(1, len(uniq) + 1, dtype=np.int32)
+        self.flag_i = ids
+        self.flag_nu = np.ascontiguousarray(uniq[:, 0])
+        self.R_i = ids
+        self.R_sigma = np.ascontiguousarray(uniq[:, 1])
+        self.ant_flag = np.ascontiguousarray(ids[uniq[:, 2] != 0], np.int32)
+        self.m = RefdrvMtrl(len(ids), self.flag_i.ctypes.data, self.flag_nu.ctypes.data, len(ids),
+                            self.R_i.ctypes.data, self.R_sigma.ctypes.data, len(self.ant_flag),
+                            self.ant_flag.ctypes.data if len(self.ant_flag) else None, float(model.omega))
+
+
+def translated_desc(model):
+    """the flat description fwh_ref_desc_from_partmesh builds from the
+    reference-shaped structs of `model`, as numpy arrays"""
+    mats = Materials(model)
+    d = model.desc()
+    out = FwhMeshDesc()
+    rc = lib().refdrv_desc(ctypes.byref(d), mats.flag.ctypes.data, ctypes.byref(mats.m), ctypes.byref(out))
+    if rc != 0:
+        from adventure_fullwave_amd import fwhip
+        raise RuntimeError(f"fwh_ref_desc_from_partmesh: {rc}: {fwhip.lib().fwh_last_error()}")
+    nel = int(model.el_off[-1])
+    nap = int(model.ap_off[-1])
+    nbc = int(model.bc_off[-1])
+    nib = int(model.ib_off[-1])
+    npart, nsub = out.npart, out.nsub
+
+    def arr(ptr, n, dt):
+        if n == 0 or not ptr:
+            return np.zeros(0, dt)
+        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                     shape=(n,)).copy()
+    res = dict(npart=npart, nsub=nsub, omega=out.omega,
+               sub_part=arr(out.sub_part, nsub, np.int32), el_off=arr(out.el_off, nsub + 1, np.int32),
+               nop=arr(out.nop, 10 * nel, np.int32), eps=arr(out.eps, nel, np.float64),
+               sigma=arr(out.sigma, nel, np.float64), src=arr(out.src, nel, np.int32),
+               ap_off=arr(out.ap_off, nsub + 1, np.int32), crd=arr(out.crd, 3 * nap, np.float64),
+               joe=arr(out.joe, 6 * nap, np.float64), nmp=arr(out.nmp, nsub, np.int32),
+               bc_off=arr(out.bc_off, nsub + 1, np.int32), ibc=arr(out.ibc, nbc, np.int32),
+               ib_off=arr(out.ib_off, nsub + 1, np.int32), ib_nfid=arr(out.ib_nfid, nib, np.int32),
+               ib_ln=arr(out.ib_ln, nib, np.int32), infree=arr(out.infree, npart, np.int32),
+               outfree=arr(out.outfree, npart, np.int32),
+               op_off=arr(out.op_off, npart * npart + 1, np.int32),
+               dgv=None if not out.dgv else arr(out.dgv, nap, np.float64), pml=out.pml)
+    res["op_ibid"] = arr(out.op_ibid, int(res["op_off"][-1]), np.int32)
+    lib().refdrv_desc_free(ctypes.byref(out))
+    return res
+
+
+def solve(model, eps=1e-8, max_loop=100000, logpath=None):
+    """hddm_solver_CS_hip on the reference-shaped structs of `model`:
+    (rc, stats, x, B, E, J)"""
+    from adventure_fullwave_amd import fwhip
+    mats = Materials(model)
+    d = model.desc()
+    ndof = int(np.sum(model.nmp))
+    nel = int(model.el_off[-1])
+    x = np.zeros(ndof, np.complex128)
+    B, E, J = (np.zeros((nel, 3), np.complex128) for _ in range(3))
+    st = fwhip.FwhStats()
+    rc = lib().refdrv_solve(ctypes.byref(d), mats.flag.ctypes.data, ctypes.byref(mats.m), float(eps), int(max_loop),
+                            None if logpath is None else logpath.encode(), x.ctypes.data, B.ctypes.data,
+                            E.ctypes.data, J.ctypes.data, ctypes.byref(st))
+    return rc, st.as_dict(), x, B, E, J
+
+
+def advlas(idx1, idx2, mat, rhs):
+    """advlas_skyline_ldl_decomposite then _solve: (factor, 1/d, solution)"""
+    idx1 = np.ascontiguousarray(idx1, np.int32).copy()
+    idx2 = np.ascontiguousarray(idx2, np.int32).copy()
+    m = np.ascontiguousarray(mat, np.complex128).copy()
+    v = np.ascontiguousarray(rhs, np.complex128).copy()
+    work = np.zeros(len(idx1), np.complex128)
+    lib().refdrv_advlas(len(idx1), idx1.ctypes.data, idx2.ctypes.data, m.ctypes.data, work.ctypes.data,
+                        v.ctypes.data)
+    return m, work, v

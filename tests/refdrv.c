@@ -1,0 +1,203 @@
+/*
+ * refdrv.c - test driver for the reference-shaped call surface of
+ * libfwhip (include/fwh_ref.h).  TEST INFRASTRUCTURE.
+ *
+ * Builds the reference's own structs from a flat model description, the
+ * way hddm_Init leaves them for hddm_solver_CS (lib/hddm/make_PartMesh.c:
+ * 58-117, lib/std/types.h): per part a PartMesh with its MTRL tables (element
+ * flags -> Permittivity / Conductor / Antenna entries), OPinfo lists and
+ * interface counts; per subdomain a DomMesh with nop / flag / crd / nmp,
+ * dbc_Axn0 = [essential edges | interface edges], Inbc entries and one
+ * ANTENNA per antenna definition (enf: local vertex -> antenna node, Jo /
+ * Jor per node).  Then calls hddm_solver_CS_hip_status and copies DomMesh.x_c
+ * and DomMesh.rvalue back out.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/fwh_ref.h"
+
+typedef struct {
+  int nflag;            /* Permittivity entries                            */
+  const int *flag_i;
+  const double *flag_nu;
+  int nR;               /* Conductor entries                               */
+  const int *R_i;
+  const double *R_sigma;
+  int nant;             /* Antenna entries (their element flags)           */
+  const int *ant_flag;
+  double omega;
+} refdrv_mtrl;
+
+static void free_parts(PartMesh *pm, int npart) {
+  for (int p = 0; p < npart; p++) {
+    for (int k = 0; k < pm[p].ndomain; k++) {
+      DomMesh *dm = &pm[p].dm[k];
+      free(dm->nop); free(dm->flag); free(dm->crd); free(dm->inbc); free(dm->dbc_Axn0.ibc);
+      for (int a = 0; a < pm[p].mtrl.nantenna; a++) {
+        free(dm->antenna[a].enf); free(dm->antenna[a].Jo); free(dm->antenna[a].Jor);
+      }
+      free(dm->antenna);
+      free(dm->x_c); free(dm->rvalue.el_B_c); free(dm->rvalue.el_E_c); free(dm->rvalue.el_Je_c);
+    }
+    for (int q = 0; q < pm[p].npart; q++) free(pm[p].op[q].ibid);
+    free(pm[p].op);
+    free(pm[p].dm);
+    free(pm[p].mtrl.antenna_def);
+  }
+  free(pm);
+}
+
+/* the reference-shaped parts of a flat model; flag: per element material flag */
+static PartMesh *make_parts(const fwh_mesh_desc *f, const int *flag, const refdrv_mtrl *mt) {
+  const int np = f->npart;
+  PartMesh *pm = (PartMesh *)calloc(np, sizeof(PartMesh));
+  for (int p = 0; p < np; p++) {
+    PartMesh *P = &pm[p];
+    P->partid = p;
+    P->npart = np;
+    P->infree = f->infree[p];
+    P->outfree = f->outfree[p];
+    P->tfree = f->infree[p] + f->outfree[p];
+    P->ap_elem = 4;
+    P->nd_elem = 10;
+    P->dimension = 3;
+    P->mtrl.nflag = mt->nflag;
+    P->mtrl.flag_i = (int *)mt->flag_i;
+    P->mtrl.flag_nu = (double *)mt->flag_nu;
+    P->mtrl.nR = mt->nR;
+    P->mtrl.R_i = (int *)mt->R_i;
+    P->mtrl.R_sigma = (double *)mt->R_sigma;
+    P->mtrl.nantenna = mt->nant;
+    P->mtrl.antenna_def = (MTRL_ANTENNA_DEFINITION *)calloc(mt->nant ? mt->nant : 1, sizeof(MTRL_ANTENNA_DEFINITION));
+    for (int a = 0; a < mt->nant; a++) P->mtrl.antenna_def[a].flag = mt->ant_flag[a];
+    P->mtrl.nantenna_omega = 1;
+    P->mtrl.antenna_omega = mt->omega;
+    P->op = (OPinfo *)calloc(np, sizeof(OPinfo));
+    for (int q = 0; q < np; q++) {
+      const int o0 = f->op_off[p * np + q], o1 = f->op_off[p * np + q + 1];
+      P->op[q].n_mofree = o1 - o0;
+      P->op[q].ibid = (int *)malloc((o1 - o0 + 1) * sizeof(int));
+      memcpy(P->op[q].ibid, f->op_ibid + o0, (o1 - o0) * sizeof(int));
+    }
+    for (int s = 0; s < f->nsub; s++) P->ndomain += f->sub_part[s] == p;
+    P->dm = (DomMesh *)calloc(P->ndomain ? P->ndomain : 1, sizeof(DomMesh));
+  }
+  int *next = (int *)calloc(np, sizeof(int));
+  for (int s = 0; s < f->nsub; s++) {
+    const int p = f->sub_part[s];
+    DomMesh *dm = &pm[p].dm[next[p]];
+    dm->partid = p;
+    dm->domid = next[p]++;
+    dm->npart = np;
+    dm->ndomain = pm[p].ndomain;
+    const int e0 = f->el_off[s], ne = f->el_off[s + 1] - e0;
+    const int a0 = f->ap_off[s], na = f->ap_off[s + 1] - a0;
+    dm->nel = ne;
+    dm->nap = na;
+    dm->nmp = f->nmp[s];
+    dm->nnd = na + f->nmp[s];
+    dm->ap_elem = 4;
+    dm->nd_elem = 10;
+    dm->dimension = 3;
+    strcpy(dm->elem_type, "3DQuadraticTetrahedron");
+    dm->nop = (int *)malloc((10 * (size_t)ne + 1) * sizeof(int));
+    memcpy(dm->nop, f->nop + 10 * (size_t)e0, 10 * (size_t)ne * sizeof(int));
+    dm->flag = (int *)malloc((ne + 1) * sizeof(int));
+    memcpy(dm->flag, flag + e0, ne * sizeof(int));
+    dm->crd = (double *)malloc((3 * (size_t)na + 1) * sizeof(double));
+    memcpy(dm->crd, f->crd + 3 * (size_t)a0, 3 * (size_t)na * sizeof(double));
+    const int b0 = f->bc_off[s], nb = f->bc_off[s + 1] - b0;
+    const int i0 = f->ib_off[s], ni = f->ib_off[s + 1] - i0;
+    dm->ninbc = ni;
+    dm->inbc = (Inbc *)calloc(ni ? ni : 1, sizeof(Inbc));
+    dm->dbc_Axn0.need = 0;
+    dm->dbc_Axn0.nbc = nb + ni;
+    dm->dbc_Axn0.ibc = (int *)malloc((nb + ni + 1) * sizeof(int));
+    memcpy(dm->dbc_Axn0.ibc, f->ibc + b0, nb * sizeof(int));
+    dm->nbc_A = nb;
+    for (int i = 0; i < ni; i++) {
+      dm->inbc[i].nfid = f->ib_nfid[i0 + i];
+      dm->inbc[i].ln_part = f->ib_ln[i0 + i];
+      dm->inbc[i].snid = -1;
+      dm->inbc[i].charge_part = -1;
+      dm->dbc_Axn0.ibc[nb + i] = f->ib_nfid[i0 + i];
+    }
+    /* the current source: Joe = -(Jo + i Jor) per vertex, one antenna node per vertex */
+    dm->antenna = (ANTENNA *)calloc(mt->nant ? mt->nant : 1, sizeof(ANTENNA));
+    for (int a = 0; a < mt->nant; a++) {
+      ANTENNA *an = &dm->antenna[a];
+      an->antenna = a;
+      an->nf = na;
+      an->enf = (int *)malloc((na + 1) * sizeof(int));
+      an->Jo = (double *)calloc(3 * (size_t)na + 1, sizeof(double));
+      an->Jor = (double *)calloc(3 * (size_t)na + 1, sizeof(double));
+      for (int v = 0; v < na; v++) {
+        an->enf[v] = v;
+        for (int j = 0; j < 3; j++) {
+          an->Jo[3 * v + j] = f->joe ? -f->joe[6 * ((size_t)a0 + v) + 2 * j] : 0.0;
+          an->Jor[3 * v + j] = f->joe ? -f->joe[6 * ((size_t)a0 + v) + 2 * j + 1] : 0.0;
+        }
+      }
+    }
+  }
+  free(next);
+  return pm;
+}
+
+/* the flat description hddm_solver_CS_hip would load from those structs */
+int refdrv_desc(const fwh_mesh_desc *f, const int *flag, const refdrv_mtrl *mt, fwh_mesh_desc *out) {
+  PartMesh *pm = make_parts(f, flag, mt);
+  const int rc = fwh_ref_desc_from_partmesh(pm, out);
+  free_parts(pm, f->npart);
+  return rc;
+}
+
+void refdrv_desc_free(fwh_mesh_desc *d) { fwh_ref_desc_free(d); }
+
+/* hddm_solver_CS_hip on those structs: x (2 n_dof doubles), B/E/J (6 per
+ * element) from DomMesh.x_c / rvalue; the residual log goes to logpath   */
+int refdrv_solve(const fwh_mesh_desc *f, const int *flag, const refdrv_mtrl *mt, double eps, int max_loop,
+                 const char *logpath, double *x, double *B, double *E, double *J, fwh_stats *st) {
+  PartMesh *pm = make_parts(f, flag, mt);
+  OPTIONS options;
+  memset(&options, 0, sizeof(options));
+  options.hddm_opt.hddm_eps = eps;
+  options.hddm_opt.hddm_max_loop = max_loop;
+  options.hddm_opt.hddm_pc = 1;
+  options.hddm_opt.hddm_log = logpath ? 0 : -1;
+  options.hddm_opt.fp = logpath ? fopen(logpath, "w") : NULL;
+  const int rc = hddm_solver_CS_hip_status(pm, options, st);
+  if (options.hddm_opt.fp) fclose(options.hddm_opt.fp);
+  if (rc == 0 && max_loop > 0) {
+    size_t xo = 0, eo = 0;
+    for (int p = 0; p < f->npart; p++)
+      for (int k = 0; k < pm[p].ndomain; k++) {
+        const DomMesh *dm = &pm[p].dm[k];
+        memcpy(x + 2 * xo, dm->x_c, dm->nmp * sizeof(Complex));
+        memcpy(B + 6 * eo, dm->rvalue.el_B_c, 3 * (size_t)dm->nel * sizeof(Complex));
+        memcpy(E + 6 * eo, dm->rvalue.el_E_c, 3 * (size_t)dm->nel * sizeof(Complex));
+        memcpy(J + 6 * eo, dm->rvalue.el_Je_c, 3 * (size_t)dm->nel * sizeof(Complex));
+        xo += dm->nmp;
+        eo += dm->nel;
+      }
+  }
+  free_parts(pm, f->npart);
+  return rc;
+}
+
+/* advlas_skyline_ldl_decomposite then advlas_skyline_ldl_solve on one
+ * DMatrix: mat (2 nlen doubles) is factored in place, work gets 1/d, v is
+ * solved in place                                                        */
+void refdrv_advlas(int n, int *idx1, int *idx2, double *mat, double *work, double *v) {
+  DMatrix dm;
+  memset(&dm, 0, sizeof(dm));
+  dm.ndim = n;
+  dm.nlen = idx2[n - 1] + n;
+  dm.idx1 = idx1;
+  dm.idx2 = idx2;
+  dm.mat_dc = (double _Complex *)mat;
+  advlas_skyline_ldl_decomposite(dm, (double _Complex *)work);
+  advlas_skyline_ldl_solve(dm, (double _Complex *)v);
+}

@@ -9,23 +9,26 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared():
-    names = set()
-    for h in ("fwhip.h",):
-        txt = open(os.path.join(ROOT, "include", h)).read()
-        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        names |= set(re.findall(r"\b(fwh_[a-z0-9_]+)\s*\(", txt))
-    return sorted(names)
+def declared(header, pattern):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(pattern + r"\s*\(", txt)))
 
 
 def test_library_exports_declared_symbols():
     from adventure_fullwave_amd import fwhip
     lib = ctypes.CDLL(fwhip.LIB_PATH)
-    names = declared()
+    names = declared("fwhip.h", r"\b(fwh_[a-z0-9_]+)")
     assert len(names) >= 15
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(names) == set(fwhip.EXPORTS)
+    # the reference-shaped surface: every function include/fwh_ref.h declares
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "fwh_ref.h")).read(), flags=re.S)
+    ref = sorted(set(re.findall(r"^(?:void|int)\s+([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt, flags=re.M)))
+    assert set(ref) == set(fwhip.REF_EXPORTS), ref
+    missing = [n for n in ref if not hasattr(lib, n)]
+    assert not missing, missing
 
 
 def test_error_without_gpu_is_loud():

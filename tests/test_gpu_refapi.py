@@ -1,0 +1,140 @@
+"""The reference-shaped interface on the GPU (run with -m gpu).
+
+hddm_solver_CS_hip, called by the C driver tests/refdrv.c with the
+reference's own structs (PartMesh / DomMesh / MTRL / OPinfo / Inbc /
+DirichletBC / ANTENNA, lib/std/types.h) as hddm_Init leaves them, must give
+the same bits as fwh_solve on the flat description: the per-subdomain
+solutions written to DomMesh.x_c, the element fields written to
+DomMesh.rvalue (post_sp), and the residual history written to
+HDDM_Opt.fp in the reference's log format (hddm_solver_s.h:205,321).  The
+load-time kernel choices that set summation orders are pinned for both
+(FWH_SMV_WAVES, FWH_SOLVE_DEPTH).  advlas_skyline_ldl_decomposite / _solve
+with DMatrix arguments against the oracle's restatement of
+advlas_skyline.c:79-167."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import fw_oracle as fo
+from tests import refapi
+from tests.helpers import with_delta_gap
+from tests.test_oracle_skyline import dense_from_sky, random_profile
+from tests.test_refapi import write_volt
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture()
+def pinned(tmp_path):
+    """pinned kernel choices and a working directory with MODEL/ (volt.dat)"""
+    keep = {k: os.environ.get(k) for k in ("FWH_SMV_WAVES", "FWH_SOLVE_DEPTH")}
+    os.environ["FWH_SMV_WAVES"] = "4"
+    os.environ["FWH_SOLVE_DEPTH"] = "16"
+    (tmp_path / "MODEL").mkdir()
+    old = os.getcwd()
+    os.chdir(tmp_path)
+    yield tmp_path
+    os.chdir(old)
+    for k, v in keep.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
+
+
+def check_same_bits(model, tmp_path, eps=1e-8, max_loop=20000):
+    from adventure_fullwave_amd import fwhip
+    logpath = str(tmp_path / "calc_log")
+    rc, st, x, B, E, J = refapi.solve(model, eps=eps, max_loop=max_loop, logpath=logpath)
+    assert rc == 0, fwhip.lib().fwh_last_error()
+    sol = fwhip.Solver(model)
+    st1 = sol.solve(eps=eps, max_loop=max_loop, graph_iters=16, schur=True)
+    say(f"[gpu] {model.name}: hddm_solver_CS_hip {st['iters']} iterations, fwh_solve {st1['iters']}")
+    assert st["iters"] == st1["iters"] and st["converged"] == 1
+    assert np.array_equal(x, sol.x())
+    for a, b in zip((B, E, J), sol.post()):
+        assert np.array_equal(a, b)
+    r = sol.resid(st1["loops"])
+    lines = open(logpath).read().splitlines()
+    assert lines[0] == f"# DOF on interface {int(sol.info['n_iface'])}"
+    assert lines[1] == f"0 {1.0:e}  # Norm {st1['r0']:e}"
+    assert len(lines) == st1["loops"] + 1
+    for k in (1, st1["loops"] // 2, st1["loops"] - 1):
+        loop, val = lines[k + 1].split()
+        assert int(loop) == k and float(val) == float(f"{r[k]:e}")
+    sol.close()
+
+
+def test_hddm_solver_cs_hip_same_bits(small_sigma_model, cfg1_sigma_model, pinned):
+    write_volt(pinned, [])
+    for model in (small_sigma_model, cfg1_sigma_model):
+        check_same_bits(model, pinned)
+
+
+def test_hddm_solver_cs_hip_delta_gap(small_sigma_model, pinned):
+    gap = with_delta_gap(small_sigma_model)
+    h, c = 1.0 / 8, 0.5
+    write_volt(pinned, [{"lo": np.array([c - 0.6 * h, c - 0.6 * h, c - 1.1 * h]),
+                         "hi": np.array([c + 0.6 * h, c + 0.6 * h, c + 1.1 * h]), "dV": 1.0},
+                        {"lo": np.array([c - 0.6 * h, c - 0.6 * h, c - 0.1 * h]),
+                         "hi": np.array([c + 0.6 * h, c + 0.6 * h, c + 1.1 * h]), "dV": 2.0}])
+    check_same_bits(gap, pinned)
+
+
+def test_hddm_solver_cs_hip_not_executed(small_sigma_model, pinned):
+    """hddm_max_loop = 0: "HDDM Solver was not executed!!!", no results
+    (hddm_solver.h:103-105)"""
+    write_volt(pinned, [])
+    rc, st, x, B, E, J = refapi.solve(small_sigma_model, max_loop=0)
+    assert rc == 0 and st["iters"] == 0 and not x.any()
+
+
+@pytest.mark.parametrize("n,maxw,seed", [(1, 0, 0), (17, 5, 2), (200, 40, 4), (1500, 120, 5)])
+def test_advlas_skyline_on_gpu(n, maxw, seed):
+    say(f"[gpu] advlas skyline n={n}")
+    rng = np.random.default_rng(seed)
+    idx1, idx2, nlen = random_profile(rng, n, maxw)
+    mat = rng.standard_normal(nlen) + 1j * rng.standard_normal(nlen)
+    A = dense_from_sky(idx1, idx2, mat)
+    A += np.diag(2.0 * np.abs(A).sum(axis=1) * np.exp(1j * rng.uniform(0, np.pi, n)))
+    mat = np.array([A[i, j] for i in range(n) for j in range(idx1[i], i + 1)])
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    fac, work, x = refapi.advlas(idx1, idx2, mat, b)
+    ref = fo.sky_factor(idx1, idx2, mat)
+    assert np.abs(fac - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.abs(work - ref[idx2 + np.arange(n)]).max() <= 1e-12 * np.abs(work).max()
+    xr = fo.sky_solve(idx1, idx2, ref, b)
+    assert np.abs(x - xr).max() <= 1e-12 * np.abs(xr).max()
+    assert np.abs(A @ x - b).max() <= 1e-11 * np.abs(b).max()
+
+
+def test_advlas_on_subdomain_factors(small_sigma_model):
+    """the GPU skyline calls on every subdomain's Dirichlet matrix of the
+    small model (profiles after SetBC, as the reference factors them)"""
+    om = fo.OracleModel(small_sigma_model)
+    rng = np.random.default_rng(8)
+    for s in range(small_sigma_model.nsub):
+        g = om.get(s)
+        n = g["n"]
+        # the unfactored skyline: rebuilt from the factor L D L^T
+        L = np.eye(n, dtype=complex)
+        dinv = np.zeros(n, complex)
+        for i in range(n):
+            for j in range(g["idx1"][i], i):
+                L[i, j] = g["factor"][g["idx2"][i] + j]
+            dinv[i] = g["factor"][g["idx2"][i] + i]
+        A = L @ np.diag(1.0 / dinv) @ L.T
+        mat = np.array([A[i, j] for i in range(n) for j in range(g["idx1"][i], i + 1)])
+        b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        fac, _, x = refapi.advlas(g["idx1"], g["idx2"], mat, b)
+        ref = fo.sky_factor(g["idx1"], g["idx2"], mat)
+        assert np.abs(fac - ref).max() <= 1e-10 * np.abs(ref).max()
+        xr = fo.sky_solve(g["idx1"], g["idx2"], ref, b)
+        assert np.abs(x - xr).max() <= 1e-10 * np.abs(xr).max()

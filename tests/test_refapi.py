@@ -1,0 +1,84 @@
+"""The reference-shaped interface (include/fwh_ref.h) on the CPU: the
+translation fwh_ref_desc_from_partmesh, from the reference's PartMesh /
+DomMesh / MTRL / OPinfo / Inbc / ANTENNA structs (built by tests/refdrv.c as
+hddm_Init leaves them) back to libfwhip's flat description, must reproduce
+the model exactly: connectivity, coordinates, per-element permittivity,
+conductivity and source flag from the MTRL flag tables
+(lib/fem/make_elem_em.c:98-115), the source vector Joe = -(Jo + i Jor) of
+every antenna vertex, Dirichlet and interface lists and the part-pair
+sharing lists; ./MODEL/volt.dat selects the current source (dg = 0) or the
+per-vertex gap voltages (dg > 0, last box wins)."""
+import os
+
+import numpy as np
+import pytest
+
+from tests import refapi
+from tests.helpers import with_delta_gap
+
+
+@pytest.fixture()
+def in_model_dir(tmp_path):
+    """a working directory with MODEL/volt.dat (the reference opens it relative to the CWD)"""
+    (tmp_path / "MODEL").mkdir()
+    old = os.getcwd()
+    os.chdir(tmp_path)
+    yield tmp_path
+    os.chdir(old)
+
+
+def write_volt(path, boxes):
+    with open(path / "MODEL" / "volt.dat", "w") as f:
+        f.write(f"{len(boxes)}\n")
+        for b in boxes:
+            for a in range(3):
+                f.write(f"{float(b['lo'][a])!r} {float(b['hi'][a])!r}\n")
+            f.write(f"{float(b['dV'])!r}\n")
+
+
+@pytest.mark.parametrize("which", ["small_sigma_model", "cfg1_model"])
+def test_translation_reproduces_the_model(which, request, in_model_dir):
+    model = request.getfixturevalue(which)
+    write_volt(in_model_dir, [])
+    t = refapi.translated_desc(model)
+    for k in ("sub_part", "el_off", "nop", "src", "ap_off", "nmp", "bc_off", "ibc", "ib_off", "ib_nfid", "ib_ln",
+              "infree", "outfree", "op_off", "op_ibid"):
+        assert np.array_equal(t[k], np.asarray(getattr(model, k), np.int32)), k
+    for k in ("eps", "sigma", "crd"):
+        assert np.array_equal(t[k], np.asarray(getattr(model, k), np.float64)), k
+    assert t["omega"] == model.omega and t["dgv"] is None and not t["pml"]
+    # Joe on every vertex of a source element, zero elsewhere
+    nop = np.asarray(model.nop).reshape(-1, 10)
+    sub_of_el = np.repeat(np.arange(model.nsub), np.diff(model.el_off))
+    vtx = np.asarray(model.ap_off)[sub_of_el][:, None] + nop[:, :4]
+    on = np.zeros(int(model.ap_off[-1]), bool)
+    on[vtx[np.asarray(model.src) != 0].ravel()] = True
+    joe = np.asarray(model.joe, np.float64).reshape(-1, 6)
+    assert np.array_equal(t["joe"].reshape(-1, 6)[on], joe[on])
+    assert not t["joe"].reshape(-1, 6)[~on].any()
+
+
+def test_translation_delta_gap(small_sigma_model, in_model_dir):
+    gap = with_delta_gap(small_sigma_model)
+    from adventure_fullwave_amd import advio
+    h = 1.0 / 8
+    c = 0.5
+    boxes = [{"lo": np.array([c - 0.6 * h, c - 0.6 * h, c - 1.1 * h]),
+              "hi": np.array([c + 0.6 * h, c + 0.6 * h, c + 1.1 * h]), "dV": 1.0},
+             {"lo": np.array([c - 0.6 * h, c - 0.6 * h, c - 0.1 * h]),
+              "hi": np.array([c + 0.6 * h, c + 0.6 * h, c + 1.1 * h]), "dV": 2.0}]
+    write_volt(in_model_dir, boxes)
+    assert [b["dV"] for b in advio.read_volt(str(in_model_dir / "MODEL" / "volt.dat"))] == [1.0, 2.0]
+    t = refapi.translated_desc(gap)
+    assert np.array_equal(t["dgv"], np.asarray(gap.dgv, np.float64))
+    assert np.array_equal(t["sigma"], np.asarray(gap.sigma, np.float64))
+
+
+def test_translation_refuses_missing_volt_dat(small_sigma_model, tmp_path):
+    old = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        with pytest.raises(RuntimeError, match="volt.dat"):
+            refapi.translated_desc(small_sigma_model)
+    finally:
+        os.chdir(old)
