@@ -8,19 +8,22 @@ An ADV file is a concatenation of documents; each document is
 property, as adv_dbox_find_by_property does for the reference
 (lib/std/fileio.c:760-990 reads the partition files this way).
 
-`read_partition` returns the raw per-part data of ``model/advhddm_in_P.adv``;
-`build_full_wave_model` turns the parts into the solver's data model with the
-reference's full-wave rules:
+`read_partition` returns the raw per-part data of ``model/advhddm_in_P.adv``
+(`write_partition` writes it); `build_full_wave_model` turns the parts into
+the solver's data model with the reference's full-wave rules:
 
 * local vertex ids follow subdomain-node order over the element vertices,
   local edge (mid-node) ids the remaining nodes in subdomain-node order
   (make_MESH_make_ap / make_MESH_change_ap, lib/fem/make_MESH.c:237-272,
   539-553), and the 8<->9 mid-node swap of make_MESH_change_nop
   (make_MESH.c:557-585) makes edge k join tet vertices tet_n[k];
-* only mid-nodes carry DOFs: an edge is an interface DOF when more than one
-  subdomain holds it (vertex interface nodes are dropped,
-  lib/hddm/make_PartMesh.c:1603-1624); it is owned by the lowest part
-  holding it;
+* only mid-nodes carry DOFs: the file's node-level interface is
+  reclassified as the reference does (reclassify_interface: essential
+  nodes and vertex nodes leave the interface, the mid-nodes keep the charge
+  part the file gives them and are renumbered per part, copies charged
+  elsewhere first, in file order; lib/hddm/make_PartMesh.c:158-858), and a
+  subdomain's Dirichlet list keeps its essential mid-nodes, sorted
+  (make_PartMesh.c:1082-1094);
 * the current source is the antenna's Parallelpiped definition
   (read_Jodefine_file_pp, fileio.c:1226-1293; make_Jo_from_definition_pp,
   fileio.c:1556-1594) on the vertices of antenna-flagged elements, with the
@@ -212,11 +215,93 @@ def gap_voltage(x: np.ndarray, boxes: List[dict]) -> np.ndarray:
     return out
 
 
+# Pinbc bc_flag values and the node-change marks of the reference's interface
+# reclassification (lib/hddm/libhddm.h:25-34)
+NOT_CHANGE_NODE = -1          # __ADVFW_HDDM_CHANGE_INNER_BC_NOT_CHANGE__: a mid-node (edge DOF)
+FLAG_ESSENTIAL = -2
+FLAG_NOT_CHANGE = -1
+FLAG_NO_DOF = 0
+
+
+def reclassify_interface(parts: List[dict]) -> List[dict]:
+    """The E-method interface of every part, from the partition file's own
+    node interface (make_PartMesh_change_innerbc_func,
+    lib/hddm/make_PartMesh.c:158-248, with the s-mode exchange of
+    make_PartMesh_s.c:51-128):
+
+    * 1st step (:393-422): an interface node in a subdomain's own Dirichlet
+      list marks its part-level entry essential;
+    * per subdomain the mid-nodes are "not change" nodes, the vertices carry
+      no edge DOF (make_nf_A_Method, :1600-1624); a part-level entry takes
+      the charge part of the inbc records that name it as a mid-node
+      (add_nf, :374-389), an essential entry no DOF;
+    * 3rd step (:491-586): essential stays, "not change" entries keep their
+      DOF, the rest (vertex nodes) become NO_DOF;
+    * 4th step (:590-640): the surviving entries are renumbered in their
+      file order, copies owned by another part (charge part != this part)
+      first, then the owned ones after them;
+    * change_Inbc / change_OPinfo (:707-858): every subdomain's inbc list
+      and every part-pair list keep their surviving entries, in file order,
+      with the new numbers.
+
+    Returns per part {"infree", "outfree", "op": [list per part],
+    "inbc": [(snid, ln_part) arrays per subdomain]}.  The flags that only
+    the magnetic-field formulations produce (ONLY_ONE, ONLY_THIS_PART,
+    INFREE, OUTFREE) cannot arise here."""
+    out = []
+    for p, P in enumerate(parts):
+        tfree = int(P["infree"]) + int(P["outfree"])
+        nf = np.zeros(tfree, np.int64)
+        flag = np.zeros(tfree, np.int64)
+        charge = np.zeros(tfree, np.int64)
+        if P["dbc"] is not None:                                  # 1st step
+            for d in range(P["nsub"]):
+                ib = P["inbc"][d]
+                hit = np.isin(ib[:, 0], P["dbc"][d])
+                flag[ib[hit, 4]] = FLAG_ESSENTIAL
+        node_nf = []
+        for d in range(P["nsub"]):                                # add_nf
+            nnd = len(P["snid2pnid"][d])
+            nfd = np.zeros(nnd, np.int64)
+            nfd[P["nop"][d][:, 4:].ravel()] = NOT_CHANGE_NODE
+            node_nf.append(nfd)
+            for snid, _coord, cp, _lnc, lnp in P["inbc"][d]:
+                if flag[lnp] == FLAG_ESSENTIAL:
+                    nf[lnp] = 0
+                elif nfd[snid] == NOT_CHANGE_NODE:
+                    nf[lnp] = NOT_CHANGE_NODE
+                    charge[lnp] = cp
+        keep = flag != FLAG_ESSENTIAL                             # 3rd step
+        flag[keep & (nf == NOT_CHANGE_NODE)] = FLAG_NOT_CHANGE
+        flag[keep & (nf == 0)] = FLAG_NO_DOF
+        if np.any(keep & (nf > 0)):
+            raise ValueError(f"part {p}: an interface entry of a magnetic-field formulation in an E-method model")
+        alive = flag == FLAG_NOT_CHANGE                           # 4th step
+        own = alive & (charge == p)
+        other = alive & (charge != p)
+        new_ln = np.full(tfree, -1, np.int64)
+        new_ln[other] = np.arange(int(other.sum()))
+        new_ln[own] = int(other.sum()) + np.arange(int(own.sum()))
+        inbc = []
+        for d in range(P["nsub"]):                                # change_Inbc
+            ib = P["inbc"][d]
+            ok = alive[ib[:, 4]] & (node_nf[d][ib[:, 0]] != 0)
+            inbc.append(np.stack([ib[ok, 0], new_ln[ib[ok, 4]]], axis=1))
+        op = [new_ln[o[alive[o]]] for o in P["op"]]               # change_OPinfo
+        out.append({"infree": int(own.sum()), "outfree": int(other.sum()), "op": op, "inbc": inbc})
+    return out
+
+
 def build_full_wave_model(parts: List[dict], mtrl: dict, antenna_pp: Dict[int, List[dict]],
                           order: str = "reference", name: str = "sample",
                           volt: List[dict] = None) -> mesh.Model:
     """Solver data model of a partitioned quadratic-tet model (full-wave
-    E method).  order="reference" keeps the reference's local numbering.
+    E method), numbered as the reference numbers it: the interface of
+    reclassify_interface, and per subdomain (make_PartMesh_FW_TetQ,
+    make_PartMesh.c:1628-1707) vertices and mid-nodes in subdomain-node
+    order, the 8<->9 mid-node swap, the essential mid-nodes of the
+    subdomain's Dirichlet list sorted (make_PartMesh.c:1082-1094) and the
+    interface mid-nodes sorted by local id (sort_inbc, :1167-1194).
     volt: gap boxes of a volt.dat with dg > 0 (delta-gap source), or
     None/empty for the current source."""
     npart = len(parts)
@@ -234,45 +319,19 @@ def build_full_wave_model(parts: List[dict], mtrl: dict, antenna_pp: Dict[int, L
             nop[:, [8, 9]] = nop[:, [9, 8]]           # edge k joins tet_n[k]
             subs.append((p, d, snid2gnid[nop], P["flag"][d], snid2gnid))
     nsub = len(subs)
-    # which subdomains / parts hold each mid-node
-    mid_sets: Dict[int, set] = {}
-    holders_sub = np.zeros(nglob, np.int64)
-    part_min = np.full(nglob, npart, np.int64)
-    part_sets = [set() for _ in range(nglob)] if False else None
-    seen = np.full(nglob, -1, np.int64)
-    for s, (p, d, gn, fl, s2g) in enumerate(subs):
-        mids = np.unique(gn[:, 4:])
-        holders_sub[mids] += 1
-        part_min[mids] = np.minimum(part_min[mids], p)
-    is_iface = holders_sub >= 2
-    # parts holding each interface mid-node
-    part_has = np.zeros((npart, nglob), bool)
-    for (p, d, gn, fl, s2g) in subs:
-        part_has[p, np.unique(gn[:, 4:])] = True
-    ln_maps = []
-    infree = np.zeros(npart, np.int64)
-    outfree = np.zeros(npart, np.int64)
-    for p in range(npart):
-        e = np.nonzero(part_has[p] & is_iface)[0]
-        nonown = e[part_min[e] != p]
-        own = e[part_min[e] == p]
-        outfree[p], infree[p] = len(nonown), len(own)
-        ln_maps.append(np.concatenate([nonown, own]))
-    lnp_all = []
-    for p in range(npart):
-        lnp = np.full(nglob, -1, np.int64)
-        lnp[ln_maps[p]] = np.arange(len(ln_maps[p]))
-        lnp_all.append(lnp)
+    iface = reclassify_interface(parts)
+    infree = np.array([x["infree"] for x in iface], np.int64)
+    outfree = np.array([x["outfree"] for x in iface], np.int64)
     op_off, op_ibid = [0], []
     for p in range(npart):
         for q in range(npart):
-            if p == q:
-                lst = np.zeros(0, np.int64)
-            else:
-                shared = np.intersect1d(ln_maps[p], ln_maps[q])
-                lst = lnp_all[p][shared]
+            lst = iface[p]["op"][q] if q != p else np.zeros(0, np.int64)
             op_ibid.append(lst)
             op_off.append(op_off[-1] + len(lst))
+    # global mid-nodes: every node that is a mid-node of some element
+    is_mid = np.zeros(nglob, bool)
+    for (p, d, gn, fl, s2g) in subs:
+        is_mid[gn[:, 4:].ravel()] = True
 
     # antenna vertices with the Parallelpiped current
     ant_flags = sorted(antenna_pp)
@@ -292,7 +351,7 @@ def build_full_wave_model(parts: List[dict], mtrl: dict, antenna_pp: Dict[int, L
     joe_g = np.stack([Jc.real, Jc.imag], axis=2).reshape(nglob, 6)
 
     nop_all, ap_off, crd_all, joe_all, nmp = [], [0], [], [], []
-    bc_off, ib_off, ib_nfid_all, ib_ln_all, ledge, mp2sn = [0], [0], [], [], [], []
+    bc_off, ib_off, ib_nfid_all, ib_ln_all, ledge, mp2sn, ibc_all = [0], [0], [], [], [], [], []
     eps_all, sig_all, src_all, el_off = [], [], [], [0]
     for (p, d, gn, fl, s2g) in subs:
         vert = np.zeros(len(s2g), bool)
@@ -313,11 +372,23 @@ def build_full_wave_model(parts: List[dict], mtrl: dict, antenna_pp: Dict[int, L
         nmp.append(len(mid_g))
         ledge.append(mid_g)
         mp2sn.append(np.nonzero(~vert)[0])
-        bc_off.append(bc_off[-1])
-        ifc = np.nonzero(is_iface[mid_g])[0]
-        ib_nfid_all.append(ifc)
-        ib_ln_all.append(lnp_all[p][mid_g[ifc]])
-        ib_off.append(ib_off[-1] + len(ifc))
+        # essential mid-nodes of the subdomain's Dirichlet list, sorted
+        # (make_MESH_change_dbc_Axn0_1st/2nd + sort_i; vertex entries go to
+        # the antenna conditions, make_MESH.c:409-432)
+        dbc = parts[p]["dbc"]
+        ess = np.zeros(0, np.int64)
+        if dbc is not None:
+            raw = np.asarray(dbc[d], np.int64)
+            ess = np.sort(mp[raw[~vert[raw]]])
+        ibc_all.append(ess)
+        bc_off.append(bc_off[-1] + len(ess))
+        # interface mid-nodes sorted by local id, with their new ln_part
+        ib = iface[p]["inbc"][d]
+        nfid = mp[ib[:, 0]]
+        srt = np.argsort(nfid, kind="stable")
+        ib_nfid_all.append(nfid[srt])
+        ib_ln_all.append(ib[srt, 1])
+        ib_off.append(ib_off[-1] + len(nfid))
         eps_all.append(np.array([mtrl["eps"][int(f)] for f in fl]))
         sig_all.append(np.array([mtrl["sigma"].get(int(f), 0.0) for f in fl]))
         src_all.append(np.isin(fl, ant_flags).astype(np.int32))
@@ -332,11 +403,11 @@ def build_full_wave_model(parts: List[dict], mtrl: dict, antenna_pp: Dict[int, L
         nop=cat(nop_all).astype(np.int32), eps=cat(eps_all), sigma=cat(sig_all),
         src=cat(src_all).astype(np.int32), pml=None, ap_off=np.array(ap_off, np.int32),
         crd=crd_cat, joe=cat(joe_all).reshape(-1), nmp=np.array(nmp, np.int32),
-        bc_off=np.array(bc_off, np.int32), ibc=np.zeros(0, np.int32), ib_off=np.array(ib_off, np.int32),
+        bc_off=np.array(bc_off, np.int32), ibc=cat(ibc_all).astype(np.int32), ib_off=np.array(ib_off, np.int32),
         ib_nfid=cat(ib_nfid_all).astype(np.int32), ib_ln=cat(ib_ln_all).astype(np.int32),
         infree=infree.astype(np.int32), outfree=outfree.astype(np.int32),
         op_off=np.array(op_off, np.int32), op_ibid=cat(op_ibid).astype(np.int32),
-        ledge_global=cat(ledge).astype(np.int64), n_edges=int(is_iface.size), n_tets=int(el_off[-1]),
+        ledge_global=cat(ledge).astype(np.int64), n_edges=int(nglob), n_tets=int(el_off[-1]),
         mp2sn=cat(mp2sn).astype(np.int32), name=name)
 
 
@@ -366,6 +437,142 @@ def load_reference_model(model_dir: str) -> mesh.Model:
     vpath = os.path.join(model_dir, "volt.dat")
     volt = read_volt(vpath) if os.path.exists(vpath) else None
     return build_full_wave_model(parts, mtrl, pp, volt=volt)
+
+
+# ---------------------------------------------------------------------------
+# partition files (the inverse of read_partition)
+# ---------------------------------------------------------------------------
+def _int_doc(records, width: int = 1) -> bytes:
+    """int32 records [n, n*width values] one after another"""
+    out = []
+    for r in records:
+        r = np.asarray(r, "<i4").ravel()
+        out.append(np.array([len(r) // width], "<i4").tobytes())
+        out.append(r.tobytes())
+    return b"".join(out)
+
+
+def write_partition(model, out_dir: str, flags: np.ndarray, essential: bool = True) -> List[str]:
+    """Writes `model` as reference partition files model/advhddm_in_P.adv
+    (the layout read_partition reads, fileio.c:760-990): node-level data as
+    ADVENTURE_Metis leaves it, before the full-wave reclassification.
+
+    Global nodes: the vertices (told apart by coordinates), then one
+    mid-node per global edge.  Per part the nodes of its subdomains in
+    global order; per subdomain its nodes in global order, elements with the
+    file's mid-node order (edges (1,3) and (2,3) in the file's positions 9
+    and 8).  Interface nodes (vertices and mid-nodes held by two or more
+    subdomains) are charged to the lowest part holding them and numbered per
+    part, copies charged elsewhere first, each group in global order; the
+    part-pair lists in global order.  flags: per element material flag.
+    essential: write DirichletBCs_Axn0 (Options NEED) with the essential
+    edges' mid-nodes and their end vertices."""
+    os.makedirs(out_dir, exist_ok=True)
+    nop = np.asarray(model.nop, np.int64).reshape(-1, 10)
+    crd = np.asarray(model.crd, np.float64).reshape(-1, 3)
+    nmp = np.asarray(model.nmp, np.int64)
+    dof0 = np.concatenate([[0], np.cumsum(nmp)])
+    lg = np.asarray(model.ledge_global, np.int64)
+    # global vertex ids by coordinates
+    uq, vinv = np.unique(np.round(crd, 12), axis=0, return_inverse=True)
+    vinv = vinv.ravel()
+    nv = len(uq)
+    ne_glob = int(lg.max()) + 1
+    gcrd = np.zeros((nv + ne_glob, 3))
+    gcrd[:nv] = uq
+    sub_nodes, sub_gnop = [], []
+    ess_nodes = []
+    for s in range(model.nsub):
+        e0, e1 = model.el_off[s], model.el_off[s + 1]
+        a0 = model.ap_off[s]
+        gv = vinv[a0 + nop[e0:e1, :4]]
+        ge = nv + lg[dof0[s] + nop[e0:e1, 4:]]
+        gn = np.concatenate([gv, ge], axis=1)
+        vtx_of_edge = gv[:, [0, 0, 0, 1, 1, 2]], gv[:, [1, 2, 3, 2, 3, 3]]
+        gcrd[ge] = 0.5 * (gcrd[vtx_of_edge[0]] + gcrd[vtx_of_edge[1]])
+        sub_gnop.append(gn)
+        sub_nodes.append(np.unique(gn))
+        ibc = np.asarray(model.ibc[model.bc_off[s]:model.bc_off[s + 1]], np.int64)
+        eg = nv + lg[dof0[s] + ibc]
+        ends = set()
+        for k in range(6):
+            hit = np.isin(ge[:, k], eg)
+            ends.update(vtx_of_edge[0][hit, k].tolist())
+            ends.update(vtx_of_edge[1][hit, k].tolist())
+        ess_nodes.append(np.union1d(eg, np.array(sorted(ends), np.int64)))
+    nglob = nv + ne_glob
+    holders = np.zeros(nglob, np.int64)
+    owner = np.full(nglob, np.iinfo(np.int64).max)
+    for s in range(model.nsub):
+        holders[sub_nodes[s]] += 1
+        owner[sub_nodes[s]] = np.minimum(owner[sub_nodes[s]], model.sub_part[s])
+    is_if = holders >= 2
+    npart = model.npart
+    part_nodes = [np.unique(np.concatenate([sub_nodes[s] for s in range(model.nsub) if model.sub_part[s] == p]))
+                  for p in range(npart)]
+    lnmap = []
+    for p in range(npart):
+        ifn = part_nodes[p][is_if[part_nodes[p]]]
+        order = np.concatenate([ifn[owner[ifn] != p], ifn[owner[ifn] == p]])
+        m = np.full(nglob, -1, np.int64)
+        m[order] = np.arange(len(order))
+        lnmap.append((m, int((owner[ifn] != p).sum()), int((owner[ifn] == p).sum())))
+    paths = []
+    nel_all = int(model.el_off[-1])
+    for p in range(npart):
+        subs = [s for s in range(model.nsub) if model.sub_part[s] == p]
+        pn = part_nodes[p]
+        g2p = np.full(nglob, -1, np.int64)
+        g2p[pn] = np.arange(len(pn))
+        m, outf, inf = lnmap[p]
+        docs = []
+        docs.append(("Node", [("content_type", "Node"), ("num_items", len(pn)), ("num_items_orig", nglob),
+                              ("format", "f8f8f8")], gcrd[pn].astype("<f8").tobytes()))
+        docs.append(("p2g", [("label", "NodeIndex_PartToGlobal"), ("num_items", len(pn))],
+                     pn.astype("<i4").tobytes()))
+        opdata = []
+        for q in range(npart):
+            if q == p:
+                opdata.append(np.zeros(0, np.int64))
+                continue
+            shared = np.intersect1d(pn[is_if[pn]], part_nodes[q])
+            opdata.append(m[shared])
+        docs.append(("ifd", [("content_type", "HDDM_InterfaceDOF"), ("num_parts", npart), ("part_number", p),
+                             ("num_subdomains", len(subs)), ("total_num_infree", inf),
+                             ("total_num_outfree", outf)], _int_doc(opdata)))
+        els, s2p, inbc, dbc, flg, e2g = [], [], [], [], [], []
+        for s in subs:
+            sn = sub_nodes[s]
+            g2s = np.full(nglob, -1, np.int64)
+            g2s[sn] = np.arange(len(sn))
+            raw = g2s[sub_gnop[s]]
+            raw[:, [8, 9]] = raw[:, [9, 8]]          # the file's mid-node order
+            els.append(raw)
+            s2p.append(g2p[sn])
+            ifs = sn[is_if[sn]]
+            rows = np.stack([g2s[ifs], np.zeros(len(ifs), np.int64), owner[ifs],
+                             np.array([lnmap[owner[g]][0][g] for g in ifs], np.int64), m[ifs]], axis=1)
+            inbc.append(rows)
+            dbc.append(g2s[np.intersect1d(ess_nodes[s], sn)])
+            flg.append(flags[model.el_off[s]:model.el_off[s + 1]])
+            e2g.append(np.arange(model.el_off[s], model.el_off[s + 1]))
+        docs.append(("el", [("content_type", "HDDM_Element"), ("element_type", "3DQuadraticTetrahedron"),
+                            ("num_nodes_per_element", 10), ("num_subdomains", len(subs)),
+                            ("num_items_orig", nel_all)], _int_doc(els, 10)))
+        docs.append(("s2p", [("label", "NodeIndex_SubdomainToPart")], _int_doc(s2p)))
+        docs.append(("inbc", [("label", "InterfaceDOF"), ("format", "i4i4i4i4")], _int_doc(inbc, 5)))
+        if essential:
+            docs.append(("dbc", [("label", "DirichletBCs_Axn0")], _int_doc(dbc)))
+        docs.append(("flag", [("label", "Flag")], _int_doc(flg)))
+        docs.append(("e2g", [("label", "ElementIndex_SubdomainToGlobal")], _int_doc(e2g)))
+        docs.append(("opt", [("label", "Options"), ("DirichletBCs_Axn0", "NEED" if essential else "NO_NEED")],
+                     b""))
+        path = os.path.join(out_dir, f"advhddm_in_{p}.adv")
+        with open(path, "wb") as f:
+            for k, (tag, props, payload) in enumerate(docs):
+                f.write(_doc_bytes(f"fwhip-part{p}-{tag}-{k}", props, payload))
+        paths.append(path)
+    return paths
 
 
 # ---------------------------------------------------------------------------
