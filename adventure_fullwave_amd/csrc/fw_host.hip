@@ -233,6 +233,18 @@ struct fwh_ctx {
   long long *sm_off = nullptr;
   int *smv_order = nullptr;  // work order of the matvec (picked at formation)
   fw::SmvWork *smv_work = nullptr;  // per work item of that order: tiles, slots
+  // work items of the matvec: nsub when no subdomain is split; per subdomain
+  // its first item and item count (a split subdomain's items are adjacent)
+  int smv_nwork = 0;
+  std::vector<int> smv_first, smv_count;
+  // (p,q) parts of an explicit iteration: one per subdomain, then one per
+  // further item of a split subdomain; the gather lists of the explicit
+  // iteration (the load's lists with the split subdomains' slot copies)
+  int npq_expl = 0;
+  int *gqx_ptr = nullptr, *gqx_slot = nullptr;
+  int4 *gqx_slot4 = nullptr;
+  std::vector<int> gq_ptr_h, gq_slot_h;  // the load's gather lists (host copy)
+  bool gather_csr = false;                // FWH_GATHER=csr: no four-wide tables
   int nbmax = 0;
   int smv_waves = 4;
   bool schur_built = false;
@@ -403,19 +415,22 @@ struct fwh_ctx {
     a.part = part;
     a.G = G;
     a.pqpart = (check_done && fused) ? pqpart : nullptr;  // an interface iteration
-    a.npq = nsub;
+    a.npq = npq_expl > 0 ? npq_expl : nsub;
     a.yslot = yslot;
     return a;
   }
-  fw::VecArgs vec_args(int only = -1) const {
+  // expl: the vector pass of an explicit iteration (its gather lists and
+  // (p,q) parts), else of the first step or an implicit iteration
+  fw::VecArgs vec_args(int only = -1, bool expl = false) const {
     fw::VecArgs v{};
     v.sys0 = only >= 0 ? only : 0;
     v.nsys = only >= 0 ? 1 : nsys;
     v.ss = ss;
     v.ng = ng;
-    v.gq_ptr = gq_ptr;
-    v.gq_slot = gq_slot;
-    v.gq_slot4 = gq_slot4;
+    const bool x = expl && gqx_ptr;
+    v.gq_ptr = x ? gqx_ptr : gq_ptr;
+    v.gq_slot = x ? gqx_slot : gq_slot;
+    v.gq_slot4 = x ? gqx_slot4 : gq_slot4;
     v.yslot = yslot;
     v.pcslot = pcslot;
     v.P = P;
@@ -426,7 +441,7 @@ struct fwh_ctx {
     v.PC = PC;
     v.part = part;
     v.pqpart = pqpart;
-    v.npq = nsub;
+    v.npq = (expl && npq_expl > 0) ? npq_expl : nsub;
     v.sc = sc;
     v.hist = hist;
     v.G = G;
@@ -451,18 +466,25 @@ struct fwh_ctx {
   // the vector part of an interface iteration after its Schur step (which
   // leaves every subdomain's (p,q) part): k_iter_ab, then k_iter_c when
   // status_step
-  hipError_t launch_vec_iter(bool status_step) { return fw::launch_iter_ab(vec_args(), st, status_step); }
+  hipError_t launch_vec_iter(bool status_step, bool expl) {
+    return fw::launch_iter_ab(vec_args(-1, expl), st, status_step);
+  }
   // the Schur step of an interface iteration (y_slot = S_s p_s): implicit
   // (Dirichlet solves through the factors) or explicit (formed S_s)
   hipError_t run_schur(bool expl, bool plain, bool check_done, bool fused, int only = -1) {
     if (!expl) return run_solve(fw::MODE_ITER, plain, check_done, fused, only);
     fw::SmvArgs a = smv_args(check_done, fused, only);
-    if (!plain) return fw::launch_smv(a, nsub, st);
-    a.order = nullptr;
-    a.work = nullptr;
+    if (!plain) return fw::launch_smv(a, smv_work ? smv_nwork : nsub, st);
+    // one launch per subdomain, in the reference's loop order (a split
+    // subdomain's items in one launch)
     for (int s = 0; s < nsub; s++) {
-      a.base = s;
-      hipError_t e = fw::launch_smv(a, 1, st);
+      if (smv_work) {
+        a.base = smv_first[s];
+      } else {
+        a.order = nullptr;
+        a.base = s;
+      }
+      hipError_t e = fw::launch_smv(a, smv_work ? smv_count[s] : 1, st);
       if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -692,12 +714,15 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         if (t == gq_ptr[g] || gq_part[t] != gq_part[t - 1]) gq_slot[t] = ~gq_slot[t];
     }
   }
+  c->gq_ptr_h = gq_ptr;
+  c->gq_slot_h = gq_slot;
   // the same lists four per unknown, so a gather reads its slot ids in one
   // load (interface edges lie in 2-4 subdomains); FWH_GATHER=csr sends every
   // unknown through the CSR, the path of unknowns in more than four
   std::vector<int4> gq_slot4(ng);
   const char *genv = getenv("FWH_GATHER");
   const bool csr_only = genv && strcmp(genv, "csr") == 0;
+  c->gather_csr = csr_only;
   for (int g = 0; g < ng; g++) {
     const int c0 = gq_ptr[g], cnt = gq_ptr[g + 1] - c0;
     int e[4] = {fw::SLOT_NONE, fw::SLOT_NONE, fw::SLOT_NONE, fw::SLOT_NONE};
@@ -1285,6 +1310,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   c->ss.cpl = ncpl;
   c->ss.grow = ngrow;
   c->ss.slot = nslots;
+  c->ss.pcs = nslots;
   c->ss.g = ng;
   c->ss.smat = 0;
   c->ss.hist = 0;
@@ -1448,6 +1474,195 @@ int read_scal(fwh_ctx *c, std::vector<fw::Scal> &s, int *all_done = nullptr) {
   }
   return 0;
 }
+// Work items of the explicit Schur matvec.  A workgroup streams one
+// subdomain's tiles, so on uneven decompositions the largest subdomains set
+// the launch's tail; they can be split into items over contiguous tile
+// ranges, each writing its own copy of the subdomain's slots and its own
+// part of (p,q).  Items follow the matvec's work order `ord`, a split
+// subdomain's items adjacent; an unsplit subdomain (and a split one's first
+// item) writes the load's slots and its subdomain's (p,q) part, further
+// items slots after them and parts after the nsub subdomain parts.  The
+// gather of the explicit iteration sums a split subdomain's copies in item
+// order where the load's lists have its slot.  Candidates: items of at most
+// total tiles / (nCU m) tiles for m = 2, 4, 8; kept when more than 3 %
+// faster than no split (a split regroups the sums, changing rounding);
+// FWH_SMV_SPLIT=0, 2, 4 or 8 fixes the choice.
+namespace {
+struct SmvSplit {
+  std::vector<fw::SmvWork> work;
+  std::vector<int> first, count;
+  long long slots = 0;  // output slots in all
+  int npq = 0;          // (p,q) parts in all
+  bool split = false;
+};
+
+SmvSplit make_smv_split(const fwh_ctx *c, const std::vector<int> &ord, const std::vector<long long> &off,
+                        const std::vector<long long> &ntile, long long tmax) {
+  SmvSplit sp;
+  sp.first.assign(c->nsub, 0);
+  sp.count.assign(c->nsub, 1);
+  sp.slots = c->nslots;
+  sp.npq = c->nsub;
+  for (int s : ord) {
+    const SubMeta &M = c->meta_h[s];
+    const int K = (tmax > 0 && ntile[s] > tmax) ? (int)((ntile[s] + tmax - 1) / tmax) : 1;
+    sp.first[s] = (int)sp.work.size();
+    sp.count[s] = K;
+    sp.split |= K > 1;
+    for (int k = 0; k < K; k++) {
+      fw::SmvWork w{};
+      w.off = off[s];
+      w.gg = M.g0;
+      w.ng = M.ng;
+      if (k == 0) {
+        w.y0 = M.g0;
+        w.pq = s;
+      } else {
+        w.y0 = (int)sp.slots;
+        sp.slots += M.ng;
+        w.pq = sp.npq++;
+      }
+      if (K > 1) {
+        w.t0 = (int)(ntile[s] * k / K);
+        w.t1 = (int)(ntile[s] * (k + 1) / K);
+      }
+      sp.work.push_back(w);
+    }
+  }
+  return sp;
+}
+
+int tune_smv_split(fwh_ctx *c, const std::vector<int> &ord, const std::vector<long long> &off, int ncu) {
+  std::vector<long long> ntile(c->nsub);
+  long long ttot = 0;
+  for (int s = 0; s < c->nsub; s++) {
+    const long long nb = (c->meta_h[s].ng + 15) / 16;
+    ntile[s] = nb * (nb + 1) / 2;
+    ttot += ntile[s];
+  }
+  const int cand[4] = {0, 2, 4, 8};
+  SmvSplit sp[4];
+  long long slots_max = c->nslots;
+  int npq_max = c->nsub;
+  for (int v = 0; v < 4; v++) {
+    const long long tmax = cand[v] ? std::max<long long>(1, (ttot + (long long)ncu * cand[v] - 1) /
+                                                               ((long long)ncu * cand[v]))
+                                   : 0;
+    sp[v] = make_smv_split(c, ord, off, ntile, tmax);
+    slots_max = std::max(slots_max, sp[v].slots);
+    npq_max = std::max(npq_max, sp[v].npq);
+  }
+  if (slots_max > (1LL << 31) - 1) return fail(FWH_E_LIMIT, "too many matvec output slots");
+  // room for every candidate's outputs (per system stride ss.slot)
+  int rc;
+  if (slots_max > c->ss.slot) {
+    c->release_graph();
+    c->free_one(c->yslot);
+    c->yslot = nullptr;
+    if ((rc = c->alloc(&c->yslot, (size_t)c->nsys * slots_max))) return rc;
+    c->ss.slot = slots_max;
+  }
+  if (npq_max > c->nsub) {
+    c->release_graph();
+    c->free_one(c->pqpart);
+    c->pqpart = nullptr;
+    if ((rc = c->alloc(&c->pqpart, (size_t)c->nsys * npq_max))) return rc;
+  }
+  // time the distinct candidates in turn (pairs of launches, best of five)
+  TempDev tmp;
+  fw::SmvWork *d_w[4] = {};
+  fw::SmvArgs sa[4];
+  EventPair ev;
+  HIPCHK(ev.create());
+  for (int v = 0; v < 4; v++) {
+    c->info.ms_smv_split[v] = -1.0;
+    if (v > 0 && !sp[v].split) continue;
+    HIPCHK(tmp.alloc(&d_w[v], sp[v].work.size()));
+    HIPCHK(hipMemcpyAsync(d_w[v], sp[v].work.data(), sp[v].work.size() * sizeof(fw::SmvWork),
+                          hipMemcpyHostToDevice, c->st));
+    c->smv_work = d_w[v];
+    sa[v] = c->smv_args(false, false);
+    HIPCHK(fw::launch_smv(sa[v], (int)sp[v].work.size(), c->st));
+    c->info.ms_smv_split[v] = 1e30;
+  }
+  for (int r = 0; r < 5; r++)
+    for (int v = 0; v < 4; v++) {
+      if (c->info.ms_smv_split[v] < 0) continue;
+      float ms = 0.f;
+      HIPCHK(hipEventRecord(ev.a, c->st));
+      HIPCHK(fw::launch_smv(sa[v], (int)sp[v].work.size(), c->st));
+      HIPCHK(fw::launch_smv(sa[v], (int)sp[v].work.size(), c->st));
+      HIPCHK(hipEventRecord(ev.b, c->st));
+      HIPCHK(hipEventSynchronize(ev.b));
+      HIPCHK(hipEventElapsedTime(&ms, ev.a, ev.b));
+      c->info.ms_smv_split[v] = std::min(c->info.ms_smv_split[v], 0.5 * (double)ms);
+    }
+  int pick = 0;
+  for (int v = 1; v < 4; v++)
+    if (c->info.ms_smv_split[v] > 0 && c->info.ms_smv_split[v] < 0.97 * c->info.ms_smv_split[pick]) pick = v;
+  if (const char *e = getenv("FWH_SMV_SPLIT")) {
+    const int m = atoi(e);
+    for (int v = 0; v < 4; v++)
+      if (cand[v] == m) pick = v;
+  }
+  const SmvSplit &S = sp[pick];
+  // every item stays inside the per-system output slots, (p,q) parts and
+  // tile ranges allocated above
+  for (const fw::SmvWork &w : S.work) {
+    const long long nb = (w.ng + 15) / 16;
+    if (w.y0 < 0 || (long long)w.y0 + w.ng > c->ss.slot || w.pq < 0 || w.pq >= S.npq ||
+        (long long)w.gg + w.ng > c->nslots || (w.t1 > w.t0 && (w.t0 < 0 || w.t1 > nb * (nb + 1) / 2)))
+      return fail(FWH_E_LIMIT, "matvec work item out of range");
+  }
+  c->info.smv_split = S.split ? cand[pick] : 0;
+  c->info.smv_nwork = (int)S.work.size();
+  fw::SmvWork *d_work = nullptr;
+  if ((rc = c->upload(&d_work, S.work))) return rc;
+  c->smv_work = d_work;
+  c->smv_nwork = (int)S.work.size();
+  c->smv_first = S.first;
+  c->smv_count = S.count;
+  c->npq_expl = S.npq;
+  c->gqx_ptr = c->gqx_slot = nullptr;
+  c->gqx_slot4 = nullptr;
+  if (S.split) {
+    // the explicit iteration's gather lists: the load's, with every slot of a
+    // split subdomain followed by its copies (part markers stay on the first)
+    std::vector<int> slot_sub(c->nslots);
+    for (int s = 0; s < c->nsub; s++)
+      for (int k = 0; k < c->meta_h[s].ng; k++) slot_sub[c->meta_h[s].g0 + k] = s;
+    std::vector<int> xptr(c->ng + 1, 0), xslot;
+    xslot.reserve(c->gq_slot_h.size());
+    for (int g = 0; g < c->ng; g++) {
+      for (int t = c->gq_ptr_h[g]; t < c->gq_ptr_h[g + 1]; t++) {
+        const int raw = c->gq_slot_h[t], sl = raw < 0 ? ~raw : raw;
+        const int s = slot_sub[sl], k = sl - c->meta_h[s].g0;
+        xslot.push_back(raw);
+        for (int j = 1; j < S.count[s]; j++) xslot.push_back(S.work[S.first[s] + j].y0 + k);
+      }
+      xptr[g + 1] = (int)xslot.size();
+    }
+    std::vector<int4> x4(c->ng);
+    for (int g = 0; g < c->ng; g++) {
+      const int c0 = xptr[g], cnt = xptr[g + 1] - c0;
+      int e[4] = {fw::SLOT_NONE, fw::SLOT_NONE, fw::SLOT_NONE, fw::SLOT_NONE};
+      if (cnt <= 4 && !c->gather_csr)
+        for (int j = 0; j < cnt; j++) e[j] = xslot[c0 + j];
+      else
+        e[1] = fw::SLOT_MORE;
+      x4[g] = make_int4(e[0], e[1], e[2], e[3]);
+    }
+    for (int raw : xslot)
+      if ((raw < 0 ? ~raw : raw) >= c->ss.slot) return fail(FWH_E_LIMIT, "matvec gather slot out of range");
+    if ((rc = c->upload(&c->gqx_ptr, xptr)) || (rc = c->upload(&c->gqx_slot, xslot)) ||
+        (rc = c->upload(&c->gqx_slot4, x4)))
+      return rc;
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+}  // namespace
+
 // Form every S_s = A_GG - A_GI A_II^-1 A_IG once: column k of all subdomains
 // in one batched launch of the Schur step on the unit vector e_k (the same
 // kernel, factors and summation order as the implicit iteration).
@@ -1764,11 +1979,12 @@ int build_schur(fwh_ctx *c) {
     std::vector<fw::SmvWork> work(all.size());
     for (size_t i = 0; i < all.size(); i++) {
       const int sd = all[i];
+      work[i] = fw::SmvWork{};
       work[i].off = off[sd];
-      work[i].g0 = c->meta_h[sd].g0;
+      work[i].y0 = c->meta_h[sd].g0;
+      work[i].gg = c->meta_h[sd].g0;
       work[i].ng = c->meta_h[sd].ng;
-      work[i].sub = sd;
-      work[i].pad = 0;
+      work[i].pq = sd;
     }
     if ((rc = c->upload(&d_orders, all)) || (rc = c->upload(&d_work, work))) return rc;
     // the orders are timed in turn over five rounds (pairs of back-to-back
@@ -1801,6 +2017,8 @@ int build_schur(fwh_ctx *c) {
     c->info.smv_order = pick;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    const std::vector<int> ord(all.begin() + (size_t)pick * c->nsub, all.begin() + (size_t)(pick + 1) * c->nsub);
+    if ((rc = tune_smv_split(c, ord, off, ncu))) return rc;
   }
   double packed = 0.0;
   for (int s = 0; s < c->nsub; s++) {
@@ -1868,7 +2086,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   if (o->plain) {
     while (it < n && !all) {
       HIPCHK(c->run_schur(expl, true, true, true));
-      HIPCHK(c->launch_vec_iter(true));
+      HIPCHK(c->launch_vec_iter(true, expl));
       if ((rc = read_scal(c, s, &all))) return rc;
       it++;
     }
@@ -1884,7 +2102,7 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
       // k_iter_c runs only after the last iteration of the graph
       for (int k = 0; k < o->graph_iters; k++) {
         hipError_t e = c->run_schur(expl, false, true, true);
-        if (e == hipSuccess) e = c->launch_vec_iter(!expl || k == o->graph_iters - 1);
+        if (e == hipSuccess) e = c->launch_vec_iter(!expl || k == o->graph_iters - 1, expl);
         if (e != hipSuccess) {
           hipGraph_t g = nullptr;
           (void)hipStreamEndCapture(c->st, &g);
@@ -1928,14 +2146,14 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
     }
     while (it < n && !all) {
       HIPCHK(c->run_schur(expl, false, true, true));
-      HIPCHK(c->launch_vec_iter(true));
+      HIPCHK(c->launch_vec_iter(true, expl));
       it++;
       if (it >= n && (rc = read_scal(c, s, &all))) return rc;
     }
   } else {
     while (it < n && !all) {
       HIPCHK(c->run_schur(expl, false, true, true));
-      HIPCHK(c->launch_vec_iter(true));
+      HIPCHK(c->launch_vec_iter(true, expl));
       it++;
       if ((it & 15) == 0 || it >= n)
         if ((rc = read_scal(c, s, &all))) return rc;
@@ -2107,7 +2325,7 @@ extern "C" int fwh_schur_apply(fwh_ctx *c, const double *p, double *q, int mode)
   const size_t o = (size_t)c->sel * c->ng;
   HIPCHK(hipMemcpyAsync(c->P + o, p, (size_t)c->ng * sizeof(double2), hipMemcpyHostToDevice, c->st));
   HIPCHK(c->run_schur(expl, (mode & 1) != 0, false, false, c->sel));
-  HIPCHK(fw::launch_gather_q(c->vec_args(c->sel), c->st));
+  HIPCHK(fw::launch_gather_q(c->vec_args(c->sel, expl), c->st));
   HIPCHK(hipMemcpyAsync(q, c->Q + o, (size_t)c->ng * sizeof(double2), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->first_done = false;

@@ -80,7 +80,8 @@ struct SysStride {
   long long nlen;  // skyline factor
   long long cpl;   // interior-interface coupling values
   long long grow;  // interface-row values
-  long long slot;  // per-slot arrays: yslot, pcslot
+  long long slot;  // per-slot outputs yslot (with the matvec's slot copies)
+  long long pcs;   // per-slot diagonal entries pcslot (the load's slots)
   long long g;     // interface vectors
   long long smat;  // Schur-complement tiles
   long long hist;  // residual history
@@ -185,12 +186,16 @@ struct AggArgs {
 // one workgroup of the explicit Schur matvec: where its subdomain's tiles,
 // slots and interface count are (one scalar load instead of order -> meta ->
 // tile offset)
+// one work item of the explicit Schur matvec: a subdomain, or a contiguous
+// range of its tiles (the largest subdomains are split over several items,
+// each writing its own copy of the subdomain's slots)
 struct SmvWork {
-  long long off;  // first tile entry
-  int g0;         // first slot
+  long long off;  // first tile entry of the subdomain
+  int y0;         // first output slot (yslot) of this item
+  int gg;         // first entry of the subdomain's interface ids (g_gid)
   int ng;         // interface DOFs
-  int sub;        // subdomain
-  int pad;
+  int pq;         // index of this item's part of (p,q) (pqpart)
+  int t0, t1;     // tiles [t0, t1) of the row-block-major sequence; t1 <= t0: all
 };
 
 struct SmvArgs {

@@ -661,7 +661,7 @@ __device__ __forceinline__ void to_system(VecArgs &a) {
   const long long s = a.sys0 + (long long)blockIdx.y;
   if (s == 0) return;
   a.yslot += s * a.ss.slot;
-  if (a.pcslot) a.pcslot += s * a.ss.slot;
+  if (a.pcslot) a.pcslot += s * a.ss.pcs;
   a.P += s * a.ss.g;
   a.Q += s * a.ss.g;
   a.R += s * a.ss.g;

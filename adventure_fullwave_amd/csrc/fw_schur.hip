@@ -277,9 +277,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   } else {
     const int w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
     wk.off = a.sm_off[w];
-    wk.g0 = a.meta[w].g0;
+    wk.y0 = a.meta[w].g0;
+    wk.gg = a.meta[w].g0;
     wk.ng = a.meta[w].ng;
-    wk.sub = w;
+    wk.pq = w;
+    wk.t0 = 0;
+    wk.t1 = 0;
   }
   const int ng = wk.ng, nb = (ng + 15) >> 4, npad = nb * 16, stride = a.nbmax * 16;
   extern __shared__ double2 lds[];
@@ -292,16 +295,18 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   if (ng == 0) {
     double2 beta;
     if (a.done && a.Z && blockIdx.x == 0 && a.base == 0) (void)iter_status(a, NT, beta, nullptr);  // it may have to stage
-    if (a.pqpart && tid == 0) a.pqpart[wk.sub] = cz();
+    if (a.pqpart && tid == 0) a.pqpart[wk.pq] = cz();
     return;
   }
 
-  // contiguous, equal shares of the tile sequence (row-block major); the
+  // contiguous, equal shares of the item's tiles (row-block major; all of
+  // the subdomain's, or the range [wk.t0, wk.t1) of a split one); the
   // first two tiles of every wave go in flight before the interface values
   // are gathered, so their latency overlaps the gather (a wave without tiles
   // loads tile 0, which every subdomain with an interface has)
   const long long ntile = (long long)nb * (nb + 1) / 2;
-  const long long t0 = ntile * wave / SMV_WAVES, t1 = ntile * (wave + 1) / SMV_WAVES;
+  const long long T0 = wk.t1 > wk.t0 ? wk.t0 : 0, T1 = wk.t1 > wk.t0 ? wk.t1 : ntile;
+  const long long t0 = T0 + (T1 - T0) * wave / SMV_WAVES, t1 = T0 + (T1 - T0) * (wave + 1) / SMV_WAVES;
   const bool active = t0 < t1;
   const long long tl = active ? t1 - 1 : 0;
   const double2 *base = a.smat + wk.off;
@@ -319,8 +324,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   const bool pre = W == 4 && a.done && a.Z;
   if (pre) status_prefetch(a, sp);
   const int k0 = tid, k1 = tid + NT;
-  const int gid0 = k0 < ng ? a.g_gid[wk.g0 + k0] : 0;
-  const int gid1 = k1 < ng ? a.g_gid[wk.g0 + k1] : 0;
+  const int gid0 = k0 < ng ? a.g_gid[wk.gg + k0] : 0;
+  const int gid1 = k1 < ng ? a.g_gid[wk.gg + k1] : 0;
   Tile A, B;
   load_next(base, zero, cur, lane, A);
   load_next(base, zero, cur, lane, B);
@@ -361,7 +366,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
         p = pb;
         z = zb;
       } else {
-        const int gid = a.g_gid[wk.g0 + k];
+        const int gid = a.g_gid[wk.gg + k];
         p = a.P[gid];
         z = a.Z ? a.Z[gid] : cz();
       }
@@ -398,7 +403,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   for (int k = tid; k < ng; k += NT) {
     double2 y = cadd(acc[k], acc[stride + k]);
     for (int v = 2; v < SMV_WAVES; v++) y = cadd(y, acc[v * stride + k]);
-    a.yslot[wk.g0 + k] = y;
+    a.yslot[wk.y0 + k] = y;
     pq = cadd(pq, cmul(pv[k], y));
   }
   if (a.pqpart) {
@@ -409,7 +414,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
     if (tid == 0) {
       double2 t = shq[0];
       for (int v = 1; v < SMV_WAVES; v++) t = cadd(t, shq[v]);
-      a.pqpart[wk.sub] = t;
+      a.pqpart[wk.pq] = t;
     }
   }
 }

@@ -321,7 +321,9 @@ def main():
                      "solve_depth": info["solve_depth"], "unit_depth": info["unit_depth"],
                      "ms_solve_by_depth_8_16": [round(v, 4) for v in info["ms_solve_depth"]],
                      "smv_order": info["smv_order"],
-                     "ms_smv_by_order": [round(v, 4) for v in info["ms_smv_order"]]},
+                     "ms_smv_by_order": [round(v, 4) for v in info["ms_smv_order"]],
+                     "smv_split": info["smv_split"], "smv_items": info["smv_nwork"],
+                     "ms_smv_by_split_0_2_4_8": [round(v, 4) for v in info["ms_smv_split"]]},
         "adjoint": adjoint,
         "sweep": sweep,
         # from the model in host memory: the whole load (host index work,

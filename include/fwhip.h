@@ -126,6 +126,12 @@ typedef struct fwh_info {
                            sum 8 ng (nlen - n)                             */
   double flops_schur_gemm; /* and of the tile products: sum over subdomains
                            8 nI ng (ng+1)/2 + 6 nI ng (nI interior rows)   */
+  int    smv_split;     /* explicit matvec: 0 one work item per subdomain;
+                           m > 0 subdomains split into items of at most
+                           total tiles / (nCU m) tiles (picked by timing)  */
+  int    smv_nwork;     /* its work items                                  */
+  double ms_smv_split[4]; /* its timings for m = 0, 2, 4, 8 (-1: no
+                           subdomain large enough to split)                */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */

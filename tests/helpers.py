@@ -1,4 +1,7 @@
 """Shared test helpers (global assembly from subdomain matrices, etc.)."""
+import contextlib
+import os
+
 import numpy as np
 import scipy.sparse as sp
 
@@ -95,3 +98,23 @@ def iface_gids(model):
                     lnmap[pp][lp] = lnmap[o][lo]
     return [lnmap[model.sub_part[s]][model.ib_ln[model.ib_off[s]:model.ib_off[s + 1]]]
             for s in range(model.nsub)]
+
+
+# load-time choices of the explicit matvec that set summation orders (waves
+# per workgroup, split of large subdomains): pinned where two contexts must
+# give the same bits
+MATVEC_PINS = {"FWH_SMV_WAVES": "4", "FWH_SMV_SPLIT": "2"}
+
+
+@contextlib.contextmanager
+def pinned_env(pins=MATVEC_PINS):
+    keep = {k: os.environ.get(k) for k in pins}
+    os.environ.update(pins)
+    try:
+        yield
+    finally:
+        for k, v in keep.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v

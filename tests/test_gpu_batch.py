@@ -4,10 +4,12 @@ Three systems share the small model's mesh and decomposition: two
 frequencies of the reference-mode absorber model and the delta-gap variant
 at a third frequency.  The batch solve (every launch serving all three) must
 reproduce three separate single-system solves: bit for bit when the
-load-time kernel choices agree (matvec waves, LDL^T prefetch depth, size
-buckets: they set summation orders), else to rel. L2 1e-9 with counts within
-2 %; and each system must match the oracle (count and solution bars per
-system from measured spreads, ORACLE_BARS).  Per-system accessors (select)
+load-time kernel choices agree (matvec waves and split, LDL^T prefetch
+depth, size buckets: they set summation orders), else with counts within
+2 % and solutions within twice the system's oracle bar (each of the two
+meets it; with other rounding the 1e-8 stop can come one iteration earlier
+or later); and each system must match the oracle (count and solution bars
+per system from measured spreads, ORACLE_BARS).  Per-system accessors (select)
 and the structure check are covered too."""
 import dataclasses
 import sys
@@ -16,11 +18,11 @@ import numpy as np
 import pytest
 
 from oracle import fw_oracle as fo
-from tests.helpers import with_delta_gap
+from tests.helpers import pinned_env, with_delta_gap
 
 pytestmark = pytest.mark.gpu
 
-CHOICES = ("smv_waves", "solve_depth", "solve_bucketed")
+CHOICES = ("smv_waves", "smv_split", "solve_depth", "solve_bucketed")
 
 
 def say(msg):
@@ -58,7 +60,7 @@ def test_batch_matches_single_solves(systems, schur):
             assert np.array_equal(rb, r1)
         else:
             assert abs(stats_b[k]["iters"] - st1["iters"]) <= max(1, 0.02 * st1["iters"])
-            assert np.linalg.norm(xb - x1) <= 1e-9 * np.linalg.norm(x1)
+            assert np.linalg.norm(xb - x1) <= 2 * ORACLE_BARS[k][1] * np.linalg.norm(x1)
         s1.close()
     b.close()
 
@@ -91,18 +93,21 @@ def test_batch_systems_match_oracle(systems):
 
 
 def test_batch_per_system_operators(systems):
-    """post, schur_apply and get_sub address the selected system"""
+    """post, schur_apply and get_sub address the selected system (the
+    matvec choices pinned, so that batch and single solves round alike)"""
     from adventure_fullwave_amd import fwhip
     say("[gpu] batch per-system operators")
-    b = fwhip.Solver(systems)
-    b.solve(eps=1e-8, max_loop=5000, graph_iters=8, schur=True)
+    with pinned_env():
+        b = fwhip.Solver(systems)
+        b.solve(eps=1e-8, max_loop=5000, graph_iters=8, schur=True)
     rng = np.random.default_rng(5)
     ng = int(b.info["n_iface"])
     p = rng.standard_normal(ng) + 1j * rng.standard_normal(ng)
     for k in (2, 0):
         m = systems[k]
-        s1 = fwhip.Solver(m)
-        s1.solve(eps=1e-8, max_loop=5000, graph_iters=8, schur=True)
+        with pinned_env():
+            s1 = fwhip.Solver(m)
+            s1.solve(eps=1e-8, max_loop=5000, graph_iters=8, schur=True)
         b.select(k)
         for fb, f1 in zip(b.post(), s1.post()):
             assert np.array_equal(fb, f1)

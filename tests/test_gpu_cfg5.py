@@ -42,7 +42,7 @@ EPS = 1e-12
 H = 1e-6          # central-difference step in eps_r
 REL_BAR = 1e-6    # relative, per element
 FLOOR = 1e-9      # of the largest gradient entry
-CHOICES = ("smv_waves", "solve_depth", "solve_bucketed")
+CHOICES = ("smv_waves", "smv_split", "solve_depth", "solve_bucketed")
 
 
 def say(msg):

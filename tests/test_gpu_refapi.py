@@ -8,7 +8,7 @@ solutions written to DomMesh.x_c, the element fields written to
 DomMesh.rvalue (post_sp), and the residual history written to
 HDDM_Opt.fp in the reference's log format (hddm_solver_s.h:205,321).  The
 load-time kernel choices that set summation orders are pinned for both
-(FWH_SMV_WAVES, FWH_SOLVE_DEPTH).  advlas_skyline_ldl_decomposite / _solve
+(FWH_SMV_WAVES, FWH_SMV_SPLIT, FWH_SOLVE_DEPTH).  advlas_skyline_ldl_decomposite / _solve
 with DMatrix arguments against the oracle's restatement of
 advlas_skyline.c:79-167."""
 import os
@@ -34,8 +34,9 @@ def say(msg):
 @pytest.fixture()
 def pinned(tmp_path):
     """pinned kernel choices and a working directory with MODEL/ (volt.dat)"""
-    keep = {k: os.environ.get(k) for k in ("FWH_SMV_WAVES", "FWH_SOLVE_DEPTH")}
+    keep = {k: os.environ.get(k) for k in ("FWH_SMV_WAVES", "FWH_SMV_SPLIT", "FWH_SOLVE_DEPTH")}
     os.environ["FWH_SMV_WAVES"] = "4"
+    os.environ["FWH_SMV_SPLIT"] = "2"
     os.environ["FWH_SOLVE_DEPTH"] = "16"
     (tmp_path / "MODEL").mkdir()
     old = os.getcwd()
