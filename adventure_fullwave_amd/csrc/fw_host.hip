@@ -1481,7 +1481,8 @@ int read_scal(fwh_ctx *c, std::vector<fw::Scal> &s, int *all_done = nullptr) {
 // part of (p,q).  Items follow the matvec's work order `ord`, a split
 // subdomain's items adjacent; an unsplit subdomain (and a split one's first
 // item) writes the load's slots and its subdomain's (p,q) part, further
-// items slots after them and parts after the nsub subdomain parts.  The
+// items slots after them and parts after the nsub subdomain parts, both
+// numbered in subdomain order (the work order schedules, nothing more).  The
 // gather of the explicit iteration sums a split subdomain's copies in item
 // order where the load's lists have its slot.  Candidates: items of at most
 // total tiles / (nCU m) tiles for m = 2, 4, 8; kept when more than 3 %
@@ -1503,12 +1504,24 @@ SmvSplit make_smv_split(const fwh_ctx *c, const std::vector<int> &ord, const std
   sp.count.assign(c->nsub, 1);
   sp.slots = c->nslots;
   sp.npq = c->nsub;
-  for (int s : ord) {
-    const SubMeta &M = c->meta_h[s];
+  // the further items' slots and (p,q) parts are numbered in subdomain
+  // order, so the sums over them (gather, alpha) do not depend on the work
+  // order, which only schedules
+  std::vector<long long> ybase(c->nsub);
+  std::vector<int> pqbase(c->nsub);
+  for (int s = 0; s < c->nsub; s++) {
     const int K = (tmax > 0 && ntile[s] > tmax) ? (int)((ntile[s] + tmax - 1) / tmax) : 1;
-    sp.first[s] = (int)sp.work.size();
     sp.count[s] = K;
     sp.split |= K > 1;
+    ybase[s] = sp.slots;
+    sp.slots += (long long)(K - 1) * c->meta_h[s].ng;
+    pqbase[s] = sp.npq;
+    sp.npq += K - 1;
+  }
+  for (int s : ord) {
+    const SubMeta &M = c->meta_h[s];
+    const int K = sp.count[s];
+    sp.first[s] = (int)sp.work.size();
     for (int k = 0; k < K; k++) {
       fw::SmvWork w{};
       w.off = off[s];
@@ -1518,9 +1531,8 @@ SmvSplit make_smv_split(const fwh_ctx *c, const std::vector<int> &ord, const std
         w.y0 = M.g0;
         w.pq = s;
       } else {
-        w.y0 = (int)sp.slots;
-        sp.slots += M.ng;
-        w.pq = sp.npq++;
+        w.y0 = (int)(ybase[s] + (long long)(k - 1) * M.ng);
+        w.pq = pqbase[s] + k - 1;
       }
       if (K > 1) {
         w.t0 = (int)(ntile[s] * k / K);
@@ -2012,6 +2024,11 @@ int build_schur(fwh_ctx *c) {
     int pick = 0;
     for (int v = 1; v < 3; v++)
       if (c->info.ms_smv_order[v] < c->info.ms_smv_order[pick]) pick = v;
+    // FWH_SMV_ORDER=0, 1 or 2 fixes the order (tests of order independence)
+    if (const char *e = getenv("FWH_SMV_ORDER")) {
+      const int v = atoi(e);
+      if (v >= 0 && v < 3) pick = v;
+    }
     c->smv_order = d_orders + (size_t)pick * c->nsub;
     c->smv_work = d_work + (size_t)pick * c->nsub;
     c->info.smv_order = pick;

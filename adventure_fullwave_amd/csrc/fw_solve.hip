@@ -100,43 +100,54 @@ __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__re
     issue_block(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
   }
 
-  // A: columns before the block; rows r = lane/4, four lanes per row
-  const int ra = lane >> 2, la = lane & 3;
-  double2 acc0 = cz(), acc1 = cz(), acc2 = cz(), acc3 = cz();
+  // A: columns before the block.  Lr lanes per row, Lr = 64 / nb rounded
+  // down to a power of two (at least 4), so a block of a few wide rows keeps
+  // every lane busy.  A lane takes every Lr-th column of its row, eight at a
+  // time: the batch's sixteen LDS reads are issued together (addresses
+  // clamped, terms past the row multiplied by zero), then added in column
+  // order into two partial sums; a butterfly over the row's lanes adds them.
+  const int lsh = nb <= 1 ? 6 : nb <= 2 ? 5 : nb <= 4 ? 4 : nb <= 8 ? 3 : 2;
+  const int Lr = 1 << lsh, ra = lane >> lsh, la = lane & (Lr - 1);
+  double2 acc0 = cz(), acc1 = cz();
   if (ra < nb) {
     const int fi = L.idx1[i0 + ra];
-    const int rbase = L.rb[i0 + ra] - e0 - fi;
-    int j = fi + la;
-    for (; j + 12 < i0; j += 16) {
-      acc0 = cfma(acc0, L.stg[rbase + j], L.v[j]);
-      acc1 = cfma(acc1, L.stg[rbase + j + 4], L.v[j + 4]);
-      acc2 = cfma(acc2, L.stg[rbase + j + 8], L.v[j + 8]);
-      acc3 = cfma(acc3, L.stg[rbase + j + 12], L.v[j + 12]);
+    const int rs = L.rb[i0 + ra] - e0;  // the row's first staged entry
+    const int rbase = rs - fi;
+    for (int j0 = fi + la; j0 < i0; j0 += 8 * Lr) {
+      double2 sv[8], vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int j = j0 + u * Lr;
+        const bool in = j < i0;
+        sv[u] = L.stg[in ? rbase + j : rs];
+        vv[u] = L.v[in ? j : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        acc0 = cfma(acc0, sv[u], j0 + u * Lr < i0 ? vv[u] : cz());
+        acc1 = cfma(acc1, sv[u + 1], j0 + (u + 1) * Lr < i0 ? vv[u + 1] : cz());
+      }
     }
-    for (; j < i0; j += 4) acc0 = cfma(acc0, L.stg[rbase + j], L.v[j]);
   }
-  double2 acc = cadd(cadd(acc0, acc1), cadd(acc2, acc3));
-  acc.x += __shfl_xor(acc.x, 1, 4);
-  acc.y += __shfl_xor(acc.y, 1, 4);
-  acc.x += __shfl_xor(acc.x, 2, 4);
-  acc.y += __shfl_xor(acc.y, 2, 4);
+  double2 acc = cadd(acc0, acc1);
+  for (int o = 1; o < Lr; o <<= 1) {
+    acc.x += __shfl_xor(acc.x, o, 64);
+    acc.y += __shfl_xor(acc.y, o, 64);
+  }
+  if (la == 0 && ra < nb) L.v[i0 + ra] = csub(L.v[i0 + ra], acc);
+  wave_sync();
 
   // B: the block's own triangle.  With s = v_B - (columns before the block)
   // and M = L_BB^-1 (unit lower, precomputed at load), x_r = s_r +
   // sum_{q<r} M_rq s_q: four lanes per row, one term of every fourth column
   // each, then a quad sum (no chain of dependent steps)
-  const int r = ra;
-  const bool own = la == 0 && r < nb;
-  double2 s = cz();
-  if (own) {
-    s = csub(L.v[i0 + r], acc);
-    L.v[i0 + r] = s;
-  }
-  wave_sync();
+  const int r = lane >> 2, l4 = lane & 3;
+  const bool own = l4 == 0 && r < nb;
+  const double2 s = own ? L.v[i0 + r] : cz();
   double2 t = cz();
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const int q = la + 4 * k;
+    const int q = l4 + 4 * k;
     const bool use = q < r && r < nb;
     const double2 mq = L.mst[use ? (r * (r - 1)) / 2 + q : 0];
     const double2 sq = L.v[use ? i0 + q : 0];

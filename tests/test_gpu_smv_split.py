@@ -84,3 +84,33 @@ def test_split_batch(small_sigma_model):
         assert np.linalg.norm(b.x() - s1.x()) <= 1e-7 * np.linalg.norm(s1.x())
         s1.close()
     b.close()
+
+
+def test_split_independent_of_work_order(small_sigma_model):
+    """The matvec's work order (picked at formation by timing) only
+    schedules: with split items, their slot copies and (p,q) parts are
+    numbered in subdomain order, so every order gives the same bits."""
+    from adventure_fullwave_amd import fwhip
+    out = []
+    for order in (0, 1, 2):
+        keep = os.environ.get("FWH_SMV_ORDER")
+        os.environ["FWH_SMV_ORDER"] = str(order)
+        try:
+            sol = formed(small_sigma_model, 2)
+        finally:
+            if keep is None:
+                del os.environ["FWH_SMV_ORDER"]
+            else:
+                os.environ["FWH_SMV_ORDER"] = keep
+        assert sol.info["smv_order"] == order and sol.info["smv_split"] == 2
+        rng = np.random.default_rng(3)
+        ng = int(sol.info["n_iface"])
+        p = rng.standard_normal(ng) + 1j * rng.standard_normal(ng)
+        q = sol.schur_apply(p, explicit=True)
+        st = sol.solve(eps=1e-8, max_loop=5000, graph_iters=16, schur=True)
+        say(f"[gpu] split m=2 order {order}: {st['iters']} iterations")
+        out.append((q, st["iters"], sol.x().copy(), sol.resid(st["loops"]).copy()))
+        sol.close()
+    for q, it, x, r in out[1:]:
+        assert np.array_equal(q, out[0][0]) and it == out[0][1]
+        assert np.array_equal(x, out[0][2]) and np.array_equal(r, out[0][3])
