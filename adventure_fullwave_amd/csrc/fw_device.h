@@ -167,4 +167,33 @@ __device__ __forceinline__ double2 readlane_c(double2 x, int lane) {
   return make_double2(readlane_d(x.x, lane), readlane_d(x.y, lane));
 }
 
+// a double moved between lanes by a DPP pattern (two 32-bit moves; every
+// lane active, patterns inside a 16-lane row)
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// the sum over each aligned group of 2^lsh lanes (lsh = 2..6) in every lane
+// of the group: quad_perm [1,0,3,2] and [2,3,0,1] (the quad sums),
+// row_half_mirror (adds the other quad of the 8-lane half-row),
+// row_mirror (the other half of the 16-lane row), then exchanges across
+// 16 and 32 lanes.  Each step adds the two partial sums of a pair of
+// disjoint lane sets, so every lane of a group ends with the same bits, the
+// bits of the xor butterfly over the same group.
+__device__ __forceinline__ double group_sum(double v, int lsh) {
+  v += dpp_mov_d<0xB1>(v);
+  v += dpp_mov_d<0x4E>(v);
+  if (lsh >= 3) v += dpp_mov_d<0x141>(v);
+  if (lsh >= 4) v += dpp_mov_d<0x140>(v);
+  if (lsh >= 5) v += __shfl_xor(v, 16, 64);
+  if (lsh >= 6) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+__device__ __forceinline__ double2 group_csum(double2 v, int lsh) {
+  return make_double2(group_sum(v.x, lsh), group_sum(v.y, lsh));
+}
+
 }  // namespace fw

@@ -181,6 +181,7 @@ struct fwh_ctx {
   // load by timing; the Schur-complement formation (hundreds of waves per
   // CU) uses 8, the depth with the higher occupancy, whenever every row fits
   int pfk_iter = 16, pfk_unit = 16;
+  int waves_iter = 1;  // waves per subdomain of the LDL^T kernel outside MODE_UNIT (1 or 4)
   bool short_ok = false;
   long long ndof = 0, nslots = 0, nnz = 0, nlen = 0, nel = 0, ncpl = 0, ngrow = 0;
   std::vector<SubMeta> meta_h;
@@ -363,6 +364,7 @@ struct fwh_ctx {
     a.nmax = nmax;
     a.ngmax = ngmax;
     a.pfk = mode == fw::MODE_UNIT ? pfk_unit : pfk_iter;
+    a.waves = mode == fw::MODE_UNIT ? 1 : waves_iter;
     a.nblkmax = a.pfk == 8 ? nblkmax_s : nblkmax;
     a.idx1 = idx1;
     a.rowbeg = rowbeg;
@@ -1188,9 +1190,9 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   }
   c->nlen = nlen_tot;
   c->wmax = wmax;
-  if (fw::solve_lds_bytes(nmax, ngmax, c->nblkmax, fw::PFK) > 160 * 1024)
+  if (fw::solve_lds_bytes(nmax, ngmax, c->nblkmax, fw::PFK, 1) > 160 * 1024)
     return fail(FWH_E_LIMIT, "subdomain of %d DOFs / %d interface DOFs exceeds the LDS budget", nmax, ngmax);
-  if (c->short_ok && fw::solve_lds_bytes(nmax, ngmax, c->nblkmax_s, 8) > 160 * 1024) c->short_ok = false;
+  if (c->short_ok && fw::solve_lds_bytes(nmax, ngmax, c->nblkmax_s, 8, 1) > 160 * 1024) c->short_ok = false;
   c->pfk_iter = fw::PFK;
   c->pfk_unit = fw::PFK;  // both picked after the factorisation, by timing
   if ((size_t)(wmax + 1 + nmax) * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "profile too wide for the factor kernel");
@@ -1210,7 +1212,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   {
     std::vector<int> klass(nsub);
     for (int s = 0; s < nsub; s++) {
-      const size_t lds = fw::solve_lds_bytes(meta[s].n, meta[s].ng, meta[s].nblk, fw::PFK);
+      const size_t lds = fw::solve_lds_bytes(meta[s].n, meta[s].ng, meta[s].nblk, fw::PFK, 1);
       klass[s] = (int)std::min<size_t>(8, (160 * 1024) / std::max<size_t>(lds, 1));
     }
     std::vector<int> bo(order);
@@ -1339,65 +1341,95 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   I.flops_factor = nsys * flops;
 
   // the batched LDL^T kernel of the iteration sweeps: prefetch depth 16 or 8
-  // (the short partition), one launch or one per size bucket, whichever is
-  // fastest on this model (one warm-up, then the best of three).  The depth
-  // sets the row blocks and with them the order of the sums, so 16 is kept
-  // unless 8 is more than 5 % faster: the choice does not flip between runs
-  // on timing noise.
+  // (the short partition), one or four waves per subdomain, one launch or
+  // one per size bucket, whichever is fastest on this model (one warm-up,
+  // then the best of three).  Depth and waves set the row blocks and the
+  // lanes per sum, and with them the order of the sums, so the choice keeps
+  // depth 16 with one wave unless another form is more than 5 % faster (in
+  // the order depth 8, then four waves at 16 and 8): it does not flip
+  // between runs on timing noise.
   c->use_buckets = false;
   {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    double best[2][2] = {{1e30, 1e30}, {1e30, 1e30}};  // [depth 16, 8][single, bucketed]
+    double best[2][2][2];  // [depth 16, 8][waves 1, 4][single, bucketed]; 1e30: not run
+    for (auto &x : best)
+      for (auto &y : x) y[0] = y[1] = 1e30;
     const int depths = c->short_ok ? 2 : 1;
     const int variants = c->buckets.size() > 1 ? 2 : 1;
+    // every launch of the form must fit a CU's LDS
+    auto fits = [&](int d, int w, int v) -> bool {
+      const int pfk = d == 0 ? fw::PFK : 8, wv = w == 0 ? 1 : 4;
+      if (v == 0) return fw::solve_lds_bytes(nmax, ngmax, d == 0 ? c->nblkmax : c->nblkmax_s, pfk, wv) <= 160 * 1024;
+      for (const auto &B : c->buckets)
+        if (fw::solve_lds_bytes(B.nmax, B.ngmax, d == 0 ? B.nblkmax : B.nblkmax_s, pfk, wv) > 160 * 1024) return false;
+      return true;
+    };
     for (int d = 0; d < depths; d++)
-      for (int v = 0; v < variants; v++) {
-        c->pfk_iter = d == 0 ? fw::PFK : 8;
-        fw::SolveArgs a = c->solve_args(fw::MODE_RHS, false);
-        // a timed warm-up; three more repetitions only for short launches
-        // (one suffices when a launch takes milliseconds)
-        float ms0 = 0.f;
-        HIPCHK(hipEventRecord(e0, st));
-        HIPCHK(c->launch_solve_all(a, v == 1));
-        HIPCHK(hipEventRecord(e1, st));
-        HIPCHK(hipEventSynchronize(e1));
-        HIPCHK(hipEventElapsedTime(&ms0, e0, e1));
-        const int reps = ms0 > 2.0f ? 1 : 3;
-        for (int r = 0; r < reps; r++) {
-          float ms = 0.f;
+      for (int w = 0; w < 2; w++)
+        for (int v = 0; v < variants; v++) {
+          if (!fits(d, w, v)) continue;
+          c->pfk_iter = d == 0 ? fw::PFK : 8;
+          c->waves_iter = w == 0 ? 1 : 4;
+          fw::SolveArgs a = c->solve_args(fw::MODE_RHS, false);
+          // a timed warm-up; three more repetitions only for short launches
+          // (one suffices when a launch takes milliseconds)
+          float ms0 = 0.f;
           HIPCHK(hipEventRecord(e0, st));
           HIPCHK(c->launch_solve_all(a, v == 1));
           HIPCHK(hipEventRecord(e1, st));
           HIPCHK(hipEventSynchronize(e1));
-          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-          best[d][v] = std::min(best[d][v], (double)ms);
+          HIPCHK(hipEventElapsedTime(&ms0, e0, e1));
+          const int reps = ms0 > 2.0f ? 1 : 3;
+          for (int r = 0; r < reps; r++) {
+            float ms = 0.f;
+            HIPCHK(hipEventRecord(e0, st));
+            HIPCHK(c->launch_solve_all(a, v == 1));
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            best[d][w][v] = std::min(best[d][w][v], (double)ms);
+          }
         }
-      }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    const double t16 = std::min(best[0][0], best[0][1]), t8 = std::min(best[1][0], best[1][1]);
-    int d = (depths == 2 && t8 < 0.95 * t16) ? 1 : 0;
-    c->pfk_iter = d == 0 ? fw::PFK : 8;
+    auto tm = [&](int d, int w) { return std::min(best[d][w][0], best[d][w][1]); };
+    int d = 0, w = 0;
+    const int cand[3][2] = {{1, 0}, {0, 1}, {1, 1}};
+    for (const auto &cw : cand)
+      if (tm(cw[0], cw[1]) < 0.95 * tm(d, w)) {
+        d = cw[0];
+        w = cw[1];
+      }
     // the Schur formation runs hundreds of waves per CU, where the 8-deep
     // kernel's higher occupancy pays unless its single solve is much slower
     // (wide profiles cut into many short blocks: cfg 3's large subdomains)
-    c->pfk_unit = (depths == 2 && t8 <= 1.2 * t16) ? 8 : fw::PFK;
-    // FWH_SOLVE_DEPTH (8 or 16) forces the depth of every launch kind (tests
-    // of each variant, experiments)
+    c->pfk_unit = (depths == 2 && tm(1, 0) <= 1.2 * tm(0, 0)) ? 8 : fw::PFK;
+    // FWH_SOLVE_DEPTH (8 or 16) forces the depth of every launch kind and
+    // FWH_SOLVE_WAVES (1 or 4) the waves outside the formation (tests of
+    // each form, experiments), where the form was timed (fits)
     if (const char *e = getenv("FWH_SOLVE_DEPTH")) {
       const int v = atoi(e);
       if (v == fw::PFK || (v == 8 && c->short_ok)) {
-        c->pfk_iter = c->pfk_unit = v;
+        c->pfk_unit = v;
         d = v == fw::PFK ? 0 : 1;
       }
     }
-    c->ms_single = best[d][0];
-    c->ms_bucketed = variants == 2 ? best[d][1] : 0.0;
-    c->use_buckets = variants == 2 && best[d][1] < best[d][0];
-    I.ms_solve_depth[0] = depths == 2 ? t8 : 0.0;
-    I.ms_solve_depth[1] = t16;
+    if (const char *e = getenv("FWH_SOLVE_WAVES")) {
+      const int v = atoi(e);
+      if (v == 1 || v == 4) w = v == 1 ? 0 : 1;
+    }
+    if (tm(d, w) >= 1e30) w = 0;  // the forced form does not fit: one wave
+    c->pfk_iter = d == 0 ? fw::PFK : 8;
+    c->waves_iter = w == 0 ? 1 : 4;
+    c->ms_single = best[d][w][0];
+    c->ms_bucketed = variants == 2 ? best[d][w][1] : 0.0;
+    c->use_buckets = variants == 2 && best[d][w][1] < best[d][w][0];
+    I.ms_solve_depth[0] = depths == 2 ? tm(1, w) : 0.0;
+    I.ms_solve_depth[1] = tm(0, w);
+    I.ms_solve_waves[0] = tm(d, 0);
+    I.ms_solve_waves[1] = tm(d, 1) < 1e30 ? tm(d, 1) : -1.0;
     HIPCHK(hipMemsetAsync(c->X, 0, (size_t)nsys * ndof * sizeof(double2), st));
     HIPCHK(hipStreamSynchronize(st));
   }
@@ -1407,6 +1439,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   I.ms_solve_bucketed = c->ms_bucketed;
   I.solve_depth = c->pfk_iter;
   I.unit_depth = c->pfk_unit;
+  I.solve_waves = c->waves_iter;
   c->loaded = true;
   return 0;
 }
@@ -2663,7 +2696,7 @@ int sky_one(fwh_ctx *c, int n, const int *idx1, const int *idx2, double *mat, do
   if (partition_rows(n, rb.data(), nlen, fw::SOLVE_THREADS * fw::PFK, blk))
     return fail(FWH_E_LIMIT, "a profile row is wider than %d entries", fw::SOLVE_THREADS * fw::PFK);
   const int nblk = (int)blk.size() - 1;
-  if (!factor && fw::solve_lds_bytes(n, 0, nblk, fw::PFK) > 160 * 1024)
+  if (!factor && fw::solve_lds_bytes(n, 0, nblk, fw::PFK, 1) > 160 * 1024)
     return fail(FWH_E_LIMIT, "a skyline of %d rows exceeds the LDS budget of the solve kernel", n);
   HIPCHK(hipSetDevice(c->device));
   SubMeta m{};
@@ -2727,6 +2760,7 @@ int sky_one(fwh_ctx *c, int n, const int *idx1, const int *idx2, double *mat, do
   a.nmax = n;
   a.nblkmax = nblk;
   a.pfk = fw::PFK;
+  a.waves = 1;
   a.idx1 = d_idx1;
   a.rowbeg = d_rb;
   a.fac = d_fac;

@@ -99,6 +99,7 @@ struct SolveArgs {
   int nmax, ngmax;       // LDS sizing
   int nblkmax;
   int pfk;               // prefetch loads per lane (8 or 16): blocks of <= 64*pfk entries
+  int waves;             // waves per subdomain: 1, or 4 (k_solve4, the same blocks)
   const int *idx1;       // per DOF, local first column of the profile
   const int *rowbeg;     // per DOF, start of row i within the subdomain's skyline
   const double2 *fac;
@@ -344,7 +345,7 @@ hipError_t launch_factor16(const FactorArgs &a, int nwork, hipStream_t st);
 hipError_t launch_blockinv(const SubMeta *meta, int nsub, const int *idx1, const int *rowbeg, const int *blk_row,
                            const double2 *fac, double2 *minv, hipStream_t st);
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st);
-size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk);
+size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk, int waves);
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st);
 hipError_t launch_blockinv16(const BlockInv16Args &a, hipStream_t st);
 hipError_t launch_schur_fwd(const FwdArgs &a, hipStream_t st);

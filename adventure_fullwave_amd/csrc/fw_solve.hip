@@ -84,6 +84,39 @@ struct SolveLds {
 // inverse triangle of block b (packed rows, MINV entries per block)
 __device__ __forceinline__ const double2 *minv_of(const SolveLds &L, int b) { return L.minv + (long long)b * MINV; }
 
+// Backward column updates of a block: v_j -= sum_r L_rj x_r for the columns
+// j in [jlo, i0) (thread t takes j = jlo + t, jlo + t + NT, ...), rows in
+// ascending order.  Row r's first column, base and solution value are on
+// lane xl * r of the wave (fR, rbR, xv).  R (4, 8 or 16, at least the
+// block's rows) rows are summed without a branch, so their R LDS reads are
+// issued together: a row past the block has fR = i0 > j and adds a valid
+// entry times zero.
+template <int R, int NT>
+__device__ __forceinline__ void col_updates(const SolveLds &L, int jlo, int i0, int fR, int rbR, double2 xv,
+                                            int xl) {
+  for (int j = jlo + (int)threadIdx.x; j < i0; j += NT) {
+    double2 acc = cz();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int f_u = readlane_i(fR, r), rb_u = readlane_i(rbR, r);
+      const double2 xr = readlane_c(xv, xl * r);
+      const bool in = j >= f_u;
+      acc = cfma(acc, L.stg[in ? rb_u + j : 0], in ? xr : cz());
+    }
+    L.v[j] = csub(L.v[j], acc);
+  }
+}
+template <int NT>
+__device__ __forceinline__ void col_updates_nb(const SolveLds &L, int nb, int jlo, int i0, int fR, int rbR,
+                                               double2 xv, int xl) {
+  if (nb <= 4)
+    col_updates<4, NT>(L, jlo, i0, fR, rbR, xv, xl);
+  else if (nb <= 8)
+    col_updates<8, NT>(L, jlo, i0, fR, rbR, xv, xl);
+  else
+    col_updates<BMAX, NT>(L, jlo, i0, fR, rbR, xv, xl);
+}
+
 // forward block b: rows [i0,i1) with skyline entries [e0,e1)
 template <int K>
 __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__restrict__ fac, int nblk, int b,
@@ -102,10 +135,10 @@ __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__re
 
   // A: columns before the block.  Lr lanes per row, Lr = 64 / nb rounded
   // down to a power of two (at least 4), so a block of a few wide rows keeps
-  // every lane busy.  A lane takes every Lr-th column of its row, eight at a
-  // time: the batch's sixteen LDS reads are issued together (addresses
+  // every lane busy.  A lane takes every Lr-th column of its row, four at a
+  // time: the batch's eight LDS reads are issued together (addresses
   // clamped, terms past the row multiplied by zero), then added in column
-  // order into two partial sums; a butterfly over the row's lanes adds them.
+  // order into two partial sums; a sum over the row's lanes adds them.
   const int lsh = nb <= 1 ? 6 : nb <= 2 ? 5 : nb <= 4 ? 4 : nb <= 8 ? 3 : 2;
   const int Lr = 1 << lsh, ra = lane >> lsh, la = lane & (Lr - 1);
   double2 acc0 = cz(), acc1 = cz();
@@ -113,27 +146,23 @@ __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__re
     const int fi = L.idx1[i0 + ra];
     const int rs = L.rb[i0 + ra] - e0;  // the row's first staged entry
     const int rbase = rs - fi;
-    for (int j0 = fi + la; j0 < i0; j0 += 8 * Lr) {
-      double2 sv[8], vv[8];
+    for (int j0 = fi + la; j0 < i0; j0 += 4 * Lr) {
+      double2 sv[4], vv[4];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
+      for (int u = 0; u < 4; u++) {
         const int j = j0 + u * Lr;
         const bool in = j < i0;
         sv[u] = L.stg[in ? rbase + j : rs];
         vv[u] = L.v[in ? j : 0];
       }
 #pragma unroll
-      for (int u = 0; u < 8; u += 2) {
+      for (int u = 0; u < 4; u += 2) {
         acc0 = cfma(acc0, sv[u], j0 + u * Lr < i0 ? vv[u] : cz());
         acc1 = cfma(acc1, sv[u + 1], j0 + (u + 1) * Lr < i0 ? vv[u + 1] : cz());
       }
     }
   }
-  double2 acc = cadd(acc0, acc1);
-  for (int o = 1; o < Lr; o <<= 1) {
-    acc.x += __shfl_xor(acc.x, o, 64);
-    acc.y += __shfl_xor(acc.y, o, 64);
-  }
+  const double2 acc = group_csum(cadd(acc0, acc1), lsh);
   if (la == 0 && ra < nb) L.v[i0 + ra] = csub(L.v[i0 + ra], acc);
   wave_sync();
 
@@ -153,10 +182,7 @@ __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__re
     const double2 sq = L.v[use ? i0 + q : 0];
     t = cfma(t, use ? mq : cz(), sq);
   }
-  t.x += __shfl_xor(t.x, 1, 4);
-  t.y += __shfl_xor(t.y, 1, 4);
-  t.x += __shfl_xor(t.x, 2, 4);
-  t.y += __shfl_xor(t.y, 2, 4);
+  t = group_csum(t, 2);
   if (own) L.v[i0 + r] = cadd(s, t);
 }
 
@@ -194,10 +220,7 @@ __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__re
     const double2 vr = L.v[use ? i0 + r : 0];
     t = cfma(t, use ? mr : cz(), vr);
   }
-  t.x += __shfl_xor(t.x, 1, 4);
-  t.y += __shfl_xor(t.y, 1, 4);
-  t.x += __shfl_xor(t.x, 2, 4);
-  t.y += __shfl_xor(t.y, 2, 4);
+  t = group_csum(t, 2);
   const bool own = la == 0 && qc < nb;
   double2 x = cz();
   if (own) x = cadd(L.v[i0 + qc], t);
@@ -211,19 +234,7 @@ __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__re
   for (int o = 32; o > 0; o >>= 1) jlo = min(jlo, __shfl_xor(jlo, o, 64));
   // (branch-free: a column outside row r's profile reads a valid entry and
   // multiplies it by zero, so the 16 LDS reads pipeline)
-  for (int j = jlo + lane; j < i0; j += WL) {
-    double2 acc = cz();
-#pragma unroll
-    for (int r = 0; r < BMAX; r++) {
-      if (r < nb) {
-        const int f_u = readlane_i(fR, r), rb_u = readlane_i(rbR, r);
-        const double2 xr = readlane_c(x, 4 * r);
-        const bool in = j >= f_u;
-        acc = cfma(acc, L.stg[in ? rb_u + j : 0], in ? xr : cz());
-      }
-    }
-    L.v[j] = csub(L.v[j], acc);
-  }
+  col_updates_nb<WL>(L, nb, jlo, i0, fR, rbR, x, 4);
 }
 
 // the value arrays of system sys0 + blockIdx.y (index arrays are shared)
@@ -246,53 +257,54 @@ __device__ __forceinline__ void to_system(SolveArgs &a) {
   a.minv += s * a.ss.minv;
 }
 
-template <int K>  // prefetch loads per lane: row blocks of at most 64*K entries
-__global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 3 : 2))) void k_solve(SolveArgs a) {
-  constexpr int CAPK = WL * K;
-  to_system(a);
-  // (the status of an iteration launch is read below, once the metadata
-  // loads are in flight)
-  const bool check_done = a.mode == MODE_ITER && a.done;
-  const int done0 = check_done ? *a.done : 0;
-  int w, uk = a.unit_k;
+// ---------------------------------------------------------------------------
+// Parts of the Schur step over a subdomain's DOFs, shared by both forms of
+// the kernel (NT threads, thread t takes DOFs t, t + NT, ...)
+
+// the subdomain's work item: (subdomain, unit vector) of MODE_UNIT, else
+// the subdomain; false: nothing to do
+__device__ __forceinline__ bool solve_item(const SolveArgs &a, int &w, int &uk) {
+  uk = a.unit_k;
   if (a.mode == MODE_UNIT && a.unit_list) {
     const int2 pr = a.unit_list[a.base + blockIdx.x];  // (subdomain, unit vector); -1: padding
-    if (pr.x < 0) return;
+    if (pr.x < 0) return false;
     w = pr.x;
     uk = pr.y;
   } else {
     w = a.order ? a.order[a.base + blockIdx.x] : a.base + blockIdx.x;
   }
-  const SubMeta m = a.meta[w];
-  if (a.mode == MODE_UNIT && uk >= m.ng) return;
-  extern __shared__ double2 lds[];
-  SolveLds L;
-  L.v = lds;                          // [nmax]
-  L.stg = L.v + a.nmax;               // [CAPK]
-  L.mst = L.stg + CAPK;               // [MINV]
-  L.idx1 = (int *)(L.mst + MINV);     // [nmax]
-  L.rb = L.idx1 + a.nmax;             // [nmax+1]
-  L.blk = L.rb + a.nmax + 1;          // [nblkmax+1]
-  const int lane = threadIdx.x;
-  const int n = m.n;
-  const double2 *fac = a.fac + m.fac0;
-  // row blocks of this prefetch depth (the host keeps both partitions)
-  const int nblk = K == 8 ? m.nblk_s : m.nblk;
-  const int blk0 = K == 8 ? m.blk0_s : m.blk0;
-  L.minv = a.minv + (long long)blk0 * MINV;
+  return true;
+}
 
-  // ---- 0. metadata --------------------------------------------------------------
-  for (int i = lane; i < n; i += WL) {
+// LDS carve-up: v[nmax], stg[cap], mst[nmst], idx1[nmax], rb[nmax+1],
+// blk[nblkmax+1]
+__device__ __forceinline__ SolveLds solve_lds(const SolveArgs &a, double2 *lds, int cap, int nmst) {
+  SolveLds L;
+  L.v = lds;
+  L.stg = L.v + a.nmax;
+  L.mst = L.stg + cap;
+  L.idx1 = (int *)(L.mst + nmst);
+  L.rb = L.idx1 + a.nmax;
+  L.blk = L.rb + a.nmax + 1;
+  return L;
+}
+
+template <int NT>
+__device__ __forceinline__ void load_meta(const SolveArgs &a, const SubMeta &m, const SolveLds &L, int nblk,
+                                          int blk0) {
+  const int t = threadIdx.x;
+  for (int i = t; i < m.n; i += NT) {
     L.idx1[i] = a.idx1[m.dof0 + i];
     L.rb[i] = a.rowbeg[m.dof0 + i];
   }
-  if (lane == 0) L.rb[n] = (int)m.nlen;
-  for (int b = lane; b <= nblk; b += WL) L.blk[b] = a.blk_row[blk0 + b];
-  __syncthreads();
-  if (done0) return;
+  if (t == 0) L.rb[m.n] = (int)m.nlen;
+  for (int b = t; b <= nblk; b += NT) L.blk[b] = a.blk_row[blk0 + b];
+}
 
-
-  // ---- 1. load vector --------------------------------------------------------------
+// the load vector (first/last step: f_I - A_IG v_G; iteration: -A_IG p_G;
+// unit vector; right-hand side) into v
+template <int NT>
+__device__ __forceinline__ void load_rhs(const SolveArgs &a, const SubMeta &m, double2 *v, int uk) {
   const bool zero_g = (a.mode == MODE_FIRST || a.mode == MODE_RHS);
   // interface values: P, or the next search direction z + beta p while the
   // interface iteration runs (formed here, not stored, see k_iter_ab)
@@ -301,7 +313,7 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
     const double2 p = a.P[gid];
     return a.Z ? zbp(a.Z[gid], beta, p) : p;
   };
-  for (int i = lane; i < n; i += WL) {
+  for (int i = threadIdx.x; i < m.n; i += NT) {
     const int d = m.dof0 + i;
     double2 b;
     if (a.mode == MODE_RHS) {
@@ -328,8 +340,78 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
             b = cfms(b, a.cp_val[c], pval(a.g_gid[m.g0 + a.cp_k[c]]));
       }
     }
-    L.v[i] = b;
+    v[i] = b;
   }
+}
+
+// outputs from the solution v: X (last step, right-hand side), a column of
+// S_s (unit vector), else y_G = (A x)_G (- f_G) per slot; returns this
+// thread's part of (p,q) (the interface iteration's sum over the
+// subdomain's slots of p y: an interface row's solution value is p itself,
+// its factor row being the identity and its column in L zero)
+template <int NT>
+__device__ __forceinline__ double2 write_outputs(const SolveArgs &a, const SubMeta &m, const double2 *v, int w,
+                                                 int uk) {
+  const int t = threadIdx.x;
+  double2 pq = cz();
+  if (a.mode == MODE_LAST || a.mode == MODE_RHS)
+    for (int i = t; i < m.n; i += NT) a.X[m.dof0 + i] = v[i];
+  if (a.mode == MODE_UNIT) {
+    // column unit_k of S_s: rows k >= unit_k into the lower-triangle tiles
+    // (layout in fw_schur.hip), the diagonal halved
+    const int j = uk, J = j >> 4, cj = j & 15;
+    const int nbk = (m.ng + 15) >> 4;
+    const bool hlf = smv_half(m.ng);
+    double2 *dst = a.smat + a.sm_off[w];
+    for (int k = j + t; k < m.ng; k += NT) {
+      const int sl = m.g0 + k;
+      double2 y = cz();
+      for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], v[a.gr_col[c]]);
+      if (k == j) y = cscale(y, 0.5);
+      dst[tile_pos(k >> 4, J, k & 15, cj, nbk, hlf)] = y;
+    }
+  } else if (a.mode != MODE_RHS) {
+    for (int k = t; k < m.ng; k += NT) {
+      const int sl = m.g0 + k;
+      double2 y = cz();
+      for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], v[a.gr_col[c]]);
+      if (a.mode != MODE_ITER) y = csub(y, a.f[m.dof0 + a.g_nfid[sl]]);
+      a.yslot[sl] = y;
+      pq = cadd(pq, cmul(v[a.g_nfid[sl]], y));
+    }
+  }
+  return pq;
+}
+
+template <int K>  // prefetch loads per lane: row blocks of at most 64*K entries
+__global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 3 : 2))) void k_solve(SolveArgs a) {
+  constexpr int CAPK = WL * K;
+  to_system(a);
+  // (the status of an iteration launch is read below, once the metadata
+  // loads are in flight)
+  const bool check_done = a.mode == MODE_ITER && a.done;
+  const int done0 = check_done ? *a.done : 0;
+  int w = 0, uk = 0;
+  if (!solve_item(a, w, uk)) return;
+  const SubMeta m = a.meta[w];
+  if (a.mode == MODE_UNIT && uk >= m.ng) return;
+  extern __shared__ double2 lds[];
+  SolveLds L = solve_lds(a, lds, CAPK, MINV);
+  const int lane = threadIdx.x;
+  const int n = m.n;
+  const double2 *fac = a.fac + m.fac0;
+  // row blocks of this prefetch depth (the host keeps both partitions)
+  const int nblk = K == 8 ? m.nblk_s : m.nblk;
+  const int blk0 = K == 8 ? m.blk0_s : m.blk0;
+  L.minv = a.minv + (long long)blk0 * MINV;
+
+  // ---- 0. metadata --------------------------------------------------------------
+  load_meta<WL>(a, m, L, nblk, blk0);
+  __syncthreads();
+  if (done0) return;
+
+  // ---- 1. load vector --------------------------------------------------------------
+  load_rhs<WL>(a, m, L.v, uk);
 
   // ---- 2. forward sweep: the first two blocks go in flight once the load
   // vector's own global reads are complete ----------------------------------
@@ -366,39 +448,230 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   wave_sync();
 
   // ---- 5. outputs ------------------------------------------------------------------------------
-  if (a.mode == MODE_LAST || a.mode == MODE_RHS)
-    for (int i = lane; i < n; i += WL) a.X[m.dof0 + i] = L.v[i];
-  if (a.mode == MODE_UNIT) {
-    // column unit_k of S_s: rows k >= unit_k into the lower-triangle tiles
-    // (layout in fw_schur.hip), the diagonal halved
-    const int j = uk, J = j >> 4, cj = j & 15;
-    const int nbk = (m.ng + 15) >> 4;
-    const bool hlf = smv_half(m.ng);
-    double2 *dst = a.smat + a.sm_off[w];
-    for (int k = j + lane; k < m.ng; k += WL) {
-      const int sl = m.g0 + k;
-      double2 y = cz();
-      for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], L.v[a.gr_col[c]]);
-      if (k == j) y = cscale(y, 0.5);
-      dst[tile_pos(k >> 4, J, k & 15, cj, nbk, hlf)] = y;
+  double2 pq = write_outputs<WL>(a, m, L.v, w, uk);
+  if (a.mode == MODE_ITER && a.pqpart) {
+    pq = wave_csum(pq);
+    if (lane == 0) a.pqpart[w] = pq;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same Schur step with four waves per subdomain (256 threads).  A sweep
+// stays a chain of row blocks, but the work of each block is spread over
+// the workgroup: every thread stages and prefetches a quarter of the block
+// (KQ loads each, blocks of at most 256*KQ entries: the partitions of the
+// single-wave kernel's depths 4*KQ), the dot products of the forward block
+// run with up to 64 lanes per row (one row per wave for blocks of at most
+// four rows), the backward block's column updates over 256 columns at a
+// time; wave 0 resolves the 16x16 triangles through L_BB^-1 as the
+// single-wave kernel does.  Workgroup barriers order the phases: forward,
+// staged -> dot products -> triangle (the next block's barrier orders the
+// triangle before its dot products; the inverse triangles are staged
+// alternately in two slots, so the next block's staging does not touch the
+// one wave 0 may still read); backward, staged -> triangle -> column updates
+// -> (all updates done before the next block is staged).  Sums run in a
+// fixed order, so the results do not depend on scheduling; the order differs
+// from the single-wave kernel's (more lanes per row), so the two forms agree
+// to rounding.
+constexpr int NT4 = 256;
+
+template <int KQ>
+struct PfSet4 {
+  Pf2 e[KQ];
+  Pf2 m;  // one entry of the block's inverted diagonal triangle (threads < MINV)
+};
+
+template <int KQ>
+__device__ __forceinline__ void issue_block4(const double2 *__restrict__ fac, int e0, int e1,
+                                             const double2 *__restrict__ minv, PfSet4<KQ> &p) {
+  const int t = threadIdx.x;
+  const int last = e1 - 1;
+#pragma unroll
+  for (int k = 0; k < KQ; k++) {
+    const double2 v = fac[min(e0 + t + k * NT4, last)];
+    p.e[k].x = v.x;
+    p.e[k].y = v.y;
+  }
+  const double2 v = minv[t & (MINV - 1)];  // threads >= MINV load a duplicate (same lines)
+  p.m.x = v.x;
+  p.m.y = v.y;
+}
+
+template <int KQ>
+__device__ __forceinline__ void stage_block4(double2 *stg, double2 *mst, const PfSet4<KQ> &p) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < KQ; k++) stg[t + k * NT4] = make_double2(p.e[k].x, p.e[k].y);
+  if (t < MINV) mst[t] = make_double2(p.m.x, p.m.y);
+}
+
+template <int KQ>
+__device__ __forceinline__ void fwd_block4(const SolveLds &L, const double2 *__restrict__ fac, int nblk, int b,
+                                           PfSet4<KQ> &p) {
+  const int t = threadIdx.x, lane = t & (WL - 1);
+  const int i0 = L.blk[b], i1 = L.blk[b + 1];
+  const int e0 = L.rb[i0];
+  const int nb = i1 - i0;
+  double2 *mst = L.mst + (b & 1) * MINV;
+  stage_block4(L.stg, mst, p);
+  __syncthreads();
+  {
+    const int bn = min(b + 2, nblk - 1);  // unconditional: see issue_block
+    issue_block4(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
+  }
+
+  // A: columns before the block, Lr lanes per row (a power of two, at most
+  // a wave: 64 for blocks of up to four rows, 32 up to eight, else 16), in
+  // batches of four columns per lane (a quarter of the single-wave kernel's
+  // columns per lane)
+  const int lsh = nb <= 4 ? 6 : nb <= 8 ? 5 : 4;
+  const int Lr = 1 << lsh, ra = t >> lsh, la = t & (Lr - 1);
+  double2 acc0 = cz(), acc1 = cz();
+  if (ra < nb) {
+    const int fi = L.idx1[i0 + ra];
+    const int rs = L.rb[i0 + ra] - e0;
+    const int rbase = rs - fi;
+    for (int j0 = fi + la; j0 < i0; j0 += 4 * Lr) {
+      double2 sv[4], vv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int j = j0 + u * Lr;
+        const bool in = j < i0;
+        sv[u] = L.stg[in ? rbase + j : rs];
+        vv[u] = L.v[in ? j : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u += 2) {
+        acc0 = cfma(acc0, sv[u], j0 + u * Lr < i0 ? vv[u] : cz());
+        acc1 = cfma(acc1, sv[u + 1], j0 + (u + 1) * Lr < i0 ? vv[u + 1] : cz());
+      }
     }
-  } else if (a.mode != MODE_RHS) {
-    // in an interface iteration also this subdomain's part of (p,q) =
-    // sum over its slots of p y: an interface row's solution value is p
-    // itself (its factor row is the identity, its column in L zero)
-    double2 pq = cz();
-    for (int k = lane; k < m.ng; k += WL) {
-      const int sl = m.g0 + k;
-      double2 y = cz();
-      for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) y = cfma(y, a.gr_val[c], L.v[a.gr_col[c]]);
-      if (a.mode != MODE_ITER) y = csub(y, a.f[m.dof0 + a.g_nfid[sl]]);
-      a.yslot[sl] = y;
-      pq = cadd(pq, cmul(L.v[a.g_nfid[sl]], y));
+  }
+  const double2 acc = group_csum(cadd(acc0, acc1), lsh);
+  if (la == 0 && ra < nb) L.v[i0 + ra] = csub(L.v[i0 + ra], acc);
+  __syncthreads();
+
+  // B: the block's triangle on wave 0 (as in fwd_block)
+  if (t < WL) {
+    const int r = lane >> 2, l4 = lane & 3;
+    const bool own = l4 == 0 && r < nb;
+    const double2 s = own ? L.v[i0 + r] : cz();
+    double2 tt = cz();
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int q = l4 + 4 * k;
+      const bool use = q < r && r < nb;
+      const double2 mq = mst[use ? (r * (r - 1)) / 2 + q : 0];
+      const double2 sq = L.v[use ? i0 + q : 0];
+      tt = cfma(tt, use ? mq : cz(), sq);
     }
-    if (a.mode == MODE_ITER && a.pqpart) {
-      pq = wave_csum(pq);
-      if (lane == 0) a.pqpart[w] = pq;
+    tt = group_csum(tt, 2);
+    if (own) L.v[i0 + r] = cadd(s, tt);
+  }
+}
+
+template <int KQ>
+__device__ __forceinline__ void bwd_block4(const SolveLds &L, const double2 *__restrict__ fac, int b,
+                                           PfSet4<KQ> &p) {
+  const int t = threadIdx.x, lane = t & (WL - 1);
+  const int i0 = L.blk[b], i1 = L.blk[b + 1];
+  const int e0 = L.rb[i0];
+  const int nb = i1 - i0;
+  double2 *mst = L.mst + (b & 1) * MINV;
+  stage_block4(L.stg, mst, p);
+  __syncthreads();
+  {
+    const int bn = max(b - 2, 0);
+    issue_block4(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
+  }
+  // row metadata of the block, in every wave: lane r holds row r's first
+  // column and base
+  int fR = i0, rbR = 0;
+  if (lane < nb) {
+    fR = L.idx1[i0 + lane];
+    rbR = L.rb[i0 + lane] - e0 - fR;
+  }
+  // A: triangle through M^T on wave 0 (as in bwd_block)
+  if (t < WL) {
+    const int qc = lane >> 2, l4 = lane & 3;
+    double2 tt = cz();
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int r = qc + 1 + l4 + 4 * k;
+      const bool use = r < nb;
+      const double2 mr = mst[use ? (r * (r - 1)) / 2 + qc : 0];
+      const double2 vr = L.v[use ? i0 + r : 0];
+      tt = cfma(tt, use ? mr : cz(), vr);
     }
+    tt = group_csum(tt, 2);
+    const bool own = l4 == 0 && qc < nb;
+    double2 x = cz();
+    if (own) x = cadd(L.v[i0 + qc], tt);
+    wave_sync();
+    if (own) L.v[i0 + qc] = x;
+  }
+  __syncthreads();
+
+  // B: every column before the block receives all rows of the block (row
+  // r's solution value on lane r of every wave)
+  const double2 xR = lane < nb ? L.v[i0 + lane] : cz();
+  int jlo = fR;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) jlo = min(jlo, __shfl_xor(jlo, o, 64));
+  col_updates_nb<NT4>(L, nb, jlo, i0, fR, rbR, xR, 1);
+  __syncthreads();
+}
+
+template <int KQ>  // prefetch loads per thread: row blocks of at most 256*KQ entries
+__global__ __launch_bounds__(NT4) __attribute__((amdgpu_waves_per_eu(4))) void k_solve4(SolveArgs a) {
+  constexpr int CAPQ = NT4 * KQ;
+  to_system(a);
+  const bool check_done = a.mode == MODE_ITER && a.done;
+  const int done0 = check_done ? *a.done : 0;
+  int w = 0, uk = 0;
+  if (!solve_item(a, w, uk)) return;
+  const SubMeta m = a.meta[w];
+  if (a.mode == MODE_UNIT && uk >= m.ng) return;
+  extern __shared__ double2 lds[];
+  SolveLds L = solve_lds(a, lds, CAPQ, 2 * MINV);
+  const int t = threadIdx.x;
+  const int n = m.n;
+  const double2 *fac = a.fac + m.fac0;
+  // the partition of the single-wave kernel whose blocks hold 256*KQ entries
+  const int nblk = KQ == 2 ? m.nblk_s : m.nblk;
+  const int blk0 = KQ == 2 ? m.blk0_s : m.blk0;
+  L.minv = a.minv + (long long)blk0 * MINV;
+
+  load_meta<NT4>(a, m, L, nblk, blk0);
+  __syncthreads();
+  if (done0) return;
+  load_rhs<NT4>(a, m, L.v, uk);
+  PfSet4<KQ> pa, pb;
+  issue_block4(fac, L.rb[L.blk[0]], L.rb[L.blk[1]], minv_of(L, 0), pa);
+  issue_block4(fac, L.rb[L.blk[1]], L.rb[L.blk[2]], minv_of(L, 1), pb);
+  __syncthreads();  // the load vector is complete
+  for (int b = 0; b < nblk; b += 2) {
+    fwd_block4(L, fac, nblk, b, pa);
+    fwd_block4(L, fac, nblk, b + 1, pb);
+  }
+  __syncthreads();  // the last triangle is complete
+  for (int i = t; i < n; i += NT4) L.v[i] = cmul(L.v[i], a.dinv[m.dof0 + i]);
+  const int nbl = nblk - 1;
+  issue_block4(fac, L.rb[L.blk[nbl]], L.rb[L.blk[nbl + 1]], minv_of(L, nbl), pa);
+  issue_block4(fac, L.rb[L.blk[nbl - 1]], L.rb[L.blk[nbl]], minv_of(L, nbl - 1), pb);
+  __syncthreads();
+  for (int b = nbl; b >= 1; b -= 2) {
+    bwd_block4(L, fac, b, pa);
+    bwd_block4(L, fac, b - 1, pb);
+  }
+  // (the last backward block ends with a barrier)
+  double2 pq = write_outputs<NT4>(a, m, L.v, w, uk);
+  if (a.mode == MODE_ITER && a.pqpart) {
+    __shared__ double2 shq[NT4 / WL];
+    pq = wave_csum(pq);
+    if ((t & (WL - 1)) == 0) shq[t / WL] = pq;
+    __syncthreads();
+    if (t == 0) a.pqpart[w] = cadd(cadd(shq[0], shq[1]), cadd(shq[2], shq[3]));
   }
 }
 
@@ -440,16 +713,30 @@ hipError_t launch_blockinv(const SubMeta *meta, int nsub, const int *idx1, const
   return hipGetLastError();
 }
 
-size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk) {
+size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk, int waves) {
   (void)ngmax;
+  // the four-wave form: the same block capacity (64 pfk entries), two slots
+  // for the inverse triangles, and the (p,q) partials of its waves
+  if (waves == 4)
+    return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + 2 * MINV + NT4 / WL) +
+           sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
   return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + MINV) +
          sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
 }
 
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
   if (nwork <= 0) return hipSuccess;
-  const size_t lds = solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk);
   const dim3 grid(nwork, a.nsys > 0 ? a.nsys : 1);
+  if (a.waves == 4) {
+    // the static (p,q) partials are not part of the dynamic size
+    const size_t lds = solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk, 4) - sizeof(double2) * (NT4 / WL);
+    if (a.pfk == 8)
+      k_solve4<2><<<grid, NT4, lds, st>>>(a);
+    else
+      k_solve4<4><<<grid, NT4, lds, st>>>(a);
+    return hipGetLastError();
+  }
+  const size_t lds = solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk, 1);
   if (a.pfk == 8)
     k_solve<8><<<grid, WL, lds, st>>>(a);
   else

@@ -68,7 +68,8 @@ class FwhInfo(ctypes.Structure):
                 ("ms_factor_kernel", ctypes.c_double), ("ms_schur_fwd", ctypes.c_double),
                 ("ms_schur_gemm", ctypes.c_double), ("flops_schur_fwd", ctypes.c_double),
                 ("flops_schur_gemm", ctypes.c_double), ("smv_split", ctypes.c_int),
-                ("smv_nwork", ctypes.c_int), ("ms_smv_split", ctypes.c_double * 4)]
+                ("smv_nwork", ctypes.c_int), ("ms_smv_split", ctypes.c_double * 4),
+                ("solve_waves", ctypes.c_int), ("ms_solve_waves", ctypes.c_double * 2)]
 
     def as_dict(self):
         out = {}

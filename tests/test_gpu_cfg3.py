@@ -110,3 +110,40 @@ def test_cfg3_residuals_and_global_residual(cfg3, cfg3_om, cfg3_gpu):
     d = np.linalg.norm(xs[True] - xs[False]) / np.linalg.norm(xs[False])
     say(f"[gpu] cfg3 explicit vs implicit solution: rel. L2 {d:.2e}")
     assert d <= 1e-6
+
+
+@pytest.mark.parametrize("waves", [1, 4])
+def test_cfg3_solve_forms(cfg3, cfg3_om, waves):
+    """Both forms of the batched LDL^T kernel (one wave per subdomain, or
+    four; forced with FWH_SOLVE_WAVES) on cfg 3, whose large subdomains have
+    blocks of one to four wide rows: the implicit Schur step against the
+    oracle (1e-10 relative) and the largest subdomain's LDL^T solve against
+    the oracle's (1e-10 relative)."""
+    from adventure_fullwave_amd import fwhip
+    from tests.helpers import pinned_env
+    with pinned_env({"FWH_SOLVE_WAVES": str(waves)}):
+        sol = fwhip.Solver(cfg3)
+    try:
+        if sol.info["solve_waves"] != waves:
+            pytest.skip(f"{waves} waves do not fit this model's LDS")
+        gids = iface_gids(cfg3)
+        ng = int(sol.info["n_iface"])
+        rng = np.random.default_rng(33)
+        p = rng.standard_normal(ng) + 1j * rng.standard_normal(ng)
+        ref = oracle_q(cfg3, cfg3_om, gids, p)
+        q = sol.schur_apply(p)
+        err = np.abs(q - ref).max() / np.abs(ref).max()
+        say(f"[gpu] cfg3 implicit Schur step, {waves} wave(s): max rel. error {err:.2e}")
+        assert err <= 1e-10
+        big = int(np.argmax(cfg3.nmp))
+        off = int(np.sum(cfg3.nmp[:big]))
+        nd = int(sol.info["n_dof"])
+        rhs = rng.standard_normal(nd) + 1j * rng.standard_normal(nd)
+        x = sol.ldl_solve_all(rhs.copy())
+        g = cfg3_om.get(big)
+        xo = fo.sky_solve(g["idx1"], g["idx2"], g["factor"], rhs[off:off + g["n"]])
+        err = np.abs(x[off:off + g["n"]] - xo).max() / np.abs(xo).max()
+        say(f"[gpu] cfg3 largest subdomain LDL^T solve, {waves} wave(s): max rel. error {err:.2e}")
+        assert err <= 1e-10
+    finally:
+        sol.close()

@@ -1,13 +1,14 @@
-"""Both variants of the batched LDL^T kernel (run with -m gpu).
+"""Every form of the batched LDL^T kernel (run with -m gpu).
 
 The load-time tuner picks the prefetch depth (8 or 16 loads per lane, with
-a row-block partition for each) by timing, so a given run may exercise only
-one of them in the iteration sweeps.  Here each depth is forced
-(FWH_SOLVE_DEPTH, read at load) for every launch kind and checked against
-the oracle: the subdomain LDL^T solves (1e-10
-relative), the implicit Schur step and the explicit S_s formed with it
-(1e-10), and the full HDDM solve (iteration count within 2 %, solution rel.
-L2 1e-7)."""
+a row-block partition for each) and the waves per subdomain (1, or 4 with
+the same row blocks) by timing, so a given run may exercise only one form
+in the iteration sweeps.  Here each form is forced (FWH_SOLVE_DEPTH for
+every launch kind, FWH_SOLVE_WAVES outside the formation, read at load) and
+checked against the oracle: the subdomain LDL^T solves (1e-10 relative; one
+launch per subdomain gives the same bits), the implicit Schur step and the
+explicit S_s formed with it (1e-10), and the full HDDM solve (iteration
+count within 2 %, solution rel. L2 1e-7)."""
 import os
 import sys
 
@@ -30,25 +31,23 @@ def sigma_ref(small_sigma_model):
     return fo.hddm_solve(small_sigma_model, eps=1e-8, max_loop=5000)
 
 
-@pytest.mark.parametrize("depth", [16, 8])
-def test_solve_variant(small_sigma_model, sigma_ref, depth):
+@pytest.mark.parametrize("depth,waves", [(16, 1), (8, 1), (16, 4), (8, 4)])
+def test_solve_variant(small_sigma_model, sigma_ref, depth, waves):
     from adventure_fullwave_amd import fwhip
     model = small_sigma_model
-    say(f"[gpu] LDL^T variant depth={depth}")
-    os.environ["FWH_SOLVE_DEPTH"] = str(depth)
-    try:
+    say(f"[gpu] LDL^T variant depth={depth} waves={waves}")
+    with pinned_env({"FWH_SOLVE_DEPTH": str(depth), "FWH_SOLVE_WAVES": str(waves)}):
         sol = fwhip.Solver(model)
-    finally:
-        del os.environ["FWH_SOLVE_DEPTH"]
-    if sol.info["solve_depth"] != depth or sol.info["unit_depth"] != depth:
+    if sol.info["solve_depth"] != depth or sol.info["unit_depth"] != depth or sol.info["solve_waves"] != waves:
         sol.close()
-        pytest.skip(f"depth {depth} not available on this model")
+        pytest.skip(f"depth {depth} with {waves} waves not available on this model")
     om = fo.OracleModel(model)
     # subdomain LDL^T solves against the oracle's factors
     rng = np.random.default_rng(depth)
     nd = int(sol.info["n_dof"])
     rhs = rng.standard_normal(nd) + 1j * rng.standard_normal(nd)
     x = sol.ldl_solve_all(rhs.copy())
+    assert np.array_equal(sol.ldl_solve_all(rhs.copy(), plain=True), x)
     off = 0
     for s in range(model.nsub):
         g = om.get(s)
