@@ -33,14 +33,23 @@ EDGE = np.array([[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]], dtype=np.int64
 EPS0 = 8.854e-12          # vacuum permittivity as written in the reference's mtrl.dat
 C0 = 299792458.0
 ETA0 = 376.730313668
+# cfg 3: radius of the sphere scatterer (box of unit side), and the share of
+# a subdomain's edge DOFs its cells' own edges make up (the rest lie on the
+# faces it shares; more of them in small subdomains) at the smallest and
+# the largest target size, so that subdomain sizes follow their targets
+CFG3_RADIUS = 0.3
+CFG3_FILL = (0.9, 0.78)
 
 
 # --------------------------------------------------------------------------
 # geometry
 # --------------------------------------------------------------------------
-def kuhn_box(xs: np.ndarray, ys: np.ndarray, zs: np.ndarray):
+def kuhn_box(xs: np.ndarray, ys: np.ndarray, zs: np.ndarray, orient: bool = True):
     """Vertices and positively oriented tetrahedra of a (possibly graded)
     tensor grid, six tetrahedra per hex cell around the cell diagonal.
+    orient=False keeps every tetrahedron in path order (cell corner 0, then
+    one, two and three unit steps to the opposite corner), the order the
+    bisection refinement needs.
 
     Returns (crd[nv,3], tets[ne,4], cell_of_tet[ne], (nx,ny,nz))."""
     nx, ny, nz = len(xs) - 1, len(ys) - 1, len(zs) - 1
@@ -61,13 +70,150 @@ def kuhn_box(xs: np.ndarray, ys: np.ndarray, zs: np.ndarray):
         b2 = b1 | (1 << perm[1])
         tets.append(np.stack([corner[0], corner[b1], corner[b2], corner[7]], axis=1))
     ncell = nx * ny * nz
-    tets = np.stack(tets, axis=1).reshape(ncell * 6, 4)
+    tets = np.stack(tets, axis=1).reshape(ncell * 6, 4).astype(np.int64)
     cell = np.repeat(np.arange(ncell), 6)
+    if orient:
+        tets = orient_tets(crd, tets)
+    return crd, tets, cell, (nx, ny, nz)
+
+
+def orient_tets(crd: np.ndarray, tets: np.ndarray) -> np.ndarray:
+    """The tetrahedra with vertices 1 and 2 swapped where the six-times
+    volume would be negative."""
+    tets = tets.copy()
     v = crd[tets]
     d = np.einsum("ij,ij->i", v[:, 1] - v[:, 0], np.cross(v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]))
     neg = d < 0
     tets[neg, 1], tets[neg, 2] = tets[neg, 2].copy(), tets[neg, 1].copy()
-    return crd, tets.astype(np.int64), cell, (nx, ny, nz)
+    return tets
+
+
+def refine_ball(crd: np.ndarray, tets: np.ndarray, cell: np.ndarray, center, radius: float, active_radius: float,
+                generations: int = 3):
+    """Conforming local refinement of a Kuhn mesh (tetrahedra in path order)
+    by newest-vertex bisection: every tetrahedron whose centroid lies within
+    `radius` of `center` is bisected until it is `generations` bisections
+    below the base mesh (three halve the mesh size).
+
+    Bisection follows Maubach's rule for tagged simplices (J. M. Maubach,
+    SIAM J. Sci. Comput. 16 (1995) 210-227): a tetrahedron (x0, x1, x2, x3)
+    with tag k is cut at the midpoint z of its refinement edge (x0, xk) into
+    (x0, .., x_{k-1}, z, x_{k+1}, .., x3) and (x1, .., xk, z, x_{k+1}, .., x3),
+    both with tag k - 1 (3 after 1); the base tetrahedra have tag 3 (their
+    refinement edge is the cell diagonal).  Conformity: before the
+    tetrahedra around an edge are cut, every one of them whose refinement
+    edge differs is refined first (recursively), so all of them share the
+    edge as refinement edge and are cut together at the same midpoint.  On
+    Kuhn meshes this closure terminates.
+
+    Only the tetrahedra with centroids within `active_radius` take part; a
+    closure that would reach past them raises.  Returns (crd, tets in path
+    order, base cell of every tetrahedron, bisections of every tetrahedron)."""
+    center = np.asarray(center, dtype=float)
+    cent = crd[tets].mean(axis=1)
+    act = np.linalg.norm(cent - center, axis=1) < active_radius
+    frozen = tets[~act]
+    verts = [tuple(c) for c in crd]
+    T: Dict[int, list] = {}
+    E: Dict[tuple, set] = {}
+    mids: Dict[tuple, int] = {}
+
+    pairs = tuple((int(i), int(j)) for i, j in EDGE)
+
+    def ekey(a, b):
+        return (a, b) if a < b else (b, a)
+
+    def add(t, rec):
+        T[t] = rec
+        x = rec[0]
+        for i, j in pairs:
+            a, b = x[i], x[j]
+            key = (a, b) if a < b else (b, a)
+            got = E.get(key)
+            if got is None:
+                E[key] = {t}
+            else:
+                got.add(t)
+
+    def remove(t):
+        rec = T.pop(t)
+        x = rec[0]
+        for i, j in pairs:
+            a, b = x[i], x[j]
+            E[(a, b) if a < b else (b, a)].discard(t)
+        return rec
+
+    nid = [0]
+    for t, c in zip(np.nonzero(act)[0], cell[act]):
+        add(nid[0], [tuple(int(v) for v in tets[t]), 3, 0, int(c)])
+        nid[0] += 1
+
+    def ref_edge(rec):
+        x, k = rec[0], rec[1]
+        return ekey(x[0], x[k])
+
+    def bisect(t):
+        x, k, g, c = remove(t)
+        e = ekey(x[0], x[k])
+        z = mids.get(e)
+        if z is None:
+            a, b = verts[e[0]], verts[e[1]]
+            z = len(verts)
+            verts.append(((a[0] + b[0]) * 0.5, (a[1] + b[1]) * 0.5, (a[2] + b[2]) * 0.5))
+            mids[e] = z
+        k2 = k - 1 if k > 1 else 3
+        add(nid[0], [x[:k] + (z,) + x[k + 1:], k2, g + 1, c])
+        add(nid[0] + 1, [x[1:k + 1] + (z,) + x[k + 1:], k2, g + 1, c])
+        nid[0] += 2
+
+    def refine(t):
+        stack = [t]
+        while stack:
+            u = stack[-1]
+            if u not in T:
+                stack.pop()
+                continue
+            e = ref_edge(T[u])
+            around = E[e]
+            blocker = next((w for w in around if ref_edge(T[w]) != e), None)
+            if blocker is not None:
+                stack.append(blocker)
+                continue
+            for w in list(around):
+                bisect(w)
+            stack.pop()
+
+    c2 = center * 1.0
+    r2 = radius * radius
+    changed = True
+    while changed:
+        changed = False
+        for t in list(T):
+            rec = T.get(t)
+            if rec is None or rec[2] >= generations:
+                continue
+            x = rec[0]
+            cx = sum(verts[v][0] for v in x) * 0.25 - c2[0]
+            cy = sum(verts[v][1] for v in x) * 0.25 - c2[1]
+            cz = sum(verts[v][2] for v in x) * 0.25 - c2[2]
+            if cx * cx + cy * cy + cz * cz < r2:
+                refine(t)
+                changed = True
+    # the frozen tetrahedra must not have an edge that was cut
+    if mids:
+        cut = np.array(sorted(mids), dtype=np.int64)
+        key_cut = cut[:, 0] * len(verts) + cut[:, 1]
+        a = frozen[:, EDGE[:, 0]]
+        b = frozen[:, EDGE[:, 1]]
+        key = np.minimum(a, b) * len(verts) + np.maximum(a, b)
+        if np.isin(key, key_cut).any():
+            raise ValueError("refinement closure reached past the active region")
+    recs = list(T.values())
+    new_tets = np.array([r[0] for r in recs], dtype=np.int64).reshape(-1, 4)
+    out_tets = np.concatenate([frozen, new_tets])
+    out_cell = np.concatenate([cell[~act], np.array([r[3] for r in recs], dtype=np.int64)])
+    out_gen = np.concatenate([np.zeros(len(frozen), np.int64), np.array([r[2] for r in recs], dtype=np.int64)])
+    return np.array(verts, dtype=float), out_tets, out_cell, out_gen
 
 
 def edges_of(tets: np.ndarray, nv: int):
@@ -363,6 +509,43 @@ def block_partition(dims, sub_grid, part_grid):
     return new_id[blk], bpart[rank]
 
 
+def morton_order(dims) -> np.ndarray:
+    """The cells (id i + nx (j + ny k)) in Morton (Z-curve) order."""
+    nx, ny, nz = dims
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    i, j, k = (a.transpose(2, 1, 0).ravel() for a in (i, j, k))
+
+    def spread(v):
+        v = v.astype(np.uint64)
+        out = np.zeros_like(v)
+        for b in range(11):
+            out |= ((v >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b)
+        return out
+    code = spread(i) | (spread(j) << np.uint64(1)) | (spread(k) << np.uint64(2))
+    return np.argsort(code, kind="stable")
+
+
+def weighted_sfc_partition(dims, cell_weight: np.ndarray, target_sizes: np.ndarray, nparts: int):
+    """Uneven subdomains of a locally refined grid: the Morton-ordered cells
+    cut where the running weight (edge DOFs of the cell's tetrahedra) passes
+    the running sum of the target sizes; parts group consecutive
+    subdomains."""
+    order = morton_order(dims)
+    ncell = len(order)
+    run = np.cumsum(cell_weight[order])
+    bounds = np.cumsum(target_sizes)
+    bounds = bounds[bounds < run[-1]]
+    sub_seq = np.searchsorted(bounds, run - 0.5 * cell_weight[order], side="right")
+    nsub = int(sub_seq.max()) + 1
+    # drop empty ids (a cell heavier than a target)
+    used, sub_seq = np.unique(sub_seq, return_inverse=True)
+    nsub = len(used)
+    sub_of_cell = np.empty(ncell, dtype=np.int64)
+    sub_of_cell[order] = sub_seq
+    part_of_sub = (np.arange(nsub) * nparts) // nsub
+    return sub_of_cell, part_of_sub
+
+
 def sfc_partition(dims, target_sizes_cells: np.ndarray, nparts: int):
     """Uneven subdomains: cut the Morton-ordered cell sequence into chunks of
     the given cell counts; parts group consecutive subdomains."""
@@ -425,21 +608,14 @@ def make_config(cfg: int, seed: Optional[int] = None, pml_mode: str = "stretch",
         if cfg == 5:
             eps_r = 1.0
     elif cfg == 3:
+        # 48^3 base grid; the tetrahedra inside a seeded sphere scatterer are
+        # refined 2x (three bisections each, conforming closure around them)
         n = 48
-        dims0 = (n, n, n)
+        dims = (n, n, n)
         h = 1.0 / n
-        center = rng.uniform(0.35, 0.65, size=3)
-        radius = 0.15
-        grids = []
-        for a in range(3):
-            base = np.linspace(0.0, 1.0, n + 1)
-            mids = 0.5 * (base[:-1] + base[1:])
-            refine = np.abs(mids - center[a]) < radius + h
-            pts = [base]
-            pts.append(mids[refine])
-            grids.append(np.unique(np.concatenate(pts)))
-        xs, ys, zs = grids
-        dims = (len(xs) - 1, len(ys) - 1, len(zs) - 1)
+        xs = ys = zs = np.linspace(0.0, 1.0, n + 1)
+        center = rng.uniform(0.4, 0.6, size=3)
+        radius = CFG3_RADIUS
         npml = 3
         eps_r = 1.0
         sub = part = None
@@ -455,7 +631,12 @@ def make_config(cfg: int, seed: Optional[int] = None, pml_mode: str = "stretch",
     else:
         raise ValueError(cfg)
 
-    crd, tets, cell, dims = kuhn_box(xs, ys, zs)
+    if cfg == 3:
+        crd, tets, cell, dims = kuhn_box(xs, ys, zs, orient=False)
+        crd, tets, cell, _ = refine_ball(crd, tets, cell, center, radius, radius + 4.0 * h)
+        tets = orient_tets(crd, tets)
+    else:
+        crd, tets, cell, dims = kuhn_box(xs, ys, zs)
     nv, ne = len(crd), len(tets)
     cent = crd[tets].mean(axis=1)
     lo = np.array([xs[0], ys[0], zs[0]])
@@ -535,10 +716,15 @@ def make_config(cfg: int, seed: Optional[int] = None, pml_mode: str = "stretch",
 
     # decomposition
     if cfg == 3:
+        # lognormal subdomain sizes (edge DOFs) clipped to [150, 4000]; a cell
+        # weighs the edges of its tetrahedra (7/6 per tetrahedron), and the
+        # targets are scaled by CFG3_FILL for the edges a subdomain shares
         ncell = dims[0] * dims[1] * dims[2]
         sizes = np.exp(rng.normal(math.log(600.0), 0.7, size=20000)).clip(150, 4000)
-        cells = np.maximum(1, np.round(sizes / 7.0)).astype(int)
-        sub_of_cell, part_of_sub = sfc_partition(dims, cells, 8)
+        weight = np.bincount(cell, minlength=ncell) * (7.0 / 6.0)
+        u = np.log(sizes / 150.0) / math.log(4000.0 / 150.0)
+        fill = CFG3_FILL[0] + (CFG3_FILL[1] - CFG3_FILL[0]) * u
+        sub_of_cell, part_of_sub = weighted_sfc_partition(dims, weight, fill * sizes, 8)
         part_of_sub = part_of_sub.astype(np.int64)
     else:
         sub_of_cell, part_of_sub = block_partition(dims, sub, part)

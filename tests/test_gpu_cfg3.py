@@ -1,9 +1,9 @@
-"""cfg 3 at full size (run with -m gpu): 48^3 base box with refined grid
-lines through a seeded sphere scatterer (eps_r = 6 - 0.5j as conductivity),
-2,537 subdomains of lognormal sizes (up to 4,925 DOFs), reference-mode
-absorber, 1.93 M edge DOFs.
+"""cfg 3 at full size (run with -m gpu): 48^3 base box whose tetrahedra
+inside a seeded sphere scatterer (eps_r = 6 - 0.5j as conductivity) are
+refined 2x by conforming bisection, 2,330 subdomains of lognormal sizes
+(176 to 3,925 DOFs), reference-mode absorber, 1.47 M edge DOFs.
 
-The oracle cannot run this case to 1e-8 within a test (about 11,000
+The oracle cannot run this case to 1e-8 within a test (about 5,000
 interface iterations at a few per second on the host), so parity is checked
 on what it can run: the interface operator q = S p (every subdomain's Schur
 step, implicit and explicit, on random interface values and on values
@@ -22,11 +22,14 @@ from tests.helpers import global_system, iface_gids, to_global
 pytestmark = pytest.mark.gpu
 
 # the global residual ||f - A x||_2 / ||f||_2 of a solution stopped at
-# ||r||_inf / ||r0||_inf < 1e-8 (measured: 4.1e-7 explicit, 3.3e-7 implicit;
+# ||r||_inf / ||r0||_inf < 1e-8 (measured: 6.8e-7 explicit, 7.0e-7 implicit;
 # the interior rows are solved to rounding, the interface rows carry the
-# COCG residual, whose 2-norm sums ~560,000 entries); a wrong operator or
+# COCG residual, whose 2-norm sums ~446,000 entries); a wrong operator or
 # load leaves O(1)
-GLOBAL_RES_BAR = 1e-6
+GLOBAL_RES_BAR = 3e-6
+# the explicit and the implicit solution, each stopped at that residual
+# (measured 7.9e-7 apart, rel. L2)
+FORMS_BAR = 3e-6
 
 
 def say(msg):
@@ -109,7 +112,7 @@ def test_cfg3_residuals_and_global_residual(cfg3, cfg3_om, cfg3_gpu):
         xs[explicit] = xg
     d = np.linalg.norm(xs[True] - xs[False]) / np.linalg.norm(xs[False])
     say(f"[gpu] cfg3 explicit vs implicit solution: rel. L2 {d:.2e}")
-    assert d <= 1e-6
+    assert d <= FORMS_BAR
 
 
 @pytest.mark.parametrize("waves", [1, 4])

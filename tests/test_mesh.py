@@ -53,3 +53,56 @@ def test_ownership_and_lists(small_model):
 def test_orders_give_same_sets(order):
     m = mesh.make_config(1, small=(4, (2, 1, 1), (1, 1, 1)), order=order)
     assert m.nsub == 2 and m.n_interface > 0
+
+
+def _faces(tets):
+    f = np.concatenate([tets[:, [0, 1, 2]], tets[:, [0, 1, 3]], tets[:, [0, 2, 3]], tets[:, [1, 2, 3]]])
+    f.sort(axis=1)
+    _, counts = np.unique(f, axis=0, return_counts=True)
+    u = np.unique(f, axis=0)
+    return u, counts
+
+
+@pytest.mark.parametrize("n,center,radius", [(6, (0.5, 0.5, 0.5), 0.3), (8, (0.4, 0.55, 0.6), 0.25)])
+def test_refine_ball_conforming(n, center, radius):
+    """Local bisection refinement of a Kuhn box: the result is a conforming
+    mesh (every face of a tetrahedron is shared by exactly one other
+    tetrahedron unless it lies on the box surface; a hanging vertex would
+    leave an interior face used once), it fills the box (volumes sum to 1),
+    every tetrahedron inside the ball is three bisections below the base
+    mesh (none deeper), and orient_tets makes every volume positive."""
+    crd, tets, cell, dims = mesh.kuhn_box(*(np.linspace(0, 1, n + 1),) * 3, orient=False)
+    crd2, tets2, cell2, gen = mesh.refine_ball(crd, tets, cell, center, radius, radius + 4.0 / n)
+    assert len(tets2) > len(tets) and gen.max() == 3
+    faces, counts = _faces(tets2)
+    assert counts.max() == 2
+    on_surface = np.zeros(len(faces), bool)
+    for a in range(3):
+        for val in (0.0, 1.0):
+            on_surface |= np.all(np.abs(crd2[faces][:, :, a] - val) < 1e-12, axis=1)
+    assert np.array_equal(counts == 1, on_surface)
+    v = crd2[tets2]
+    d = np.einsum("ij,ij->i", v[:, 1] - v[:, 0], np.cross(v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]))
+    assert abs(np.abs(d).sum() / 6.0 - 1.0) < 1e-12
+    cent = v.mean(axis=1)
+    inside = np.linalg.norm(cent - np.asarray(center), axis=1) < radius
+    assert (gen[inside] == 3).all()
+    # base cells: every tetrahedron lies in the hex cell it came from
+    i, j, k = (np.floor(cent[:, a] * n).astype(int) for a in range(3))
+    assert np.array_equal(cell2, i + n * (j + n * k))
+    t3 = mesh.orient_tets(crd2, tets2)
+    v = crd2[t3]
+    d = np.einsum("ij,ij->i", v[:, 1] - v[:, 0], np.cross(v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]))
+    assert (d > 0).all()
+
+
+def test_cfg3_sizes():
+    """cfg 3: 48^3 base grid with the sphere scatterer's tetrahedra refined
+    2x (about 1.5 M edge DOFs) and about 2,000 subdomains of lognormal sizes
+    within [150, 4000] edge DOFs."""
+    m = mesh.make_config(3, pml_mode="sigma")
+    nmp = np.asarray(m.nmp)
+    assert m.meta["dims"] == (48, 48, 48)
+    assert m.n_edges == 1465571 and m.n_tets == 1237386 and m.nsub == 2330
+    assert nmp.min() >= 150 and nmp.max() <= 4000
+    assert 600 < np.median(nmp) < 800
