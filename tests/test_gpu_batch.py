@@ -5,11 +5,11 @@ frequencies of the reference-mode absorber model and the delta-gap variant
 at a third frequency.  The batch solve (every launch serving all three) must
 reproduce three separate single-system solves: bit for bit when the
 load-time kernel choices agree (matvec waves and split, LDL^T prefetch
-depth, size buckets: they set summation orders), else with counts within
-2 % and solutions within twice the system's oracle bar (each of the two
-meets it; with other rounding the 1e-8 stop can come one iteration earlier
-or later); and each system must match the oracle (count and solution bars
-per system from measured spreads, ORACLE_BARS).  Per-system accessors (select)
+depth, size buckets: they set summation orders), else with counts and
+solutions within twice the system's oracle bars (each of the two meets
+them; with other rounding the 1e-8 stop comes some iterations earlier or
+later); and each system must match the oracle (count and solution bars per
+system from measured spreads, ORACLE_BARS).  Per-system accessors (select)
 and the structure check are covered too."""
 import dataclasses
 import sys
@@ -59,7 +59,7 @@ def test_batch_matches_single_solves(systems, schur):
             assert np.array_equal(xb, x1)
             assert np.array_equal(rb, r1)
         else:
-            assert abs(stats_b[k]["iters"] - st1["iters"]) <= max(1, 0.02 * st1["iters"])
+            assert abs(stats_b[k]["iters"] - st1["iters"]) <= max(1, 2 * ORACLE_BARS[k][0] * st1["iters"])
             assert np.linalg.norm(xb - x1) <= 2 * ORACLE_BARS[k][1] * np.linalg.norm(x1)
         s1.close()
     b.close()
