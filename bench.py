@@ -102,6 +102,40 @@ def frequency_sweep(args, model, fwhip, expl):
             "max_rel_diff_batch_vs_separate": diff}
 
 
+def kernel_extras(args, fwhip, mesh):
+    """Kernel rates of other configurations (the model loaded, its S_s formed,
+    the kernels timed with HIP events; no solve): the explicit matvec against
+    the per-iteration algorithmic bytes (the headline roofline's definition)
+    and its own stream, the implicit Schur step and the LDL^T solve."""
+    extras = {}
+    for c in [int(x) for x in args.extra_cfg.split(",") if x.strip()]:
+        if c == args.cfg:
+            continue
+        t0 = time.time()
+        m = mesh.make_config(c, pml_mode=args.pml)
+        sol = fwhip.Solver(m)
+        sol.schur_form()
+        ms_solve, ms_schur, ms_smv = sol.time_kernels(reps=5)
+        info = dict(sol.info)
+        sol.close()
+        b_it = info["bytes_iter"] - 128.0 * info["n_iface"]
+        extras[f"cfg{c}"] = {
+            "workload": f"{m.meta.get('dims')} cell box, {m.n_edges} edge DOFs, {m.nsub} subdomains",
+            "k_smv": {"ms_per_launch": round(ms_smv, 4), "achieved": round(b_it / (ms_smv * 1e-3) / 1e9, 1),
+                      "unit": "GB/s", "frac": round(b_it / (ms_smv * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "explicit_stream_GBs": round(info["bytes_smv"] / (ms_smv * 1e-3) / 1e9, 1),
+                      "smv_split": info["smv_split"], "smv_waves": info["smv_waves"]},
+            "k_solve_iter": {"ms_per_launch": round(ms_schur, 4), "bytes_per_launch": b_it,
+                             "GBs": round(b_it / (ms_schur * 1e-3) / 1e9, 1)},
+            "ldl_solve": {"ms": round(ms_solve, 4), "bytes": info["bytes_solve"],
+                          "GBs": round(info["bytes_solve"] / (ms_solve * 1e-3) / 1e9, 1),
+                          "solve_depth": info["solve_depth"], "solve_waves": info["solve_waves"]},
+        }
+        log(f"[bench] extras cfg{c}: k_smv {ms_smv:.4f} ms, k_solve {ms_schur:.4f} ms, LDL^T {ms_solve:.4f} ms "
+            f"({time.time() - t0:.1f}s)")
+    return extras
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +155,9 @@ def main():
     ap.add_argument("--implicit-steps", type=int, default=1)
     ap.add_argument("--adjoint", action="store_true",
                     help="also time an adjoint permittivity gradient (forward solve, adjoint solve, gradient)")
+    ap.add_argument("--extra-cfg", default="3",
+                    help="comma-separated configurations whose kernels (explicit matvec, implicit Schur step, "
+                         "LDL^T solve) are also timed and reported under extras ('' for none)")
     ap.add_argument("--sweep", type=int, default=0,
                     help="also solve a sweep of this many frequencies f0*(0.9 + 0.2k/(n-1)) as one batch "
                          "(fwh_load_batch) and as separate solves")
@@ -333,8 +370,9 @@ def main():
         # formation and one solve
         "time_to_solution_ms": round(setup["load_wall"] + setup.get("schur_form", 0.0) + 1000 * dt / args.steps, 1),
     }
-    print(json.dumps(out), flush=True)
     sol.close()
+    out["extras"] = kernel_extras(args, fwhip, mesh)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
