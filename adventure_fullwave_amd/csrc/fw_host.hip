@@ -181,7 +181,7 @@ struct fwh_ctx {
   // load by timing; the Schur-complement formation (hundreds of waves per
   // CU) uses 8, the depth with the higher occupancy, whenever every row fits
   int pfk_iter = 16, pfk_unit = 16;
-  int waves_iter = 1;  // waves per subdomain of the LDL^T kernel outside MODE_UNIT (1 or 4)
+  int waves_iter = 1;  // waves per subdomain of the LDL^T kernel outside MODE_UNIT (1, 2 or 4)
   bool short_ok = false;
   long long ndof = 0, nslots = 0, nnz = 0, nlen = 0, nel = 0, ncpl = 0, ngrow = 0;
   std::vector<SubMeta> meta_h;
@@ -1341,37 +1341,38 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   I.flops_factor = nsys * flops;
 
   // the batched LDL^T kernel of the iteration sweeps: prefetch depth 16 or 8
-  // (the short partition), one or four waves per subdomain, one launch or
-  // one per size bucket, whichever is fastest on this model (one warm-up,
-  // then the best of three).  Depth and waves set the row blocks and the
-  // lanes per sum, and with them the order of the sums, so the choice keeps
-  // depth 16 with one wave unless another form is more than 5 % faster (in
-  // the order depth 8, then four waves at 16 and 8): it does not flip
-  // between runs on timing noise.
+  // (the short partition), one, two or four waves per subdomain, one launch
+  // or one per size bucket, whichever is fastest on this model (one
+  // warm-up, then the best of three).  Depth and waves set the row blocks
+  // and the lanes per sum, and with them the order of the sums, so the
+  // choice keeps depth 16 with one wave unless another form is more than 5 %
+  // faster (in the order depth 8, then two waves at 16 and 8, then four):
+  // it does not flip between runs on timing noise.
   c->use_buckets = false;
   {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    double best[2][2][2];  // [depth 16, 8][waves 1, 4][single, bucketed]; 1e30: not run
+    const int wv[3] = {1, 2, 4};
+    double best[2][3][2];  // [depth 16, 8][waves 1, 2, 4][single, bucketed]; 1e30: not run
     for (auto &x : best)
       for (auto &y : x) y[0] = y[1] = 1e30;
     const int depths = c->short_ok ? 2 : 1;
     const int variants = c->buckets.size() > 1 ? 2 : 1;
     // every launch of the form must fit a CU's LDS
     auto fits = [&](int d, int w, int v) -> bool {
-      const int pfk = d == 0 ? fw::PFK : 8, wv = w == 0 ? 1 : 4;
-      if (v == 0) return fw::solve_lds_bytes(nmax, ngmax, d == 0 ? c->nblkmax : c->nblkmax_s, pfk, wv) <= 160 * 1024;
+      const int pfk = d == 0 ? fw::PFK : 8;
+      if (v == 0) return fw::solve_lds_bytes(nmax, ngmax, d == 0 ? c->nblkmax : c->nblkmax_s, pfk, wv[w]) <= 160 * 1024;
       for (const auto &B : c->buckets)
-        if (fw::solve_lds_bytes(B.nmax, B.ngmax, d == 0 ? B.nblkmax : B.nblkmax_s, pfk, wv) > 160 * 1024) return false;
+        if (fw::solve_lds_bytes(B.nmax, B.ngmax, d == 0 ? B.nblkmax : B.nblkmax_s, pfk, wv[w]) > 160 * 1024) return false;
       return true;
     };
     for (int d = 0; d < depths; d++)
-      for (int w = 0; w < 2; w++)
+      for (int w = 0; w < 3; w++)
         for (int v = 0; v < variants; v++) {
           if (!fits(d, w, v)) continue;
           c->pfk_iter = d == 0 ? fw::PFK : 8;
-          c->waves_iter = w == 0 ? 1 : 4;
+          c->waves_iter = wv[w];
           fw::SolveArgs a = c->solve_args(fw::MODE_RHS, false);
           // a timed warm-up; three more repetitions only for short launches
           // (one suffices when a launch takes milliseconds)
@@ -1396,7 +1397,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     (void)hipEventDestroy(e1);
     auto tm = [&](int d, int w) { return std::min(best[d][w][0], best[d][w][1]); };
     int d = 0, w = 0;
-    const int cand[3][2] = {{1, 0}, {0, 1}, {1, 1}};
+    const int cand[5][2] = {{1, 0}, {0, 1}, {1, 1}, {0, 2}, {1, 2}};
     for (const auto &cw : cand)
       if (tm(cw[0], cw[1]) < 0.95 * tm(d, w)) {
         d = cw[0];
@@ -1407,7 +1408,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     // (wide profiles cut into many short blocks: cfg 3's large subdomains)
     c->pfk_unit = (depths == 2 && tm(1, 0) <= 1.2 * tm(0, 0)) ? 8 : fw::PFK;
     // FWH_SOLVE_DEPTH (8 or 16) forces the depth of every launch kind and
-    // FWH_SOLVE_WAVES (1 or 4) the waves outside the formation (tests of
+    // FWH_SOLVE_WAVES (1, 2 or 4) the waves outside the formation (tests of
     // each form, experiments), where the form was timed (fits)
     if (const char *e = getenv("FWH_SOLVE_DEPTH")) {
       const int v = atoi(e);
@@ -1418,18 +1419,18 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     }
     if (const char *e = getenv("FWH_SOLVE_WAVES")) {
       const int v = atoi(e);
-      if (v == 1 || v == 4) w = v == 1 ? 0 : 1;
+      for (int k = 0; k < 3; k++)
+        if (wv[k] == v) w = k;
     }
     if (tm(d, w) >= 1e30) w = 0;  // the forced form does not fit: one wave
     c->pfk_iter = d == 0 ? fw::PFK : 8;
-    c->waves_iter = w == 0 ? 1 : 4;
+    c->waves_iter = wv[w];
     c->ms_single = best[d][w][0];
     c->ms_bucketed = variants == 2 ? best[d][w][1] : 0.0;
     c->use_buckets = variants == 2 && best[d][w][1] < best[d][w][0];
     I.ms_solve_depth[0] = depths == 2 ? tm(1, w) : 0.0;
     I.ms_solve_depth[1] = tm(0, w);
-    I.ms_solve_waves[0] = tm(d, 0);
-    I.ms_solve_waves[1] = tm(d, 1) < 1e30 ? tm(d, 1) : -1.0;
+    for (int k = 0; k < 3; k++) I.ms_solve_waves[k] = tm(d, k) < 1e30 ? tm(d, k) : -1.0;
     HIPCHK(hipMemsetAsync(c->X, 0, (size_t)nsys * ndof * sizeof(double2), st));
     HIPCHK(hipStreamSynchronize(st));
   }

@@ -456,15 +456,14 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
 }
 
 // ---------------------------------------------------------------------------
-// The same Schur step with four waves per subdomain (256 threads).  A sweep
-// stays a chain of row blocks, but the work of each block is spread over
-// the workgroup: every thread stages and prefetches a quarter of the block
-// (KQ loads each, blocks of at most 256*KQ entries: the partitions of the
-// single-wave kernel's depths 4*KQ), the dot products of the forward block
-// run with up to 64 lanes per row (one row per wave for blocks of at most
-// four rows), the backward block's column updates over 256 columns at a
-// time; wave 0 resolves the 16x16 triangles through L_BB^-1 as the
-// single-wave kernel does.  Workgroup barriers order the phases: forward,
+// The same Schur step with two or four waves per subdomain (NT = 128 or 256
+// threads).  A sweep stays a chain of row blocks, but the work of each block
+// is spread over the workgroup: every thread stages and prefetches its share
+// of the block (KQ loads each, blocks of at most NT*KQ entries: the
+// partitions of the single-wave kernel's depths 16 and 8), the dot products
+// of the forward block run with up to 64 lanes per row, the backward
+// block's column updates over NT columns at a time; wave 0 resolves the
+// 16x16 triangles through L_BB^-1 as the single-wave kernel does.  Workgroup barriers order the phases: forward,
 // staged -> dot products -> triangle (the next block's barrier orders the
 // triangle before its dot products; the inverse triangles are staged
 // alternately in two slots, so the next block's staging does not touch the
@@ -473,7 +472,6 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
 // fixed order, so the results do not depend on scheduling; the order differs
 // from the single-wave kernel's (more lanes per row), so the two forms agree
 // to rounding.
-constexpr int NT4 = 256;
 
 template <int KQ>
 struct PfSet4 {
@@ -481,14 +479,14 @@ struct PfSet4 {
   Pf2 m;  // one entry of the block's inverted diagonal triangle (threads < MINV)
 };
 
-template <int KQ>
+template <int KQ, int NT>
 __device__ __forceinline__ void issue_block4(const double2 *__restrict__ fac, int e0, int e1,
                                              const double2 *__restrict__ minv, PfSet4<KQ> &p) {
   const int t = threadIdx.x;
   const int last = e1 - 1;
 #pragma unroll
   for (int k = 0; k < KQ; k++) {
-    const double2 v = fac[min(e0 + t + k * NT4, last)];
+    const double2 v = fac[min(e0 + t + k * NT, last)];
     p.e[k].x = v.x;
     p.e[k].y = v.y;
   }
@@ -497,15 +495,15 @@ __device__ __forceinline__ void issue_block4(const double2 *__restrict__ fac, in
   p.m.y = v.y;
 }
 
-template <int KQ>
+template <int KQ, int NT>
 __device__ __forceinline__ void stage_block4(double2 *stg, double2 *mst, const PfSet4<KQ> &p) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int k = 0; k < KQ; k++) stg[t + k * NT4] = make_double2(p.e[k].x, p.e[k].y);
+  for (int k = 0; k < KQ; k++) stg[t + k * NT] = make_double2(p.e[k].x, p.e[k].y);
   if (t < MINV) mst[t] = make_double2(p.m.x, p.m.y);
 }
 
-template <int KQ>
+template <int KQ, int NT>
 __device__ __forceinline__ void fwd_block4(const SolveLds &L, const double2 *__restrict__ fac, int nblk, int b,
                                            PfSet4<KQ> &p) {
   const int t = threadIdx.x, lane = t & (WL - 1);
@@ -513,18 +511,19 @@ __device__ __forceinline__ void fwd_block4(const SolveLds &L, const double2 *__r
   const int e0 = L.rb[i0];
   const int nb = i1 - i0;
   double2 *mst = L.mst + (b & 1) * MINV;
-  stage_block4(L.stg, mst, p);
+  stage_block4<KQ, NT>(L.stg, mst, p);
   __syncthreads();
   {
     const int bn = min(b + 2, nblk - 1);  // unconditional: see issue_block
-    issue_block4(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
+    issue_block4<KQ, NT>(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
   }
 
-  // A: columns before the block, Lr lanes per row (a power of two, at most
-  // a wave: 64 for blocks of up to four rows, 32 up to eight, else 16), in
-  // batches of four columns per lane (a quarter of the single-wave kernel's
-  // columns per lane)
-  const int lsh = nb <= 4 ? 6 : nb <= 8 ? 5 : 4;
+  // A: columns before the block, Lr lanes per row: NT / nb rounded down to
+  // a power of two, at most a wave (256 threads: 64 for blocks of up to four
+  // rows, 32 up to eight, else 16), in batches of four columns per lane
+  constexpr int LOG2NT = NT == 128 ? 7 : 8;
+  const int lgnb = nb <= 1 ? 0 : nb <= 2 ? 1 : nb <= 4 ? 2 : nb <= 8 ? 3 : 4;
+  const int lsh = min(6, LOG2NT - lgnb);
   const int Lr = 1 << lsh, ra = t >> lsh, la = t & (Lr - 1);
   double2 acc0 = cz(), acc1 = cz();
   if (ra < nb) {
@@ -570,7 +569,7 @@ __device__ __forceinline__ void fwd_block4(const SolveLds &L, const double2 *__r
   }
 }
 
-template <int KQ>
+template <int KQ, int NT>
 __device__ __forceinline__ void bwd_block4(const SolveLds &L, const double2 *__restrict__ fac, int b,
                                            PfSet4<KQ> &p) {
   const int t = threadIdx.x, lane = t & (WL - 1);
@@ -578,11 +577,11 @@ __device__ __forceinline__ void bwd_block4(const SolveLds &L, const double2 *__r
   const int e0 = L.rb[i0];
   const int nb = i1 - i0;
   double2 *mst = L.mst + (b & 1) * MINV;
-  stage_block4(L.stg, mst, p);
+  stage_block4<KQ, NT>(L.stg, mst, p);
   __syncthreads();
   {
     const int bn = max(b - 2, 0);
-    issue_block4(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
+    issue_block4<KQ, NT>(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
   }
   // row metadata of the block, in every wave: lane r holds row r's first
   // column and base
@@ -618,13 +617,13 @@ __device__ __forceinline__ void bwd_block4(const SolveLds &L, const double2 *__r
   int jlo = fR;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) jlo = min(jlo, __shfl_xor(jlo, o, 64));
-  col_updates_nb<NT4>(L, nb, jlo, i0, fR, rbR, xR, 1);
+  col_updates_nb<NT>(L, nb, jlo, i0, fR, rbR, xR, 1);
   __syncthreads();
 }
 
-template <int KQ>  // prefetch loads per thread: row blocks of at most 256*KQ entries
-__global__ __launch_bounds__(NT4) __attribute__((amdgpu_waves_per_eu(4))) void k_solve4(SolveArgs a) {
-  constexpr int CAPQ = NT4 * KQ;
+template <int KQ, int NT>  // prefetch loads per thread; threads: row blocks of at most NT*KQ entries
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 128 ? 3 : 4))) void k_solve4(SolveArgs a) {
+  constexpr int CAPQ = NT * KQ;
   to_system(a);
   const bool check_done = a.mode == MODE_ITER && a.done;
   const int done0 = check_done ? *a.done : 0;
@@ -637,41 +636,45 @@ __global__ __launch_bounds__(NT4) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int t = threadIdx.x;
   const int n = m.n;
   const double2 *fac = a.fac + m.fac0;
-  // the partition of the single-wave kernel whose blocks hold 256*KQ entries
-  const int nblk = KQ == 2 ? m.nblk_s : m.nblk;
-  const int blk0 = KQ == 2 ? m.blk0_s : m.blk0;
+  // the partition of the single-wave kernel whose blocks hold NT*KQ entries
+  const int nblk = CAPQ == 512 ? m.nblk_s : m.nblk;
+  const int blk0 = CAPQ == 512 ? m.blk0_s : m.blk0;
   L.minv = a.minv + (long long)blk0 * MINV;
 
-  load_meta<NT4>(a, m, L, nblk, blk0);
+  load_meta<NT>(a, m, L, nblk, blk0);
   __syncthreads();
   if (done0) return;
-  load_rhs<NT4>(a, m, L.v, uk);
+  load_rhs<NT>(a, m, L.v, uk);
   PfSet4<KQ> pa, pb;
-  issue_block4(fac, L.rb[L.blk[0]], L.rb[L.blk[1]], minv_of(L, 0), pa);
-  issue_block4(fac, L.rb[L.blk[1]], L.rb[L.blk[2]], minv_of(L, 1), pb);
+  issue_block4<KQ, NT>(fac, L.rb[L.blk[0]], L.rb[L.blk[1]], minv_of(L, 0), pa);
+  issue_block4<KQ, NT>(fac, L.rb[L.blk[1]], L.rb[L.blk[2]], minv_of(L, 1), pb);
   __syncthreads();  // the load vector is complete
   for (int b = 0; b < nblk; b += 2) {
-    fwd_block4(L, fac, nblk, b, pa);
-    fwd_block4(L, fac, nblk, b + 1, pb);
+    fwd_block4<KQ, NT>(L, fac, nblk, b, pa);
+    fwd_block4<KQ, NT>(L, fac, nblk, b + 1, pb);
   }
   __syncthreads();  // the last triangle is complete
-  for (int i = t; i < n; i += NT4) L.v[i] = cmul(L.v[i], a.dinv[m.dof0 + i]);
+  for (int i = t; i < n; i += NT) L.v[i] = cmul(L.v[i], a.dinv[m.dof0 + i]);
   const int nbl = nblk - 1;
-  issue_block4(fac, L.rb[L.blk[nbl]], L.rb[L.blk[nbl + 1]], minv_of(L, nbl), pa);
-  issue_block4(fac, L.rb[L.blk[nbl - 1]], L.rb[L.blk[nbl]], minv_of(L, nbl - 1), pb);
+  issue_block4<KQ, NT>(fac, L.rb[L.blk[nbl]], L.rb[L.blk[nbl + 1]], minv_of(L, nbl), pa);
+  issue_block4<KQ, NT>(fac, L.rb[L.blk[nbl - 1]], L.rb[L.blk[nbl]], minv_of(L, nbl - 1), pb);
   __syncthreads();
   for (int b = nbl; b >= 1; b -= 2) {
-    bwd_block4(L, fac, b, pa);
-    bwd_block4(L, fac, b - 1, pb);
+    bwd_block4<KQ, NT>(L, fac, b, pa);
+    bwd_block4<KQ, NT>(L, fac, b - 1, pb);
   }
   // (the last backward block ends with a barrier)
-  double2 pq = write_outputs<NT4>(a, m, L.v, w, uk);
+  double2 pq = write_outputs<NT>(a, m, L.v, w, uk);
   if (a.mode == MODE_ITER && a.pqpart) {
-    __shared__ double2 shq[NT4 / WL];
+    __shared__ double2 shq[NT / WL];
     pq = wave_csum(pq);
     if ((t & (WL - 1)) == 0) shq[t / WL] = pq;
     __syncthreads();
-    if (t == 0) a.pqpart[w] = cadd(cadd(shq[0], shq[1]), cadd(shq[2], shq[3]));
+    if (t == 0) {
+      double2 sum = shq[0];
+      for (int v = 1; v < NT / WL; v++) sum = cadd(sum, shq[v]);
+      a.pqpart[w] = sum;
+    }
   }
 }
 
@@ -715,10 +718,11 @@ hipError_t launch_blockinv(const SubMeta *meta, int nsub, const int *idx1, const
 
 size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk, int waves) {
   (void)ngmax;
-  // the four-wave form: the same block capacity (64 pfk entries), two slots
-  // for the inverse triangles, and the (p,q) partials of its waves
-  if (waves == 4)
-    return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + 2 * MINV + NT4 / WL) +
+  // the two- and four-wave forms: the same block capacity (64 pfk
+  // entries), two slots for the inverse triangles, and the (p,q) partials
+  // of their waves
+  if (waves == 2 || waves == 4)
+    return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + 2 * MINV + waves) +
            sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
   return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + MINV) +
          sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
@@ -727,13 +731,20 @@ size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk, int waves) {
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
   if (nwork <= 0) return hipSuccess;
   const dim3 grid(nwork, a.nsys > 0 ? a.nsys : 1);
-  if (a.waves == 4) {
+  if (a.waves == 2 || a.waves == 4) {
     // the static (p,q) partials are not part of the dynamic size
-    const size_t lds = solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk, 4) - sizeof(double2) * (NT4 / WL);
-    if (a.pfk == 8)
-      k_solve4<2><<<grid, NT4, lds, st>>>(a);
-    else
-      k_solve4<4><<<grid, NT4, lds, st>>>(a);
+    const size_t lds = solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk, a.waves) - sizeof(double2) * a.waves;
+    if (a.waves == 4) {
+      if (a.pfk == 8)
+        k_solve4<2, 256><<<grid, 256, lds, st>>>(a);
+      else
+        k_solve4<4, 256><<<grid, 256, lds, st>>>(a);
+    } else {
+      if (a.pfk == 8)
+        k_solve4<4, 128><<<grid, 128, lds, st>>>(a);
+      else
+        k_solve4<8, 128><<<grid, 128, lds, st>>>(a);
+    }
     return hipGetLastError();
   }
   const size_t lds = solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk, 1);

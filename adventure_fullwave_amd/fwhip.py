@@ -69,7 +69,7 @@ class FwhInfo(ctypes.Structure):
                 ("ms_schur_gemm", ctypes.c_double), ("flops_schur_fwd", ctypes.c_double),
                 ("flops_schur_gemm", ctypes.c_double), ("smv_split", ctypes.c_int),
                 ("smv_nwork", ctypes.c_int), ("ms_smv_split", ctypes.c_double * 4),
-                ("solve_waves", ctypes.c_int), ("ms_solve_waves", ctypes.c_double * 2)]
+                ("solve_waves", ctypes.c_int), ("ms_solve_waves", ctypes.c_double * 3)]
 
     def as_dict(self):
         out = {}

@@ -357,7 +357,7 @@ def main():
                                                      round(info["ms_solve_bucketed"], 4)],
                      "solve_depth": info["solve_depth"], "unit_depth": info["unit_depth"],
                      "solve_waves": info["solve_waves"],
-                     "ms_solve_by_waves_1_4": [round(v, 4) for v in info["ms_solve_waves"]],
+                     "ms_solve_by_waves_1_2_4": [round(v, 4) for v in info["ms_solve_waves"]],
                      "ms_solve_by_depth_8_16": [round(v, 4) for v in info["ms_solve_depth"]],
                      "smv_order": info["smv_order"],
                      "ms_smv_by_order": [round(v, 4) for v in info["ms_smv_order"]],

@@ -132,10 +132,11 @@ typedef struct fwh_info {
   int    smv_nwork;     /* its work items                                  */
   double ms_smv_split[4]; /* its timings for m = 0, 2, 4, 8 (-1: no
                            subdomain large enough to split)                */
-  int    solve_waves;   /* waves per subdomain (1 or 4) of the LDL^T kernel
-                           outside the formation, picked by timing at load */
-  double ms_solve_waves[2]; /* best batched solve time with 1 and 4 waves
-                           at the chosen depth (-1: 4 waves do not fit)    */
+  int    solve_waves;   /* waves per subdomain (1, 2 or 4) of the LDL^T
+                           kernel outside the formation, picked by timing
+                           at load                                         */
+  double ms_solve_waves[3]; /* best batched solve time with 1, 2 and 4
+                           waves at the chosen depth (-1: does not fit)    */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */

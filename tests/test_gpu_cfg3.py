@@ -115,10 +115,10 @@ def test_cfg3_residuals_and_global_residual(cfg3, cfg3_om, cfg3_gpu):
     assert d <= FORMS_BAR
 
 
-@pytest.mark.parametrize("waves", [1, 4])
+@pytest.mark.parametrize("waves", [1, 2, 4])
 def test_cfg3_solve_forms(cfg3, cfg3_om, waves):
-    """Both forms of the batched LDL^T kernel (one wave per subdomain, or
-    four; forced with FWH_SOLVE_WAVES) on cfg 3, whose large subdomains have
+    """Every form of the batched LDL^T kernel (one, two or four waves per
+    subdomain; forced with FWH_SOLVE_WAVES) on cfg 3, whose large subdomains have
     blocks of one to four wide rows: the implicit Schur step against the
     oracle (1e-10 relative) and the largest subdomain's LDL^T solve against
     the oracle's (1e-10 relative)."""

@@ -1,8 +1,8 @@
 """Every form of the batched LDL^T kernel (run with -m gpu).
 
 The load-time tuner picks the prefetch depth (8 or 16 loads per lane, with
-a row-block partition for each) and the waves per subdomain (1, or 4 with
-the same row blocks) by timing, so a given run may exercise only one form
+a row-block partition for each) and the waves per subdomain (1, 2 or 4,
+with the same row blocks) by timing, so a given run may exercise only one form
 in the iteration sweeps.  Here each form is forced (FWH_SOLVE_DEPTH for
 every launch kind, FWH_SOLVE_WAVES outside the formation, read at load) and
 checked against the oracle: the subdomain LDL^T solves (1e-10 relative; one
@@ -31,7 +31,7 @@ def sigma_ref(small_sigma_model):
     return fo.hddm_solve(small_sigma_model, eps=1e-8, max_loop=5000)
 
 
-@pytest.mark.parametrize("depth,waves", [(16, 1), (8, 1), (16, 4), (8, 4)])
+@pytest.mark.parametrize("depth,waves", [(16, 1), (8, 1), (16, 2), (8, 2), (16, 4), (8, 4)])
 def test_solve_variant(small_sigma_model, sigma_ref, depth, waves):
     from adventure_fullwave_amd import fwhip
     model = small_sigma_model
