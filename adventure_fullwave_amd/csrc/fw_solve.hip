@@ -31,10 +31,15 @@ __device__ __forceinline__ int readlane_i(int x, int lane) { return __builtin_am
 struct Pf2 {
   double x, y;
 };
+// a row block: rows [i0, i1), skyline entries [e0, e1)
+struct BlkDesc {
+  int i0, i1, e0, e1;
+};
 template <int K>
 struct PfSet {
   Pf2 e[K];
   Pf2 m[MINV / 64];  // the block's inverted diagonal triangle
+  BlkDesc q;         // the block itself (read with the addresses of its loads)
 };
 
 // Branch-free block loads: addresses past the block are clamped to its last
@@ -42,10 +47,11 @@ struct PfSet {
 // flow and the compiler can keep the other register set's loads in flight
 // (a counted vmcnt wait instead of a full drain).
 template <int K>
-__device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, int e0, int e1,
+__device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, const BlkDesc q,
                                             const double2 *__restrict__ minv, PfSet<K> &p) {
   const int lane = threadIdx.x;
-  const int last = e1 - 1;
+  const int e0 = q.e0, last = q.e1 - 1;
+  p.q = q;
 #pragma unroll
   for (int k = 0; k < K; k++) {
     const double2 t = fac[min(e0 + lane + k * WL, last)];
@@ -79,6 +85,7 @@ __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 struct SolveLds {
   double2 *v, *stg, *mst;       // LDS: vector, staged block, staged inverse triangle
   int *idx1, *rb, *blk;         // LDS: row metadata, block rows
+  BlkDesc *bt;                  // LDS: every row block's rows and entries
   const double2 *minv;          // global: inverse triangles of this subdomain's blocks
 };
 // inverse triangle of block b (packed rows, MINV entries per block)
@@ -122,15 +129,15 @@ template <int K>
 __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__restrict__ fac, int nblk, int b,
                                           PfSet<K> &p) {
   const int lane = threadIdx.x;
-  const int i0 = L.blk[b], i1 = L.blk[b + 1];
-  const int e0 = L.rb[i0];
+  const int i0 = p.q.i0, i1 = p.q.i1;
+  const int e0 = p.q.e0;
   const int nb = i1 - i0;
   wave_sync();
   stage_block(L.stg, L.mst, p);
   wave_sync();
   {
     const int bn = min(b + 2, nblk - 1);  // unconditional: see issue_block
-    issue_block(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
+    issue_block(fac, L.bt[bn], minv_of(L, bn), p);
   }
 
   // A: columns before the block.  Lr lanes per row, Lr = 64 / nb rounded
@@ -191,15 +198,15 @@ template <int K>
 __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__restrict__ fac, int b,
                                           PfSet<K> &p) {
   const int lane = threadIdx.x;
-  const int i0 = L.blk[b], i1 = L.blk[b + 1];
-  const int e0 = L.rb[i0];
+  const int i0 = p.q.i0, i1 = p.q.i1;
+  const int e0 = p.q.e0;
   const int nb = i1 - i0;
   wave_sync();
   stage_block(L.stg, L.mst, p);
   wave_sync();
   {
     const int bn = max(b - 2, 0);
-    issue_block(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
+    issue_block(fac, L.bt[bn], minv_of(L, bn), p);
   }
 
   // row metadata of the block: lane r holds row r's first column and base
@@ -276,17 +283,27 @@ __device__ __forceinline__ bool solve_item(const SolveArgs &a, int &w, int &uk) 
   return true;
 }
 
-// LDS carve-up: v[nmax], stg[cap], mst[nmst], idx1[nmax], rb[nmax+1],
-// blk[nblkmax+1]
+// LDS carve-up: v[nmax], stg[cap], mst[nmst], bt[nblkmax+1] (16-byte
+// entries), idx1[nmax], rb[nmax+1], blk[nblkmax+1]
 __device__ __forceinline__ SolveLds solve_lds(const SolveArgs &a, double2 *lds, int cap, int nmst) {
   SolveLds L;
   L.v = lds;
   L.stg = L.v + a.nmax;
   L.mst = L.stg + cap;
-  L.idx1 = (int *)(L.mst + nmst);
+  L.bt = (BlkDesc *)(L.mst + nmst);
+  L.idx1 = (int *)(L.bt + a.nblkmax + 1);
   L.rb = L.idx1 + a.nmax;
   L.blk = L.rb + a.nmax + 1;
   return L;
+}
+// the row-block table from the block rows and row starts in LDS (call
+// after a barrier that follows load_meta)
+template <int NT>
+__device__ __forceinline__ void load_blocks(const SolveLds &L, int nblk) {
+  for (int b = threadIdx.x; b < nblk; b += NT) {
+    const int i0 = L.blk[b], i1 = L.blk[b + 1];
+    L.bt[b] = BlkDesc{i0, i1, L.rb[i0], L.rb[i1]};
+  }
 }
 
 template <int NT>
@@ -409,6 +426,8 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   load_meta<WL>(a, m, L, nblk, blk0);
   __syncthreads();
   if (done0) return;
+  load_blocks<WL>(L, nblk);
+  __syncthreads();
 
   // ---- 1. load vector --------------------------------------------------------------
   load_rhs<WL>(a, m, L.v, uk);
@@ -419,8 +438,8 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   // (the host pads every subdomain to an even number >= 2 of row blocks, so
   // the sweeps run over block pairs without conditional loads: a branch
   // around a load would make the compiler drain every load in flight)
-  issue_block(fac, L.rb[L.blk[0]], L.rb[L.blk[1]], minv_of(L, 0), pa);
-  issue_block(fac, L.rb[L.blk[1]], L.rb[L.blk[2]], minv_of(L, 1), pb);
+  issue_block(fac, L.bt[0], minv_of(L, 0), pa);
+  issue_block(fac, L.bt[1], minv_of(L, 1), pb);
   for (int b = 0; b < nblk; b += 2) {
     fwd_block(L, fac, nblk, b, pa);
     fwd_block(L, fac, nblk, b + 1, pb);
@@ -436,8 +455,8 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
   // in pa, the one before in pb, alternating from there (the register sets
   // are always named statically, so they stay in registers)
   const int nbl = nblk - 1;
-  issue_block(fac, L.rb[L.blk[nbl]], L.rb[L.blk[nbl + 1]], minv_of(L, nbl), pa);
-  issue_block(fac, L.rb[L.blk[nbl - 1]], L.rb[L.blk[nbl]], minv_of(L, nbl - 1), pb);
+  issue_block(fac, L.bt[nbl], minv_of(L, nbl), pa);
+  issue_block(fac, L.bt[nbl - 1], minv_of(L, nbl - 1), pb);
   wave_sync();
 
   // ---- 4. backward sweep ---------------------------------------------------------------------
@@ -476,14 +495,16 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
 template <int KQ>
 struct PfSet4 {
   Pf2 e[KQ];
-  Pf2 m;  // one entry of the block's inverted diagonal triangle (threads < MINV)
+  Pf2 m;      // one entry of the block's inverted diagonal triangle (threads < MINV)
+  BlkDesc q;  // the block itself
 };
 
 template <int KQ, int NT>
-__device__ __forceinline__ void issue_block4(const double2 *__restrict__ fac, int e0, int e1,
+__device__ __forceinline__ void issue_block4(const double2 *__restrict__ fac, const BlkDesc q,
                                              const double2 *__restrict__ minv, PfSet4<KQ> &p) {
   const int t = threadIdx.x;
-  const int last = e1 - 1;
+  const int e0 = q.e0, last = q.e1 - 1;
+  p.q = q;
 #pragma unroll
   for (int k = 0; k < KQ; k++) {
     const double2 v = fac[min(e0 + t + k * NT, last)];
@@ -507,15 +528,15 @@ template <int KQ, int NT>
 __device__ __forceinline__ void fwd_block4(const SolveLds &L, const double2 *__restrict__ fac, int nblk, int b,
                                            PfSet4<KQ> &p) {
   const int t = threadIdx.x, lane = t & (WL - 1);
-  const int i0 = L.blk[b], i1 = L.blk[b + 1];
-  const int e0 = L.rb[i0];
+  const int i0 = p.q.i0, i1 = p.q.i1;
+  const int e0 = p.q.e0;
   const int nb = i1 - i0;
   double2 *mst = L.mst + (b & 1) * MINV;
   stage_block4<KQ, NT>(L.stg, mst, p);
   __syncthreads();
   {
     const int bn = min(b + 2, nblk - 1);  // unconditional: see issue_block
-    issue_block4<KQ, NT>(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
+    issue_block4<KQ, NT>(fac, L.bt[bn], minv_of(L, bn), p);
   }
 
   // A: columns before the block, Lr lanes per row: NT / nb rounded down to
@@ -573,15 +594,15 @@ template <int KQ, int NT>
 __device__ __forceinline__ void bwd_block4(const SolveLds &L, const double2 *__restrict__ fac, int b,
                                            PfSet4<KQ> &p) {
   const int t = threadIdx.x, lane = t & (WL - 1);
-  const int i0 = L.blk[b], i1 = L.blk[b + 1];
-  const int e0 = L.rb[i0];
+  const int i0 = p.q.i0, i1 = p.q.i1;
+  const int e0 = p.q.e0;
   const int nb = i1 - i0;
   double2 *mst = L.mst + (b & 1) * MINV;
   stage_block4<KQ, NT>(L.stg, mst, p);
   __syncthreads();
   {
     const int bn = max(b - 2, 0);
-    issue_block4<KQ, NT>(fac, L.rb[L.blk[bn]], L.rb[L.blk[bn + 1]], minv_of(L, bn), p);
+    issue_block4<KQ, NT>(fac, L.bt[bn], minv_of(L, bn), p);
   }
   // row metadata of the block, in every wave: lane r holds row r's first
   // column and base
@@ -644,10 +665,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 128 ? 
   load_meta<NT>(a, m, L, nblk, blk0);
   __syncthreads();
   if (done0) return;
+  load_blocks<NT>(L, nblk);
+  __syncthreads();
   load_rhs<NT>(a, m, L.v, uk);
   PfSet4<KQ> pa, pb;
-  issue_block4<KQ, NT>(fac, L.rb[L.blk[0]], L.rb[L.blk[1]], minv_of(L, 0), pa);
-  issue_block4<KQ, NT>(fac, L.rb[L.blk[1]], L.rb[L.blk[2]], minv_of(L, 1), pb);
+  issue_block4<KQ, NT>(fac, L.bt[0], minv_of(L, 0), pa);
+  issue_block4<KQ, NT>(fac, L.bt[1], minv_of(L, 1), pb);
   __syncthreads();  // the load vector is complete
   for (int b = 0; b < nblk; b += 2) {
     fwd_block4<KQ, NT>(L, fac, nblk, b, pa);
@@ -656,8 +679,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 128 ? 
   __syncthreads();  // the last triangle is complete
   for (int i = t; i < n; i += NT) L.v[i] = cmul(L.v[i], a.dinv[m.dof0 + i]);
   const int nbl = nblk - 1;
-  issue_block4<KQ, NT>(fac, L.rb[L.blk[nbl]], L.rb[L.blk[nbl + 1]], minv_of(L, nbl), pa);
-  issue_block4<KQ, NT>(fac, L.rb[L.blk[nbl - 1]], L.rb[L.blk[nbl]], minv_of(L, nbl - 1), pb);
+  issue_block4<KQ, NT>(fac, L.bt[nbl], minv_of(L, nbl), pa);
+  issue_block4<KQ, NT>(fac, L.bt[nbl - 1], minv_of(L, nbl - 1), pb);
   __syncthreads();
   for (int b = nbl; b >= 1; b -= 2) {
     bwd_block4<KQ, NT>(L, fac, b, pa);
@@ -721,11 +744,9 @@ size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk, int waves) {
   // the two- and four-wave forms: the same block capacity (64 pfk
   // entries), two slots for the inverse triangles, and the (p,q) partials
   // of their waves
-  if (waves == 2 || waves == 4)
-    return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + 2 * MINV + waves) +
-           sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
-  return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + MINV) +
-         sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1);
+  const size_t ints = sizeof(int) * ((size_t)2 * nmax + 1 + nblkmax + 1) + sizeof(BlkDesc) * ((size_t)nblkmax + 1);
+  if (waves == 2 || waves == 4) return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + 2 * MINV + waves) + ints;
+  return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + MINV) + ints;
 }
 
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
