@@ -182,6 +182,17 @@ struct fwh_ctx {
   // CU) uses 8, the depth with the higher occupancy, whenever every row fits
   int pfk_iter = 16, pfk_unit = 16;
   int waves_iter = 1;  // waves per subdomain of the LDL^T kernel outside MODE_UNIT (1, 2 or 4)
+  // persistent form of the LDL^T kernel outside MODE_UNIT (picked at load by
+  // timing; pgrid 0: not used): pgrid workgroups of pthreads threads per
+  // system claim the subdomains of porder (the first nlarge solved by a
+  // whole workgroup of 256 threads, the rest one wave each) from
+  // solve_queue (4 ints per system, zero between launches)
+  struct SolvePersist {
+    int pgrid = 0, pthreads = 64, nlarge = 0, pfk_l = 16, nmax_l = 0, nblkmax_l = 0;
+    int nmax = 0, nblkmax = 0, wave_lds = 0;
+    int *porder = nullptr;
+  } sp;
+  int *solve_queue = nullptr;
   bool short_ok = false;
   long long ndof = 0, nslots = 0, nnz = 0, nlen = 0, nel = 0, ncpl = 0, ngrow = 0;
   std::vector<SubMeta> meta_h;
@@ -248,6 +259,11 @@ struct fwh_ctx {
   bool gather_csr = false;                // FWH_GATHER=csr: no four-wide tables
   int nbmax = 0;
   int smv_waves = 4;
+  // persistent form of the matvec: smv_pgrid workgroups per system claiming
+  // the work items from smv_queue (2 ints per system, zero between launches);
+  // 0: one workgroup per item
+  int smv_pgrid = 0;
+  int *smv_queue = nullptr;
   bool schur_built = false;
 
   // size buckets of the batched LDL^T kernel: subdomains grouped by how many
@@ -331,6 +347,7 @@ struct fwh_ctx {
   }
   // one batched k_solve launch over all subdomains, or one per size bucket
   hipError_t launch_solve_all(fw::SolveArgs a, bool bucketed) {
+    if (a.pgrid > 0) return fw::launch_solve(a, a.nwork, st);
     if (!bucketed || buckets.size() <= 1) return fw::launch_solve(a, nsub, st);
     hipError_t e = hipEventRecord(ev_fork, st);
     for (size_t b = 0; b < buckets.size() && e == hipSuccess; b++) {
@@ -394,6 +411,20 @@ struct fwh_ctx {
     a.sm_off = sm_off;
     a.pqpart = (mode == fw::MODE_ITER && check_done && fused) ? pqpart : nullptr;  // an interface iteration
     a.npq = nsub;
+    a.queue = solve_queue;
+    if (sp.pgrid > 0 && mode != fw::MODE_UNIT) {
+      a.pgrid = sp.pgrid;
+      a.pthreads = sp.pthreads;
+      a.porder = sp.porder;
+      a.nwork = nsub;
+      a.nlarge = sp.nlarge;
+      a.pfk_l = sp.pfk_l;
+      a.nmax_l = sp.nmax_l;
+      a.nblkmax_l = sp.nblkmax_l;
+      a.nmax = sp.nmax;
+      a.nblkmax = sp.nblkmax;
+      a.wave_lds = sp.wave_lds;
+    }
     return a;
   }
   fw::SmvArgs smv_args(bool check_done, bool fused, int only = -1) const {
@@ -419,6 +450,8 @@ struct fwh_ctx {
     a.pqpart = (check_done && fused) ? pqpart : nullptr;  // an interface iteration
     a.npq = npq_expl > 0 ? npq_expl : nsub;
     a.yslot = yslot;
+    a.pgrid = smv_pgrid;
+    a.queue = smv_queue;
     return a;
   }
   // expl: the vector pass of an explicit iteration (its gather lists and
@@ -455,6 +488,7 @@ struct fwh_ctx {
     fw::SolveArgs a = solve_args(mode, check_done, fused, only);
     if (!plain) return launch_solve_all(a, use_buckets);
     a.order = nullptr;
+    a.pgrid = 0;
     for (int s = 0; s < nsub; s++) {
       a.base = s;
       a.nmax = meta_h[s].n;
@@ -479,6 +513,7 @@ struct fwh_ctx {
     if (!plain) return fw::launch_smv(a, smv_work ? smv_nwork : nsub, st);
     // one launch per subdomain, in the reference's loop order (a split
     // subdomain's items in one launch)
+    a.pgrid = 0;
     for (int s = 0; s < nsub; s++) {
       if (smv_work) {
         a.base = smv_first[s];
@@ -596,6 +631,193 @@ int check_batch(const fwh_mesh_desc *d, int nsys) {
         !same_array(q.op_off, m.op_off, npp + 1) || !same_array(q.op_ibid, m.op_ibid, (size_t)m.op_off[npp]))
       return fail(FWH_E_ARG, "system %d: mesh or decomposition differs from system 0", k);
   }
+  return 0;
+}
+
+// The persistent form of the batched LDL^T kernel (k_solveq), against the
+// static grid picked before (static_ms: its best time).  Candidates, each at
+// depth 16 and (every row fitting) 8:
+//   1: workgroups of one wave claiming subdomains largest skyline first;
+//   2: workgroups of 256 threads with two size-bucketed queues: subdomains
+//      too large for four of them to share the LDS of one large one are
+//      solved by the whole workgroup (blocks of up to 1,024 entries, as the
+//      four-wave kernel), the others by one wave each, four to a workgroup
+//      (as the one-wave kernel), so the small ones do not hold a CU's LDS
+//      sized for the largest.
+// The grid is the resident capacity (occupancy x CUs).  A candidate is kept
+// when more than 5 % faster than the static form (its row blocks and lanes
+// per sum differ, and with them the rounding: the choice must not flip on
+// timing noise).  FWH_SOLVE_PERSIST=0, 1 or 2 fixes the form (at the depth
+// FWH_SOLVE_DEPTH names, else the faster).  Runs MODE_RHS on X (the caller
+// clears it).
+int tune_solve_persist(fwh_ctx *c, double static_ms, int ncu) {
+  fwh_info &I = c->info;
+  I.solve_persist = 0;
+  I.solve_pgrid = 0;
+  I.solve_nlarge = 0;
+  I.ms_solve_persist[0] = I.ms_solve_persist[1] = -1.0;
+  c->sp = fwh_ctx::SolvePersist{};
+  int rc;
+  if ((rc = c->alloc(&c->solve_queue, (size_t)4 * c->nsys))) return rc;
+  HIPCHK(hipMemsetAsync(c->solve_queue, 0, sizeof(int) * 4 * c->nsys, c->st));
+  const std::vector<SubMeta> &M = c->meta_h;
+  const int nsub = c->nsub;
+  auto nblk_of = [&](int s, int pfk) { return pfk == 8 ? M[s].nblk_s : M[s].nblk; };
+  auto lds1 = [&](int n, int nb, int pfk) { return fw::solve_lds_bytes(n, c->ngmax, nb, pfk, 1); };
+  // candidates [form 1, 2][depth 16, 8]
+  fwh_ctx::SolvePersist cand[2][2];
+  std::vector<int> pord[2][2];
+  bool ok[2][2] = {{false, false}, {false, false}};
+  for (int d = 0; d < 2; d++) {
+    const int pfk = d == 0 ? fw::PFK : 8;
+    if (pfk == 8 && !c->short_ok) continue;
+    // form 1: every subdomain on one wave
+    {
+      fwh_ctx::SolvePersist &P = cand[0][d];
+      P.pthreads = 64;
+      P.nlarge = 0;
+      P.nmax = c->nmax;
+      P.nblkmax = pfk == 8 ? c->nblkmax_s : c->nblkmax;
+      P.wave_lds = (int)((lds1(P.nmax, P.nblkmax, pfk) + 15) / 16);
+      pord[0][d] = c->order_h;
+      ok[0][d] = (size_t)P.wave_lds * 16 <= 160 * 1024;
+    }
+    // form 2: the large ones set the workgroup's LDS, the small ones share
+    // it.  FWH_SOLVE_NLARGE=k puts the k largest (by DOFs) in the large
+    // queue and sizes the LDS for both (tests of the form on even models).
+    {
+      fwh_ctx::SolvePersist &P = cand[1][d];
+      P.pthreads = 256;
+      P.pfk_l = fw::PFK;
+      std::vector<int> by_n(c->order_h);
+      std::stable_sort(by_n.begin(), by_n.end(), [&](int x, int y) { return M[x].n < M[y].n; });
+      std::vector<char> small(nsub, 0);
+      int nm = 0, nbm = 0, cnt = 0;
+      const char *fl = getenv("FWH_SOLVE_NLARGE");
+      const int kforce = fl ? atoi(fl) : 0;
+      if (kforce > 0) {
+        for (int i = 0; i < nsub - std::min(kforce, nsub); i++) {
+          const int s = by_n[i];
+          if (nblk_of(s, pfk) <= 0) continue;
+          nm = std::max(nm, M[s].n);
+          nbm = std::max(nbm, nblk_of(s, pfk));
+          small[s] = 1;
+          cnt++;
+        }
+      } else {
+        // small, by size, while four of the running maxima's layouts fit
+        // the largest subdomain's layout
+        const size_t region = fw::solve_lds_bytes(c->nmax, c->ngmax, c->nblkmax, P.pfk_l, 4) - 16 * 4;
+        for (int s : by_n) {
+          if (nblk_of(s, pfk) <= 0) continue;  // a row too wide for the short blocks
+          const int n2 = std::max(nm, M[s].n), nb2 = std::max(nbm, nblk_of(s, pfk));
+          if (4 * (((lds1(n2, nb2, pfk) + 15) / 16) * 16) > region) break;
+          nm = n2;
+          nbm = nb2;
+          small[s] = 1;
+          cnt++;
+        }
+      }
+      P.nmax = std::max(nm, 1);
+      P.nblkmax = std::max(nbm, 1);
+      P.wave_lds = (int)((lds1(P.nmax, P.nblkmax, pfk) + 15) / 16);
+      P.nlarge = nsub - cnt;
+      P.nmax_l = P.nblkmax_l = 1;
+      for (int s : c->order_h)
+        if (!small[s]) {
+          pord[1][d].push_back(s);
+          P.nmax_l = std::max(P.nmax_l, M[s].n);
+          P.nblkmax_l = std::max(P.nblkmax_l, M[s].nblk);
+        }
+      for (int s : c->order_h)
+        if (small[s]) pord[1][d].push_back(s);
+      const size_t region = std::max(fw::solve_lds_bytes(P.nmax_l, c->ngmax, P.nblkmax_l, P.pfk_l, 4) - 16 * 4,
+                                     (size_t)4 * 16 * P.wave_lds);
+      // (form 2 is switched off: an iterative solve through it went silent
+      // on the GPU and the cause is not established; see the design notes)
+      ok[1][d] = false && cnt > 0 && P.nlarge > 0 && region <= 160 * 1024;
+    }
+  }
+  EventPair ev;
+  HIPCHK(ev.create());
+  TempDev tmp;
+  double best[2][2] = {{1e30, 1e30}, {1e30, 1e30}};
+  for (int f = 0; f < 2; f++)
+    for (int d = 0; d < 2; d++) {
+      if (!ok[f][d]) continue;
+      fwh_ctx::SolvePersist P = cand[f][d];
+      int *dord = nullptr;
+      HIPCHK(tmp.alloc(&dord, pord[f][d].size()));
+      HIPCHK(hipMemcpyAsync(dord, pord[f][d].data(), pord[f][d].size() * sizeof(int), hipMemcpyHostToDevice, c->st));
+      P.porder = dord;
+      c->sp = P;
+      c->sp.pgrid = 1;  // (solve_args fills the persistent fields only with a grid)
+      const int pfk_keep = c->pfk_iter;
+      c->pfk_iter = d == 0 ? fw::PFK : 8;
+      fw::SolveArgs a = c->solve_args(fw::MODE_RHS, false);
+      c->pfk_iter = pfk_keep;
+      const int occ = fw::solve_persistent_occupancy(a);
+      c->sp = fwh_ctx::SolvePersist{};
+      if (occ <= 0) continue;
+      a.pgrid = cand[f][d].pgrid = occ * ncu;
+      float ms0 = 0.f;
+      HIPCHK(hipEventRecord(ev.a, c->st));
+      HIPCHK(fw::launch_solve(a, a.nwork, c->st));
+      HIPCHK(hipEventRecord(ev.b, c->st));
+      HIPCHK(hipEventSynchronize(ev.b));
+      HIPCHK(hipEventElapsedTime(&ms0, ev.a, ev.b));
+      const int reps = ms0 > 2.0f ? 1 : 3;
+      for (int r = 0; r < reps; r++) {
+        float ms = 0.f;
+        HIPCHK(hipEventRecord(ev.a, c->st));
+        HIPCHK(fw::launch_solve(a, a.nwork, c->st));
+        HIPCHK(hipEventRecord(ev.b, c->st));
+        HIPCHK(hipEventSynchronize(ev.b));
+        HIPCHK(hipEventElapsedTime(&ms, ev.a, ev.b));
+        best[f][d] = std::min(best[f][d], (double)ms);
+      }
+    }
+  for (int f = 0; f < 2; f++) {
+    const double b = std::min(best[f][0], best[f][1]);
+    I.ms_solve_persist[f] = b < 1e30 ? b : -1.0;
+  }
+  // the pick: the faster depth of each form; a form only when 5 % faster
+  int pf = -1, pd = 0;
+  double pbest = 0.95 * static_ms;
+  for (int f = 0; f < 2; f++)
+    for (int d = 0; d < 2; d++)
+      if (best[f][d] < pbest) {
+        pbest = best[f][d];
+        pf = f;
+        pd = d;
+      }
+  // a static form forced by FWH_SOLVE_DEPTH / FWH_SOLVE_WAVES stays static
+  if (getenv("FWH_SOLVE_DEPTH") || getenv("FWH_SOLVE_WAVES")) pf = -1;
+  if (const char *e = getenv("FWH_SOLVE_PERSIST")) {
+    const int v = atoi(e);
+    if (v == 2) return fail(FWH_E_ARG, "FWH_SOLVE_PERSIST=2: the size-bucketed persistent form is switched off");
+    pf = -1;
+    if (v == 1 || v == 2) {
+      int dd = -1;
+      if (const char *e2 = getenv("FWH_SOLVE_DEPTH")) dd = atoi(e2) == 8 ? 1 : 0;
+      for (int d = 0; d < 2; d++)
+        if (best[v - 1][d] < 1e30 && (dd < 0 || dd == d) && (pf < 0 || best[v - 1][d] < best[pf][pd])) {
+          pf = v - 1;
+          pd = d;
+        }
+    }
+  }
+  if (pf >= 0) {
+    fwh_ctx::SolvePersist P = cand[pf][pd];
+    if ((rc = c->upload(&P.porder, pord[pf][pd]))) return rc;
+    c->sp = P;
+    c->pfk_iter = pd == 0 ? fw::PFK : 8;
+    c->use_buckets = false;
+    I.solve_persist = pf + 1;
+    I.solve_pgrid = P.pgrid;
+    I.solve_nlarge = P.nlarge;
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }
 
@@ -1431,6 +1653,11 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     I.ms_solve_depth[0] = depths == 2 ? tm(1, w) : 0.0;
     I.ms_solve_depth[1] = tm(0, w);
     for (int k = 0; k < 3; k++) I.ms_solve_waves[k] = tm(d, k) < 1e30 ? tm(d, k) : -1.0;
+    {
+      int ncu = 0;
+      HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+      if ((rc = tune_solve_persist(c, std::min(best[d][w][0], best[d][w][1]), std::max(ncu, 1)))) return rc;
+    }
     HIPCHK(hipMemsetAsync(c->X, 0, (size_t)nsys * ndof * sizeof(double2), st));
     HIPCHK(hipStreamSynchronize(st));
   }
@@ -1707,6 +1934,52 @@ int tune_smv_split(fwh_ctx *c, const std::vector<int> &ord, const std::vector<lo
   HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }
+
+// The persistent form of the matvec (k_smvq): as many workgroups as are
+// resident at once (occupancy x CUs) take the work items of the chosen list
+// in its order, each claiming the next unclaimed item when it finishes one,
+// so the items are list-scheduled largest first and the status of the
+// iteration is reduced once per workgroup instead of once per item.  Every
+// item writes its own outputs, so both forms give the same bits; the faster
+// is kept (pairs of back-to-back launches, best of five, alternating).
+// FWH_SMV_PERSIST=0 or 1 fixes the choice.
+int tune_smv_persist(fwh_ctx *c, int ncu) {
+  const int nw = c->smv_work ? c->smv_nwork : c->nsub;
+  const int pg = fw::smv_persistent_grid(c->smv_waves, c->nbmax, ncu);
+  c->info.smv_pgrid = pg;
+  c->info.ms_smv_persist[0] = c->info.ms_smv_persist[1] = -1.0;
+  c->smv_pgrid = 0;
+  if (pg > 0) {
+    EventPair ev;
+    HIPCHK(ev.create());
+    fw::SmvArgs sa[2];
+    for (int v = 0; v < 2; v++) {
+      c->smv_pgrid = v ? pg : 0;
+      sa[v] = c->smv_args(false, false);
+      HIPCHK(fw::launch_smv(sa[v], nw, c->st));
+      c->info.ms_smv_persist[v] = 1e30;
+    }
+    for (int r = 0; r < 5; r++)
+      for (int v = 0; v < 2; v++) {
+        float ms = 0.f;
+        HIPCHK(hipEventRecord(ev.a, c->st));
+        HIPCHK(fw::launch_smv(sa[v], nw, c->st));
+        HIPCHK(fw::launch_smv(sa[v], nw, c->st));
+        HIPCHK(hipEventRecord(ev.b, c->st));
+        HIPCHK(hipEventSynchronize(ev.b));
+        HIPCHK(hipEventElapsedTime(&ms, ev.a, ev.b));
+        c->info.ms_smv_persist[v] = std::min(c->info.ms_smv_persist[v], 0.5 * (double)ms);
+      }
+    c->smv_pgrid = c->info.ms_smv_persist[1] < c->info.ms_smv_persist[0] ? pg : 0;
+  }
+  if (const char *e = getenv("FWH_SMV_PERSIST")) {
+    const int v = atoi(e);
+    if (v == 0 || (v == 1 && pg > 0)) c->smv_pgrid = v ? pg : 0;
+  }
+  c->info.smv_persist = c->smv_pgrid > 0 ? 1 : 0;
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
 }  // namespace
 
 // Form every S_s = A_GG - A_GI A_II^-1 A_IG once: column k of all subdomains
@@ -1731,6 +2004,9 @@ int build_schur(fwh_ctx *c) {
   if ((rc = c->alloc(&c->smat, (size_t)c->nsys * tot)) || (rc = c->upload(&c->sm_off, off))) return rc;
   c->nbmax = nbmax;
   c->ss.smat = tot;  // every system: 64 zeros, then its tiles
+  c->smv_pgrid = 0;  // one workgroup per item until the persistent form is timed
+  if ((rc = c->alloc(&c->smv_queue, (size_t)2 * c->nsys))) return rc;
+  HIPCHK(hipMemsetAsync(c->smv_queue, 0, sizeof(int) * 2 * c->nsys, c->st));
   HIPCHK(fw::launch_fill(c->smat, (long long)c->nsys * tot, make_double2(0.0, 0.0), c->st));
   constexpr int NXCD = 8;
   // GEMM formation: W = L^-1 (-A_IG) by blocked forward substitution on the
@@ -2070,6 +2346,7 @@ int build_schur(fwh_ctx *c) {
     (void)hipEventDestroy(e1);
     const std::vector<int> ord(all.begin() + (size_t)pick * c->nsub, all.begin() + (size_t)(pick + 1) * c->nsub);
     if ((rc = tune_smv_split(c, ord, off, ncu))) return rc;
+    if ((rc = tune_smv_persist(c, ncu))) return rc;
   }
   double packed = 0.0;
   for (int s = 0; s < c->nsub; s++) {

@@ -128,6 +128,20 @@ struct SolveArgs {
   const long long *sm_off;  // per subdomain first entry of its tiles
   double2 *pqpart;       // MODE_ITER of an interface iteration: per subdomain its (p,q) part (may be null)
   int npq;               // per system: subdomains
+  // persistent form (pgrid > 0, batched launches other than MODE_UNIT):
+  // pgrid workgroups of pthreads threads per system claim the subdomains
+  // of porder from queue[4 s], queue[4 s + 1] (workgroups finished in
+  // queue[4 s + 2]; all back to zero when the last one ends).  The first
+  // nlarge entries of porder are solved by a whole workgroup of 256 threads
+  // (blocks of at most 64 pfk_l entries, LDS sized by nmax_l, nblkmax_l),
+  // the other nwork - nlarge by one wave each in its own wave_lds entries
+  // of LDS (blocks of at most 64 pfk entries, sized by nmax, nblkmax)
+  int pgrid, pthreads;
+  int *queue;
+  const int *porder;
+  int nwork, nlarge;
+  int pfk_l, nmax_l, nblkmax_l;
+  int wave_lds;
 };
 
 // S_s = A_GG - W^T D^-1 W with W = L^-1 (-A_IG), all on the FP64 matrix
@@ -220,6 +234,12 @@ struct SmvArgs {
   double2 *pqpart;       // per subdomain its part of (p,q) = sum_slots p y (explicit iteration; may be null)
   int npq;               // per system: subdomains
   double2 *yslot;
+  // persistent form (pgrid > 0): pgrid workgroups per system take the work
+  // items from a queue, workgroup b first item b, then the next unclaimed
+  // one (queue[2 s]: items claimed past the first pgrid, queue[2 s + 1]:
+  // workgroups finished; both back to zero when the last one ends)
+  int pgrid;
+  int *queue;
 };
 
 struct FactorArgs {
@@ -346,12 +366,19 @@ hipError_t launch_blockinv(const SubMeta *meta, int nsub, const int *idx1, const
                            const double2 *fac, double2 *minv, hipStream_t st);
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st);
 size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk, int waves);
+// the persistent form: LDS bytes of one workgroup (the large items' layout,
+// or pthreads/64 waves' own layouts, whichever is larger) and its resident
+// workgroups per CU (0: does not fit)
+size_t solve_persistent_lds(const SolveArgs &a);
+int solve_persistent_occupancy(const SolveArgs &a);
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st);
 hipError_t launch_blockinv16(const BlockInv16Args &a, hipStream_t st);
 hipError_t launch_schur_fwd(const FwdArgs &a, hipStream_t st);
 hipError_t launch_schur_gemm(const GemmArgs &a, hipStream_t st);
 hipError_t launch_schur_agg(const AggArgs &a, int nsub, hipStream_t st);
 size_t smv_lds_bytes(int nbmax, int waves);
+// workgroups of the persistent matvec: its resident capacity, occupancy x CUs (0: query failed)
+int smv_persistent_grid(int waves, int nbmax, int ncu);
 hipError_t launch_pcslot(int nslots, const int *slot_dof, const int *rowptr, const double2 *val,
                          double2 *pcslot, hipStream_t st);
 hipError_t launch_first(const VecArgs &a, hipStream_t st);

@@ -419,6 +419,207 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(5))) voi
   }
 }
 
+// the status of an interface iteration as the first work item of a
+// workgroup needs it (false: the system is finished; beta: the coefficient
+// of the next search direction).  Contains barriers (iter_status): call
+// uniformly.
+__device__ __forceinline__ bool smv_status(const SmvArgs &a, int NT, double2 &beta, const StatusPre *pre) {
+  if (a.done) {
+    if (a.Z) return iter_status(a, NT, beta, pre);
+    return *a.done == 0;
+  }
+  if (a.Z) beta = a.sc->beta;  // a timing run: the current status as it stands
+  return true;
+}
+
+// work item i of the launch: an entry of the work list, or a subdomain of
+// the order
+__device__ __forceinline__ SmvWork smv_item(const SmvArgs &a, int i) {
+  SmvWork wk;
+  if (a.work) {
+    wk = a.work[a.base + i];
+  } else {
+    const int w = a.order ? a.order[a.base + i] : a.base + i;
+    wk.off = a.sm_off[w];
+    wk.y0 = a.meta[w].g0;
+    wk.gg = a.meta[w].g0;
+    wk.ng = a.meta[w].ng;
+    wk.pq = w;
+    wk.t0 = 0;
+    wk.t1 = 0;
+  }
+  return wk;
+}
+
+// One work item of the persistent matvec (k_smvq): y = S_s p over the
+// item's tiles, as k_smv computes it.  first: the workgroup's first item,
+// which also takes the status of the iteration (after its first tiles are
+// in flight, or at once when the item has no interface DOFs) into beta;
+// later items use that beta.  Returns false when the system is finished
+// (the workgroup stops).  Contains barriers: call uniformly.  (k_smv keeps
+// its own copy of this body: at five waves per SIMD it sits at the register
+// limit, and the shared form spilled there.)
+template <int W>
+__device__ __forceinline__ bool smv_run(const SmvArgs &a, const SmvWork &wk, bool first, double2 &beta) {
+  constexpr int SMV_WAVES = W, NT = 64 * W;
+  const int ng = wk.ng, nb = (ng + 15) >> 4, npad = nb * 16, stride = a.nbmax * 16;
+  extern __shared__ double2 lds[];
+  double2 *pv = lds;          // [nbmax*16] interface values of the subdomain
+  double2 *acc = pv + stride; // [SMV_WAVES][nbmax*16] per-wave accumulators
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+
+  if (ng == 0) {
+    if (first && !smv_status(a, NT, beta, nullptr)) return false;
+    if (a.pqpart && tid == 0) a.pqpart[wk.pq] = cz();
+    return true;
+  }
+
+  // contiguous, equal shares of the item's tiles (row-block major; all of
+  // the subdomain's, or the range [wk.t0, wk.t1) of a split one); the
+  // first two tiles of every wave go in flight before the interface values
+  // are gathered, so their latency overlaps the gather (a wave without tiles
+  // loads tile 0, which every subdomain with an interface has)
+  const long long ntile = (long long)nb * (nb + 1) / 2;
+  const long long T0 = wk.t1 > wk.t0 ? wk.t0 : 0, T1 = wk.t1 > wk.t0 ? wk.t1 : ntile;
+  const long long t0 = T0 + (T1 - T0) * wave / SMV_WAVES, t1 = T0 + (T1 - T0) * (wave + 1) / SMV_WAVES;
+  const bool active = t0 < t1;
+  const long long tl = active ? t1 - 1 : 0;
+  const double2 *base = a.smat + wk.off;
+  Cursor cur;
+  cur.t = active ? t0 : 0;
+  cur.tl = tl;
+  cur.nb = nb;
+  cur.half = smv_half(ng);
+  tile_ij(cur.t, cur.I, cur.J);
+  const double2 *zero = a.smat;  // the 64 zero entries ahead of every subdomain's tiles
+  // requested before the tiles (loads complete in issue order): the status
+  // partials of an interface iteration (256 threads, first item) and the ids
+  // of this thread's first two slots
+  StatusPre sp;
+  const bool pre = W == 4 && first && a.done && a.Z;
+  if (pre) status_prefetch(a, sp);
+  const int k0 = tid, k1 = tid + NT;
+  const int gid0 = k0 < ng ? a.g_gid[wk.gg + k0] : 0;
+  const int gid1 = k1 < ng ? a.g_gid[wk.gg + k1] : 0;
+  Tile A, B;
+  load_next(base, zero, cur, lane, A);
+  load_next(base, zero, cur, lane, B);
+  // the values of those two slots (they do not depend on the status) follow
+  // the first tiles, ahead of the status fields
+  double2 pa = cz(), za = cz(), pb = cz(), zb = cz();
+  if (k0 < ng) {
+    pa = a.P[gid0];
+    if (a.Z) za = a.Z[gid0];
+  }
+  if (k1 < ng) {
+    pb = a.P[gid1];
+    if (a.Z) zb = a.Z[gid1];
+  }
+  // the status is read once the first tiles are in flight (a finished
+  // system's workgroups discard them)
+  // (two calls, each with a fixed pointer, so the prefetched partials stay
+  // in registers)
+  if (first && !(pre ? smv_status(a, NT, beta, &sp) : smv_status(a, NT, beta, nullptr))) return false;
+
+  // interface values: P, or the next search direction z + beta p while the
+  // interface iteration runs (formed here, not stored, see k_iter_ab)
+  for (int k = tid; k < npad; k += NT) {
+    double2 p = cz();
+    if (k < ng) {
+      double2 z;
+      if (k == k0) {
+        p = pa;
+        z = za;
+      } else if (k == k1) {
+        p = pb;
+        z = zb;
+      } else {
+        const int gid = a.g_gid[wk.gg + k];
+        p = a.P[gid];
+        z = a.Z ? a.Z[gid] : cz();
+      }
+      if (a.Z) p = zbp(z, beta, p);
+    }
+    pv[k] = p;
+  }
+  for (int k = tid; k < SMV_WAVES * stride; k += NT) acc[k] = cz();
+  __syncthreads();
+
+  if (active) {
+    double2 *my = acc + wave * stride;
+    SmvState st;
+    tile_ij(t0, st.I, st.J);
+    st.q0 = cz();
+    st.q1 = cz();
+    st.pI0 = pv[st.I * 16 + (lane & 7)];
+    st.pI1 = pv[st.I * 16 + (lane & 7) + 8];
+    // two tiles in flight in two statically named register sets; the loads
+    // are unconditional (clamped to the wave's last tile) so the compiler can
+    // wait for one set while the other stays in flight
+    for (long long t = t0; t < t1; t += 2) {
+      tile_step(A, st, pv, my, nb, t == tl, lane);
+      load_next(base, zero, cur, lane, A);
+      if (t + 1 < t1) tile_step(B, st, pv, my, nb, t + 1 == tl, lane);
+      load_next(base, zero, cur, lane, B);
+    }
+  }
+  __syncthreads();
+  // y, and in an explicit iteration this subdomain's part of (p,q) =
+  // sum over its slots of p y (the interface sum of p q, regrouped by
+  // subdomain; fixed order: per thread ascending, waves in order)
+  double2 pq = cz();
+  for (int k = tid; k < ng; k += NT) {
+    double2 y = cadd(acc[k], acc[stride + k]);
+    for (int v = 2; v < SMV_WAVES; v++) y = cadd(y, acc[v * stride + k]);
+    a.yslot[wk.y0 + k] = y;
+    pq = cadd(pq, cmul(pv[k], y));
+  }
+  if (a.pqpart) {
+    __shared__ double2 shq[SMV_WAVES];
+    pq = wave_csum(pq);
+    if (lane == 0) shq[wave] = pq;
+    __syncthreads();
+    if (tid == 0) {
+      double2 t = shq[0];
+      for (int v = 1; v < SMV_WAVES; v++) t = cadd(t, shq[v]);
+      a.pqpart[wk.pq] = t;
+    }
+  }
+  return true;
+}
+
+// Persistent form: pgrid workgroups per system, workgroup b takes item b,
+// then claims the next unclaimed item from the system's queue (items in
+// work order, largest first, so the claims are a longest-first list
+// schedule) until none is left.  Each item's outputs go where the item
+// says, whichever workgroup computes it, so the results are those of k_smv.
+// The status of the iteration is taken once per workgroup (with its first
+// item) instead of once per item.  The workgroup that ends last puts the
+// queue back to zero for the next launch (no memset, so graphs replay it).
+template <int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void k_smvq(SmvArgs a, int nwork) {
+  int *q = a.queue + 2 * blockIdx.y;
+  to_system(a);
+  __shared__ int next;
+  double2 beta = cz();
+  // a finished system: every workgroup of it reads the same status at its
+  // first item and stops before claiming anything, so the queue stays zero
+  if (!smv_run<W>(a, smv_item(a, blockIdx.x), true, beta)) return;
+  for (;;) {
+    __syncthreads();  // the item's LDS reads are done, and the last claim was read
+    if (threadIdx.x == 0) next = (int)gridDim.x + atomicAdd(q, 1);
+    __syncthreads();
+    const int i = next;
+    if (i >= nwork) break;
+    (void)smv_run<W>(a, smv_item(a, i), false, beta);
+  }
+  if (threadIdx.x == 0 && atomicAdd(q + 1, 1) == (int)gridDim.x - 1) {
+    // every workgroup has made its last claim
+    atomicExch(q, 0);
+    atomicExch(q + 1, 0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // GEMM formation of S_s.  With the Dirichlet matrix factored as L D L^T and
 // W = L^-1 (-A_IG), S_s = A_GG - W^T D^-1 W.  The interface and essential
@@ -658,9 +859,31 @@ hipError_t launch_schur_agg(const AggArgs &a, int nsub, hipStream_t st) {
 
 size_t smv_lds_bytes(int nbmax, int waves) { return sizeof(double2) * (size_t)(1 + waves) * nbmax * 16; }
 
+int smv_persistent_grid(int waves, int nbmax, int ncu) {
+  const size_t lds = smv_lds_bytes(nbmax, waves);
+  int nb = 0;
+  hipError_t e;
+  switch (waves) {
+    case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_smvq<2>, 128, lds); break;
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_smvq<8>, 512, lds); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_smvq<4>, 256, lds); break;
+  }
+  return (e == hipSuccess && nb > 0) ? nb * ncu : 0;
+}
+
 hipError_t launch_smv(const SmvArgs &a, int nwork, hipStream_t st) {
   if (nwork <= 0) return hipSuccess;
   const size_t lds = smv_lds_bytes(a.nbmax, a.waves);
+  if (a.pgrid > 0) {
+    if (!a.queue || a.base != 0) return hipErrorInvalidValue;
+    const dim3 grid(std::min(nwork, a.pgrid), a.nsys > 0 ? a.nsys : 1);
+    switch (a.waves) {
+      case 2: k_smvq<2><<<grid, 128, lds, st>>>(a, nwork); break;
+      case 8: k_smvq<8><<<grid, 512, lds, st>>>(a, nwork); break;
+      default: k_smvq<4><<<grid, 256, lds, st>>>(a, nwork); break;
+    }
+    return hipGetLastError();
+  }
   const dim3 grid(nwork, a.nsys > 0 ? a.nsys : 1);
   switch (a.waves) {
     case 2: k_smv<2><<<grid, 128, lds, st>>>(a); break;

@@ -17,6 +17,8 @@
 // workgroup barriers, block-row metadata in registers (broadcast with
 // v_readlane), the triangle of each block resolved by one lane per row with
 // its operands preloaded into registers.
+#include <algorithm>
+
 #include "fw_kernels.h"
 #include "fw_device.h"
 
@@ -49,7 +51,7 @@ struct PfSet {
 template <int K>
 __device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, const BlkDesc q,
                                             const double2 *__restrict__ minv, PfSet<K> &p) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (WL - 1);
   const int e0 = q.e0, last = q.e1 - 1;
   p.q = q;
 #pragma unroll
@@ -69,7 +71,7 @@ __device__ __forceinline__ void issue_block(const double2 *__restrict__ fac, con
 // Unconditional staging: entries past the block land in unused LDS slots.
 template <int K>
 __device__ __forceinline__ void stage_block(double2 *stg, double2 *mst, const PfSet<K> &p) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (WL - 1);
 #pragma unroll
   for (int k = 0; k < K; k++) stg[lane + k * WL] = make_double2(p.e[k].x, p.e[k].y);
 #pragma unroll
@@ -81,6 +83,24 @@ __device__ __forceinline__ void stage_block(double2 *stg, double2 *mst, const Pf
 // s_barrier and, above all, no memory fence that would drain the loads in
 // flight).
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
+// A team is the group of threads that solves one subdomain: a workgroup of
+// NT threads, or (NT = 64) one wave, alone in its workgroup or one of the
+// waves of a persistent workgroup that solve subdomains each on their own.
+template <int NT>
+__device__ __forceinline__ int team_tid() {
+  return NT == WL ? (int)(threadIdx.x & (WL - 1)) : (int)threadIdx.x;
+}
+// ordering point of a single-wave team: a barrier when the wave is the
+// workgroup, else a wave barrier (LDS traffic of one wave is processed in
+// order, so the other waves need not wait)
+template <bool TEAM>
+__device__ __forceinline__ void team_sync() {
+  if (TEAM)
+    wave_sync();
+  else
+    __syncthreads();
+}
 
 struct SolveLds {
   double2 *v, *stg, *mst;       // LDS: vector, staged block, staged inverse triangle
@@ -101,7 +121,7 @@ __device__ __forceinline__ const double2 *minv_of(const SolveLds &L, int b) { re
 template <int R, int NT>
 __device__ __forceinline__ void col_updates(const SolveLds &L, int jlo, int i0, int fR, int rbR, double2 xv,
                                             int xl) {
-  for (int j = jlo + (int)threadIdx.x; j < i0; j += NT) {
+  for (int j = jlo + team_tid<NT>(); j < i0; j += NT) {
     double2 acc = cz();
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -128,7 +148,7 @@ __device__ __forceinline__ void col_updates_nb(const SolveLds &L, int nb, int jl
 template <int K>
 __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__restrict__ fac, int nblk, int b,
                                           PfSet<K> &p) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (WL - 1);
   const int i0 = p.q.i0, i1 = p.q.i1;
   const int e0 = p.q.e0;
   const int nb = i1 - i0;
@@ -197,7 +217,7 @@ __device__ __forceinline__ void fwd_block(const SolveLds &L, const double2 *__re
 template <int K>
 __device__ __forceinline__ void bwd_block(const SolveLds &L, const double2 *__restrict__ fac, int b,
                                           PfSet<K> &p) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (WL - 1);
   const int i0 = p.q.i0, i1 = p.q.i1;
   const int e0 = p.q.e0;
   const int nb = i1 - i0;
@@ -285,22 +305,25 @@ __device__ __forceinline__ bool solve_item(const SolveArgs &a, int &w, int &uk) 
 
 // LDS carve-up: v[nmax], stg[cap], mst[nmst], bt[nblkmax+1] (16-byte
 // entries), idx1[nmax], rb[nmax+1], blk[nblkmax+1]
-__device__ __forceinline__ SolveLds solve_lds(const SolveArgs &a, double2 *lds, int cap, int nmst) {
+__device__ __forceinline__ SolveLds solve_lds(double2 *lds, int nmax, int nblkmax, int cap, int nmst) {
   SolveLds L;
   L.v = lds;
-  L.stg = L.v + a.nmax;
+  L.stg = L.v + nmax;
   L.mst = L.stg + cap;
   L.bt = (BlkDesc *)(L.mst + nmst);
-  L.idx1 = (int *)(L.bt + a.nblkmax + 1);
-  L.rb = L.idx1 + a.nmax;
-  L.blk = L.rb + a.nmax + 1;
+  L.idx1 = (int *)(L.bt + nblkmax + 1);
+  L.rb = L.idx1 + nmax;
+  L.blk = L.rb + nmax + 1;
   return L;
+}
+__device__ __forceinline__ SolveLds solve_lds(const SolveArgs &a, double2 *lds, int cap, int nmst) {
+  return solve_lds(lds, a.nmax, a.nblkmax, cap, nmst);
 }
 // the row-block table from the block rows and row starts in LDS (call
 // after a barrier that follows load_meta)
 template <int NT>
 __device__ __forceinline__ void load_blocks(const SolveLds &L, int nblk) {
-  for (int b = threadIdx.x; b < nblk; b += NT) {
+  for (int b = team_tid<NT>(); b < nblk; b += NT) {
     const int i0 = L.blk[b], i1 = L.blk[b + 1];
     L.bt[b] = BlkDesc{i0, i1, L.rb[i0], L.rb[i1]};
   }
@@ -309,7 +332,7 @@ __device__ __forceinline__ void load_blocks(const SolveLds &L, int nblk) {
 template <int NT>
 __device__ __forceinline__ void load_meta(const SolveArgs &a, const SubMeta &m, const SolveLds &L, int nblk,
                                           int blk0) {
-  const int t = threadIdx.x;
+  const int t = team_tid<NT>();
   for (int i = t; i < m.n; i += NT) {
     L.idx1[i] = a.idx1[m.dof0 + i];
     L.rb[i] = a.rowbeg[m.dof0 + i];
@@ -330,7 +353,7 @@ __device__ __forceinline__ void load_rhs(const SolveArgs &a, const SubMeta &m, d
     const double2 p = a.P[gid];
     return a.Z ? zbp(a.Z[gid], beta, p) : p;
   };
-  for (int i = threadIdx.x; i < m.n; i += NT) {
+  for (int i = team_tid<NT>(); i < m.n; i += NT) {
     const int d = m.dof0 + i;
     double2 b;
     if (a.mode == MODE_RHS) {
@@ -369,7 +392,7 @@ __device__ __forceinline__ void load_rhs(const SolveArgs &a, const SubMeta &m, d
 template <int NT>
 __device__ __forceinline__ double2 write_outputs(const SolveArgs &a, const SubMeta &m, const double2 *v, int w,
                                                  int uk) {
-  const int t = threadIdx.x;
+  const int t = team_tid<NT>();
   double2 pq = cz();
   if (a.mode == MODE_LAST || a.mode == MODE_RHS)
     for (int i = t; i < m.n; i += NT) a.X[m.dof0 + i] = v[i];
@@ -400,21 +423,14 @@ __device__ __forceinline__ double2 write_outputs(const SolveArgs &a, const SubMe
   return pq;
 }
 
-template <int K>  // prefetch loads per lane: row blocks of at most 64*K entries
-__global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 3 : 2))) void k_solve(SolveArgs a) {
-  constexpr int CAPK = WL * K;
-  to_system(a);
-  // (the status of an iteration launch is read below, once the metadata
-  // loads are in flight)
-  const bool check_done = a.mode == MODE_ITER && a.done;
-  const int done0 = check_done ? *a.done : 0;
-  int w = 0, uk = 0;
-  if (!solve_item(a, w, uk)) return;
-  const SubMeta m = a.meta[w];
-  if (a.mode == MODE_UNIT && uk >= m.ng) return;
-  extern __shared__ double2 lds[];
-  SolveLds L = solve_lds(a, lds, CAPK, MINV);
-  const int lane = threadIdx.x;
+// The Schur step of subdomain w on one wave (K prefetch loads per lane: row
+// blocks of at most 64 K entries), LDS carved in L.  TEAM: the wave is one
+// of several in its workgroup, each on its own subdomain.  done0: the
+// system is finished (the launch stops once the metadata loads are issued).
+template <int K, bool TEAM>
+__device__ __forceinline__ void solve1_body(const SolveArgs &a, const SubMeta &m, int w, int uk, SolveLds L,
+                                            int done0) {
+  const int lane = threadIdx.x & (WL - 1);
   const int n = m.n;
   const double2 *fac = a.fac + m.fac0;
   // row blocks of this prefetch depth (the host keeps both partitions)
@@ -424,10 +440,10 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
 
   // ---- 0. metadata --------------------------------------------------------------
   load_meta<WL>(a, m, L, nblk, blk0);
-  __syncthreads();
+  team_sync<TEAM>();
   if (done0) return;
   load_blocks<WL>(L, nblk);
-  __syncthreads();
+  team_sync<TEAM>();
 
   // ---- 1. load vector --------------------------------------------------------------
   load_rhs<WL>(a, m, L.v, uk);
@@ -472,6 +488,22 @@ __global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 
     pq = wave_csum(pq);
     if (lane == 0) a.pqpart[w] = pq;
   }
+}
+
+template <int K>  // prefetch loads per lane: row blocks of at most 64*K entries
+__global__ __launch_bounds__(WL) __attribute__((amdgpu_waves_per_eu(2, K == 8 ? 3 : 2))) void k_solve(SolveArgs a) {
+  constexpr int CAPK = WL * K;
+  to_system(a);
+  // (the status of an iteration launch is read below, once the metadata
+  // loads are in flight)
+  const bool check_done = a.mode == MODE_ITER && a.done;
+  const int done0 = check_done ? *a.done : 0;
+  int w = 0, uk = 0;
+  if (!solve_item(a, w, uk)) return;
+  const SubMeta m = a.meta[w];
+  if (a.mode == MODE_UNIT && uk >= m.ng) return;
+  extern __shared__ double2 lds[];
+  solve1_body<K, false>(a, m, w, uk, solve_lds(a, lds, CAPK, MINV), done0);
 }
 
 // ---------------------------------------------------------------------------
@@ -642,22 +674,16 @@ __device__ __forceinline__ void bwd_block4(const SolveLds &L, const double2 *__r
   __syncthreads();
 }
 
-template <int KQ, int NT>  // prefetch loads per thread; threads: row blocks of at most NT*KQ entries
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 128 ? 3 : 4))) void k_solve4(SolveArgs a) {
+// The Schur step of subdomain w on a workgroup of NT threads (KQ prefetch
+// loads per thread: row blocks of at most NT KQ entries, the partition of
+// the single-wave kernel with blocks of that size), LDS carved in L.
+template <int KQ, int NT>
+__device__ __forceinline__ void solve4_body(const SolveArgs &a, const SubMeta &m, int w, int uk, SolveLds L,
+                                            int done0) {
   constexpr int CAPQ = NT * KQ;
-  to_system(a);
-  const bool check_done = a.mode == MODE_ITER && a.done;
-  const int done0 = check_done ? *a.done : 0;
-  int w = 0, uk = 0;
-  if (!solve_item(a, w, uk)) return;
-  const SubMeta m = a.meta[w];
-  if (a.mode == MODE_UNIT && uk >= m.ng) return;
-  extern __shared__ double2 lds[];
-  SolveLds L = solve_lds(a, lds, CAPQ, 2 * MINV);
   const int t = threadIdx.x;
   const int n = m.n;
   const double2 *fac = a.fac + m.fac0;
-  // the partition of the single-wave kernel whose blocks hold NT*KQ entries
   const int nblk = CAPQ == 512 ? m.nblk_s : m.nblk;
   const int blk0 = CAPQ == 512 ? m.blk0_s : m.blk0;
   L.minv = a.minv + (long long)blk0 * MINV;
@@ -698,6 +724,78 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 128 ? 
       for (int v = 1; v < NT / WL; v++) sum = cadd(sum, shq[v]);
       a.pqpart[w] = sum;
     }
+  }
+}
+
+template <int KQ, int NT>  // prefetch loads per thread; threads: row blocks of at most NT*KQ entries
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 128 ? 3 : 4))) void k_solve4(SolveArgs a) {
+  to_system(a);
+  const bool check_done = a.mode == MODE_ITER && a.done;
+  const int done0 = check_done ? *a.done : 0;
+  int w = 0, uk = 0;
+  if (!solve_item(a, w, uk)) return;
+  const SubMeta m = a.meta[w];
+  if (a.mode == MODE_UNIT && uk >= m.ng) return;
+  extern __shared__ double2 lds[];
+  solve4_body<KQ, NT>(a, m, w, uk, solve_lds(a, lds, NT * KQ, 2 * MINV), done0);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent form with size-bucketed queues: pgrid workgroups per system
+// (as many as are resident at once) claim subdomains until none is left.
+// NT = 256: first the large subdomains (porder[0, nlarge), largest first),
+// each solved by the whole workgroup as in k_solve4 (KL loads per thread,
+// blocks of at most 256 KL entries); then the small ones, each wave on its
+// own subdomain in its own part of the workgroup's LDS as in k_solve (KS
+// loads per lane), so a CU holds one large subdomain or four small ones in
+// the same LDS.  NT = 64: small subdomains only.  Each subdomain's outputs
+// go where it says, whichever workgroup solves it, and its sums are those of
+// the kernel form that solves it.  The workgroup that ends last puts the
+// queue back to zero for the next launch (no memset, so graphs replay it).
+template <int KL, int KS, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 2 : 1))) void
+k_solveq(SolveArgs a) {
+  int *q = a.queue + 4 * blockIdx.y;
+  to_system(a);
+  // a finished system: every workgroup of it stops before claiming
+  // anything, so the queue stays zero
+  if (a.mode == MODE_ITER && a.done && *a.done) return;
+  extern __shared__ double2 lds[];
+  const int t = threadIdx.x;
+  if (NT > WL) {
+    __shared__ int next;
+    for (;;) {
+      if (t == 0) next = atomicAdd(q, 1);
+      __syncthreads();
+      // (uniform: in scalar registers, so the subdomain's metadata is too)
+      const int i = __builtin_amdgcn_readfirstlane(next);
+      __syncthreads();  // every thread has read the claim before the next one is written
+      if (i >= a.nlarge) break;
+      const int w = a.porder[i];
+      const SubMeta m = a.meta[w];
+      solve4_body<KL, NT>(a, m, w, 0, solve_lds(lds, a.nmax_l, a.nblkmax_l, NT * KL, 2 * MINV), 0);
+    }
+    __syncthreads();  // the large items' LDS is free for the waves' own parts
+  }
+  const int lane = t & (WL - 1);
+  double2 *mine = lds + (size_t)a.wave_lds * (t / WL);
+  const int nsmall = a.nwork - a.nlarge;
+  for (;;) {
+    int i = 0;
+    if (lane == 0) i = atomicAdd(q + 1, 1);
+    i = __builtin_amdgcn_readfirstlane(i);  // lane 0's claim, uniform
+    if (i >= nsmall) break;
+    const int w = a.porder[a.nlarge + i];
+    const SubMeta m = a.meta[w];
+    solve1_body<KS, true>(a, m, w, 0, solve_lds(mine, a.nmax, a.nblkmax, WL * KS, MINV), 0);
+    wave_sync();
+  }
+  __syncthreads();
+  if (t == 0 && atomicAdd(q + 2, 1) == (int)gridDim.x - 1) {
+    // every workgroup has made its last claims
+    atomicExch(q, 0);
+    atomicExch(q + 1, 0);
+    atomicExch(q + 2, 0);
   }
 }
 
@@ -749,8 +847,58 @@ size_t solve_lds_bytes(int nmax, int ngmax, int nblkmax, int pfk, int waves) {
   return sizeof(double2) * ((size_t)nmax + (size_t)WL * pfk + MINV) + ints;
 }
 
+size_t solve_persistent_lds(const SolveArgs &a) {
+  // every wave's own part: wave_lds entries (at least its layout)
+  const size_t one = std::max(solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk, 1),
+                              sizeof(double2) * (size_t)a.wave_lds);
+  size_t b = one * (size_t)(a.pthreads / WL);
+  if (a.pthreads > WL && sizeof(double2) * (size_t)a.wave_lds < solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk, 1))
+    return (size_t)1 << 30;  // a wave's part would overlap the next: never fits
+  if (a.nlarge > 0) {
+    // the large items' layout (its static (p,q) partials are not dynamic LDS)
+    const size_t large = solve_lds_bytes(a.nmax_l, a.ngmax, a.nblkmax_l, a.pfk_l, 4) - sizeof(double2) * 4;
+    b = std::max(b, large);
+  }
+  return b;
+}
+
+// the persistent kernel of the args' shape (null: not instantiated)
+static const void *solveq_kernel(const SolveArgs &a) {
+  // (the size-bucketed form, 256 threads with a large queue, is switched
+  // off: an iterative solve through it went silent on the GPU and the cause
+  // is not established; it is not instantiated)
+  if (a.pthreads == 64 && a.nlarge == 0) {
+    if (a.pfk == 8) return (const void *)k_solveq<4, 8, 64>;
+    if (a.pfk == 16) return (const void *)k_solveq<4, 16, 64>;
+  }
+  return nullptr;
+}
+
+int solve_persistent_occupancy(const SolveArgs &a) {
+  const void *k = solveq_kernel(a);
+  const size_t lds = solve_persistent_lds(a);
+  if (!k || lds > 160 * 1024) return 0;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, a.pthreads, lds) != hipSuccess) return 0;
+  return nb;
+}
+
 hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
   if (nwork <= 0) return hipSuccess;
+  if (a.pgrid > 0) {
+    // (one subdomain per entry of porder; the caller's item count must match)
+    const void *k = solveq_kernel(a);
+    if (!k || !a.queue || !a.porder || a.mode == MODE_UNIT || nwork != a.nwork || a.nlarge < 0 ||
+        a.nlarge > a.nwork || (a.pthreads == 64 && a.nlarge > 0))
+      return hipErrorInvalidValue;
+    const dim3 grid(a.pgrid, a.nsys > 0 ? a.nsys : 1);
+    const size_t lds = solve_persistent_lds(a);
+    if (a.pfk == 8)
+      k_solveq<4, 8, 64><<<grid, 64, lds, st>>>(a);
+    else
+      k_solveq<4, 16, 64><<<grid, 64, lds, st>>>(a);
+    return hipGetLastError();
+  }
   const dim3 grid(nwork, a.nsys > 0 ? a.nsys : 1);
   if (a.waves == 2 || a.waves == 4) {
     // the static (p,q) partials are not part of the dynamic size

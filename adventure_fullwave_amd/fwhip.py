@@ -69,7 +69,11 @@ class FwhInfo(ctypes.Structure):
                 ("ms_schur_gemm", ctypes.c_double), ("flops_schur_fwd", ctypes.c_double),
                 ("flops_schur_gemm", ctypes.c_double), ("smv_split", ctypes.c_int),
                 ("smv_nwork", ctypes.c_int), ("ms_smv_split", ctypes.c_double * 4),
-                ("solve_waves", ctypes.c_int), ("ms_solve_waves", ctypes.c_double * 3)]
+                ("solve_waves", ctypes.c_int), ("ms_solve_waves", ctypes.c_double * 3),
+                ("smv_persist", ctypes.c_int), ("smv_pgrid", ctypes.c_int),
+                ("ms_smv_persist", ctypes.c_double * 2),
+                ("solve_persist", ctypes.c_int), ("solve_pgrid", ctypes.c_int),
+                ("solve_nlarge", ctypes.c_int), ("ms_solve_persist", ctypes.c_double * 2)]
 
     def as_dict(self):
         out = {}

@@ -124,12 +124,17 @@ def kernel_extras(args, fwhip, mesh):
             "k_smv": {"ms_per_launch": round(ms_smv, 4), "achieved": round(b_it / (ms_smv * 1e-3) / 1e9, 1),
                       "unit": "GB/s", "frac": round(b_it / (ms_smv * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                       "explicit_stream_GBs": round(info["bytes_smv"] / (ms_smv * 1e-3) / 1e9, 1),
-                      "smv_split": info["smv_split"], "smv_waves": info["smv_waves"]},
+                      "smv_split": info["smv_split"], "smv_waves": info["smv_waves"],
+                      "smv_persist": info["smv_persist"],
+                      "ms_static_vs_persistent": [round(v, 4) for v in info["ms_smv_persist"]]},
             "k_solve_iter": {"ms_per_launch": round(ms_schur, 4), "bytes_per_launch": b_it,
                              "GBs": round(b_it / (ms_schur * 1e-3) / 1e9, 1)},
             "ldl_solve": {"ms": round(ms_solve, 4), "bytes": info["bytes_solve"],
                           "GBs": round(info["bytes_solve"] / (ms_solve * 1e-3) / 1e9, 1),
-                          "solve_depth": info["solve_depth"], "solve_waves": info["solve_waves"]},
+                          "solve_depth": info["solve_depth"], "solve_waves": info["solve_waves"],
+                          "solve_persist": info["solve_persist"], "solve_large_queue": info["solve_nlarge"],
+                          "ms_static_by_waves_1_2_4": [round(v, 4) for v in info["ms_solve_waves"]],
+                          "ms_persistent_forms_1_2": [round(v, 4) for v in info["ms_solve_persist"]]},
         }
         log(f"[bench] extras cfg{c}: k_smv {ms_smv:.4f} ms, k_solve {ms_schur:.4f} ms, LDL^T {ms_solve:.4f} ms "
             f"({time.time() - t0:.1f}s)")
@@ -362,7 +367,12 @@ def main():
                      "smv_order": info["smv_order"],
                      "ms_smv_by_order": [round(v, 4) for v in info["ms_smv_order"]],
                      "smv_split": info["smv_split"], "smv_items": info["smv_nwork"],
-                     "ms_smv_by_split_0_2_4_8": [round(v, 4) for v in info["ms_smv_split"]]},
+                     "ms_smv_by_split_0_2_4_8": [round(v, 4) for v in info["ms_smv_split"]],
+                     "smv_persist": info["smv_persist"], "smv_persistent_grid": info["smv_pgrid"],
+                     "ms_smv_static_vs_persistent": [round(v, 4) for v in info["ms_smv_persist"]],
+                     "solve_persist": info["solve_persist"], "solve_persistent_grid": info["solve_pgrid"],
+                     "solve_large_queue": info["solve_nlarge"],
+                     "ms_solve_persistent_forms_1_2": [round(v, 4) for v in info["ms_solve_persist"]]},
         "adjoint": adjoint,
         "sweep": sweep,
         # from the model in host memory: the whole load (host index work,

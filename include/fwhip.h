@@ -137,6 +137,26 @@ typedef struct fwh_info {
                            at load                                         */
   double ms_solve_waves[3]; /* best batched solve time with 1, 2 and 4
                            waves at the chosen depth (-1: does not fit)    */
+  int    smv_persist;   /* 1: the explicit matvec runs as a persistent
+                           grid of smv_pgrid workgroups that claim the work
+                           items from a queue (same items, same results);
+                           0: one workgroup per item.  Picked by timing    */
+  int    smv_pgrid;     /* workgroups of the persistent grid (resident
+                           capacity: occupancy x CUs)                      */
+  double ms_smv_persist[2]; /* its timings: one workgroup per item,
+                           persistent (-1: not timed)                      */
+  int    solve_persist; /* the batched LDL^T kernel outside the formation:
+                           0 one workgroup per subdomain (solve_depth,
+                           solve_waves); 1 persistent, workgroups of one
+                           wave claiming subdomains from a queue; 2
+                           persistent with two size-bucketed queues (the
+                           solve_nlarge largest subdomains by a workgroup
+                           of four waves, the others one wave each).
+                           Picked by timing at load                        */
+  int    solve_pgrid;   /* workgroups of the persistent grid               */
+  int    solve_nlarge;  /* subdomains of the large queue (form 2)          */
+  double ms_solve_persist[2]; /* best LDL^T solve time of forms 1 and 2
+                           (-1: does not fit)                              */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */

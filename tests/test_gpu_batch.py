@@ -22,7 +22,8 @@ from tests.helpers import pinned_env, with_delta_gap
 
 pytestmark = pytest.mark.gpu
 
-CHOICES = ("smv_waves", "smv_split", "solve_depth", "solve_bucketed")
+CHOICES = ("smv_waves", "smv_split", "solve_depth", "solve_waves", "unit_depth", "solve_bucketed", "schur_gemm",
+           "solve_persist", "solve_nlarge")
 
 
 def say(msg):
@@ -52,7 +53,8 @@ def test_batch_matches_single_solves(systems, schur):
         xb, x1 = b.x(), s1.x()
         rb, r1 = b.resid(st1["loops"]), s1.resid(st1["loops"])
         same = all(b.info[c] == s1.info[c] for c in CHOICES)
-        say(f"[gpu] system {k}: iters batch={stats_b[k]['iters']} single={st1['iters']} same_choices={same}")
+        say(f"[gpu] system {k}: iters batch={stats_b[k]['iters']} single={st1['iters']} same_choices={same} "
+            f"{ {c: (b.info[c], s1.info[c]) for c in CHOICES} }")
         assert stats_b[k]["converged"] == 1 and st1["converged"] == 1
         if same:
             assert stats_b[k]["iters"] == st1["iters"]

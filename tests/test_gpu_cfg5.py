@@ -42,7 +42,8 @@ EPS = 1e-12
 H = 1e-6          # central-difference step in eps_r
 REL_BAR = 1e-6    # relative, per element
 FLOOR = 1e-9      # of the largest gradient entry
-CHOICES = ("smv_waves", "smv_split", "solve_depth", "solve_bucketed")
+CHOICES = ("smv_waves", "smv_split", "solve_depth", "solve_waves", "unit_depth", "solve_bucketed", "schur_gemm",
+           "solve_persist", "solve_nlarge")
 
 
 def say(msg):
@@ -179,7 +180,8 @@ def test_cfg5_frequency_batch(cfg5):
         b.select(k)
         xb, x1 = b.x(), s1.x()
         same = all(b.info[c] == s1.info[c] for c in CHOICES)
-        say(f"[gpu] frequency {k}: iters batch={stats_b[k]['iters']} single={st1['iters']} same_choices={same}")
+        say(f"[gpu] frequency {k}: iters batch={stats_b[k]['iters']} single={st1['iters']} same_choices={same} "
+            f"{ {c: (b.info[c], s1.info[c]) for c in CHOICES} }")
         assert stats_b[k]["converged"] == 1 and st1["converged"] == 1
         if same:
             assert stats_b[k]["iters"] == st1["iters"] and np.array_equal(xb, x1)
