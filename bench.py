@@ -33,11 +33,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(cfg, waves, split):
+def pmc_traffic(cfg, waves, split, persist):
     """HBM bytes per k_smv launch from the committed rocprofv3 --pmc summary
     (FETCH_SIZE, WRITE_SIZE, scaled by the copy-kernel calibration), used only
     while the kernel source is the one that was profiled and the run uses the
-    profiled waves per workgroup and split of large subdomains; else None."""
+    profiled waves per workgroup, split of large subdomains and grid form
+    (one workgroup per item or persistent); else None."""
     import hashlib
     path = os.path.join(ROOT, "profiles", f"k_smv_cfg{cfg}_pmc.json")
     src = os.path.join(ROOT, "adventure_fullwave_amd", "csrc", "fw_schur.hip")
@@ -45,7 +46,7 @@ def pmc_traffic(cfg, waves, split):
         d = json.load(open(path))
         if d.get("source_sha256") != hashlib.sha256(open(src, "rb").read()).hexdigest():
             return None
-        if d.get("smv_waves") != waves or d.get("smv_split", 0) != split:
+        if d.get("smv_waves") != waves or d.get("smv_split", 0) != split or d.get("smv_persist", 0) != persist:
             return None
         return d["traffic_bytes_per_launch"]
     except (OSError, ValueError, KeyError):
@@ -248,7 +249,7 @@ def main():
         # form's own stream (packed S_s) is reported beside it
         achieved = bytes_iter_kernel / (ms_smv * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.cfg, info["smv_waves"], info["smv_split"]),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.cfg, info["smv_waves"], info["smv_split"], info["smv_persist"]),
                     "kernel": "k_smv (explicit Schur matvec, all subdomains: one interface iteration's Schur step)",
                     "ms_per_launch": round(ms_smv, 4), "bytes_per_launch": bytes_iter_kernel,
                     "explicit_stream": {"bytes_per_launch": info["bytes_smv"],
