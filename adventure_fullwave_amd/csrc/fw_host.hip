@@ -265,6 +265,16 @@ struct fwh_ctx {
   int smv_pgrid = 0;
   int *smv_queue = nullptr;
   bool schur_built = false;
+  // factors released after the Schur formation (fwh_release_factors): the
+  // solves that need them (first and last steps, right-hand sides) recompute
+  // them chunk by chunk of subdomains into the c* buffers, at most a budget
+  // of bytes at a time; the explicit iteration needs none
+  bool fac_resident = true;
+  std::vector<int> fchunk;            // chunk boundaries (subdomains), back() == nsub
+  double2 *cfac = nullptr, *cminv = nullptr, *cm16 = nullptr;
+  long long *mf_off_d = nullptr;      // per subdomain its first 16-row block of cm16's layout
+  std::vector<long long> mf_off_h;
+  long long nblk_tot = 0;             // entries of blk_row
 
   // size buckets of the batched LDL^T kernel: subdomains grouped by how many
   // of their workgroups fit a CU's LDS, one launch per bucket on its own
@@ -330,6 +340,10 @@ struct fwh_ctx {
     release_graph();
     for (void *p : allocs) (void)hipFree(p);
     allocs.clear();
+    fac_resident = true;
+    fchunk.clear();
+    cfac = cminv = cm16 = nullptr;
+    mf_off_d = nullptr;
     for (int b = 0; b < 8; b++) {
       if (bst[b]) (void)hipStreamDestroy(bst[b]);
       if (ev_join[b]) (void)hipEventDestroy(ev_join[b]);
@@ -484,7 +498,92 @@ struct fwh_ctx {
   }
   // all subdomains: one launch (batched, size-ordered) or one launch per
   // subdomain in the reference's loop order (plain)
+  // Recompute the factors of subdomains [s0, s1) of system sys into the
+  // chunk buffers: the load's steps (skyline fill from the assembled matrix,
+  // identity rows, LDL^T factor kernel, block inverses) restricted to the
+  // chunk, so the factors and pivots come out as the load made them.  The
+  // kernels address a subdomain by its offsets in the whole model, so the
+  // buffers are passed shifted back by the chunk's first offsets (they only
+  // touch the chunk's own entries).
+  hipError_t refactor_chunk(int sys, int s0, int s1) {
+    const SubMeta &A = meta_h[s0], &B = meta_h[s1 - 1];
+    const long long f0 = A.fac0, f1 = B.fac0 + B.nlen;
+    const long long e0 = A.nnz0, e1 = (long long)B.nnz0 + B.nnz;
+    const int d0 = A.dof0, d1 = B.dof0 + B.n;
+    const long long b0 = A.blk0;
+    double2 *facc = cfac - f0;
+    hipError_t e = fw::launch_fill(cfac, f1 - f0, make_double2(0.0, 0.0), st);
+    if (e == hipSuccess)
+      e = fw::launch_skyfill((int)(e1 - e0), ent_row + e0, dof_sub, meta, col + e0, aval + (size_t)sys * nnz + e0,
+                             bcflag, idx1, rowbeg, facc, st);
+    if (e == hipSuccess) e = fw::launch_skydiag_bc(d0, d1 - d0, dof_sub, meta, bcflag, rowbeg, idx1, facc, st);
+    if (e != hipSuccess) return e;
+    fw::FactorArgs fa{};
+    fa.meta = meta;
+    fa.order = nullptr;
+    fa.base = s0;
+    fa.idx1 = idx1;
+    fa.rowbeg = rowbeg;
+    fa.fac = facc;
+    fa.dinv = dinv + (size_t)sys * ndof;
+    fa.wmax = wmax;
+    fa.nmax = nmax;
+    fa.m16 = cm16 - mf_off_h[s0] * 256;
+    fa.mf_off = mf_off_d;
+    e = info.factor_blocked ? fw::launch_factor16(fa, s1 - s0, st) : fw::launch_factor(fa, s1 - s0, st);
+    if (e == hipSuccess)
+      e = fw::launch_blockinv(meta + s0, s1 - s0, idx1, rowbeg, blk_row, facc, cminv - b0 * fw::MINV, st);
+    return e;
+  }
+  // device bytes to recompute subdomain s's factors: its skyline entries,
+  // the block inverses of both row-block partitions and the factor
+  // kernel's 16x16 block inverses
+  long long refactor_bytes(int s) const {
+    const SubMeta &M = meta_h[s];
+    const long long b = (s + 1 < nsub ? (long long)meta_h[s + 1].blk0 : nblk_tot) - M.blk0;
+    return 16 * (M.nlen + b * fw::MINV + 256LL * ((M.n + 15) / 16));
+  }
+  // a solve that needs the factors after they were released: per system and
+  // chunk, the chunk's factors are recomputed, then its subdomains solved by
+  // the static kernel form the load picked (one wave when the pick was the
+  // persistent one), one launch per chunk or (plain) per subdomain.  Not
+  // for the implicit iteration or the Schur formation.
+  hipError_t run_solve_chunked(int mode, bool plain, int only) {
+    if (mode == fw::MODE_ITER || mode == fw::MODE_UNIT) return hipErrorNotSupported;
+    for (int sys = only >= 0 ? only : 0; sys < (only >= 0 ? only + 1 : nsys); sys++)
+      for (size_t k = 0; k + 1 < fchunk.size(); k++) {
+        const int s0 = fchunk[k], s1 = fchunk[k + 1];
+        hipError_t e = refactor_chunk(sys, s0, s1);
+        if (e != hipSuccess) return e;
+        fw::SolveArgs a = solve_args(mode, false, false, sys);
+        a.pgrid = 0;
+        a.order = nullptr;
+        a.waves = sp.pgrid > 0 ? 1 : waves_iter;
+        a.ss.nlen = 0;  // one system per launch: the chunk buffers are that system's
+        a.ss.minv = 0;
+        a.fac = cfac - meta_h[s0].fac0;
+        a.minv = cminv - (long long)meta_h[s0].blk0 * fw::MINV;
+        if (!plain) {
+          a.base = s0;
+          a.nmax = nmax;
+          a.ngmax = ngmax;
+          a.nblkmax = a.pfk == 8 ? nblkmax_s : nblkmax;
+          e = fw::launch_solve(a, s1 - s0, st);
+        } else {
+          for (int s = s0; s < s1 && e == hipSuccess; s++) {
+            a.base = s;
+            a.nmax = meta_h[s].n;
+            a.ngmax = meta_h[s].ng;
+            a.nblkmax = a.pfk == 8 ? meta_h[s].nblk_s : meta_h[s].nblk;
+            e = fw::launch_solve(a, 1, st);
+          }
+        }
+        if (e != hipSuccess) return e;
+      }
+    return hipSuccess;
+  }
   hipError_t run_solve(int mode, bool plain, bool check_done, bool fused = false, int only = -1) {
+    if (!fac_resident) return run_solve_chunked(mode, plain, only);
     fw::SolveArgs a = solve_args(mode, check_done, fused, only);
     if (!plain) return launch_solve_all(a, use_buckets);
     a.order = nullptr;
@@ -1466,7 +1565,8 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   // keeps the inverted diagonal blocks of the finished block rows
   const char *fenv = getenv("FWH_FACTOR");
   const bool blocked = !(fenv && strcmp(fenv, "row") == 0);
-  std::vector<long long> mf_off(nsub);
+  std::vector<long long> &mf_off = c->mf_off_h;
+  mf_off.assign(nsub, 0);
   long long mf_tot = 0;
   for (int s = 0; s < nsub; s++) {
     mf_off[s] = mf_tot;
@@ -1493,7 +1593,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     double2 *fac = c->fac + k * nlen_tot;
     HIPCHK(fw::launch_skyfill((int)nnz, c->ent_row, c->dof_sub, c->meta, c->col, c->aval + k * nnz, c->bcflag,
                               c->idx1, c->rowbeg, fac, st));
-    HIPCHK(fw::launch_skydiag_bc((int)ndof, c->dof_sub, c->meta, c->bcflag, c->rowbeg, c->idx1, fac, st));
+    HIPCHK(fw::launch_skydiag_bc(0, (int)ndof, c->dof_sub, c->meta, c->bcflag, c->rowbeg, c->idx1, fac, st));
     fw::FactorArgs fa{};
     fa.meta = c->meta;
     fa.order = c->order;
@@ -1539,6 +1639,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   c->ss.smat = 0;
   c->ss.hist = 0;
   c->ss.minv = (long long)blk_row.size() * fw::MINV;
+  c->nblk_tot = (long long)blk_row.size();
   c->sys_stats.assign(nsys, fwh_stats{});
   HIPCHK(hipMemsetAsync(c->X, 0, (size_t)nsys * ndof * sizeof(double2), st));
   HIPCHK(hipMemsetAsync(c->sc, 0, nsys * sizeof(fw::Scal), st));
@@ -1668,6 +1769,13 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   I.solve_depth = c->pfk_iter;
   I.unit_depth = c->pfk_unit;
   I.solve_waves = c->waves_iter;
+  I.factors_resident = 1;
+  I.factor_need_max = I.factor_need_total = 0.0;
+  for (int s = 0; s < nsub; s++) {
+    const double b = (double)c->refactor_bytes(s);
+    I.factor_need_max = std::max(I.factor_need_max, b);
+    I.factor_need_total += b;
+  }
   c->loaded = true;
   return 0;
 }
@@ -2368,10 +2476,68 @@ extern "C" int fwh_schur_form(fwh_ctx *c) {
   return build_schur(c);
 }
 
+// Release the skyline factors and block inverses once the S_s are formed.
+// The explicit iteration needs neither; the solves that do (first and last
+// steps, fwh_ldl_solve_all) then recompute them chunk by chunk: consecutive
+// subdomains whose factors, block inverses and the factor kernel's
+// temporaries fit budget_bytes, into buffers of that size.
+extern "C" int fwh_release_factors(fwh_ctx *c, long long budget_bytes) {
+  if (!c) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  if (!c->schur_built) return fail(FWH_E_STATE, "form the Schur complements before releasing the factors");
+  if (!c->fac_resident) return fail(FWH_E_STATE, "the factors are released already");
+  if (budget_bytes <= 0) return fail(FWH_E_ARG, "budget must be positive");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->st));
+  // per subdomain: factor entries, block inverses (both partitions), the
+  // factor kernel's 16x16 block inverses
+  auto blk_end = [&](int s) { return s + 1 < c->nsub ? (long long)c->meta_h[s + 1].blk0 : c->nblk_tot; };
+  std::vector<int> ch{0};
+  long long fmax = 0, bmax = 0, mmax = 0, fsum = 0, bsum = 0, msum = 0;
+  for (int s = 0; s < c->nsub; s++) {
+    const SubMeta &M = c->meta_h[s];
+    const long long f = M.nlen, b = blk_end(s) - M.blk0, m = (M.n + 15) / 16;
+    const long long one = c->refactor_bytes(s);
+    if (one > budget_bytes)
+      return fail(FWH_E_LIMIT, "subdomain %d needs %lld bytes of factors, above the budget %lld", s, one,
+                  budget_bytes);
+    if (16 * (fsum + f + (bsum + b) * fw::MINV + 256 * (msum + m)) > budget_bytes) {
+      ch.push_back(s);
+      fsum = bsum = msum = 0;
+    }
+    fsum += f;
+    bsum += b;
+    msum += m;
+    fmax = std::max(fmax, fsum);
+    bmax = std::max(bmax, bsum);
+    mmax = std::max(mmax, msum);
+  }
+  ch.push_back(c->nsub);
+  int rc;
+  if ((rc = c->alloc(&c->cfac, (size_t)fmax)) || (rc = c->alloc(&c->cminv, (size_t)bmax * fw::MINV)) ||
+      (rc = c->alloc(&c->cm16, (size_t)mmax * 256)) || (rc = c->upload(&c->mf_off_d, c->mf_off_h)))
+    return rc;
+  c->release_graph();  // graphs of the implicit iteration hold the factor pointers
+  const double freed = 16.0 * c->nsys * ((double)c->ss.nlen + (double)c->ss.minv);
+  c->free_one(c->fac);
+  c->free_one(c->minv);
+  c->fac = nullptr;
+  c->minv = nullptr;
+  c->fchunk = ch;
+  c->fac_resident = false;
+  c->info.factors_resident = 0;
+  c->info.factor_chunks = (int)ch.size() - 1;
+  c->info.factor_bytes_released = freed;
+  c->info.factor_chunk_bytes = 16.0 * ((double)fmax + (double)bmax * fw::MINV + 256.0 * (double)mmax);
+  return 0;
+}
+
 extern "C" int fwh_first(fwh_ctx *c, const fwh_opts *o) {
   if (!c || !o) return fail(FWH_E_ARG, "null argument");
   if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
   if (o->hddm_max_loop < 0) return fail(FWH_E_ARG, "hddm_max_loop < 0");
+  if (!c->fac_resident && !o->schur)
+    return fail(FWH_E_STATE, "the implicit iteration needs the factors, which were released");
   HIPCHK(hipSetDevice(c->device));
   int rc = ensure_hist(c, o->hddm_max_loop);
   if (rc) return rc;
@@ -2409,6 +2575,8 @@ extern "C" int fwh_iterate(fwh_ctx *c, const fwh_opts *o, int n, int *done) {
   int it = 0;
   const bool expl = o->schur != 0;
   if (expl && !c->schur_built) return fail(FWH_E_STATE, "explicit Schur mode requested after fwh_first without it");
+  if (!expl && !c->fac_resident)
+    return fail(FWH_E_STATE, "the implicit iteration needs the factors, which were released");
   // every launch serves all systems; a system whose status is LAST skips
   // its work inside the kernels, and the loop ends when all are LAST
   if (o->plain) {
@@ -2646,6 +2814,8 @@ extern "C" int fwh_schur_apply(fwh_ctx *c, const double *p, double *q, int mode)
   if (!c || !p || !q) return fail(FWH_E_ARG, "null argument");
   if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
   if (mode < 0 || mode > 3) return fail(FWH_E_ARG, "bad mode %d", mode);
+  if (mode < 2 && !c->fac_resident)
+    return fail(FWH_E_STATE, "the implicit Schur step needs the factors, which were released");
   HIPCHK(hipSetDevice(c->device));
   const bool expl = mode >= 2;
   int rc;
@@ -2676,6 +2846,7 @@ extern "C" int fwh_ldl_solve_all(fwh_ctx *c, double *rhs, int plain) {
 extern "C" int fwh_time_kernels(fwh_ctx *c, int reps, double *ms_solve, double *ms_schur, double *ms_smv) {
   if (!c || reps <= 0) return fail(FWH_E_ARG, "bad argument");
   if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  if (!c->fac_resident) return fail(FWH_E_STATE, "the kernel timings need the factors, which were released");
   HIPCHK(hipSetDevice(c->device));
   EventPair ev;
   HIPCHK(ev.create());
@@ -2748,6 +2919,7 @@ extern "C" int fwh_get_sub(const fwh_ctx *c, int s, int what, void *out) {
       break;
     }
     case 6:
+      if (!c->fac_resident) return fail(FWH_E_STATE, "the factors were released");
       HIPCHK(hipMemcpy(out, c->fac + (size_t)c->sel * c->nlen + M.fac0, (size_t)M.nlen * sizeof(double2),
                        hipMemcpyDeviceToHost));
       break;

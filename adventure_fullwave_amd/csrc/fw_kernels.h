@@ -354,7 +354,8 @@ hipError_t launch_profile(int ndof, const int *dof_sub, const SubMeta *meta, con
 hipError_t launch_skyfill(int nnz_tot, const int *ent_row, const int *dof_sub, const SubMeta *meta,
                           const int *col, const double2 *val, const int8_t *bcflag,
                           const int *idx1, const int *rowbeg, double2 *fac, hipStream_t st);
-hipError_t launch_skydiag_bc(int ndof, const int *dof_sub, const SubMeta *meta,
+// identity diagonals of the Dirichlet rows among DOFs [d0, d0 + ndof)
+hipError_t launch_skydiag_bc(int d0, int ndof, const int *dof_sub, const SubMeta *meta,
                              const int8_t *bcflag, const int *rowbeg, const int *idx1, double2 *fac,
                              hipStream_t st);
 hipError_t launch_factor(const FactorArgs &a, int nwork, hipStream_t st);

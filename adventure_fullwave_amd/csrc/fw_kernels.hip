@@ -447,19 +447,22 @@ hipError_t launch_skyfill(int nnz_tot, const int *ent_row, const int *dof_sub, c
   return hipGetLastError();
 }
 
-__global__ void k_skydiag_bc(int ndof, const int *__restrict__ dof_sub, const SubMeta *__restrict__ meta,
+// DOFs [d0, d0 + ndof)
+__global__ void k_skydiag_bc(int d0, int ndof, const int *__restrict__ dof_sub, const SubMeta *__restrict__ meta,
                              const int8_t *__restrict__ bcflag, const int *__restrict__ rowbeg,
                              const int *__restrict__ idx1, double2 *__restrict__ fac) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= ndof || bcflag[d] == 0) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ndof) return;
+  const int d = d0 + t;
+  if (bcflag[d] == 0) return;
   const SubMeta m = meta[dof_sub[d]];
   const int i = d - m.dof0;
   fac[m.fac0 + rowbeg[d] + (i - idx1[d])] = make_double2(1.0, 0.0);
 }
-hipError_t launch_skydiag_bc(int ndof, const int *dof_sub, const SubMeta *meta, const int8_t *bcflag,
+hipError_t launch_skydiag_bc(int d0, int ndof, const int *dof_sub, const SubMeta *meta, const int8_t *bcflag,
                              const int *rowbeg, const int *idx1, double2 *fac, hipStream_t st) {
   if (ndof <= 0) return hipSuccess;
-  k_skydiag_bc<<<(ndof + 255) / 256, 256, 0, st>>>(ndof, dof_sub, meta, bcflag, rowbeg, idx1, fac);
+  k_skydiag_bc<<<(ndof + 255) / 256, 256, 0, st>>>(d0, ndof, dof_sub, meta, bcflag, rowbeg, idx1, fac);
   return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ EXPORTS = ("fwh_create", "fwh_destroy", "fwh_last_error", "fwh_load", "fwh_get_i
            "fwh_get_resid", "fwh_schur_apply", "fwh_ldl_solve_all", "fwh_time_kernels", "fwh_get_sub",
            "fwh_element_batch", "fwh_cocg_icc_all", "fwh_schur_form", "fwh_post", "fwh_set_load",
            "fwh_grad_eps", "fwh_load_batch", "fwh_select", "fwh_element_batch_static", "fwh_sky_factor",
-           "fwh_sky_solve")
+           "fwh_sky_solve", "fwh_release_factors")
 # the reference-shaped call surface (include/fwh_ref.h)
 REF_EXPORTS = ("hddm_solver_CS_hip", "hddm_solver_CS_hip_status", "fwh_ref_desc_from_partmesh",
                "fwh_ref_desc_free", "advlas_skyline_ldl_decomposite", "advlas_skyline_ldl_solve")
@@ -73,7 +73,10 @@ class FwhInfo(ctypes.Structure):
                 ("smv_persist", ctypes.c_int), ("smv_pgrid", ctypes.c_int),
                 ("ms_smv_persist", ctypes.c_double * 2),
                 ("solve_persist", ctypes.c_int), ("solve_pgrid", ctypes.c_int),
-                ("solve_nlarge", ctypes.c_int), ("ms_solve_persist", ctypes.c_double * 2)]
+                ("solve_nlarge", ctypes.c_int), ("ms_solve_persist", ctypes.c_double * 2),
+                ("factors_resident", ctypes.c_int), ("factor_chunks", ctypes.c_int),
+                ("factor_bytes_released", ctypes.c_double), ("factor_chunk_bytes", ctypes.c_double),
+                ("factor_need_max", ctypes.c_double), ("factor_need_total", ctypes.c_double)]
 
     def as_dict(self):
         out = {}
@@ -117,6 +120,7 @@ def lib():
         L.fwh_ldl_solve_all.argtypes = [P, P, I]
         L.fwh_time_kernels.argtypes = [P, I, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)]
         L.fwh_schur_form.argtypes = [P]
+        L.fwh_release_factors.argtypes = [P, ctypes.c_longlong]
         L.fwh_post.argtypes = [P, P, P, P, P]
         L.fwh_set_load.argtypes = [P, P]
         L.fwh_grad_eps.argtypes = [P, P, P, P]
@@ -241,6 +245,12 @@ class Solver:
         _check(lib().fwh_schur_form(self.ctx.h), "fwh_schur_form")
         self._refresh_info()
 
+    def release_factors(self, budget_bytes):
+        """free the skyline factors after the Schur formation; the solves that
+        need them recompute them in chunks of at most budget_bytes"""
+        _check(lib().fwh_release_factors(self.ctx.h, int(budget_bytes)), "fwh_release_factors")
+        self._refresh_info()
+
     def iterate(self, n):
         done = ctypes.c_int()
         _check(lib().fwh_iterate(self.ctx.h, ctypes.byref(self.last_opts), int(n), ctypes.byref(done)),
@@ -345,7 +355,9 @@ class Solver:
                "fwh_time_kernels")
         return a.value, b.value, c.value
 
-    def sub(self, s):
+    def sub(self, s, factor=True):
+        """subdomain s: assembled matrix, load, profile and (factor=True) its
+        skyline factor"""
         L = lib()
         sz = np.zeros(4, np.int32)
         _check(L.fwh_get_sub(self.ctx.h, s, 7, _ptr(sz)), "fwh_get_sub")
@@ -357,10 +369,11 @@ class Solver:
         idx1 = np.zeros(n, np.int32)
         idx2 = np.zeros(n, np.int32)
         fac = np.zeros(max(nlen, 1), np.complex128)
-        for what, arr in ((0, val), (1, rowptr), (2, col), (3, f), (4, idx1), (5, idx2), (6, fac)):
+        parts = [(0, val), (1, rowptr), (2, col), (3, f), (4, idx1), (5, idx2)] + ([(6, fac)] if factor else [])
+        for what, arr in parts:
             _check(L.fwh_get_sub(self.ctx.h, s, what, _ptr(arr)), "fwh_get_sub")
         return dict(n=n, nnz=nnz, nlen=nlen, ng=ng, val=val[:nnz], rowptr=rowptr, col=col[:nnz], f=f,
-                    idx1=idx1, idx2=idx2, factor=fac[:nlen])
+                    idx1=idx1, idx2=idx2, factor=fac[:nlen] if factor else None)
 
     def close(self):
         self.ctx.close()

@@ -164,6 +164,9 @@ def main():
     ap.add_argument("--extra-cfg", default="3",
                     help="comma-separated configurations whose kernels (explicit matvec, implicit Schur step, "
                          "LDL^T solve) are also timed and reported under extras ('' for none)")
+    ap.add_argument("--factor-budget-mb", type=float, default=256.0,
+                    help="after the measurements, release the skyline factors (explicit form) and solve again with "
+                         "the first/last steps recomputing them in chunks of this many MB (0: skip)")
     ap.add_argument("--sweep", type=int, default=0,
                     help="also solve a sweep of this many frequencies f0*(0.9 + 0.2k/(n-1)) as one batch "
                          "(fwh_load_batch) and as separate solves")
@@ -309,6 +312,30 @@ def main():
     sweep = None
     if args.sweep > 1:
         sweep = frequency_sweep(args, model, fwhip, expl)
+    # factors released after the formation: the explicit iteration runs on
+    # the S_s alone, the first and last steps recompute the factors chunk by
+    # chunk within the budget; same bits as the resident solve
+    released = None
+    if expl and args.factor_budget_mb > 0:
+        import numpy as np
+        kw = dict(eps=args.eps, max_loop=args.max_loop, graph_iters=args.graph_iters, schur=True)
+        st0 = sol.solve(**kw)
+        x0 = sol.x()
+        budget = int(max(args.factor_budget_mb * 1e6, sol.info["factor_need_max"]))
+        sol.release_factors(budget)
+        t_r = time.perf_counter()
+        st1 = sol.solve(**kw)
+        dt_r = time.perf_counter() - t_r
+        inf_r = sol.info
+        released = {"budget_bytes": budget, "bytes_released": inf_r["factor_bytes_released"],
+                    "chunks": inf_r["factor_chunks"], "chunk_buffer_bytes": inf_r["factor_chunk_bytes"],
+                    "solve_ms": round(1000 * dt_r, 2), "iters": st1["iters"],
+                    "first_ms": [round(st0["ms_first"], 2), round(st1["ms_first"], 2)],
+                    "last_ms": [round(st0["ms_last"], 2), round(st1["ms_last"], 2)],
+                    "same_bits_as_resident": bool(st0["iters"] == st1["iters"] and np.array_equal(x0, sol.x())),
+                    "note": "first_ms and last_ms: resident factors, then recomputed in chunks"}
+        log(f"[bench] factors released: {released['chunks']} chunks, first step {released['first_ms']} ms, "
+            f"last step {released['last_ms']} ms, same bits {released['same_bits_as_resident']}")
     setup = {"assemble": round(info["ms_assemble"], 2), "factor": round(info["ms_factor"], 2)}
 
     def fp64(flops, ms, kernel):
@@ -376,6 +403,7 @@ def main():
                      "ms_solve_persistent_forms_1_2": [round(v, 4) for v in info["ms_solve_persist"]]},
         "adjoint": adjoint,
         "sweep": sweep,
+        "factors_released": released,
         # from the model in host memory: the whole load (host index work,
         # uploads, assembly, factorisation, load-time tuning), the Schur
         # formation and one solve

@@ -157,6 +157,16 @@ typedef struct fwh_info {
   int    solve_nlarge;  /* subdomains of the large queue (form 2)          */
   double ms_solve_persist[2]; /* best LDL^T solve time of forms 1 and 2
                            (-1: does not fit)                              */
+  int    factors_resident; /* 1: the skyline factors are in device memory;
+                           0: released (fwh_release_factors), recomputed
+                           chunk by chunk by the solves that need them     */
+  int    factor_chunks; /* chunks of that recomputation                    */
+  double factor_bytes_released; /* device bytes the release freed          */
+  double factor_chunk_bytes;    /* bytes of the recomputation buffers      */
+  double factor_need_max;  /* recomputation bytes of the largest subdomain
+                           (factor entries, block inverses, the factor
+                           kernel's temporaries): the smallest budget      */
+  double factor_need_total; /* the same summed over all subdomains         */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */
@@ -255,6 +265,17 @@ int  fwh_schur_apply(fwh_ctx *ctx, const double *p, double *q, int mode);
  * (lib/hddm/hddm_solver.h:559-622) applied to every interface unit vector.
  * Idempotent; fwh_first forms them when opts.schur = 1.                   */
 int  fwh_schur_form(fwh_ctx *ctx);
+/* Release the skyline factors and block inverses after the Schur
+ * complements are formed (the device memory of the factors, 16 bytes per
+ * skyline entry and system).  No reference counterpart: the reference keeps
+ * dm->dmat for the whole solve (hddm_solver.h:512-523).  The explicit
+ * iteration needs no factors; the solves that do (fwh_first, fwh_last,
+ * fwh_ldl_solve_all, the adjoint's solves through fwh_set_load) recompute
+ * them chunk by chunk, consecutive subdomains whose factors fit
+ * budget_bytes, with the load's kernels (same factors and pivots).  After
+ * it the implicit Schur step is refused (FWH_E_STATE); a new load restores
+ * resident factors.                                                       */
+int  fwh_release_factors(fwh_ctx *ctx, long long budget_bytes);
 /* advlas_skyline_ldl_solve (lib/hddm/advlas_skyline.c:134-167) applied to
  * every subdomain at once: rhs/x are 2*n_dof doubles (in place).          */
 int  fwh_ldl_solve_all(fwh_ctx *ctx, double *rhs, int plain);
