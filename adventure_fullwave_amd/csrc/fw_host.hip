@@ -270,6 +270,7 @@ struct fwh_ctx {
   // them chunk by chunk of subdomains into the c* buffers, at most a budget
   // of bytes at a time; the explicit iteration needs none
   bool fac_resident = true;
+  long long factor_budget = 0;        // fwh_set_factor_budget: above it, a load streams its factors
   std::vector<int> fchunk;            // chunk boundaries (subdomains), back() == nsub
   double2 *cfac = nullptr, *cminv = nullptr, *cm16 = nullptr;
   long long *mf_off_d = nullptr;      // per subdomain its first 16-row block of cm16's layout
@@ -691,6 +692,9 @@ extern "C" void fwh_destroy(fwh_ctx *c) {
 // ---------------------------------------------------------------------------
 // model load
 // ---------------------------------------------------------------------------
+// chunks and buffers of a factor recomputation (defined with fwh_release_factors)
+int plan_refactor(fwh_ctx *c, long long budget_bytes);
+
 namespace {
 
 struct Key {
@@ -1518,9 +1522,17 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   c->pfk_unit = fw::PFK;  // both picked after the factorisation, by timing
   if ((size_t)(wmax + 1 + nmax) * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "profile too wide for the factor kernel");
   HIPCHK(hipMemcpyAsync(c->meta, meta.data(), nsub * sizeof(SubMeta), hipMemcpyHostToDevice, st));
+  // factors above the budget (fwh_set_factor_budget): the load streams them
+  // (computed chunk by chunk when the S_s are formed and again by the solves
+  // that need them) instead of keeping them resident
+  c->nblk_tot = (long long)blk_row.size();
+  const double fac_bytes = 16.0 * nsys * ((double)nlen_tot + (double)blk_row.size() * fw::MINV);
+  const bool streamed = c->factor_budget > 0 && fac_bytes > (double)c->factor_budget;
   if ((rc = c->upload(&c->rowbeg, rowbeg)) || (rc = c->upload(&c->blk_row, blk_row)) ||
-      (rc = c->alloc(&c->fac, (size_t)nsys * nlen_tot)) || (rc = c->alloc(&c->dinv, (size_t)nsys * ndof)) ||
-      (rc = c->alloc(&c->minv, (size_t)nsys * blk_row.size() * fw::MINV)))
+      (rc = c->alloc(&c->dinv, (size_t)nsys * ndof)))
+    return rc;
+  if (!streamed && ((rc = c->alloc(&c->fac, (size_t)nsys * nlen_tot)) ||
+                    (rc = c->alloc(&c->minv, (size_t)nsys * blk_row.size() * fw::MINV))))
     return rc;
   // work order: largest skyline first
   std::vector<int> order(nsub);
@@ -1588,8 +1600,8 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   EventPair fev;
   HIPCHK(fev.create());
   double ms_fk = 0.0;
-  HIPCHK(hipMemsetAsync(c->fac, 0, (size_t)nsys * nlen_tot * sizeof(double2), st));
-  for (int k = 0; k < nsys; k++) {  // one factor set per system
+  if (!streamed) HIPCHK(hipMemsetAsync(c->fac, 0, (size_t)nsys * nlen_tot * sizeof(double2), st));
+  for (int k = 0; k < (streamed ? 0 : nsys); k++) {  // one factor set per system
     double2 *fac = c->fac + k * nlen_tot;
     HIPCHK(fw::launch_skyfill((int)nnz, c->ent_row, c->dof_sub, c->meta, c->col, c->aval + k * nnz, c->bcflag,
                               c->idx1, c->rowbeg, fac, st));
@@ -1672,7 +1684,16 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   // faster (in the order depth 8, then two waves at 16 and 8, then four):
   // it does not flip between runs on timing noise.
   c->use_buckets = false;
-  {
+  if (streamed) {
+    // no resident factors to time the LDL^T kernel on: depth 16, one wave,
+    // static grid; the S_s are formed chunk by chunk (build_schur)
+    c->sp = fwh_ctx::SolvePersist{};
+    I.solve_persist = 0;
+    I.ms_solve_persist[0] = I.ms_solve_persist[1] = -1.0;
+    for (int k = 0; k < 3; k++) I.ms_solve_waves[k] = -1.0;
+    if ((rc = plan_refactor(c, c->factor_budget))) return rc;
+    c->fac_resident = false;
+  } else {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -1769,7 +1790,8 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   I.solve_depth = c->pfk_iter;
   I.unit_depth = c->pfk_unit;
   I.solve_waves = c->waves_iter;
-  I.factors_resident = 1;
+  I.factors_resident = streamed ? 0 : 1;
+  I.factor_bytes_released = 0.0;
   I.factor_need_max = I.factor_need_total = 0.0;
   for (int s = 0; s < nsub; s++) {
     const double b = (double)c->refactor_bytes(s);
@@ -2090,6 +2112,157 @@ int tune_smv_persist(fwh_ctx *c, int ncu) {
 }
 }  // namespace
 
+// The GEMM formation for the subdomains `subs` (largest first) of system k
+// with factors at fac (offsets as in the whole model): W = L^-1 (-A_IG) by
+// blocked forward substitution on the matrix cores (k_blockinv16, then
+// k_schur_fwd: one wave per (subdomain, 16-column chunk), the chunks of
+// subdomain s dealt to XCD s mod 8 so that its factor is read from one L2),
+// then S_s = A_GG - W^T D^-1 W (k_schur_gemm, one wave per lower 16x16 tile,
+// dealt the same way) and A_GG added (k_schur_agg).  W and the block
+// inverses are temporary: W covers each subdomain's rows up to its last
+// interior one (nsw).  The two matrix-core kernels' event times are added to
+// ms_fwd, ms_gemm.
+int form_gemm(fwh_ctx *c, const std::vector<int> &subs, int k, const double2 *fac, long long tot,
+              const std::vector<int> &nsw, double &ms_fwd, double &ms_gemm) {
+  constexpr int NXCD = 8;
+  std::vector<long long> woff(c->nsub, 0), moff(c->nsub, 0);
+  std::vector<int2> blist;
+  long long wtot = 0;
+  for (int s : subs) {
+    const SubMeta &M = c->meta_h[s];
+    woff[s] = wtot;
+    wtot += (long long)((M.ng + 15) / 16) * nsw[s] * 16;
+    moff[s] = (long long)blist.size();
+    for (int b = 0; b < (nsw[s] + 15) / 16; b++) blist.push_back(make_int2(s, b));
+  }
+  const long long mtot = (long long)blist.size() * 256;
+  // per XCD lanes of work, subdomains largest first
+  auto deal = [&](auto item_count, auto make) {
+    using T = decltype(make(0, 0));
+    std::vector<std::vector<T>> lane(NXCD);
+    for (int s : subs)
+      for (int q = 0; q < item_count(s); q++) lane[s % NXCD].push_back(make(s, q));
+    size_t len = 0;
+    for (auto &v : lane) len = std::max(len, v.size());
+    std::vector<T> out(len * NXCD, make(-1, 0));
+    for (int x = 0; x < NXCD; x++)
+      for (size_t i = 0; i < lane[x].size(); i++) out[i * NXCD + x] = lane[x][i];
+    return out;
+  };
+  const std::vector<int2> fwork = deal([&](int s) { return nsw[s] > 0 ? (c->meta_h[s].ng + 15) / 16 : 0; },
+                                       [](int s, int q) { return make_int2(s, q); });
+  std::vector<int4> tiles = deal(
+      [&](int s) {
+        const int nb = (c->meta_h[s].ng + 15) / 16;
+        return nb * (nb + 1) / 2;
+      },
+      [&](int s, int q) {
+        if (s < 0) return make_int4(-1, 0, 0, 0);
+        int I = 0;
+        while ((I + 1) * (I + 2) / 2 <= q) I++;
+        return make_int4(s, I, q - I * (I + 1) / 2, 0);
+      });
+  // the aggregation kernel takes one workgroup per subdomain of the list
+  std::vector<SubMeta> agg_meta;
+  for (int s : subs) agg_meta.push_back(c->meta_h[s]);
+  std::vector<long long> agg_off;
+  for (int s : subs) agg_off.push_back(0);
+  double2 *W = nullptr, *M16 = nullptr;
+  long long *d_woff = nullptr, *d_moff = nullptr;
+  int *d_nsw = nullptr;
+  int2 *d_blist = nullptr, *d_fwork = nullptr;
+  int4 *d_tiles = nullptr;
+  int *d_subs = nullptr;
+  hipError_t e = hipMalloc(&W, std::max<long long>(wtot, 1) * sizeof(double2));
+  if (e == hipSuccess) e = hipMalloc(&M16, std::max<long long>(mtot, 1) * sizeof(double2));
+  auto put = [&](auto **dst, const auto &v) {
+    if (e != hipSuccess) return;
+    using T = typename std::decay_t<decltype(v)>::value_type;
+    e = hipMalloc(dst, std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (e == hipSuccess && !v.empty())
+      e = hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->st);
+  };
+  put(&d_woff, woff);
+  put(&d_moff, moff);
+  put(&d_nsw, nsw);
+  put(&d_blist, blist);
+  put(&d_fwork, fwork);
+  put(&d_tiles, tiles);
+  put(&d_subs, subs);
+  // the substitution and the tile products are timed with events
+  EventPair ev0, ev1;
+  if (e == hipSuccess) e = ev0.create();
+  if (e == hipSuccess) e = ev1.create();
+  if (e == hipSuccess) {
+    fw::BlockInv16Args bi{};
+    bi.blist = d_blist;
+    bi.nblk = (int)blist.size();
+    bi.meta = c->meta;
+    bi.nsw = d_nsw;
+    bi.idx1 = c->idx1;
+    bi.rowbeg = c->rowbeg;
+    bi.fac = fac;
+    bi.m16 = M16;
+    e = fw::launch_blockinv16(bi, c->st);
+    fw::FwdArgs fa{};
+    fa.work = d_fwork;
+    fa.nwork = (int)fwork.size();
+    fa.meta = c->meta;
+    fa.nsw = d_nsw;
+    fa.idx1 = c->idx1;
+    fa.rowbeg = c->rowbeg;
+    fa.fac = fac;
+    fa.bcflag = c->bcflag;
+    fa.cp_ptr = c->cp_ptr;
+    fa.cp_k = c->cp_k;
+    fa.cp_val = c->cp_val + (size_t)k * c->ss.cpl;
+    fa.m16 = M16;
+    fa.m16_off = d_moff;
+    fa.wmat = W;
+    fa.w_off = d_woff;
+    if (e == hipSuccess) e = hipEventRecord(ev0.a, c->st);
+    if (e == hipSuccess) e = fw::launch_schur_fwd(fa, c->st);
+    if (e == hipSuccess) e = hipEventRecord(ev0.b, c->st);
+    fw::GemmArgs g{};
+    g.meta = c->meta;
+    g.tiles = d_tiles;
+    g.ntile = (int)tiles.size();
+    g.nsw = d_nsw;
+    g.wmat = W;
+    g.w_off = d_woff;
+    g.dinv = c->dinv + (size_t)k * c->ndof;
+    g.smat = c->smat + (size_t)k * tot;
+    g.sm_off = c->sm_off;
+    if (e == hipSuccess) e = hipEventRecord(ev1.a, c->st);
+    if (e == hipSuccess) e = fw::launch_schur_gemm(g, c->st);
+    if (e == hipSuccess) e = hipEventRecord(ev1.b, c->st);
+    fw::AggArgs q{};
+    q.meta = c->meta;
+    q.subs = d_subs;
+    q.gr_ptr = c->gr_ptr;
+    q.gr_col = c->gr_col;
+    q.gr_val = c->gr_val + (size_t)k * c->ngrow;
+    q.bcflag = c->bcflag;
+    q.kidx = c->kidx;
+    q.smat = c->smat + (size_t)k * tot;
+    q.sm_off = c->sm_off;
+    if (e == hipSuccess) e = fw::launch_schur_agg(q, (int)subs.size(), c->st);
+    float t_fwd = 0.f, t_gemm = 0.f;
+    if (e == hipSuccess) e = hipEventSynchronize(ev1.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t_fwd, ev0.a, ev0.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t_gemm, ev1.a, ev1.b);
+    ms_fwd += t_fwd;
+    ms_gemm += t_gemm;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  (void)hipStreamSynchronize(c->st);
+  for (void *ptr : {(void *)W, (void *)M16, (void *)d_woff, (void *)d_moff, (void *)d_nsw, (void *)d_blist,
+                    (void *)d_fwork, (void *)d_tiles, (void *)d_subs})
+    if (ptr) (void)hipFree(ptr);
+  if (e != hipSuccess) return fail(FWH_E_HIP, "Schur formation (GEMM): %s", hipGetErrorString(e));
+  return 0;
+}
+
 // Form every S_s = A_GG - A_GI A_II^-1 A_IG once: column k of all subdomains
 // in one batched launch of the Schur step on the unit vector e_k (the same
 // kernel, factors and summation order as the implicit iteration).
@@ -2117,29 +2290,21 @@ int build_schur(fwh_ctx *c) {
   HIPCHK(hipMemsetAsync(c->smv_queue, 0, sizeof(int) * 2 * c->nsys, c->st));
   HIPCHK(fw::launch_fill(c->smat, (long long)c->nsys * tot, make_double2(0.0, 0.0), c->st));
   constexpr int NXCD = 8;
-  // GEMM formation: W = L^-1 (-A_IG) by blocked forward substitution on the
-  // matrix cores (k_blockinv16, then k_schur_fwd: one wave per (subdomain,
-  // 16-column chunk), the chunks of subdomain s dealt to XCD s mod 8 so that
-  // its factor is read from one L2), then S_s = A_GG - W^T D^-1 W
-  // (k_schur_gemm, one wave per lower 16x16 tile, dealt the same way) and
-  // A_GG added (k_schur_agg).  W and the block inverses are temporary: W
-  // covers each subdomain's rows up to its last interior one (nsw).  The
-  // unit-vector Schur step (MODE_UNIT, forward and backward sweeps per
-  // column) remains for FWH_SCHUR_FORM=solve or when they would take more
-  // than half the free device memory.
+  // GEMM formation (form_gemm) on the matrix cores, its temporaries W and
+  // the 16x16 block inverses permitting: the unit-vector Schur step
+  // (MODE_UNIT, forward and backward sweeps per column) remains for
+  // FWH_SCHUR_FORM=solve or when they would take more than half the free
+  // device memory.  A streamed load (factors never all resident) forms the
+  // S_s chunk by chunk, right after each chunk's factors are computed.
   const std::vector<int8_t> &bcf = c->bcflag_h;
   std::vector<int> nsw(c->nsub, 0);
-  std::vector<long long> woff(c->nsub), moff(c->nsub);
-  std::vector<int2> blist;
-  long long wtot = 0;
+  long long wtot = 0, nblk16 = 0;
   for (int s = 0; s < c->nsub; s++) {
     const SubMeta &M = c->meta_h[s];
     for (int i = 0; i < M.n; i++)
       if (bcf[M.dof0 + i] == 0) nsw[s] = i + 1;
-    woff[s] = wtot;
     wtot += (long long)((M.ng + 15) / 16) * nsw[s] * 16;
-    moff[s] = (long long)blist.size();
-    for (int b = 0; b < (nsw[s] + 15) / 16; b++) blist.push_back(make_int2(s, b));
+    nblk16 += (nsw[s] + 15) / 16;
   }
   // algorithmic flops of the formation: the substitution touches every
   // off-diagonal profile entry once per interface column, the tile products
@@ -2156,132 +2321,34 @@ int build_schur(fwh_ctx *c) {
     c->info.flops_schur_fwd = c->nsys * ff;
     c->info.flops_schur_gemm = c->nsys * fg;
   }
-  const long long mtot = (long long)blist.size() * 256;
+  const long long mtot = nblk16 * 256;
   size_t mem_free = 0, mem_tot = 0;
   HIPCHK(hipMemGetInfo(&mem_free, &mem_tot));
   const char *fe = getenv("FWH_SCHUR_FORM");
-  bool gemm = !(fe && strcmp(fe, "solve") == 0) && 16.0 * (double)(wtot + mtot) < 0.5 * (double)mem_free;
+  bool gemm = !c->fac_resident ||
+              (!(fe && strcmp(fe, "solve") == 0) && 16.0 * (double)(wtot + mtot) < 0.5 * (double)mem_free);
   if (gemm) {
-    // per XCD lanes of work, subdomains largest first
-    auto deal = [&](auto item_count, auto make) {
-      using T = decltype(make(0, 0));
-      std::vector<std::vector<T>> lane(NXCD);
-      for (int s : c->order_h)
-        for (int k = 0; k < item_count(s); k++) lane[s % NXCD].push_back(make(s, k));
-      size_t len = 0;
-      for (auto &v : lane) len = std::max(len, v.size());
-      std::vector<T> out(len * NXCD, make(-1, 0));
-      for (int x = 0; x < NXCD; x++)
-        for (size_t i = 0; i < lane[x].size(); i++) out[i * NXCD + x] = lane[x][i];
-      return out;
-    };
-    const std::vector<int2> fwork = deal([&](int s) { return nsw[s] > 0 ? (c->meta_h[s].ng + 15) / 16 : 0; },
-                                         [](int s, int k) { return make_int2(s, k); });
-    std::vector<int4> tiles = deal(
-        [&](int s) {
-          const int nb = (c->meta_h[s].ng + 15) / 16;
-          return nb * (nb + 1) / 2;
-        },
-        [&](int s, int k) {
-          if (s < 0) return make_int4(-1, 0, 0, 0);
-          int I = 0;
-          while ((I + 1) * (I + 2) / 2 <= k) I++;
-          return make_int4(s, I, k - I * (I + 1) / 2, 0);
-        });
-    double2 *W = nullptr, *M16 = nullptr;
-    long long *d_woff = nullptr, *d_moff = nullptr;
-    int *d_nsw = nullptr;
-    int2 *d_blist = nullptr, *d_fwork = nullptr;
-    int4 *d_tiles = nullptr;
-    hipError_t e = hipMalloc(&W, std::max<long long>(wtot, 1) * sizeof(double2));
-    if (e == hipSuccess) e = hipMalloc(&M16, std::max<long long>(mtot, 1) * sizeof(double2));
-    auto put = [&](auto **dst, const auto &v) {
-      if (e != hipSuccess) return;
-      using T = typename std::decay_t<decltype(v)>::value_type;
-      e = hipMalloc(dst, std::max<size_t>(v.size(), 1) * sizeof(T));
-      if (e == hipSuccess && !v.empty())
-        e = hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->st);
-    };
-    put(&d_woff, woff);
-    put(&d_moff, moff);
-    put(&d_nsw, nsw);
-    put(&d_blist, blist);
-    put(&d_fwork, fwork);
-    put(&d_tiles, tiles);
-    // the substitution and the tile products are timed with events
-    EventPair ev0, ev1;
-    if (e == hipSuccess) e = ev0.create();
-    if (e == hipSuccess) e = ev1.create();
     double ms_fwd = 0.0, ms_gemm = 0.0;
-    for (int k = 0; k < c->nsys && e == hipSuccess; k++) {
-      const double2 *fac = c->fac + (size_t)k * c->ss.nlen;
-      fw::BlockInv16Args bi{};
-      bi.blist = d_blist;
-      bi.nblk = (int)blist.size();
-      bi.meta = c->meta;
-      bi.nsw = d_nsw;
-      bi.idx1 = c->idx1;
-      bi.rowbeg = c->rowbeg;
-      bi.fac = fac;
-      bi.m16 = M16;
-      e = fw::launch_blockinv16(bi, c->st);
-      fw::FwdArgs fa{};
-      fa.work = d_fwork;
-      fa.nwork = (int)fwork.size();
-      fa.meta = c->meta;
-      fa.nsw = d_nsw;
-      fa.idx1 = c->idx1;
-      fa.rowbeg = c->rowbeg;
-      fa.fac = fac;
-      fa.bcflag = c->bcflag;
-      fa.cp_ptr = c->cp_ptr;
-      fa.cp_k = c->cp_k;
-      fa.cp_val = c->cp_val + (size_t)k * c->ss.cpl;
-      fa.m16 = M16;
-      fa.m16_off = d_moff;
-      fa.wmat = W;
-      fa.w_off = d_woff;
-      if (e == hipSuccess) e = hipEventRecord(ev0.a, c->st);
-      if (e == hipSuccess) e = fw::launch_schur_fwd(fa, c->st);
-      if (e == hipSuccess) e = hipEventRecord(ev0.b, c->st);
-      fw::GemmArgs g{};
-      g.meta = c->meta;
-      g.tiles = d_tiles;
-      g.ntile = (int)tiles.size();
-      g.nsw = d_nsw;
-      g.wmat = W;
-      g.w_off = d_woff;
-      g.dinv = c->dinv + (size_t)k * c->ndof;
-      g.smat = c->smat + (size_t)k * tot;
-      g.sm_off = c->sm_off;
-      if (e == hipSuccess) e = hipEventRecord(ev1.a, c->st);
-      if (e == hipSuccess) e = fw::launch_schur_gemm(g, c->st);
-      if (e == hipSuccess) e = hipEventRecord(ev1.b, c->st);
-      fw::AggArgs q{};
-      q.meta = c->meta;
-      q.gr_ptr = c->gr_ptr;
-      q.gr_col = c->gr_col;
-      q.gr_val = c->gr_val + (size_t)k * c->ngrow;
-      q.bcflag = c->bcflag;
-      q.kidx = c->kidx;
-      q.smat = c->smat + (size_t)k * tot;
-      q.sm_off = c->sm_off;
-      if (e == hipSuccess) e = fw::launch_schur_agg(q, c->nsub, c->st);
-      float t_fwd = 0.f, t_gemm = 0.f;
-      if (e == hipSuccess) e = hipEventSynchronize(ev1.b);
-      if (e == hipSuccess) e = hipEventElapsedTime(&t_fwd, ev0.a, ev0.b);
-      if (e == hipSuccess) e = hipEventElapsedTime(&t_gemm, ev1.a, ev1.b);
-      ms_fwd += t_fwd;
-      ms_gemm += t_gemm;
+    if (c->fac_resident) {
+      std::vector<int> all(c->order_h);
+      for (int k = 0; k < c->nsys && rc == 0; k++)
+        rc = form_gemm(c, all, k, c->fac + (size_t)k * c->ss.nlen, tot, nsw, ms_fwd, ms_gemm);
+    } else {
+      // streamed load: the factors of each chunk are computed, its S_s
+      // formed, and the chunk buffers reused by the next
+      for (int k = 0; k < c->nsys && rc == 0; k++)
+        for (size_t ch = 0; ch + 1 < c->fchunk.size() && rc == 0; ch++) {
+          const int s0 = c->fchunk[ch], s1 = c->fchunk[ch + 1];
+          HIPCHK(c->refactor_chunk(k, s0, s1));
+          std::vector<int> sub;
+          for (int s : c->order_h)
+            if (s >= s0 && s < s1) sub.push_back(s);
+          rc = form_gemm(c, sub, k, c->cfac - c->meta_h[s0].fac0, tot, nsw, ms_fwd, ms_gemm);
+        }
     }
+    if (rc) return rc;
     c->info.ms_schur_fwd = ms_fwd;
     c->info.ms_schur_gemm = ms_gemm;
-    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-    (void)hipStreamSynchronize(c->st);
-    for (void *ptr : {(void *)W, (void *)M16, (void *)d_woff, (void *)d_moff, (void *)d_nsw, (void *)d_blist,
-                      (void *)d_fwork, (void *)d_tiles})
-      if (ptr) (void)hipFree(ptr);
-    if (e != hipSuccess) return fail(FWH_E_HIP, "Schur formation (GEMM): %s", hipGetErrorString(e));
   } else {
     // one launch over all (subdomain, unit vector) pairs.  Workgroups are
     // dealt to the XCDs round-robin (workgroup b on XCD b mod 8), so the pairs
@@ -2481,16 +2548,11 @@ extern "C" int fwh_schur_form(fwh_ctx *c) {
 // steps, fwh_ldl_solve_all) then recompute them chunk by chunk: consecutive
 // subdomains whose factors, block inverses and the factor kernel's
 // temporaries fit budget_bytes, into buffers of that size.
-extern "C" int fwh_release_factors(fwh_ctx *c, long long budget_bytes) {
-  if (!c) return fail(FWH_E_ARG, "null argument");
-  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
-  if (!c->schur_built) return fail(FWH_E_STATE, "form the Schur complements before releasing the factors");
-  if (!c->fac_resident) return fail(FWH_E_STATE, "the factors are released already");
-  if (budget_bytes <= 0) return fail(FWH_E_ARG, "budget must be positive");
-  HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipStreamSynchronize(c->st));
-  // per subdomain: factor entries, block inverses (both partitions), the
-  // factor kernel's 16x16 block inverses
+// The chunks of a factor recomputation within budget_bytes (consecutive
+// subdomains, refactor_bytes each) and their buffers (sized for the largest
+// chunk); the per-subdomain offsets of the factor kernel's temporaries go to
+// the device.
+int plan_refactor(fwh_ctx *c, long long budget_bytes) {
   auto blk_end = [&](int s) { return s + 1 < c->nsub ? (long long)c->meta_h[s + 1].blk0 : c->nblk_tot; };
   std::vector<int> ch{0};
   long long fmax = 0, bmax = 0, mmax = 0, fsum = 0, bsum = 0, msum = 0;
@@ -2517,18 +2579,45 @@ extern "C" int fwh_release_factors(fwh_ctx *c, long long budget_bytes) {
   if ((rc = c->alloc(&c->cfac, (size_t)fmax)) || (rc = c->alloc(&c->cminv, (size_t)bmax * fw::MINV)) ||
       (rc = c->alloc(&c->cm16, (size_t)mmax * 256)) || (rc = c->upload(&c->mf_off_d, c->mf_off_h)))
     return rc;
+  c->fchunk = ch;
+  c->info.factor_chunks = (int)ch.size() - 1;
+  c->info.factor_chunk_bytes = 16.0 * ((double)fmax + (double)bmax * fw::MINV + 256.0 * (double)mmax);
+  return 0;
+}
+
+// Release the skyline factors and block inverses once the S_s are formed.
+// The explicit iteration needs neither; the solves that do (first and last
+// steps, fwh_ldl_solve_all) then recompute them chunk by chunk: consecutive
+// subdomains whose factors, block inverses and the factor kernel's
+// temporaries fit budget_bytes, into buffers of that size.
+extern "C" int fwh_release_factors(fwh_ctx *c, long long budget_bytes) {
+  if (!c) return fail(FWH_E_ARG, "null argument");
+  if (!c->loaded) return fail(FWH_E_STATE, "no model loaded");
+  if (!c->schur_built) return fail(FWH_E_STATE, "form the Schur complements before releasing the factors");
+  if (!c->fac_resident) return fail(FWH_E_STATE, "the factors are released already");
+  if (budget_bytes <= 0) return fail(FWH_E_ARG, "budget must be positive");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->st));
+  int rc;
+  if ((rc = plan_refactor(c, budget_bytes))) return rc;
   c->release_graph();  // graphs of the implicit iteration hold the factor pointers
   const double freed = 16.0 * c->nsys * ((double)c->ss.nlen + (double)c->ss.minv);
   c->free_one(c->fac);
   c->free_one(c->minv);
   c->fac = nullptr;
   c->minv = nullptr;
-  c->fchunk = ch;
   c->fac_resident = false;
   c->info.factors_resident = 0;
-  c->info.factor_chunks = (int)ch.size() - 1;
   c->info.factor_bytes_released = freed;
-  c->info.factor_chunk_bytes = 16.0 * ((double)fmax + (double)bmax * fw::MINV + 256.0 * (double)mmax);
+  return 0;
+}
+
+// Device bytes the factors of a model may take: a load whose factors (all
+// systems, with their block inverses) exceed it streams them (fwh_load).
+extern "C" int fwh_set_factor_budget(fwh_ctx *c, long long budget_bytes) {
+  if (!c) return fail(FWH_E_ARG, "null argument");
+  if (budget_bytes < 0) return fail(FWH_E_ARG, "budget must not be negative");
+  c->factor_budget = budget_bytes;
   return 0;
 }
 

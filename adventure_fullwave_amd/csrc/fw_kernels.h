@@ -187,9 +187,10 @@ struct GemmArgs {
   double2 *smat;
   const long long *sm_off;
 };
-// A_GG added to the tiles: one workgroup per subdomain
+// A_GG added to the tiles: one workgroup per subdomain (of subs, or all)
 struct AggArgs {
   const SubMeta *meta;
+  const int *subs;       // the subdomains, or null: every subdomain in order
   const int *gr_ptr, *gr_col;
   const double2 *gr_val;
   const int8_t *bcflag;

@@ -814,10 +814,11 @@ __global__ __launch_bounds__(256) void k_schur_gemm(GemmArgs a) {
 // j <= i (the diagonal halved), added to -W^T D^-1 W; one workgroup per
 // subdomain, one thread per interface row (each tile entry has one writer)
 __global__ __launch_bounds__(256) void k_schur_agg(AggArgs a) {
-  const SubMeta m = a.meta[blockIdx.x];
+  const int w = a.subs ? a.subs[blockIdx.x] : (int)blockIdx.x;
+  const SubMeta m = a.meta[w];
   const int ng = m.ng, nb = (ng + 15) >> 4;
   const bool half = smv_half(ng);
-  double2 *dst = a.smat + a.sm_off[blockIdx.x];
+  double2 *dst = a.smat + a.sm_off[w];
   for (int i = threadIdx.x; i < ng; i += blockDim.x) {
     const int sl = m.g0 + i;
     for (int c = a.gr_ptr[sl]; c < a.gr_ptr[sl + 1]; c++) {

@@ -28,7 +28,7 @@ EXPORTS = ("fwh_create", "fwh_destroy", "fwh_last_error", "fwh_load", "fwh_get_i
            "fwh_get_resid", "fwh_schur_apply", "fwh_ldl_solve_all", "fwh_time_kernels", "fwh_get_sub",
            "fwh_element_batch", "fwh_cocg_icc_all", "fwh_schur_form", "fwh_post", "fwh_set_load",
            "fwh_grad_eps", "fwh_load_batch", "fwh_select", "fwh_element_batch_static", "fwh_sky_factor",
-           "fwh_sky_solve", "fwh_release_factors")
+           "fwh_sky_solve", "fwh_release_factors", "fwh_set_factor_budget")
 # the reference-shaped call surface (include/fwh_ref.h)
 REF_EXPORTS = ("hddm_solver_CS_hip", "hddm_solver_CS_hip_status", "fwh_ref_desc_from_partmesh",
                "fwh_ref_desc_free", "advlas_skyline_ldl_decomposite", "advlas_skyline_ldl_solve")
@@ -121,6 +121,7 @@ def lib():
         L.fwh_time_kernels.argtypes = [P, I, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(D)]
         L.fwh_schur_form.argtypes = [P]
         L.fwh_release_factors.argtypes = [P, ctypes.c_longlong]
+        L.fwh_set_factor_budget.argtypes = [P, ctypes.c_longlong]
         L.fwh_post.argtypes = [P, P, P, P, P]
         L.fwh_set_load.argtypes = [P, P]
         L.fwh_grad_eps.argtypes = [P, P, P, P]
@@ -190,10 +191,13 @@ class Solver:
     methods (stats, x, u, resid, post, set_load, grad_eps, schur_apply,
     ldl_solve_all, sub) address."""
 
-    def __init__(self, model, device: int = 0, ctx: Optional[Context] = None):
+    def __init__(self, model, device: int = 0, ctx: Optional[Context] = None, factor_budget: int = 0):
+        """factor_budget (bytes, 0: none): a model whose factors exceed it is
+        loaded streamed (fwh_set_factor_budget)"""
         self.models = list(model) if isinstance(model, (list, tuple)) else [model]
         self.model = self.models[0]
         self.ctx = ctx or Context(device)
+        _check(lib().fwh_set_factor_budget(self.ctx.h, int(factor_budget)), "fwh_set_factor_budget")
         if len(self.models) == 1:
             self._desc = self.model.desc()
             _check(lib().fwh_load(self.ctx.h, ctypes.byref(self._desc)), "fwh_load")

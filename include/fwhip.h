@@ -276,6 +276,17 @@ int  fwh_schur_form(fwh_ctx *ctx);
  * it the implicit Schur step is refused (FWH_E_STATE); a new load restores
  * resident factors.                                                       */
 int  fwh_release_factors(fwh_ctx *ctx, long long budget_bytes);
+/* Device bytes the skyline factors may take (all systems, with the block
+ * inverses of the solve kernel; 0, the default: no limit).  Read by the
+ * next fwh_load / fwh_load_batch: a model whose factors exceed it is loaded
+ * streamed -- the factors are never all resident: the Schur complements are
+ * formed chunk by chunk right after each chunk's factors are computed (in
+ * buffers of at most budget_bytes), and the solves that need the factors
+ * recompute them as after fwh_release_factors (the LDL^T kernel runs at
+ * depth 16 with one wave per subdomain; the implicit Schur step is refused).
+ * No reference counterpart: the reference keeps every subdomain's factor in
+ * host memory (hddm_solver.h:512).                                         */
+int  fwh_set_factor_budget(fwh_ctx *ctx, long long budget_bytes);
 /* advlas_skyline_ldl_solve (lib/hddm/advlas_skyline.c:134-167) applied to
  * every subdomain at once: rhs/x are 2*n_dof doubles (in place).          */
 int  fwh_ldl_solve_all(fwh_ctx *ctx, double *rhs, int plain);

@@ -9,7 +9,9 @@ The recomputed factors are the load's, so every result must equal the resident
 path's bit for bit: a solve, an adjoint solve on the same context, and a batch of
 right-hand sides, with budgets that cut the model into several chunks.  The
 implicit Schur step is refused afterwards, as are a budget smaller than one
-subdomain's factors and a release before the formation.
+subdomain's factors and a release before the formation.  A load under a
+budget below the model's factors (fwh_set_factor_budget) never holds them all:
+its S_s, solutions and adjoint gradient equal the resident load's bit for bit.
 """
 import dataclasses
 import sys
@@ -106,3 +108,48 @@ def test_release_refusals(small_sigma_model):
     with pytest.raises(fwhip.FwhError, match="FWH_E_STATE"):
         sol.release_factors(1 << 30)
     sol.close()
+
+
+@pytest.mark.parametrize("nchunk", [2, 5])
+def test_streamed_load_same_bits(small_sigma_model, cfg1_sigma_model, nchunk):
+    """A model whose factors exceed the budget is loaded streamed: its S_s are
+    formed chunk by chunk and the factors are never all resident.  With the
+    LDL^T kernel form a streamed load uses (depth 16, one wave) pinned on the
+    resident load, every result is the same bits."""
+    from adventure_fullwave_amd import fwhip
+    pins = dict(PINS, FWH_SOLVE_DEPTH="16", FWH_SOLVE_WAVES="1")
+    for model in (small_sigma_model, cfg1_sigma_model):
+        with pinned_env(pins):
+            a = fwhip.Solver(model)
+            a.schur_form()
+            total = a.info["factor_need_total"]
+            budget = int(max(a.info["factor_need_max"], total / nchunk))
+            b = fwhip.Solver(model, factor_budget=budget)
+            b.schur_form()
+        say(f"[gpu] {model.name} streamed: {b.info['factor_chunks']} chunks of at most "
+            f"{b.info['factor_chunk_bytes'] / 1e6:.2f} MB (factors {total / 1e6:.2f} MB)")
+        assert a.info["factors_resident"] == 1 and b.info["factors_resident"] == 0
+        assert b.info["factor_chunks"] >= 2 and b.info["schur_gemm"] == 1
+        rng = np.random.default_rng(9)
+        ng = int(a.info["n_iface"])
+        p = rng.standard_normal(ng) + 1j * rng.standard_normal(ng)
+        assert np.array_equal(a.schur_apply(p, explicit=True), b.schur_apply(p, explicit=True))
+        st_a = a.solve(eps=1e-8, max_loop=20000, graph_iters=16, schur=True)
+        st_b = b.solve(eps=1e-8, max_loop=20000, graph_iters=16, schur=True)
+        assert st_a["iters"] == st_b["iters"] and np.array_equal(a.x(), b.x())
+        # the adjoint gradient of J = sum |E|^2 over 8 edges, on the same contexts
+        E = a.x()
+        probes = rng.choice(E.size, 8, replace=False)
+        f = np.zeros_like(E)
+        f[probes] = np.conj(E[probes])
+        grads = []
+        for sol in (a, b):
+            sol.set_load(f)
+            sol.solve(eps=1e-10, max_loop=20000, graph_iters=16, schur=True)
+            grads.append(sol.grad_eps(E, sol.x()))
+            sol.set_load(None)
+        assert np.array_equal(grads[0], grads[1]) and np.abs(grads[0]).max() > 0
+        with pytest.raises(fwhip.FwhError, match="FWH_E_STATE"):
+            b.solve(eps=1e-8, max_loop=100, graph_iters=0, schur=False)
+        a.close()
+        b.close()
