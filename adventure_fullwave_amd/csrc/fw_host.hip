@@ -272,8 +272,8 @@ struct fwh_ctx {
   bool fac_resident = true;
   long long factor_budget = 0;        // fwh_set_factor_budget: above it, a load streams its factors
   std::vector<int> fchunk;            // chunk boundaries (subdomains), back() == nsub
-  double2 *cfac = nullptr, *cminv = nullptr, *cm16 = nullptr;
-  long long *mf_off_d = nullptr;      // per subdomain its first 16-row block of cm16's layout
+  double2 *farena = nullptr;          // one chunk's factors, block inverses and factor temporaries
+  long long *mf_off_d = nullptr;      // per subdomain its first 16-row block of the temporaries' layout
   std::vector<long long> mf_off_h;
   long long nblk_tot = 0;             // entries of blk_row
 
@@ -343,7 +343,7 @@ struct fwh_ctx {
     allocs.clear();
     fac_resident = true;
     fchunk.clear();
-    cfac = cminv = cm16 = nullptr;
+    farena = nullptr;
     mf_off_d = nullptr;
     for (int b = 0; b < 8; b++) {
       if (bst[b]) (void)hipStreamDestroy(bst[b]);
@@ -506,7 +506,19 @@ struct fwh_ctx {
   // kernels address a subdomain by its offsets in the whole model, so the
   // buffers are passed shifted back by the chunk's first offsets (they only
   // touch the chunk's own entries).
+  // the arena's parts for chunk [s0, s1): its skyline entries, then its
+  // block inverses, then the factor kernel's temporaries
+  long long blk_end(int s) const { return s + 1 < nsub ? (long long)meta_h[s + 1].blk0 : nblk_tot; }
+  void chunk_bufs(int s0, int s1, double2 **cfac, double2 **cminv, double2 **cm16) const {
+    const long long f = meta_h[s1 - 1].fac0 + meta_h[s1 - 1].nlen - meta_h[s0].fac0;
+    const long long b = blk_end(s1 - 1) - meta_h[s0].blk0;
+    *cfac = farena;
+    *cminv = farena + f;
+    *cm16 = farena + f + b * fw::MINV;
+  }
   hipError_t refactor_chunk(int sys, int s0, int s1) {
+    double2 *cfac, *cminv, *cm16;
+    chunk_bufs(s0, s1, &cfac, &cminv, &cm16);
     const SubMeta &A = meta_h[s0], &B = meta_h[s1 - 1];
     const long long f0 = A.fac0, f1 = B.fac0 + B.nlen;
     const long long e0 = A.nnz0, e1 = (long long)B.nnz0 + B.nnz;
@@ -556,6 +568,8 @@ struct fwh_ctx {
         const int s0 = fchunk[k], s1 = fchunk[k + 1];
         hipError_t e = refactor_chunk(sys, s0, s1);
         if (e != hipSuccess) return e;
+        double2 *cfac, *cminv, *cm16;
+        chunk_bufs(s0, s1, &cfac, &cminv, &cm16);
         fw::SolveArgs a = solve_args(mode, false, false, sys);
         a.pgrid = 0;
         a.order = nullptr;
@@ -2343,7 +2357,9 @@ int build_schur(fwh_ctx *c) {
           std::vector<int> sub;
           for (int s : c->order_h)
             if (s >= s0 && s < s1) sub.push_back(s);
-          rc = form_gemm(c, sub, k, c->cfac - c->meta_h[s0].fac0, tot, nsw, ms_fwd, ms_gemm);
+          double2 *cfac, *cminv, *cm16;
+          c->chunk_bufs(s0, s1, &cfac, &cminv, &cm16);
+          rc = form_gemm(c, sub, k, cfac - c->meta_h[s0].fac0, tot, nsw, ms_fwd, ms_gemm);
         }
     }
     if (rc) return rc;
@@ -2543,45 +2559,31 @@ extern "C" int fwh_schur_form(fwh_ctx *c) {
   return build_schur(c);
 }
 
-// Release the skyline factors and block inverses once the S_s are formed.
-// The explicit iteration needs neither; the solves that do (first and last
-// steps, fwh_ldl_solve_all) then recompute them chunk by chunk: consecutive
-// subdomains whose factors, block inverses and the factor kernel's
-// temporaries fit budget_bytes, into buffers of that size.
 // The chunks of a factor recomputation within budget_bytes (consecutive
 // subdomains, refactor_bytes each) and their buffers (sized for the largest
 // chunk); the per-subdomain offsets of the factor kernel's temporaries go to
 // the device.
 int plan_refactor(fwh_ctx *c, long long budget_bytes) {
-  auto blk_end = [&](int s) { return s + 1 < c->nsub ? (long long)c->meta_h[s + 1].blk0 : c->nblk_tot; };
   std::vector<int> ch{0};
-  long long fmax = 0, bmax = 0, mmax = 0, fsum = 0, bsum = 0, msum = 0;
+  long long sum = 0, cmax = 0;  // bytes of the running chunk, of the largest
   for (int s = 0; s < c->nsub; s++) {
-    const SubMeta &M = c->meta_h[s];
-    const long long f = M.nlen, b = blk_end(s) - M.blk0, m = (M.n + 15) / 16;
     const long long one = c->refactor_bytes(s);
     if (one > budget_bytes)
       return fail(FWH_E_LIMIT, "subdomain %d needs %lld bytes of factors, above the budget %lld", s, one,
                   budget_bytes);
-    if (16 * (fsum + f + (bsum + b) * fw::MINV + 256 * (msum + m)) > budget_bytes) {
+    if (sum + one > budget_bytes) {
       ch.push_back(s);
-      fsum = bsum = msum = 0;
+      sum = 0;
     }
-    fsum += f;
-    bsum += b;
-    msum += m;
-    fmax = std::max(fmax, fsum);
-    bmax = std::max(bmax, bsum);
-    mmax = std::max(mmax, msum);
+    sum += one;
+    cmax = std::max(cmax, sum);
   }
   ch.push_back(c->nsub);
   int rc;
-  if ((rc = c->alloc(&c->cfac, (size_t)fmax)) || (rc = c->alloc(&c->cminv, (size_t)bmax * fw::MINV)) ||
-      (rc = c->alloc(&c->cm16, (size_t)mmax * 256)) || (rc = c->upload(&c->mf_off_d, c->mf_off_h)))
-    return rc;
+  if ((rc = c->alloc(&c->farena, (size_t)(cmax / 16))) || (rc = c->upload(&c->mf_off_d, c->mf_off_h))) return rc;
   c->fchunk = ch;
   c->info.factor_chunks = (int)ch.size() - 1;
-  c->info.factor_chunk_bytes = 16.0 * ((double)fmax + (double)bmax * fw::MINV + 256.0 * (double)mmax);
+  c->info.factor_chunk_bytes = (double)cmax;
   return 0;
 }
 

@@ -54,6 +54,7 @@ def test_released_factors_same_bits(small_sigma_model, cfg1_sigma_model, nchunk)
         say(f"[gpu] {model.name}: released {b.info['factor_bytes_released'] / 1e6:.2f} MB, "
             f"{b.info['factor_chunks']} chunks of at most {b.info['factor_chunk_bytes'] / 1e6:.2f} MB")
         assert b.info["factors_resident"] == 0 and b.info["factor_chunks"] >= 2
+        assert b.info["factor_chunk_bytes"] <= budget
         st_a = a.solve(eps=1e-8, max_loop=20000, graph_iters=16, schur=True)
         st_b = b.solve(eps=1e-8, max_loop=20000, graph_iters=16, schur=True)
         assert st_a["iters"] == st_b["iters"] and st_b["converged"] == 1
@@ -130,6 +131,7 @@ def test_streamed_load_same_bits(small_sigma_model, cfg1_sigma_model, nchunk):
             f"{b.info['factor_chunk_bytes'] / 1e6:.2f} MB (factors {total / 1e6:.2f} MB)")
         assert a.info["factors_resident"] == 1 and b.info["factors_resident"] == 0
         assert b.info["factor_chunks"] >= 2 and b.info["schur_gemm"] == 1
+        assert b.info["factor_chunk_bytes"] <= budget
         rng = np.random.default_rng(9)
         ng = int(a.info["n_iface"])
         p = rng.standard_normal(ng) + 1j * rng.standard_normal(ng)
