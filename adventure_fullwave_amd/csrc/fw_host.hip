@@ -3316,6 +3316,115 @@ int sky_one(fwh_ctx *c, int n, const int *idx1, const int *idx2, double *mat, do
 }
 }  // namespace
 
+// One system A x = b by the subdomain COCG kernel (k_cocg, one 256-thread
+// workgroup, vectors in LDS) with IC(0) (pc 3, k_icc_make) or no
+// preconditioner (pc 0: the IC(0) solve run on the identity pattern, L = I,
+// which returns r exactly).  A is the reference's AIJ_MAT storage: row i's
+// entries of the lower triangle ai[i] .. ai[i+1]-1, columns ascending with the
+// diagonal last (aij), values vij (2 doubles each); x is the initial guess
+// in and the solution out.  *status: the reference's return value (>= 0
+// iterations, SOLVER_STATUS_* < 0).
+extern "C" int fwh_aij_cocg(fwh_ctx *c, int nf, const int *ai, const int *aij, const double *vij, const double *b,
+                            double *x, int pc, double eps, double div, double ic_shift, int *status) {
+  if (!c || !ai || !aij || !vij || !b || !x || !status) return fail(FWH_E_ARG, "null argument");
+  if (nf <= 0) return fail(FWH_E_ARG, "nf must be positive");
+  if (pc != 0 && pc != 3) return fail(FWH_E_ARG, "preconditioner %d: only none (0) and ICC (3) run on the GPU", pc);
+  if ((size_t)5 * nf * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "system of %d unknowns too large for the COCG kernel", nf);
+  if (ai[0] != 0) return fail(FWH_E_ARG, "ai[0] must be 0");
+  for (int i = 0; i < nf; i++) {
+    if (ai[i + 1] <= ai[i]) return fail(FWH_E_ARG, "row %d is empty (its diagonal is required)", i);
+    if (aij[ai[i + 1] - 1] != i) return fail(FWH_E_ARG, "row %d does not end with its diagonal", i);
+    for (int t = ai[i]; t < ai[i + 1] - 1; t++)
+      if (aij[t] < 0 || aij[t] >= aij[t + 1]) return fail(FWH_E_ARG, "row %d: columns not ascending below i", i);
+  }
+  const int nnz = ai[nf];
+  HIPCHK(hipSetDevice(c->device));
+  // full symmetric rows for the products: the row's lower entries, then the
+  // transposed entries of the later rows in row order
+  std::vector<std::vector<std::pair<int, double2>>> up(nf);
+  const double2 *v2 = (const double2 *)vij;
+  for (int i = 0; i < nf; i++)
+    for (int t = ai[i]; t < ai[i + 1] - 1; t++) up[aij[t]].push_back({i, v2[t]});
+  std::vector<int> fptr(nf + 1, 0), fcol;
+  std::vector<double2> fval;
+  for (int i = 0; i < nf; i++) {
+    for (int t = ai[i]; t < ai[i + 1]; t++) {
+      fcol.push_back(aij[t]);
+      fval.push_back(v2[t]);
+    }
+    for (auto &pv : up[i]) {
+      fcol.push_back(pv.first);
+      fval.push_back(pv.second);
+    }
+    fptr[i + 1] = (int)fcol.size();
+  }
+  // the identity pattern of an unpreconditioned solve
+  std::vector<double2> Lid, dgid;
+  if (pc == 0) {
+    Lid.assign(nnz, make_double2(0.0, 0.0));
+    for (int i = 0; i < nf; i++) Lid[ai[i + 1] - 1] = make_double2(1.0, 0.0);
+    dgid.assign(nf, make_double2(1.0, 0.0));
+  }
+  SubMeta m{};
+  m.n = nf;
+  TempDev t;
+  SubMeta *d_meta;
+  int *d_ai, *d_aij, *d_fptr, *d_fcol, *d_st;
+  double2 *d_v, *d_L, *d_dg, *d_fval, *d_b, *d_x;
+  HIPCHK(t.alloc(&d_meta, 1));
+  HIPCHK(t.alloc(&d_ai, nf + 1));
+  HIPCHK(t.alloc(&d_aij, nnz));
+  HIPCHK(t.alloc(&d_v, nnz));
+  HIPCHK(t.alloc(&d_L, nnz));
+  HIPCHK(t.alloc(&d_dg, nf));
+  HIPCHK(t.alloc(&d_fptr, nf + 1));
+  HIPCHK(t.alloc(&d_fcol, fcol.size()));
+  HIPCHK(t.alloc(&d_fval, fval.size()));
+  HIPCHK(t.alloc(&d_b, nf));
+  HIPCHK(t.alloc(&d_x, nf));
+  HIPCHK(t.alloc(&d_st, 1));
+  auto up_ = [&](void *d, const void *h, size_t bytes) {
+    return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->st);
+  };
+  HIPCHK(up_(d_meta, &m, sizeof(m)));
+  HIPCHK(up_(d_ai, ai, sizeof(int) * (nf + 1)));
+  HIPCHK(up_(d_aij, aij, sizeof(int) * nnz));
+  HIPCHK(up_(d_v, vij, sizeof(double2) * nnz));
+  HIPCHK(up_(d_fptr, fptr.data(), sizeof(int) * fptr.size()));
+  HIPCHK(up_(d_fcol, fcol.data(), sizeof(int) * fcol.size()));
+  HIPCHK(up_(d_fval, fval.data(), sizeof(double2) * fval.size()));
+  HIPCHK(up_(d_b, b, sizeof(double2) * nf));
+  HIPCHK(up_(d_x, x, sizeof(double2) * nf));
+  if (pc == 0) {
+    HIPCHK(up_(d_L, Lid.data(), sizeof(double2) * nnz));
+    HIPCHK(up_(d_dg, dgid.data(), sizeof(double2) * nf));
+  }
+  fw::IccArgs a{};
+  a.meta = d_meta;
+  a.dptr = d_ai;
+  a.dcol = d_aij;
+  a.dval = d_v;
+  a.L = d_L;
+  a.dg = d_dg;
+  a.fptr = d_fptr;
+  a.fcol = d_fcol;
+  a.fval = d_fval;
+  // acceleration factor outside [1, 3] is replaced by 1.05 (aij_pc_func.h:66-73)
+  a.af = (ic_shift > 3.0 || ic_shift < 1.0) ? 1.05 : ic_shift;
+  a.b = d_b;
+  a.x = d_x;
+  a.status = d_st;
+  a.eps = eps;
+  a.div = div;
+  a.nmax = nf;
+  if (pc == 3) HIPCHK(fw::launch_icc_make(a, 1, c->st));
+  HIPCHK(fw::launch_cocg(a, 1, c->st));
+  HIPCHK(hipMemcpyAsync(x, d_x, sizeof(double2) * nf, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(status, d_st, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
 extern "C" int fwh_sky_factor(fwh_ctx *c, int n, const int *idx1, const int *idx2, double *mat, double *dinv) {
   return sky_one(c, n, idx1, idx2, mat, nullptr, dinv, true);
 }

@@ -27,6 +27,9 @@
 #define REF_NU 7.957747e+05  /* 1/mu0 as fixed in make_elem_em.c:37 and fullwave.c:670-675 */
 #define REF_NEED 0           /* NEED, lib/std/defs.h:111 */
 #define REF_VOLT "./MODEL/volt.dat"
+#define REF_SOLVER_PC_NONE 0             /* lib/std/defs.h:137 */
+#define REF_SOLVER_PC_ICC 3              /* lib/std/defs.h:140 */
+#define REF_SOLVER_STATUS_UNKNOWN_PC (-4) /* lib/std/defs.h:167 */
 
 /* records the message where fwh_last_error() reports it (fw_host.hip) */
 int fwhi_fail(int code, const char *msg);
@@ -408,4 +411,26 @@ void advlas_skyline_ldl_solve(DMatrix dmat, double _Complex *solution) {
     fflush(stdout);
     exit(1);
   }
+}
+
+/* ---- the subdomain COCG of mymt_aij_solver on the GPU ------------------ */
+/* mymt_CS_AIJ_Solver_COCG (mymt_aij_cs.h:64, aij_solver_cg.h:9-44): the
+ * preconditioner switch, then COCG on aij_mat->x_c from aij_mat->b_c.  ICC
+ * and none run on the GPU (fwh_aij_cocg); MRIC0 and the others return
+ * SOLVER_STATUS_UNKNOWN_PC, as the reference's switch does for a PC it does
+ * not know.  The monitor log is not written. */
+int mymt_CS_AIJ_Solver_COCG(AIJ_MAT *aij_mat, SolvOpt solvopt, FILE *monitor) {
+  (void)monitor;
+  if (solvopt.solver_pc != REF_SOLVER_PC_NONE && solvopt.solver_pc != REF_SOLVER_PC_ICC)
+    return REF_SOLVER_STATUS_UNKNOWN_PC;
+  int status = 0;
+  const int rc = fwh_aij_cocg(sky_ctx(), aij_mat->nf, aij_mat->ai, aij_mat->aij, (const double *)aij_mat->vij_c,
+                              (const double *)aij_mat->b_c, (double *)aij_mat->x_c, solvopt.solver_pc,
+                              solvopt.solver_eps, solvopt.solver_div, solvopt.solver_ic_shift, &status);
+  if (rc) {
+    fprintf(stdout, "ERROR: mymt_CS_AIJ_Solver_COCG (%d): %s\n", rc, fwh_last_error());
+    fflush(stdout);
+    exit(1);
+  }
+  return status;
 }

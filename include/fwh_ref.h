@@ -428,4 +428,15 @@ void fwh_ref_desc_free(fwh_mesh_desc *d);
 void advlas_skyline_ldl_decomposite(DMatrix dmat, double _Complex *work);
 void advlas_skyline_ldl_solve(DMatrix dmat, double _Complex *solution);
 
+/* mymt_CS_AIJ_Solver_COCG (lib/mymt_aij_solver/mymt_aij_cs.h:64; the
+ * preconditioner switch aij_solver_cg.h:9-44 and the COCG of :113-355) on
+ * the GPU of the same process-wide context: the system in aij_mat (ai, aij,
+ * vij_c; b_c; x_c the initial guess, overwritten by the solution), stopping
+ * on ||r||_2/||b||_2 <= solvopt.solver_eps, diverged at solver_div, at most
+ * 10 nf iterations, IC(0) with acceleration solver_ic_shift for
+ * SOLVER_PC_ICC or none for SOLVER_PC_NONE; returns the reference's value
+ * (>= 0 iterations, SOLVER_STATUS_* < 0; SOLVER_STATUS_UNKNOWN_PC for the
+ * other preconditioners).  At most 2,048 unknowns.                         */
+int mymt_CS_AIJ_Solver_COCG(AIJ_MAT *aij_mat, SolvOpt solvopt, FILE *monitor);
+
 #endif /* FWH_REF_H */

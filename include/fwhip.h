@@ -349,6 +349,17 @@ int  fwh_element_batch_static(fwh_ctx *ctx, int nel, const double *xyz,
  * return value (>=0 iterations, <0 error).                                */
 int  fwh_cocg_icc_all(fwh_ctx *ctx, const double *b, double *x, int *status,
                       double eps, double ic_shift);
+/* One system in the reference's AIJ_MAT storage (lib/std/types.h:44-61:
+ * ai[nf+1], lower-triangle columns aij with the diagonal last, values vij,
+ * 2 doubles each) by the same GPU COCG: mymt_CS_AIJ_Solver_COCG
+ * (aij_solver_cg.h:9-44, 113-355) with pc = SOLVER_PC_ICC (3, IC(0) with
+ * acceleration ic_shift, aij_pc_func.h:36-167) or SOLVER_PC_NONE (0); x is
+ * the initial guess in, the solution out (2*nf doubles); *status the
+ * reference's return value (>= 0 iterations, SOLVER_STATUS_* < 0).  At most
+ * 2,048 unknowns (the kernel's vectors live in LDS).                      */
+int  fwh_aij_cocg(fwh_ctx *ctx, int nf, const int *ai, const int *aij, const double *vij,
+                  const double *b, double *x, int pc, double eps, double div, double ic_shift,
+                  int *status);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,8 @@ def lib():
         L.refdrv_solve.restype = ctypes.c_int
         L.refdrv_advlas.argtypes = [ctypes.c_int, P, P, P, P, P]
         L.refdrv_advlas.restype = None
+        L.refdrv_mymt_cocg.argtypes = [ctypes.c_int, P, P, P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.refdrv_mymt_cocg.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -123,3 +125,16 @@ def advlas(idx1, idx2, mat, rhs):
     lib().refdrv_advlas(len(idx1), idx1.ctypes.data, idx2.ctypes.data, m.ctypes.data, work.ctypes.data,
                         v.ctypes.data)
     return m, work, v
+
+
+def mymt_cocg(rowptr, col, val, b, x0=None, pc=3, eps=1e-9, ic_shift=1.2):
+    """mymt_CS_AIJ_Solver_COCG through the reference's AIJ_MAT / SolvOpt
+    (tests/refdrv.c): (return value, solution)"""
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    cl = np.ascontiguousarray(col, np.int32)
+    vl = np.ascontiguousarray(val, np.complex128)
+    bb = np.ascontiguousarray(b, np.complex128)
+    x = np.zeros(len(rp) - 1, np.complex128) if x0 is None else np.array(x0, np.complex128)
+    st = lib().refdrv_mymt_cocg(len(rp) - 1, rp.ctypes.data, cl.ctypes.data, vl.ctypes.data, bb.ctypes.data,
+                                x.ctypes.data, int(pc), float(eps), float(ic_shift))
+    return st, x

@@ -201,3 +201,26 @@ void refdrv_advlas(int n, int *idx1, int *idx2, double *mat, double *work, doubl
   advlas_skyline_ldl_decomposite(dm, (double _Complex *)work);
   advlas_skyline_ldl_solve(dm, (double _Complex *)v);
 }
+
+/* mymt_CS_AIJ_Solver_COCG on an AIJ_MAT and SolvOpt filled as the reference's
+ * callers fill them (the Dirichlet matrix of aij_matrix_func.h:140-235,
+ * solver options of main.c:40-60): returns the solver's value, x in/out */
+int refdrv_mymt_cocg(int nf, int *ai, int *aij, double *vij, double *b, double *x, int pc, double eps,
+                     double ic_shift) {
+  AIJ_MAT am;
+  SolvOpt so;
+  memset(&am, 0, sizeof(am));
+  memset(&so, 0, sizeof(so));
+  am.nf = nf;
+  am.ai = ai;
+  am.aij = aij;
+  am.vij_c = (Complex *)vij;
+  am.b_c = (Complex *)b;
+  am.x_c = (Complex *)x;
+  so.solver_pc = pc;
+  so.solver_eps = eps;
+  so.solver_div = 1.0e+10; /* DEFAULT_SOLVER_DIV, lib/std/defs.h:157 */
+  so.solver_ic_shift = ic_shift;
+  so.solver_max_loop = 10 * nf;
+  return mymt_CS_AIJ_Solver_COCG(&am, so, NULL);
+}

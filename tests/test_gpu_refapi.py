@@ -139,3 +139,31 @@ def test_advlas_on_subdomain_factors(small_sigma_model):
         assert np.abs(fac - ref).max() <= 1e-10 * np.abs(ref).max()
         xr = fo.sky_solve(g["idx1"], g["idx2"], ref, b)
         assert np.abs(x - xr).max() <= 1e-10 * np.abs(xr).max()
+
+
+@pytest.mark.parametrize("pc", [3, 0])
+def test_mymt_cocg_on_subdomain_matrices(small_sigma_model, pc):
+    """mymt_CS_AIJ_Solver_COCG on the reference's AIJ_MAT (filled by the C
+    driver) for every subdomain's Dirichlet matrix of the small model (the
+    reference's SetBC + DeleteZero + Assem storage), with IC(0) and with no
+    preconditioner: iteration counts against the oracle's COCG within 6 %
+    (the CPU spread under summation-order permutations, as for the batched
+    subdomain COCG), solutions against a dense solve (rel. L2 1e-7 at a 1e-9
+    stop), and an unknown preconditioner returns SOLVER_STATUS_UNKNOWN_PC."""
+    from tests.helpers import dense_lower
+    from tests.test_gpu_parity import dirichlet_csr
+    om = fo.OracleModel(small_sigma_model)
+    rng = np.random.default_rng(12)
+    for s in range(small_sigma_model.nsub):
+        rp, cl, vl = dirichlet_csr(small_sigma_model, om, s)
+        n = len(rp) - 1
+        b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        st, x = refapi.mymt_cocg(rp, cl, vl, b, pc=pc, eps=1e-9, ic_shift=1.2)
+        sto, xo, _ = fo.cocg_icc(rp, cl, vl, b, eps=1e-9, ic_shift=1.2, use_icc=pc == 3)
+        say(f"[gpu] mymt COCG pc={pc} subdomain {s}: {st} iterations, oracle {sto}")
+        assert st > 0 and sto > 0
+        assert abs(st - sto) <= max(2, 0.06 * sto)
+        xd = np.linalg.solve(dense_lower(rp, cl, vl, n), b)
+        assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-7
+    st, _ = refapi.mymt_cocg(rp, cl, vl, b, pc=4)
+    assert st == -4
