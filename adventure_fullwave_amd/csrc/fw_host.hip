@@ -1949,6 +1949,41 @@ SmvSplit make_smv_split(const fwh_ctx *c, const std::vector<int> &ord, const std
   return sp;
 }
 
+// The explicit iteration's gather lists for a split: the load's, with every
+// slot of a split subdomain followed by its copies (part markers stay on the
+// first), and the four-wide table of the same ids.
+int split_gather(const fwh_ctx *c, const SmvSplit &S, std::vector<int> &xptr, std::vector<int> &xslot,
+                 std::vector<int4> &x4) {
+  std::vector<int> slot_sub(c->nslots);
+  for (int s = 0; s < c->nsub; s++)
+    for (int k = 0; k < c->meta_h[s].ng; k++) slot_sub[c->meta_h[s].g0 + k] = s;
+  xptr.assign(c->ng + 1, 0);
+  xslot.clear();
+  xslot.reserve(c->gq_slot_h.size());
+  for (int g = 0; g < c->ng; g++) {
+    for (int t = c->gq_ptr_h[g]; t < c->gq_ptr_h[g + 1]; t++) {
+      const int raw = c->gq_slot_h[t], sl = raw < 0 ? ~raw : raw;
+      const int s = slot_sub[sl], k = sl - c->meta_h[s].g0;
+      xslot.push_back(raw);
+      for (int j = 1; j < S.count[s]; j++) xslot.push_back(S.work[S.first[s] + j].y0 + k);
+    }
+    xptr[g + 1] = (int)xslot.size();
+  }
+  x4.assign(c->ng, make_int4(0, 0, 0, 0));
+  for (int g = 0; g < c->ng; g++) {
+    const int c0 = xptr[g], cnt = xptr[g + 1] - c0;
+    int e[4] = {fw::SLOT_NONE, fw::SLOT_NONE, fw::SLOT_NONE, fw::SLOT_NONE};
+    if (cnt <= 4 && !c->gather_csr)
+      for (int j = 0; j < cnt; j++) e[j] = xslot[c0 + j];
+    else
+      e[1] = fw::SLOT_MORE;
+    x4[g] = make_int4(e[0], e[1], e[2], e[3]);
+  }
+  for (int raw : xslot)
+    if ((raw < 0 ? ~raw : raw) >= c->ss.slot) return fail(FWH_E_LIMIT, "matvec gather slot out of range");
+  return 0;
+}
+
 int tune_smv_split(fwh_ctx *c, const std::vector<int> &ord, const std::vector<long long> &off, int ncu) {
   std::vector<long long> ntile(c->nsub);
   long long ttot = 0;
@@ -1985,10 +2020,20 @@ int tune_smv_split(fwh_ctx *c, const std::vector<int> &ord, const std::vector<lo
     c->pqpart = nullptr;
     if ((rc = c->alloc(&c->pqpart, (size_t)c->nsys * npq_max))) return rc;
   }
-  // time the distinct candidates in turn (pairs of launches, best of five)
+  // time the distinct candidates in turn as an explicit iteration runs them:
+  // the matvec, then the vector pass with the candidate's gather lists (a
+  // split multiplies the slots each interface unknown gathers), two such
+  // pairs back to back, best of five rounds.  The vector pass runs on a
+  // zeroed status of its own (it commits nothing, writes no history) and
+  // overwrites the interface vectors, which the first step of every solve
+  // sets again.
   TempDev tmp;
   fw::SmvWork *d_w[4] = {};
   fw::SmvArgs sa[4];
+  fw::VecArgs va[4];
+  fw::Scal *d_sc = nullptr;
+  HIPCHK(tmp.alloc(&d_sc, (size_t)c->nsys));
+  HIPCHK(hipMemsetAsync(d_sc, 0, sizeof(fw::Scal) * c->nsys, c->st));
   EventPair ev;
   HIPCHK(ev.create());
   for (int v = 0; v < 4; v++) {
@@ -1999,7 +2044,27 @@ int tune_smv_split(fwh_ctx *c, const std::vector<int> &ord, const std::vector<lo
                           hipMemcpyHostToDevice, c->st));
     c->smv_work = d_w[v];
     sa[v] = c->smv_args(false, false);
+    va[v] = c->vec_args(-1, false);  // the load's gather lists, one (p,q) part per subdomain
+    va[v].sc = d_sc;
+    va[v].npq = sp[v].npq;
+    if (sp[v].split) {
+      std::vector<int> xptr, xslot;
+      std::vector<int4> x4;
+      if ((rc = split_gather(c, sp[v], xptr, xslot, x4))) return rc;
+      int *d_ptr = nullptr, *d_slot = nullptr;
+      int4 *d_s4 = nullptr;
+      HIPCHK(tmp.alloc(&d_ptr, xptr.size()));
+      HIPCHK(tmp.alloc(&d_slot, xslot.size()));
+      HIPCHK(tmp.alloc(&d_s4, x4.size()));
+      HIPCHK(hipMemcpyAsync(d_ptr, xptr.data(), xptr.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(d_slot, xslot.data(), xslot.size() * sizeof(int), hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(d_s4, x4.data(), x4.size() * sizeof(int4), hipMemcpyHostToDevice, c->st));
+      va[v].gq_ptr = d_ptr;
+      va[v].gq_slot = d_slot;
+      va[v].gq_slot4 = d_s4;
+    }
     HIPCHK(fw::launch_smv(sa[v], (int)sp[v].work.size(), c->st));
+    HIPCHK(fw::launch_iter_ab(va[v], c->st, false));
     c->info.ms_smv_split[v] = 1e30;
   }
   for (int r = 0; r < 5; r++)
@@ -2007,8 +2072,10 @@ int tune_smv_split(fwh_ctx *c, const std::vector<int> &ord, const std::vector<lo
       if (c->info.ms_smv_split[v] < 0) continue;
       float ms = 0.f;
       HIPCHK(hipEventRecord(ev.a, c->st));
-      HIPCHK(fw::launch_smv(sa[v], (int)sp[v].work.size(), c->st));
-      HIPCHK(fw::launch_smv(sa[v], (int)sp[v].work.size(), c->st));
+      for (int k = 0; k < 2; k++) {
+        HIPCHK(fw::launch_smv(sa[v], (int)sp[v].work.size(), c->st));
+        HIPCHK(fw::launch_iter_ab(va[v], c->st, false));
+      }
       HIPCHK(hipEventRecord(ev.b, c->st));
       HIPCHK(hipEventSynchronize(ev.b));
       HIPCHK(hipEventElapsedTime(&ms, ev.a, ev.b));
@@ -2043,34 +2110,9 @@ int tune_smv_split(fwh_ctx *c, const std::vector<int> &ord, const std::vector<lo
   c->gqx_ptr = c->gqx_slot = nullptr;
   c->gqx_slot4 = nullptr;
   if (S.split) {
-    // the explicit iteration's gather lists: the load's, with every slot of a
-    // split subdomain followed by its copies (part markers stay on the first)
-    std::vector<int> slot_sub(c->nslots);
-    for (int s = 0; s < c->nsub; s++)
-      for (int k = 0; k < c->meta_h[s].ng; k++) slot_sub[c->meta_h[s].g0 + k] = s;
-    std::vector<int> xptr(c->ng + 1, 0), xslot;
-    xslot.reserve(c->gq_slot_h.size());
-    for (int g = 0; g < c->ng; g++) {
-      for (int t = c->gq_ptr_h[g]; t < c->gq_ptr_h[g + 1]; t++) {
-        const int raw = c->gq_slot_h[t], sl = raw < 0 ? ~raw : raw;
-        const int s = slot_sub[sl], k = sl - c->meta_h[s].g0;
-        xslot.push_back(raw);
-        for (int j = 1; j < S.count[s]; j++) xslot.push_back(S.work[S.first[s] + j].y0 + k);
-      }
-      xptr[g + 1] = (int)xslot.size();
-    }
-    std::vector<int4> x4(c->ng);
-    for (int g = 0; g < c->ng; g++) {
-      const int c0 = xptr[g], cnt = xptr[g + 1] - c0;
-      int e[4] = {fw::SLOT_NONE, fw::SLOT_NONE, fw::SLOT_NONE, fw::SLOT_NONE};
-      if (cnt <= 4 && !c->gather_csr)
-        for (int j = 0; j < cnt; j++) e[j] = xslot[c0 + j];
-      else
-        e[1] = fw::SLOT_MORE;
-      x4[g] = make_int4(e[0], e[1], e[2], e[3]);
-    }
-    for (int raw : xslot)
-      if ((raw < 0 ? ~raw : raw) >= c->ss.slot) return fail(FWH_E_LIMIT, "matvec gather slot out of range");
+    std::vector<int> xptr, xslot;
+    std::vector<int4> x4;
+    if ((rc = split_gather(c, S, xptr, xslot, x4))) return rc;
     if ((rc = c->upload(&c->gqx_ptr, xptr)) || (rc = c->upload(&c->gqx_slot, xslot)) ||
         (rc = c->upload(&c->gqx_slot4, x4)))
       return rc;
