@@ -102,6 +102,9 @@ void parallel_for(int n, F f, const int *order = nullptr) {
   for (auto &x : th) x.join();
 }
 
+// the row blocks of subdomain m that prefetch depth pfk sweeps
+inline int nblk_for(const SubMeta &m, int pfk) { return pfk == 8 ? m.nblk_s : pfk == 32 ? m.nblk_w : m.nblk; }
+
 double now_ms() {
   using namespace std::chrono;
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
@@ -175,9 +178,12 @@ struct fwh_ctx {
   bool loaded = false, first_done = false, last_done = false, p_final = false;
 
   // sizes
-  int nsub = 0, npart = 0, ng = 0, nmax = 0, ngmax = 0, wmax = 0, G = 1, nblkmax = 0, nblkmax_s = 0;
+  int nsub = 0, npart = 0, ng = 0, nmax = 0, ngmax = 0, wmax = 0, G = 1, nblkmax = 0, nblkmax_s = 0, nblkmax_w = 0;
+  // the most row blocks of the partition that prefetch depth pfk sweeps
+  // (8: short blocks, 32: wide blocks for four waves, else the long ones)
+  int nblkmax_for(int pfk) const { return pfk == 8 ? nblkmax_s : pfk == 32 ? nblkmax_w : nblkmax; }
   // prefetch depth of the LDL^T kernel (8 or 16 loads per lane, row blocks
-  // of 64*depth entries): the depth of the per-iteration sweeps is picked at
+  // of 64*depth entries; 32 with four waves: 8 loads per thread): the depth of the per-iteration sweeps is picked at
   // load by timing; the Schur-complement formation (hundreds of waves per
   // CU) uses 8, the depth with the higher occupancy, whenever every row fits
   int pfk_iter = 16, pfk_unit = 16;
@@ -194,6 +200,7 @@ struct fwh_ctx {
   } sp;
   int *solve_queue = nullptr;
   bool short_ok = false;
+  bool wide_ok = false;  // the wide row blocks are built (four-wave form at depth 32)
   long long ndof = 0, nslots = 0, nnz = 0, nlen = 0, nel = 0, ncpl = 0, ngrow = 0;
   std::vector<SubMeta> meta_h;
   std::vector<int8_t> bcflag_h;  // per DOF: 0 interior, 1 interface, 2 essential
@@ -282,7 +289,8 @@ struct fwh_ctx {
   // stream (fork/join on the library stream, so graphs capture them as
   // parallel branches); use_buckets is picked at load by timing
   struct Bucket {
-    int first, count, nmax, ngmax, nblkmax, nblkmax_s;
+    int first, count, nmax, ngmax, nblkmax, nblkmax_s, nblkmax_w;
+    int nblkmax_for(int pfk) const { return pfk == 8 ? nblkmax_s : pfk == 32 ? nblkmax_w : nblkmax; }
   };
   std::vector<Bucket> buckets;
   int *border = nullptr;          // subdomains bucket by bucket, largest first
@@ -372,7 +380,7 @@ struct fwh_ctx {
       ab.base = B.first;
       ab.nmax = B.nmax;
       ab.ngmax = B.ngmax;
-      ab.nblkmax = a.pfk == 8 ? B.nblkmax_s : B.nblkmax;
+      ab.nblkmax = B.nblkmax_for(a.pfk);
       e = hipStreamWaitEvent(bst[b], ev_fork, 0);
       if (e == hipSuccess) e = fw::launch_solve(ab, B.count, bst[b]);
       if (e == hipSuccess) e = hipEventRecord(ev_join[b], bst[b]);
@@ -397,7 +405,7 @@ struct fwh_ctx {
     a.ngmax = ngmax;
     a.pfk = mode == fw::MODE_UNIT ? pfk_unit : pfk_iter;
     a.waves = mode == fw::MODE_UNIT ? 1 : waves_iter;
-    a.nblkmax = a.pfk == 8 ? nblkmax_s : nblkmax;
+    a.nblkmax = nblkmax_for(a.pfk);
     a.idx1 = idx1;
     a.rowbeg = rowbeg;
     a.fac = fac;
@@ -582,14 +590,14 @@ struct fwh_ctx {
           a.base = s0;
           a.nmax = nmax;
           a.ngmax = ngmax;
-          a.nblkmax = a.pfk == 8 ? nblkmax_s : nblkmax;
+          a.nblkmax = nblkmax_for(a.pfk);
           e = fw::launch_solve(a, s1 - s0, st);
         } else {
           for (int s = s0; s < s1 && e == hipSuccess; s++) {
             a.base = s;
             a.nmax = meta_h[s].n;
             a.ngmax = meta_h[s].ng;
-            a.nblkmax = a.pfk == 8 ? meta_h[s].nblk_s : meta_h[s].nblk;
+            a.nblkmax = nblk_for(meta_h[s], a.pfk);
             e = fw::launch_solve(a, 1, st);
           }
         }
@@ -607,7 +615,7 @@ struct fwh_ctx {
       a.base = s;
       a.nmax = meta_h[s].n;
       a.ngmax = meta_h[s].ng;
-      a.nblkmax = a.pfk == 8 ? meta_h[s].nblk_s : meta_h[s].nblk;
+      a.nblkmax = nblk_for(meta_h[s], a.pfk);
       hipError_t e = fw::launch_solve(a, 1, st);
       if (e != hipSuccess) return e;
     }
@@ -1462,7 +1470,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   // integers, so the sum over subdomains is exact in any order)
   {
     struct SubProfile {
-      std::vector<int> blk, blk_s;
+      std::vector<int> blk, blk_s, blk_w;
       double flops = 0.0;
       int wmax = 0, bad = 0;
     };
@@ -1494,6 +1502,10 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         P.bad = 1;
         return;
       }
+      if (partition(M, fw::SOLVE_THREADS * 32, P.blk_w)) {
+        P.bad = 1;
+        return;
+      }
       for (int i = 0; i < M.n; i++) {
         const int fi = idx1[M.dof0 + i];
         long long f = 0;
@@ -1501,6 +1513,20 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         P.flops += 8.0 * (double)f + 14.0 * (i - fi);
       }
     }, big_first.data());
+    // the wide blocks (<= 2,048 entries, swept by four waves) only where
+    // they cut the block count by 5 % or more: profile rows long enough to
+    // stop the 1,024-entry blocks short of 16 rows (each block costs a 2 KB
+    // inverse triangle per system; the block count sets the length of the
+    // sweeps' dependency chain); FWH_SOLVE_DEPTH=32 builds them anyway
+    {
+      long long nb = 0, nbw = 0;
+      for (int s = 0; s < nsub; s++) {
+        nb += (long long)sp[s].blk.size() - 1;
+        nbw += (long long)sp[s].blk_w.size() - 1;
+      }
+      const char *e = getenv("FWH_SOLVE_DEPTH");
+      c->wide_ok = (e && atoi(e) == 32) || (double)nbw < 0.95 * (double)nb;
+    }
     for (int s = 0; s < nsub; s++) {
       SubMeta &M = meta[s];
       const SubProfile &P = sp[s];
@@ -1524,6 +1550,14 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
         M.nblk_s = (int)P.blk_s.size() - 1;
         c->nblkmax_s = std::max(c->nblkmax_s, M.nblk_s);
       }
+      M.nblk_w = 0;
+      M.blk0_w = M.blk0;
+      if (c->wide_ok) {
+        M.blk0_w = (int)blk_row.size();
+        blk_row.insert(blk_row.end(), P.blk_w.begin(), P.blk_w.end());
+        M.nblk_w = (int)P.blk_w.size() - 1;
+        c->nblkmax_w = std::max(c->nblkmax_w, M.nblk_w);
+      }
       flops += P.flops;
     }
   }
@@ -1532,6 +1566,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
   if (fw::solve_lds_bytes(nmax, ngmax, c->nblkmax, fw::PFK, 1) > 160 * 1024)
     return fail(FWH_E_LIMIT, "subdomain of %d DOFs / %d interface DOFs exceeds the LDS budget", nmax, ngmax);
   if (c->short_ok && fw::solve_lds_bytes(nmax, ngmax, c->nblkmax_s, 8, 1) > 160 * 1024) c->short_ok = false;
+  if (c->wide_ok && fw::solve_lds_bytes(nmax, ngmax, c->nblkmax_w, 32, 4) > 160 * 1024) c->wide_ok = false;
   c->pfk_iter = fw::PFK;
   c->pfk_unit = fw::PFK;  // both picked after the factorisation, by timing
   if ((size_t)(wmax + 1 + nmax) * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "profile too wide for the factor kernel");
@@ -1568,13 +1603,14 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     for (int i = 0; i < nsub; i++) {
       const int s = bo[i];
       if (c->buckets.empty() || klass[bo[c->buckets.back().first]] != klass[s])
-        c->buckets.push_back({i, 0, 0, 0, 0, 0});
+        c->buckets.push_back({i, 0, 0, 0, 0, 0, 0});
       fwh_ctx::Bucket &B = c->buckets.back();
       B.count++;
       B.nmax = std::max(B.nmax, meta[s].n);
       B.ngmax = std::max(B.ngmax, meta[s].ng);
       B.nblkmax = std::max(B.nblkmax, meta[s].nblk);
       B.nblkmax_s = std::max(B.nblkmax_s, meta[s].nblk_s);
+      B.nblkmax_w = std::max(B.nblkmax_w, meta[s].nblk_w);
     }
     if ((rc = c->upload(&c->border, bo))) return rc;
     if (c->buckets.size() > 1) {
@@ -1705,6 +1741,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     I.solve_persist = 0;
     I.ms_solve_persist[0] = I.ms_solve_persist[1] = -1.0;
     for (int k = 0; k < 3; k++) I.ms_solve_waves[k] = -1.0;
+    I.ms_solve_wide = -1.0;
     if ((rc = plan_refactor(c, c->factor_budget))) return rc;
     c->fac_resident = false;
   } else {
@@ -1712,24 +1749,26 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     const int wv[3] = {1, 2, 4};
-    double best[2][3][2];  // [depth 16, 8][waves 1, 2, 4][single, bucketed]; 1e30: not run
+    const int dv[3] = {fw::PFK, 8, 32};
+    double best[3][3][2];  // [depth 16, 8, 32][waves 1, 2, 4][single, bucketed]; 1e30: not run
     for (auto &x : best)
       for (auto &y : x) y[0] = y[1] = 1e30;
-    const int depths = c->short_ok ? 2 : 1;
     const int variants = c->buckets.size() > 1 ? 2 : 1;
-    // every launch of the form must fit a CU's LDS
+    // every launch of the form must fit a CU's LDS; depth 8 needs the short
+    // blocks, depth 32 the wide ones and four waves
     auto fits = [&](int d, int w, int v) -> bool {
-      const int pfk = d == 0 ? fw::PFK : 8;
-      if (v == 0) return fw::solve_lds_bytes(nmax, ngmax, d == 0 ? c->nblkmax : c->nblkmax_s, pfk, wv[w]) <= 160 * 1024;
+      if ((d == 1 && !c->short_ok) || (d == 2 && (!c->wide_ok || w != 2))) return false;
+      const int pfk = dv[d];
+      if (v == 0) return fw::solve_lds_bytes(nmax, ngmax, c->nblkmax_for(pfk), pfk, wv[w]) <= 160 * 1024;
       for (const auto &B : c->buckets)
-        if (fw::solve_lds_bytes(B.nmax, B.ngmax, d == 0 ? B.nblkmax : B.nblkmax_s, pfk, wv[w]) > 160 * 1024) return false;
+        if (fw::solve_lds_bytes(B.nmax, B.ngmax, B.nblkmax_for(pfk), pfk, wv[w]) > 160 * 1024) return false;
       return true;
     };
-    for (int d = 0; d < depths; d++)
+    for (int d = 0; d < 3; d++)
       for (int w = 0; w < 3; w++)
         for (int v = 0; v < variants; v++) {
           if (!fits(d, w, v)) continue;
-          c->pfk_iter = d == 0 ? fw::PFK : 8;
+          c->pfk_iter = dv[d];
           c->waves_iter = wv[w];
           fw::SolveArgs a = c->solve_args(fw::MODE_RHS, false);
           // a timed warm-up; three more repetitions only for short launches
@@ -1755,7 +1794,7 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     (void)hipEventDestroy(e1);
     auto tm = [&](int d, int w) { return std::min(best[d][w][0], best[d][w][1]); };
     int d = 0, w = 0;
-    const int cand[5][2] = {{1, 0}, {0, 1}, {1, 1}, {0, 2}, {1, 2}};
+    const int cand[6][2] = {{1, 0}, {0, 1}, {1, 1}, {0, 2}, {1, 2}, {2, 2}};
     for (const auto &cw : cand)
       if (tm(cw[0], cw[1]) < 0.95 * tm(d, w)) {
         d = cw[0];
@@ -1764,30 +1803,42 @@ int build_model(fwh_ctx *c, const fwh_mesh_desc *descs, int nsys) {
     // the Schur formation runs hundreds of waves per CU, where the 8-deep
     // kernel's higher occupancy pays unless its single solve is much slower
     // (wide profiles cut into many short blocks: cfg 3's large subdomains)
-    c->pfk_unit = (depths == 2 && tm(1, 0) <= 1.2 * tm(0, 0)) ? 8 : fw::PFK;
-    // FWH_SOLVE_DEPTH (8 or 16) forces the depth of every launch kind and
-    // FWH_SOLVE_WAVES (1, 2 or 4) the waves outside the formation (tests of
-    // each form, experiments), where the form was timed (fits)
+    c->pfk_unit = (c->short_ok && tm(1, 0) <= 1.2 * tm(0, 0)) ? 8 : fw::PFK;
+    // FWH_SOLVE_DEPTH (8 or 16) forces the depth of every launch kind, 32
+    // that of the iteration sweeps (with four waves), and FWH_SOLVE_WAVES
+    // (1, 2 or 4) the waves outside the formation (tests of each form,
+    // experiments), where the form was timed (fits)
     if (const char *e = getenv("FWH_SOLVE_DEPTH")) {
       const int v = atoi(e);
       if (v == fw::PFK || (v == 8 && c->short_ok)) {
         c->pfk_unit = v;
         d = v == fw::PFK ? 0 : 1;
+      } else if (v == 32 && tm(2, 2) < 1e30) {
+        d = 2;
+        w = 2;
       }
     }
     if (const char *e = getenv("FWH_SOLVE_WAVES")) {
       const int v = atoi(e);
+      const char *ed = getenv("FWH_SOLVE_DEPTH");
+      const bool wide_forced = ed && atoi(ed) == 32;
       for (int k = 0; k < 3; k++)
-        if (wv[k] == v) w = k;
+        if (wv[k] == v && (d != 2 || k == 2 || !wide_forced)) {
+          w = k;
+          // the wide blocks were the pick, not forced: the faster of the
+          // other two depths with the forced waves
+          if (d == 2 && k != 2) d = tm(1, k) < tm(0, k) ? 1 : 0;
+        }
     }
-    if (tm(d, w) >= 1e30) w = 0;  // the forced form does not fit: one wave
-    c->pfk_iter = d == 0 ? fw::PFK : 8;
+    if (tm(d, w) >= 1e30) d = w = 0;  // the forced form does not fit: depth 16, one wave
+    c->pfk_iter = dv[d];
     c->waves_iter = wv[w];
     c->ms_single = best[d][w][0];
     c->ms_bucketed = variants == 2 ? best[d][w][1] : 0.0;
     c->use_buckets = variants == 2 && best[d][w][1] < best[d][w][0];
-    I.ms_solve_depth[0] = depths == 2 ? tm(1, w) : 0.0;
+    I.ms_solve_depth[0] = c->short_ok ? tm(1, w) : 0.0;
     I.ms_solve_depth[1] = tm(0, w);
+    I.ms_solve_wide = tm(2, 2) < 1e30 ? tm(2, 2) : -1.0;
     for (int k = 0; k < 3; k++) I.ms_solve_waves[k] = tm(d, k) < 1e30 ? tm(d, k) : -1.0;
     {
       int ncu = 0;
@@ -2453,7 +2504,7 @@ int build_schur(fwh_ctx *c) {
         ab.base = (int)ufirst[b];
         ab.nmax = c->buckets[b].nmax;
         ab.ngmax = c->buckets[b].ngmax;
-        ab.nblkmax = a.pfk == 8 ? c->buckets[b].nblkmax_s : c->buckets[b].nblkmax;
+        ab.nblkmax = c->buckets[b].nblkmax_for(a.pfk);
         HIPCHK(hipStreamWaitEvent(c->bst[b], c->ev_fork, 0));
         HIPCHK(fw::launch_solve(ab, (int)(ufirst[b + 1] - ufirst[b]), c->bst[b]));
         HIPCHK(hipEventRecord(c->ev_join[b], c->bst[b]));

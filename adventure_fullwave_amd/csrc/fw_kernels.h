@@ -21,6 +21,8 @@ struct SubMeta {
   int nnz;        // entries
   int nblk_s;     // short row blocks (<= 64*8 entries; 0 when a row is wider)
   int blk0_s;     // offset of the short blocks in blk_row
+  int nblk_w;     // wide row blocks (<= 64*32 entries, for four waves; 0 when not built)
+  int blk0_w;     // offset of the wide blocks in blk_row
 };
 
 // MODE_UNIT: the Schur step applied to interface unit vector unit_k, output
@@ -98,7 +100,7 @@ struct SolveArgs {
   const int *done;       // iteration mode: skip when *done != 0 (may be null)
   int nmax, ngmax;       // LDS sizing
   int nblkmax;
-  int pfk;               // prefetch loads per lane (8 or 16): blocks of <= 64*pfk entries
+  int pfk;               // prefetch loads per lane (8 or 16; 32 with four waves): blocks of <= 64*pfk entries
   int waves;             // waves per subdomain: 1, or 4 (k_solve4, the same blocks)
   const int *idx1;       // per DOF, local first column of the profile
   const int *rowbeg;     // per DOF, start of row i within the subdomain's skyline

@@ -684,8 +684,8 @@ __device__ __forceinline__ void solve4_body(const SolveArgs &a, const SubMeta &m
   const int t = threadIdx.x;
   const int n = m.n;
   const double2 *fac = a.fac + m.fac0;
-  const int nblk = CAPQ == 512 ? m.nblk_s : m.nblk;
-  const int blk0 = CAPQ == 512 ? m.blk0_s : m.blk0;
+  const int nblk = CAPQ == 512 ? m.nblk_s : CAPQ == 2048 ? m.nblk_w : m.nblk;
+  const int blk0 = CAPQ == 512 ? m.blk0_s : CAPQ == 2048 ? m.blk0_w : m.blk0;
   L.minv = a.minv + (long long)blk0 * MINV;
 
   load_meta<NT>(a, m, L, nblk, blk0);
@@ -727,8 +727,12 @@ __device__ __forceinline__ void solve4_body(const SolveArgs &a, const SubMeta &m
   }
 }
 
+// (four waves hold two prefetch sets of KQ loads per thread: the 1,024-entry
+// blocks need more than 128 VGPRs, so three workgroups per CU, the wide
+// blocks two, which is what their LDS allows anyway)
 template <int KQ, int NT>  // prefetch loads per thread; threads: row blocks of at most NT*KQ entries
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 128 ? 3 : 4))) void k_solve4(SolveArgs a) {
+__global__ __launch_bounds__(NT)
+__attribute__((amdgpu_waves_per_eu(NT == 256 ? (KQ == 8 ? 2 : KQ == 4 ? 3 : 4) : 3))) void k_solve4(SolveArgs a) {
   to_system(a);
   const bool check_done = a.mode == MODE_ITER && a.done;
   const int done0 = check_done ? *a.done : 0;
@@ -803,14 +807,16 @@ k_solveq(SolveArgs a) {
 // lower entries inside the block, unit diagonal), packed by rows: entry
 // (r, q), q < r, at r(r-1)/2 + q, the rest of the MINV slots zero.
 // (L^-1)_rq = -(L_rq + sum_{q<j<r} L_rj (L^-1)_jq).  One workgroup per
-// subdomain, one thread per block, both row-block partitions.
+// subdomain, one thread per block, every row-block partition (long, short,
+// wide).
 __global__ __launch_bounds__(64) void k_blockinv(const SubMeta *__restrict__ meta, const int *__restrict__ idx1,
                                                  const int *__restrict__ rowbeg, const int *__restrict__ blk_row,
                                                  const double2 *__restrict__ fac_all, double2 *__restrict__ minv) {
   const SubMeta m = meta[blockIdx.x];
   const double2 *fac = fac_all + m.fac0;
-  for (int part = 0; part < 2; part++) {
-    const int nblk = part ? m.nblk_s : m.nblk, blk0 = part ? m.blk0_s : m.blk0;
+  for (int part = 0; part < 3; part++) {
+    const int nblk = part == 0 ? m.nblk : part == 1 ? m.nblk_s : m.nblk_w;
+    const int blk0 = part == 0 ? m.blk0 : part == 1 ? m.blk0_s : m.blk0_w;
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
       const int i0 = blk_row[blk0 + b], nb = blk_row[blk0 + b + 1] - i0;
       double2 M[MINV];
@@ -906,9 +912,12 @@ hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
     if (a.waves == 4) {
       if (a.pfk == 8)
         k_solve4<2, 256><<<grid, 256, lds, st>>>(a);
+      else if (a.pfk == 32)
+        k_solve4<8, 256><<<grid, 256, lds, st>>>(a);  // the wide blocks (<= 2,048 entries)
       else
         k_solve4<4, 256><<<grid, 256, lds, st>>>(a);
     } else {
+      if (a.pfk == 32) return hipErrorInvalidValue;  // wide blocks only with four waves
       if (a.pfk == 8)
         k_solve4<4, 128><<<grid, 128, lds, st>>>(a);
       else
@@ -917,6 +926,7 @@ hipError_t launch_solve(const SolveArgs &a, int nwork, hipStream_t st) {
     return hipGetLastError();
   }
   const size_t lds = solve_lds_bytes(a.nmax, a.ngmax, a.nblkmax, a.pfk, 1);
+  if (a.pfk == 32) return hipErrorInvalidValue;
   if (a.pfk == 8)
     k_solve<8><<<grid, WL, lds, st>>>(a);
   else

@@ -77,7 +77,8 @@ class FwhInfo(ctypes.Structure):
                 ("solve_nlarge", ctypes.c_int), ("ms_solve_persist", ctypes.c_double * 2),
                 ("factors_resident", ctypes.c_int), ("factor_chunks", ctypes.c_int),
                 ("factor_bytes_released", ctypes.c_double), ("factor_chunk_bytes", ctypes.c_double),
-                ("factor_need_max", ctypes.c_double), ("factor_need_total", ctypes.c_double)]
+                ("factor_need_max", ctypes.c_double), ("factor_need_total", ctypes.c_double),
+                ("ms_solve_wide", ctypes.c_double)]
 
     def as_dict(self):
         out = {}

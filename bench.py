@@ -135,6 +135,7 @@ def kernel_extras(args, fwhip, mesh):
                           "solve_depth": info["solve_depth"], "solve_waves": info["solve_waves"],
                           "solve_persist": info["solve_persist"], "solve_large_queue": info["solve_nlarge"],
                           "ms_static_by_waves_1_2_4": [round(v, 4) for v in info["ms_solve_waves"]],
+                          "ms_wide_depth_32": round(info["ms_solve_wide"], 4),
                           "ms_persistent_forms_1_2": [round(v, 4) for v in info["ms_solve_persist"]]},
         }
         log(f"[bench] extras cfg{c}: k_smv {ms_smv:.4f} ms, k_solve {ms_schur:.4f} ms, LDL^T {ms_solve:.4f} ms "
@@ -410,6 +411,7 @@ def main():
                      "solve_waves": info["solve_waves"],
                      "ms_solve_by_waves_1_2_4": [round(v, 4) for v in info["ms_solve_waves"]],
                      "ms_solve_by_depth_8_16": [round(v, 4) for v in info["ms_solve_depth"]],
+                     "ms_solve_wide_depth_32": round(info["ms_solve_wide"], 4),
                      "smv_order": info["smv_order"],
                      "ms_smv_by_order": [round(v, 4) for v in info["ms_smv_order"]],
                      "smv_split": info["smv_split"], "smv_items": info["smv_nwork"],

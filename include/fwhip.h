@@ -167,6 +167,9 @@ typedef struct fwh_info {
                            (factor entries, block inverses, the factor
                            kernel's temporaries): the smallest budget      */
   double factor_need_total; /* the same summed over all subdomains         */
+  double ms_solve_wide; /* best batched solve time of the four-wave form
+                           over the wide row blocks (<= 2,048 entries,
+                           depth 32; -1: the blocks are not built)         */
 } fwh_info;
 
 /* ---- context ------------------------------------------------------------ */

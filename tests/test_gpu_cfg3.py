@@ -115,20 +115,24 @@ def test_cfg3_residuals_and_global_residual(cfg3, cfg3_om, cfg3_gpu):
     assert d <= FORMS_BAR
 
 
-@pytest.mark.parametrize("waves", [1, 2, 4])
-def test_cfg3_solve_forms(cfg3, cfg3_om, waves):
+@pytest.mark.parametrize("depth,waves", [(16, 1), (16, 2), (16, 4), (32, 4)])
+def test_cfg3_solve_forms(cfg3, cfg3_om, depth, waves):
     """Every form of the batched LDL^T kernel (one, two or four waves per
-    subdomain; forced with FWH_SOLVE_WAVES) on cfg 3, whose large subdomains have
-    blocks of one to four wide rows: the implicit Schur step against the
-    oracle (1e-10 relative) and the largest subdomain's LDL^T solve against
-    the oracle's (1e-10 relative)."""
+    subdomain over row blocks of up to 1,024 entries, or four waves over
+    the wide row blocks of up to 2,048; forced with FWH_SOLVE_WAVES and
+    FWH_SOLVE_DEPTH) on cfg 3,
+    whose large subdomains have blocks of one to four wide rows: the
+    implicit Schur step against the oracle (1e-10 relative) and the largest
+    subdomain's LDL^T solve against the oracle's (1e-10 relative)."""
     from adventure_fullwave_amd import fwhip
     from tests.helpers import pinned_env
-    with pinned_env({"FWH_SOLVE_WAVES": str(waves)}):
+    with pinned_env({"FWH_SOLVE_WAVES": str(waves), "FWH_SOLVE_DEPTH": str(depth)}):
         sol = fwhip.Solver(cfg3)
     try:
-        if sol.info["solve_waves"] != waves:
-            pytest.skip(f"{waves} waves do not fit this model's LDS")
+        say(f"[gpu] cfg3 form depth={sol.info['solve_depth']} waves={sol.info['solve_waves']} "
+            f"wide {sol.info['ms_solve_wide']:.3f} ms, by waves {list(sol.info['ms_solve_waves'])} ms")
+        if sol.info["solve_waves"] != waves or sol.info["solve_depth"] != depth:
+            pytest.skip(f"depth {depth} with {waves} waves does not fit this model's LDS")
         gids = iface_gids(cfg3)
         ng = int(sol.info["n_iface"])
         rng = np.random.default_rng(33)
@@ -136,7 +140,7 @@ def test_cfg3_solve_forms(cfg3, cfg3_om, waves):
         ref = oracle_q(cfg3, cfg3_om, gids, p)
         q = sol.schur_apply(p)
         err = np.abs(q - ref).max() / np.abs(ref).max()
-        say(f"[gpu] cfg3 implicit Schur step, {waves} wave(s): max rel. error {err:.2e}")
+        say(f"[gpu] cfg3 implicit Schur step, {waves} wave(s), depth {depth}: max rel. error {err:.2e}")
         assert err <= 1e-10
         big = int(np.argmax(cfg3.nmp))
         off = int(np.sum(cfg3.nmp[:big]))

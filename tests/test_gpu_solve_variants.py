@@ -2,7 +2,8 @@
 
 The load-time tuner picks the prefetch depth (8 or 16 loads per lane, with
 a row-block partition for each) and the waves per subdomain (1, 2 or 4,
-with the same row blocks) by timing, so a given run may exercise only one form
+with the same row blocks; four waves also over wide blocks of up to 2,048
+entries, depth 32) by timing, so a given run may exercise only one form
 in the iteration sweeps.  Here each form is forced (FWH_SOLVE_DEPTH for
 every launch kind, FWH_SOLVE_WAVES outside the formation, read at load) and
 checked against the oracle: the subdomain LDL^T solves (1e-10 relative; one
@@ -31,14 +32,16 @@ def sigma_ref(small_sigma_model):
     return fo.hddm_solve(small_sigma_model, eps=1e-8, max_loop=5000)
 
 
-@pytest.mark.parametrize("depth,waves", [(16, 1), (8, 1), (16, 2), (8, 2), (16, 4), (8, 4)])
+@pytest.mark.parametrize("depth,waves", [(16, 1), (8, 1), (16, 2), (8, 2), (16, 4), (8, 4), (32, 4)])
 def test_solve_variant(small_sigma_model, sigma_ref, depth, waves):
     from adventure_fullwave_amd import fwhip
     model = small_sigma_model
     say(f"[gpu] LDL^T variant depth={depth} waves={waves}")
     with pinned_env({"FWH_SOLVE_DEPTH": str(depth), "FWH_SOLVE_WAVES": str(waves)}):
         sol = fwhip.Solver(model)
-    if sol.info["solve_depth"] != depth or sol.info["unit_depth"] != depth or sol.info["solve_waves"] != waves:
+    # (depth 32, the wide blocks, is a form of the iteration sweeps only)
+    unit_ok = sol.info["unit_depth"] in (8, 16) if depth == 32 else sol.info["unit_depth"] == depth
+    if sol.info["solve_depth"] != depth or not unit_ok or sol.info["solve_waves"] != waves:
         sol.close()
         pytest.skip(f"depth {depth} with {waves} waves not available on this model")
     om = fo.OracleModel(model)
