@@ -3417,9 +3417,11 @@ int sky_one(fwh_ctx *c, int n, const int *idx1, const int *idx2, double *mat, do
 // diagonal last (aij), values vij (2 doubles each); x is the initial guess
 // in and the solution out.  *status: the reference's return value (>= 0
 // iterations, SOLVER_STATUS_* < 0).
-extern "C" int fwh_aij_cocg(fwh_ctx *c, int nf, const int *ai, const int *aij, const double *vij, const double *b,
-                            double *x, int pc, double eps, double div, double ic_shift, int *status) {
+extern "C" int fwh_aij_cocg_pc(fwh_ctx *c, int nf, const int *ai, const int *aij, const double *vij, const double *b,
+                               double *x, int pc, double eps, double div, double ic_shift, double *pcm, int make,
+                               int *status) {
   if (!c || !ai || !aij || !vij || !b || !x || !status) return fail(FWH_E_ARG, "null argument");
+  if (pc == 3 && !make && !pcm) return fail(FWH_E_ARG, "no preconditioner matrix to reuse (pcm is null)");
   if (nf <= 0) return fail(FWH_E_ARG, "nf must be positive");
   if (pc != 0 && pc != 3) return fail(FWH_E_ARG, "preconditioner %d: only none (0) and ICC (3) run on the GPU", pc);
   if ((size_t)5 * nf * 16 > 160 * 1024) return fail(FWH_E_LIMIT, "system of %d unknowns too large for the COCG kernel", nf);
@@ -3491,6 +3493,10 @@ extern "C" int fwh_aij_cocg(fwh_ctx *c, int nf, const int *ai, const int *aij, c
   if (pc == 0) {
     HIPCHK(up_(d_L, Lid.data(), sizeof(double2) * nnz));
     HIPCHK(up_(d_dg, dgid.data(), sizeof(double2) * nf));
+  } else if (!make) {
+    // the caller's factor: L in the matrix's pattern, then d
+    HIPCHK(up_(d_L, pcm, sizeof(double2) * nnz));
+    HIPCHK(up_(d_dg, pcm + 2 * (size_t)nnz, sizeof(double2) * nf));
   }
   fw::IccArgs a{};
   a.meta = d_meta;
@@ -3510,12 +3516,23 @@ extern "C" int fwh_aij_cocg(fwh_ctx *c, int nf, const int *ai, const int *aij, c
   a.eps = eps;
   a.div = div;
   a.nmax = nf;
-  if (pc == 3) HIPCHK(fw::launch_icc_make(a, 1, c->st));
+  if (pc == 3 && make) {
+    HIPCHK(fw::launch_icc_make(a, 1, c->st));
+    if (pcm) {
+      HIPCHK(hipMemcpyAsync(pcm, d_L, sizeof(double2) * nnz, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(pcm + 2 * (size_t)nnz, d_dg, sizeof(double2) * nf, hipMemcpyDeviceToHost, c->st));
+    }
+  }
   HIPCHK(fw::launch_cocg(a, 1, c->st));
   HIPCHK(hipMemcpyAsync(x, d_x, sizeof(double2) * nf, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(status, d_st, sizeof(int), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   return 0;
+}
+
+extern "C" int fwh_aij_cocg(fwh_ctx *c, int nf, const int *ai, const int *aij, const double *vij, const double *b,
+                            double *x, int pc, double eps, double div, double ic_shift, int *status) {
+  return fwh_aij_cocg_pc(c, nf, ai, aij, vij, b, x, pc, eps, div, ic_shift, nullptr, 1, status);
 }
 
 extern "C" int fwh_sky_factor(fwh_ctx *c, int n, const int *idx1, const int *idx2, double *mat, double *dinv) {

@@ -416,21 +416,73 @@ void advlas_skyline_ldl_solve(DMatrix dmat, double _Complex *solution) {
 /* ---- the subdomain COCG of mymt_aij_solver on the GPU ------------------ */
 /* mymt_CS_AIJ_Solver_COCG (mymt_aij_cs.h:64, aij_solver_cg.h:9-44): the
  * preconditioner switch, then COCG on aij_mat->x_c from aij_mat->b_c.  ICC
- * and none run on the GPU (fwh_aij_cocg); MRIC0 and the others return
- * SOLVER_STATUS_UNKNOWN_PC, as the reference's switch does for a PC it does
- * not know.  The monitor log is not written. */
-int mymt_CS_AIJ_Solver_COCG(AIJ_MAT *aij_mat, SolvOpt solvopt, FILE *monitor) {
-  (void)monitor;
-  if (solvopt.solver_pc != REF_SOLVER_PC_NONE && solvopt.solver_pc != REF_SOLVER_PC_ICC)
-    return REF_SOLVER_STATUS_UNKNOWN_PC;
+ * and none run on the GPU (fwh_aij_cocg_pc); the IC(0) factor is left in
+ * aij_mat->pc_c for the _nmpcm form, as the reference's make step leaves
+ * it; MRIC0 and the others return SOLVER_STATUS_UNKNOWN_PC, as the
+ * reference's switch does for a PC it does not know.  The monitor log is
+ * not written. */
+/* the reference's counted allocator (lib/std/mem.c:86) when the library is
+ * linked into the reference, else malloc: the IC(0) factor left in
+ * aij_mat->pc_c is released by mymt_CS_AIJ_Free (aij_matrix_func.h:253-255)
+ * with the reference's Free, i.e. free() */
+extern void *__MemoryAllocate__(size_t mem, char *name, char *fname, int line) __attribute__((weak));
+
+static Complex *pc_alloc(size_t count) {
+  const size_t bytes = sizeof(Complex) * count;
+  return (Complex *)(__MemoryAllocate__ ? __MemoryAllocate__(bytes, NULL, (char *)__FILE__, __LINE__)
+                                        : malloc(bytes));
+}
+
+static int aij_cocg(AIJ_MAT *aij_mat, SolvOpt solvopt, int make, const char *who) {
+  const int nf = aij_mat->nf;
+  const int pc_size = aij_mat->ai[nf] + nf;
+  double *pcm = NULL;
+  if (solvopt.solver_pc == REF_SOLVER_PC_ICC) {
+    if (make) {
+      /* PC_ICC make (aij_pc_func.h:52-58): a factor of ai[nf] + nf
+       * entries in aij_mat->pc_c (a buffer of that size is reused) */
+      if (!aij_mat->pc_c || aij_mat->pc_size != pc_size) {
+        aij_mat->pc_c = pc_alloc((size_t)pc_size);
+        if (!aij_mat->pc_c) {
+          fprintf(stdout, "ERROR: %s: out of host memory for the preconditioner\n", who);
+          fflush(stdout);
+          exit(1);
+        }
+      }
+      aij_mat->pc_size = pc_size;
+    } else if (!aij_mat->pc_c || aij_mat->pc_size != pc_size) {
+      fprintf(stdout, "ERROR: %s: no preconditioner matrix of %d entries in aij_mat\n", who, pc_size);
+      fflush(stdout);
+      exit(1);
+    }
+    pcm = (double *)aij_mat->pc_c;
+  }
   int status = 0;
-  const int rc = fwh_aij_cocg(sky_ctx(), aij_mat->nf, aij_mat->ai, aij_mat->aij, (const double *)aij_mat->vij_c,
-                              (const double *)aij_mat->b_c, (double *)aij_mat->x_c, solvopt.solver_pc,
-                              solvopt.solver_eps, solvopt.solver_div, solvopt.solver_ic_shift, &status);
+  const int rc = fwh_aij_cocg_pc(sky_ctx(), nf, aij_mat->ai, aij_mat->aij, (const double *)aij_mat->vij_c,
+                                 (const double *)aij_mat->b_c, (double *)aij_mat->x_c, solvopt.solver_pc,
+                                 solvopt.solver_eps, solvopt.solver_div, solvopt.solver_ic_shift, pcm, make, &status);
   if (rc) {
-    fprintf(stdout, "ERROR: mymt_CS_AIJ_Solver_COCG (%d): %s\n", rc, fwh_last_error());
+    fprintf(stdout, "ERROR: %s (%d): %s\n", who, rc, fwh_last_error());
     fflush(stdout);
     exit(1);
   }
   return status;
+}
+
+int mymt_CS_AIJ_Solver_COCG(AIJ_MAT *aij_mat, SolvOpt solvopt, FILE *monitor) {
+  (void)monitor;
+  if (solvopt.solver_pc != REF_SOLVER_PC_NONE && solvopt.solver_pc != REF_SOLVER_PC_ICC)
+    return REF_SOLVER_STATUS_UNKNOWN_PC;
+  return aij_cocg(aij_mat, solvopt, 1, "mymt_CS_AIJ_Solver_COCG");
+}
+
+/* mymt_CS_AIJ_Solver_COCG_nmpcm (mymt_aij_cs.h:67, aij_solver_cg.h:68-92):
+ * the same COCG with the preconditioner the make step left in aij_mat
+ * (pc_c, pc_size) for SOLVER_PC_ICC, none for SOLVER_PC_NONE,
+ * SOLVER_STATUS_UNKNOWN_PC for the others. */
+int mymt_CS_AIJ_Solver_COCG_nmpcm(AIJ_MAT *aij_mat, SolvOpt solvopt, FILE *monitor) {
+  (void)monitor;
+  if (solvopt.solver_pc != REF_SOLVER_PC_NONE && solvopt.solver_pc != REF_SOLVER_PC_ICC)
+    return REF_SOLVER_STATUS_UNKNOWN_PC;
+  return aij_cocg(aij_mat, solvopt, 0, "mymt_CS_AIJ_Solver_COCG_nmpcm");
 }

@@ -28,11 +28,12 @@ EXPORTS = ("fwh_create", "fwh_destroy", "fwh_last_error", "fwh_load", "fwh_get_i
            "fwh_get_resid", "fwh_schur_apply", "fwh_ldl_solve_all", "fwh_time_kernels", "fwh_get_sub",
            "fwh_element_batch", "fwh_cocg_icc_all", "fwh_schur_form", "fwh_post", "fwh_set_load",
            "fwh_grad_eps", "fwh_load_batch", "fwh_select", "fwh_element_batch_static", "fwh_sky_factor",
-           "fwh_sky_solve", "fwh_release_factors", "fwh_set_factor_budget", "fwh_aij_cocg")
+           "fwh_sky_solve", "fwh_release_factors", "fwh_set_factor_budget", "fwh_aij_cocg",
+           "fwh_aij_cocg_pc")
 # the reference-shaped call surface (include/fwh_ref.h)
 REF_EXPORTS = ("hddm_solver_CS_hip", "hddm_solver_CS_hip_status", "fwh_ref_desc_from_partmesh",
                "fwh_ref_desc_free", "advlas_skyline_ldl_decomposite", "advlas_skyline_ldl_solve",
-               "mymt_CS_AIJ_Solver_COCG")
+               "mymt_CS_AIJ_Solver_COCG", "mymt_CS_AIJ_Solver_COCG_nmpcm")
 
 
 class FwhOpts(ctypes.Structure):

@@ -438,5 +438,15 @@ void advlas_skyline_ldl_solve(DMatrix dmat, double _Complex *solution);
  * (>= 0 iterations, SOLVER_STATUS_* < 0; SOLVER_STATUS_UNKNOWN_PC for the
  * other preconditioners).  At most 2,048 unknowns.                         */
 int mymt_CS_AIJ_Solver_COCG(AIJ_MAT *aij_mat, SolvOpt solvopt, FILE *monitor);
+/* With SOLVER_PC_ICC the IC(0) factor is left in aij_mat->pc_c (pc_size =
+ * ai[nf] + nf entries: L in the matrix's pattern, then d = 1/diag(L)), as
+ * the reference's make step (aij_pc_func.h:52-58) leaves it; allocated by
+ * the reference's counted allocator when the library is linked into it,
+ * else by malloc, and released by mymt_CS_AIJ_Free.
+ *
+ * mymt_CS_AIJ_Solver_COCG_nmpcm (mymt_aij_cs.h:67; aij_solver_cg.h:68-92):
+ * the same solve reusing that factor (no make step), as the reference's
+ * iteration steps call it through HDDM_FUNC.dom_solver_nmpcm.              */
+int mymt_CS_AIJ_Solver_COCG_nmpcm(AIJ_MAT *aij_mat, SolvOpt solvopt, FILE *monitor);
 
 #endif /* FWH_REF_H */

@@ -363,6 +363,17 @@ int  fwh_cocg_icc_all(fwh_ctx *ctx, const double *b, double *x, int *status,
 int  fwh_aij_cocg(fwh_ctx *ctx, int nf, const int *ai, const int *aij, const double *vij,
                   const double *b, double *x, int pc, double eps, double div, double ic_shift,
                   int *status);
+/* The same with the IC(0) factor in the reference's AIJ_PC layout
+ * (aij_pc_func.h:52-58: the factor L in the matrix's pattern, ai[nf]
+ * entries, then d = 1/diag(L), nf entries; 2 doubles each): make = 1
+ * computes it on the GPU and, when pcm is not null, writes it to pcm, as
+ * the PC_ICC make step of mymt_CS_AIJ_Solver_COCG leaves it in
+ * aij_mat->pc_c; make = 0 takes it from pcm and skips the make step, as
+ * mymt_CS_AIJ_Solver_COCG_nmpcm (aij_solver_cg.h:68-92) reuses it.
+ * pcm is not read or written with pc = SOLVER_PC_NONE.                   */
+int  fwh_aij_cocg_pc(fwh_ctx *ctx, int nf, const int *ai, const int *aij, const double *vij,
+                     const double *b, double *x, int pc, double eps, double div, double ic_shift,
+                     double *pcm, int make, int *status);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,9 @@ def lib():
         L.refdrv_advlas.restype = None
         L.refdrv_mymt_cocg.argtypes = [ctypes.c_int, P, P, P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.refdrv_mymt_cocg.restype = ctypes.c_int
+        L.refdrv_mymt_cocg_nmpcm.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, P, P, P]
+        L.refdrv_mymt_cocg_nmpcm.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -138,3 +141,24 @@ def mymt_cocg(rowptr, col, val, b, x0=None, pc=3, eps=1e-9, ic_shift=1.2):
     st = lib().refdrv_mymt_cocg(len(rp) - 1, rp.ctypes.data, cl.ctypes.data, vl.ctypes.data, bb.ctypes.data,
                                 x.ctypes.data, int(pc), float(eps), float(ic_shift))
     return st, x
+
+
+def mymt_cocg_nmpcm(rowptr, col, val, b1, b2, pc=3, eps=1e-9, ic_shift=1.2):
+    """mymt_CS_AIJ_Solver_COCG on b1, then mymt_CS_AIJ_Solver_COCG_nmpcm on
+    b2 with the preconditioner the first left in the AIJ_MAT (tests/refdrv.c):
+    (value 1, x1, value 2, x2, the preconditioner matrix or None)"""
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    cl = np.ascontiguousarray(col, np.int32)
+    vl = np.ascontiguousarray(val, np.complex128)
+    n = len(rp) - 1
+    bb1 = np.ascontiguousarray(b1, np.complex128)
+    bb2 = np.ascontiguousarray(b2, np.complex128)
+    x1 = np.zeros(n, np.complex128)
+    x2 = np.zeros(n, np.complex128)
+    pcm = np.zeros(int(rp[n]) + n, np.complex128)
+    size = ctypes.c_int(0)
+    st2 = ctypes.c_int(0)
+    st1 = lib().refdrv_mymt_cocg_nmpcm(n, rp.ctypes.data, cl.ctypes.data, vl.ctypes.data, bb1.ctypes.data,
+                                       x1.ctypes.data, bb2.ctypes.data, x2.ctypes.data, int(pc), float(eps),
+                                       float(ic_shift), pcm.ctypes.data, ctypes.byref(size), ctypes.byref(st2))
+    return st1, x1, st2.value, x2, (pcm if size.value else None)

@@ -224,3 +224,36 @@ int refdrv_mymt_cocg(int nf, int *ai, int *aij, double *vij, double *b, double *
   so.solver_max_loop = 10 * nf;
   return mymt_CS_AIJ_Solver_COCG(&am, so, NULL);
 }
+
+/* the reference's order of the two subdomain solver hooks
+ * (HDDM_FUNC.dom_solver, then dom_solver_nmpcm on the same AIJ_MAT with
+ * another right-hand side): mymt_CS_AIJ_Solver_COCG from b1 into x1, its
+ * preconditioner copied to pcm (pc_size entries, 0 when none was left),
+ * then mymt_CS_AIJ_Solver_COCG_nmpcm from b2 into x2; returns the first
+ * value, *st2 the second */
+int refdrv_mymt_cocg_nmpcm(int nf, int *ai, int *aij, double *vij, double *b1, double *x1, double *b2, double *x2,
+                           int pc, double eps, double ic_shift, double *pcm, int *pc_size, int *st2) {
+  AIJ_MAT am;
+  SolvOpt so;
+  memset(&am, 0, sizeof(am));
+  memset(&so, 0, sizeof(so));
+  am.nf = nf;
+  am.ai = ai;
+  am.aij = aij;
+  am.vij_c = (Complex *)vij;
+  am.b_c = (Complex *)b1;
+  am.x_c = (Complex *)x1;
+  so.solver_pc = pc;
+  so.solver_eps = eps;
+  so.solver_div = 1.0e+10;
+  so.solver_ic_shift = ic_shift;
+  so.solver_max_loop = 10 * nf;
+  const int st1 = mymt_CS_AIJ_Solver_COCG(&am, so, NULL);
+  *pc_size = am.pc_size;
+  if (am.pc_size > 0) memcpy(pcm, am.pc_c, sizeof(Complex) * (size_t)am.pc_size);
+  am.b_c = (Complex *)b2;
+  am.x_c = (Complex *)x2;
+  *st2 = mymt_CS_AIJ_Solver_COCG_nmpcm(&am, so, NULL);
+  free(am.pc_c); /* mymt_CS_AIJ_Free's Free of the factor */
+  return st1;
+}

@@ -167,3 +167,41 @@ def test_mymt_cocg_on_subdomain_matrices(small_sigma_model, pc):
         assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-7
     st, _ = refapi.mymt_cocg(rp, cl, vl, b, pc=4)
     assert st == -4
+
+
+@pytest.mark.parametrize("pc", [3, 0])
+def test_mymt_cocg_nmpcm_reuses_the_factor(small_sigma_model, pc):
+    """The two subdomain solver hooks in the reference's order
+    (HDDM_FUNC.dom_solver, then dom_solver_nmpcm on the same AIJ_MAT):
+    mymt_CS_AIJ_Solver_COCG leaves the IC(0) factor in aij_mat->pc_c in the
+    reference's layout (L in the matrix's pattern, then d = 1/diag(L)), which
+    matches the oracle's make step (fwo_icc_make, aij_pc_func.h:36-102) to
+    1e-10 relative, L and d each; mymt_CS_AIJ_Solver_COCG_nmpcm with it on
+    another right-hand side returns the same value and bits as a
+    make-and-solve call on that right-hand side (same factor, same kernel);
+    with no preconditioner nothing is left and the same holds."""
+    from tests.test_gpu_parity import dirichlet_csr
+    om = fo.OracleModel(small_sigma_model)
+    rng = np.random.default_rng(21)
+    for s in (0, small_sigma_model.nsub - 1):
+        rp, cl, vl = dirichlet_csr(small_sigma_model, om, s)
+        n = len(rp) - 1
+        b1 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        b2 = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        st1, x1, st2, x2, pcm = refapi.mymt_cocg_nmpcm(rp, cl, vl, b1, b2, pc=pc, eps=1e-9, ic_shift=1.2)
+        say(f"[gpu] mymt COCG then _nmpcm, pc={pc}, subdomain {s}: {st1} and {st2} iterations")
+        assert st1 > 0 and st2 > 0
+        if pc == 3:
+            ref = fo.icc_make(rp, cl, vl, ic_shift=1.2)
+            nnz = int(rp[n])
+            assert pcm is not None and len(pcm) == len(ref)
+            for part in (slice(0, nnz), slice(nnz, nnz + n)):
+                err = np.abs(pcm[part] - ref[part]).max() / np.abs(ref[part]).max()
+                say(f"[gpu]   IC(0) factor part vs oracle: {err:.2e}")
+                assert err <= 1e-10
+        else:
+            assert pcm is None
+        st_a, x_a = refapi.mymt_cocg(rp, cl, vl, b1, pc=pc, eps=1e-9, ic_shift=1.2)
+        st_b, x_b = refapi.mymt_cocg(rp, cl, vl, b2, pc=pc, eps=1e-9, ic_shift=1.2)
+        assert (st1, st2) == (st_a, st_b)
+        assert np.array_equal(x1, x_a) and np.array_equal(x2, x_b)
