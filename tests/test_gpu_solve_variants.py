@@ -75,6 +75,56 @@ def test_solve_variant(small_sigma_model, sigma_ref, depth, waves):
     sol.close()
 
 
+def test_wide_blocks_batch_and_released(small_sigma_model):
+    """The four-wave form over the wide row blocks (forced: FWH_SOLVE_DEPTH=32,
+    FWH_SOLVE_WAVES=4) where the systems of a batch and the chunked
+    recomputation address the blocks: a batch of two systems (two
+    frequencies) gives each system's single-solve bits, implicit and
+    explicit, and with the factors released in three chunks the explicit
+    solve and the right-hand-side solves (batched and one launch per
+    subdomain) give the resident bits.  The matvec's own load-time choices
+    are pinned too, for the whole test: they are read when the S_s are
+    formed, at the first explicit solve."""
+    import dataclasses
+    from adventure_fullwave_amd import fwhip
+    pins = {"FWH_SOLVE_DEPTH": "32", "FWH_SOLVE_WAVES": "4", "FWH_SOLVE_PERSIST": "0", "FWH_SMV_WAVES": "4",
+            "FWH_SMV_SPLIT": "0", "FWH_SMV_PERSIST": "0"}
+    m = small_sigma_model
+    systems = [dataclasses.replace(m, omega=0.95 * m.omega), m]
+    with pinned_env(pins):
+        b = fwhip.Solver(systems)
+        singles = [fwhip.Solver(x) for x in systems]
+        assert b.info["solve_depth"] == 32 and b.info["solve_waves"] == 4
+        for schur in (False, True):
+            b.solve(eps=1e-8, max_loop=5000, graph_iters=8, schur=schur)
+            stats = b.stats_all()
+            for k, s1 in enumerate(singles):
+                assert s1.info["solve_depth"] == 32
+                st1 = s1.solve(eps=1e-8, max_loop=5000, graph_iters=8, schur=schur)
+                b.select(k)
+                say(f"[gpu] wide blocks, batch system {k}, explicit={schur}: "
+                    f"{stats[k]['iters']} / {st1['iters']} iterations")
+                assert stats[k]["iters"] == st1["iters"] and st1["converged"] == 1
+                assert np.array_equal(b.x(), s1.x())
+        b.close()
+        a = singles[1]
+        r = fwhip.Solver(m)
+        r.schur_form()
+        budget = int(max(r.info["factor_need_max"], r.info["factor_need_total"] / 3))
+        r.release_factors(budget)
+        assert r.info["factors_resident"] == 0 and r.info["factor_chunks"] >= 2
+        st_a = a.solve(eps=1e-8, max_loop=5000, graph_iters=8, schur=True)
+        st_r = r.solve(eps=1e-8, max_loop=5000, graph_iters=8, schur=True)
+    say(f"[gpu] wide blocks, released in {r.info['factor_chunks']} chunks: {st_r['iters']} / {st_a['iters']} iterations")
+    assert st_a["iters"] == st_r["iters"] and np.array_equal(a.x(), r.x())
+    rhs = np.random.default_rng(9).standard_normal(int(a.info["n_dof"])) * (1 + 0.5j)
+    assert np.array_equal(a.ldl_solve_all(rhs), r.ldl_solve_all(rhs))
+    assert np.array_equal(a.ldl_solve_all(rhs, plain=True), r.ldl_solve_all(rhs, plain=True))
+    for s1 in singles:
+        s1.close()
+    r.close()
+
+
 def test_schur_formation_paths(small_sigma_model):
     """S_s formed on the matrix cores (A_GG - W^T D^-1 W, W by forward
     sweeps) and by the Schur step on every interface unit vector give the same
