@@ -100,10 +100,11 @@ def iface_gids(model):
             for s in range(model.nsub)]
 
 
-# load-time choices of the explicit matvec that set summation orders (waves
-# per workgroup, split of large subdomains): pinned where two contexts must
-# give the same bits
-MATVEC_PINS = {"FWH_SMV_WAVES": "4", "FWH_SMV_SPLIT": "2"}
+# load-time choices that set summation orders, pinned where two contexts must
+# give the same bits: the explicit matvec's waves per workgroup and split of
+# large subdomains, and the LDL^T form of the first and last steps (depth and
+# waves, which the load otherwise picks by timing)
+MATVEC_PINS = {"FWH_SMV_WAVES": "4", "FWH_SMV_SPLIT": "2", "FWH_SOLVE_DEPTH": "16", "FWH_SOLVE_WAVES": "1"}
 
 
 @contextlib.contextmanager
