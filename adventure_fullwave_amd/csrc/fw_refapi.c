@@ -13,11 +13,18 @@
  * reference's element kernel looks up per call (lib/fem/make_elem_em.c:
  * 74-115: volt.dat, permittivity, conductivity, antenna).
  *
+ * set_hddm_func_fullwave_hip fills the reference's hook table
+ * (lib/std/types.h:629-730) so that the reference's own driver loop runs the
+ * same batched GPU solve: the per-subdomain hooks queue, the iteration
+ * hooks run the work for all subdomains.
+ *
  * advlas_skyline_ldl_decomposite / _solve keep the reference's prototypes
  * (lib/hddm/advlas_skyline.h:11-12) and run on the GPU of a process-wide
- * context.
+ * context, as do mymt_CS_AIJ_Solver_COCG / _nmpcm (mymt_aij_cs.h:64-68).
  */
+#define _GNU_SOURCE /* RTLD_DEFAULT */
 #include <complex.h>
+#include <dlfcn.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -485,4 +492,326 @@ int mymt_CS_AIJ_Solver_COCG_nmpcm(AIJ_MAT *aij_mat, SolvOpt solvopt, FILE *monit
   if (solvopt.solver_pc != REF_SOLVER_PC_NONE && solvopt.solver_pc != REF_SOLVER_PC_ICC)
     return REF_SOLVER_STATUS_UNKNOWN_PC;
   return aij_cocg(aij_mat, solvopt, 0, "mymt_CS_AIJ_Solver_COCG_nmpcm");
+}
+
+/* ---- the hook table: set_hddm_func_fullwave_hip ------------------------ */
+#define REF_HDDM_SW_ITERATION 1 /* __HDDM_SW_ITERATION__, lib/hddm/libhddm.h:38 */
+#define REF_HDDM_SW_LAST 2      /* __HDDM_SW_LAST__, lib/hddm/libhddm.h:39 */
+#define REF_HDDM_PC_DIAG 1      /* HDDM_PC_DIAG, lib/std/defs.h */
+#define HOOK_GRAPH 16           /* interface iterations per hipGraph */
+
+/* the reference's own, when the library is linked into it: the solver
+ * state (lib/hddm/make_PartMesh.c:16), the full-wave hook table
+ * (main/fullwave/fullwave.c:23-146) and the counted Calloc (lib/std/mem.c) */
+extern HDDM_SW hddm_sw __attribute__((weak));
+extern void set_hddm_func_fullwave(HDDM_FUNC *hddm_func, OPTIONS options) __attribute__((weak));
+extern void *__ClearAllocate__(size_t mem, char *name, char *fname, int line) __attribute__((weak));
+
+/* host arrays the reference frees with its Free (free()) */
+static void *ref_calloc(size_t n, size_t size) {
+  if (n == 0) n = 1;
+  if (__ClearAllocate__) return __ClearAllocate__(n * size, NULL, (char *)__FILE__, __LINE__);
+  return calloc(n, size);
+}
+
+/* one model per process, from general_func_1st to general_func_5th */
+static struct {
+  fwh_ctx *ctx;
+  fwh_mesh_desc d;
+  int have_desc;
+  PartMesh *pm;   /* the parts general_func_1st was given */
+  int npart;
+  int n_iface;
+  long *xoff;     /* per part: its first DOF in the flat solution (npart + 1) */
+  long *eoff;     /* per part: its first element (npart + 1) */
+  fwh_opts o;
+  int calls;      /* iteration_s calls since iteration_s_1st */
+  int run;        /* interface iterations launched on the GPU */
+  int done;       /* the GPU's status is LAST ... */
+  int k_stop;     /* ... since the iteration of this loop */
+  double *hist;   /* residual history read back so far (hist_n entries) */
+  int hist_n;
+  double *x, *B, *E, *J; /* the results, read back once */
+} H;
+
+static void hooks_release(void) {
+  if (H.ctx) fwh_destroy(H.ctx);
+  if (H.have_desc) fwh_ref_desc_free(&H.d);
+  free(H.xoff); free(H.eoff); free(H.hist);
+  free(H.x); free(H.B); free(H.E); free(H.J);
+  memset(&H, 0, sizeof(H));
+}
+
+static void hook_fail(const char *who, int rc) {
+  /* emessage(code, abort, str) -> COM_Exit (lib/std/errormessage.h:13) */
+  fprintf(stdout, "ERROR: %s (%d): %s\n", who, rc, fwh_last_error());
+  fflush(stdout);
+  exit(1);
+}
+
+static void hook_check(const char *who, int rc) {
+  if (rc) hook_fail(who, rc);
+}
+
+/* the reference's hddm_sw: the library's own reference to it when the
+ * process defines it at load, else a symbol of the process (a driver
+ * loaded later with global symbols) */
+static HDDM_SW *ref_sw(const char *who) {
+  HDDM_SW *sw = &hddm_sw ? &hddm_sw : (HDDM_SW *)dlsym(RTLD_DEFAULT, "hddm_sw");
+  if (!sw) {
+    fprintf(stdout, "ERROR: %s: the reference's hddm_sw is not a symbol of this process\n", who);
+    fflush(stdout);
+    exit(1);
+  }
+  return sw;
+}
+
+static int hook_part(const PartMesh *pm, const char *who) {
+  const long p = H.pm ? (long)(pm - H.pm) : -1;
+  if (!H.ctx || p < 0 || p >= H.npart) {
+    fprintf(stdout, "ERROR: %s: not a part of the model general_func_1st loaded\n", who);
+    fflush(stdout);
+    exit(1);
+  }
+  return (int)p;
+}
+
+/* general_func_1st (hddm_solver.h:77): the whole model onto the GPU, as
+ * hddm_solver_CS_hip loads it */
+static void hk_general_1st(PartMesh *pm, HDDM_VEC *v, OPTIONS options, HDDM_Opt ho, SolvOpt so) {
+  (void)v; (void)options; (void)ho; (void)so;
+  hooks_release();
+  hook_check("general_func_1st", fwh_ref_desc_from_partmesh(pm, &H.d));
+  H.have_desc = 1;
+  hook_check("general_func_1st", fwh_create(&H.ctx, device_from_env()));
+  hook_check("general_func_1st", fwh_load(H.ctx, &H.d));
+  fwh_info info;
+  hook_check("general_func_1st", fwh_get_info(H.ctx, &info));
+  H.pm = pm;
+  H.npart = H.d.npart;
+  H.n_iface = info.n_iface;
+  H.xoff = (long *)zalloc(H.npart + 1, sizeof(long));
+  H.eoff = (long *)zalloc(H.npart + 1, sizeof(long));
+  if (!H.xoff || !H.eoff) hook_fail("general_func_1st", ref_fail(FWH_E_ARG, "out of host memory"));
+  for (int p = 0; p < H.npart; p++) {
+    long nx = 0, ne = 0;
+    for (int k = 0; k < pm[p].ndomain; k++) {
+      nx += pm[p].dm[k].nmp;
+      ne += pm[p].dm[k].nel;
+    }
+    H.xoff[p + 1] = H.xoff[p] + nx;
+    H.eoff[p + 1] = H.eoff[p] + ne;
+  }
+}
+
+/* general_func_5th (hddm_solver.h:157): the GPU released once the results
+ * are in the DomMesh structs */
+static void hk_general_5th(PartMesh *pm, HDDM_VEC *v, OPTIONS options, HDDM_Opt ho, SolvOpt so) {
+  (void)pm; (void)v; (void)options; (void)ho; (void)so;
+  hooks_release();
+}
+
+static void hk_general_none(PartMesh *pm, HDDM_VEC *v, OPTIONS options, HDDM_Opt ho, SolvOpt so) {
+  (void)pm; (void)v; (void)options; (void)ho; (void)so;
+}
+
+/* make_HDDM_VEC (fullwave.c:168-174, hddm_solver.h:292-313): the part's
+ * host vectors, sized as the reference sizes them (the iterates themselves
+ * live on the GPU) */
+static void hk_make_vec(HDDM_VEC *v, PartMesh pm, HDDM_Opt ho) {
+  v->infree = pm.infree;
+  v->outfree = pm.outfree;
+  v->tfree = pm.infree + pm.outfree;
+  v->total_free = pm.total_free > 0 ? pm.total_free : H.n_iface;
+  v->p_c = (Complex *)ref_calloc(v->tfree, sizeof(Complex));
+  v->q_c = (Complex *)ref_calloc(v->tfree, sizeof(Complex));
+  v->r_c = (Complex *)ref_calloc(v->infree, sizeof(Complex));
+  v->z_c = (Complex *)ref_calloc(v->infree, sizeof(Complex));
+  v->u_c = (Complex *)ref_calloc(v->infree, sizeof(Complex));
+  if (ho.hddm_pc == REF_HDDM_PC_DIAG) {
+    v->pc_size = v->tfree;
+    v->pc_c = (Complex *)ref_calloc(v->tfree, sizeof(Complex));
+  }
+}
+
+/* the per-subdomain hooks of the loops: their work is queued for the
+ * iteration hooks, which run it for every subdomain at once */
+static void hk_dom_none(DomMesh *dm) { (void)dm; }
+static void hk_make_tmp_aij(DomMesh *dm, MTRL mtrl) { (void)dm; (void)mtrl; }
+static void hk_renewal(DomMesh *dm, HDDM_VEC v) { (void)dm; (void)v; }
+static void hk_solve_dom_1st(DomMesh *dm, HDDM_Opt ho, SolvOpt so, fwh_ref_dom_solver_fn ds, HDDM_DOM_VEC *dv,
+                             fwh_ref_make_pc_fn mk, FILE *fp) {
+  (void)dm; (void)ho; (void)so; (void)ds; (void)dv; (void)mk; (void)fp;
+}
+static void hk_solve_dom(DomMesh *dm, SolvOpt so, fwh_ref_dom_solver_fn ds, HDDM_DOM_VEC *dv, FILE *fp) {
+  (void)dm; (void)so; (void)ds; (void)dv; (void)fp;
+}
+static void hk_com_res(HDDM_VEC *v, HDDM_DOM_VEC dv, DomMesh dm, HDDM_Opt ho) {
+  (void)v; (void)dv; (void)dm; (void)ho;
+}
+static int hk_make_pc(TMP_AIJ t, HDDM_DOM_VEC *dv) { (void)t; (void)dv; return 0; }
+static int hk_set_pc(HDDM_VEC *v) { (void)v; return 0; }
+static void hk_gather_pc(OPinfo **oop, HDDM_VEC *v) { (void)oop; (void)v; }
+
+/* iteration_s_1st (hddm_solver_s.h:158-256): the first step of every
+ * subdomain and the interface residual r0; status ITERATION, or LAST when
+ * ||r0|| < NEARLY_ZERO */
+static void hk_iteration_1st(PartMesh *pm, HDDM_VEC *v, HDDM_Opt ho, fwh_ref_set_pc_fn setPC,
+                             void (*gatherPC_s)(OPinfo **oop, HDDM_VEC *hddm_vec)) {
+  (void)v; (void)setPC; (void)gatherPC_s;
+  (void)hook_part(pm, "iteration_s_1st");
+  HDDM_SW *sw = ref_sw("iteration_s_1st");
+  memset(&H.o, 0, sizeof(H.o));
+  H.o.hddm_eps = ho.hddm_eps;
+  H.o.hddm_max_loop = ho.hddm_max_loop;
+  H.o.plain = 0;
+  H.o.graph_iters = HOOK_GRAPH;
+  H.o.schur = 1;
+  int done = 0;
+  fwh_stats st;
+  hook_check("iteration_s_1st", fwh_first(H.ctx, &H.o));
+  hook_check("iteration_s_1st", fwh_iterate(H.ctx, &H.o, 0, &done));
+  hook_check("iteration_s_1st", fwh_get_stats(H.ctx, &st));
+  if (ho.hddm_log == REF_NEED && ho.fp) fprintf(ho.fp, "%d %le  # Norm %le\n", 0, 1.0, st.r0);
+  H.calls = 0;
+  H.run = 0;
+  H.done = done;
+  H.k_stop = 0;
+  sw->status = done ? REF_HDDM_SW_LAST : REF_HDDM_SW_ITERATION;
+}
+
+/* iteration_s (hddm_solver_s.h:260-400), the k-th call of the loop: the
+ * GPU runs ahead in graphs of HOOK_GRAPH iterations (its status machine
+ * makes the iterations after the stop no-ops); the call logs the k-th
+ * residual and sets LAST at the loop where the GPU stopped (convergence,
+ * or hddm_max_loop, where the reference's loop sets it too) */
+static void hk_iteration(PartMesh *pm, HDDM_VEC *v, HDDM_Opt ho, fwh_ref_set_pc_fn setPC) {
+  (void)v; (void)setPC;
+  (void)hook_part(pm, "iteration_s");
+  HDDM_SW *sw = ref_sw("iteration_s");
+  const int k = ++H.calls;
+  if (!H.done && k > H.run) {
+    int done = 0;
+    fwh_stats st;
+    hook_check("iteration_s", fwh_iterate(H.ctx, &H.o, HOOK_GRAPH, &done));
+    H.run += HOOK_GRAPH;
+    hook_check("iteration_s", fwh_get_stats(H.ctx, &st));
+    if (st.loops > H.hist_n) {
+      double *h = (double *)realloc(H.hist, sizeof(double) * (size_t)st.loops);
+      if (!h) hook_fail("iteration_s", ref_fail(FWH_E_ARG, "out of host memory"));
+      H.hist = h;
+      hook_check("iteration_s", fwh_get_resid(H.ctx, H.hist, st.loops));
+      H.hist_n = st.loops;
+    }
+    if (done) {
+      H.done = 1;
+      H.k_stop = st.loops - 1;
+    }
+  }
+  if (ho.hddm_log == REF_NEED && ho.fp && k < H.hist_n) fprintf(ho.fp, "%d %le\n", k, H.hist[k]);
+  if (H.done && k >= H.k_stop) sw->status = REF_HDDM_SW_LAST;
+}
+
+/* get_result_sp (fullwave.c:500-514): after the last step, dm->x_c of the
+ * part's subdomains */
+static void hk_get_result(PartMesh *pm) {
+  const int p = hook_part(pm, "get_result_sp");
+  if (!H.x) {
+    fwh_info info;
+    hook_check("get_result_sp", fwh_last(H.ctx));
+    hook_check("get_result_sp", fwh_get_info(H.ctx, &info));
+    H.x = (double *)zalloc(2 * (size_t)info.n_dof, sizeof(double));
+    if (!H.x) hook_fail("get_result_sp", ref_fail(FWH_E_ARG, "out of host memory"));
+    hook_check("get_result_sp", fwh_get_x(H.ctx, H.x));
+  }
+  long xo = H.xoff[p];
+  for (int k = 0; k < pm->ndomain; k++) {
+    DomMesh *dm = &pm->dm[k];
+    dm->x_c = (Complex *)ref_calloc((size_t)dm->nmp, sizeof(Complex));
+    if (!dm->x_c) hook_fail("get_result_sp", ref_fail(FWH_E_ARG, "out of host memory"));
+    memcpy(dm->x_c, H.x + 2 * xo, (size_t)dm->nmp * sizeof(Complex));
+    xo += dm->nmp;
+  }
+}
+
+/* post_sp (fullwave.c:607-635): dm->rvalue.el_B_c / el_E_c / el_Je_c of
+ * the part's elements */
+static void hk_post(PartMesh *pm) {
+  const int p = hook_part(pm, "post_sp");
+  if (!H.x) hook_fail("post_sp", ref_fail(FWH_E_STATE, "post_sp before get_result_sp"));
+  if (!H.B) {
+    fwh_info info;
+    hook_check("post_sp", fwh_get_info(H.ctx, &info));
+    const size_t ne = (size_t)info.n_elem;
+    double *nu = (double *)zalloc(ne, sizeof(double));
+    H.B = (double *)zalloc(6 * ne, sizeof(double));
+    H.E = (double *)zalloc(6 * ne, sizeof(double));
+    H.J = (double *)zalloc(6 * ne, sizeof(double));
+    if (!nu || !H.B || !H.E || !H.J) {
+      free(nu);
+      hook_fail("post_sp", ref_fail(FWH_E_ARG, "out of host memory"));
+    }
+    long el = 0;
+    for (int q = 0; q < H.npart; q++)
+      for (int k = 0; k < H.pm[q].ndomain; k++)
+        for (int e = 0; e < H.pm[q].dm[k].nel; e++, el++)
+          nu[el] = has_flag(&H.pm[q].mtrl, H.pm[q].dm[k].flag[e]) ? REF_NU : 0.0;
+    const int rc = fwh_post(H.ctx, nu, H.B, H.E, H.J);
+    free(nu);
+    hook_check("post_sp", rc);
+  }
+  long eo = H.eoff[p];
+  for (int k = 0; k < pm->ndomain; k++) {
+    DomMesh *dm = &pm->dm[k];
+    const size_t ne = (size_t)dm->nel;
+    dm->rvalue.el_B_c = (Complex *)ref_calloc(3 * ne, sizeof(Complex));
+    dm->rvalue.el_E_c = (Complex *)ref_calloc(3 * ne, sizeof(Complex));
+    dm->rvalue.el_Je_c = (Complex *)ref_calloc(3 * ne, sizeof(Complex));
+    if (!dm->rvalue.el_B_c || !dm->rvalue.el_E_c || !dm->rvalue.el_Je_c)
+      hook_fail("post_sp", ref_fail(FWH_E_ARG, "out of host memory"));
+    memcpy(dm->rvalue.el_B_c, H.B + 6 * eo, 3 * ne * sizeof(Complex));
+    memcpy(dm->rvalue.el_E_c, H.E + 6 * eo, 3 * ne * sizeof(Complex));
+    memcpy(dm->rvalue.el_Je_c, H.J + 6 * eo, 3 * ne * sizeof(Complex));
+    eo += (long)ne;
+  }
+}
+
+/* alone (not linked into the reference): options copied, no result files */
+static void hk_set_hddm_opt(HDDM_Opt *ho, OPTIONS options) { *ho = options.hddm_opt; }
+static void hk_set_solvopt(SolvOpt *so, OPTIONS options) { *so = options.solvopt; }
+static void hk_out_result(int t_loop, PartMesh pm, OPTIONS options) { (void)t_loop; (void)pm; (void)options; }
+
+void set_hddm_func_fullwave_hip(HDDM_FUNC *f, OPTIONS options) {
+  if (set_hddm_func_fullwave) {
+    set_hddm_func_fullwave(f, options);
+  } else {
+    memset(f, 0, sizeof(*f));
+    f->set_HDDM_Opt = hk_set_hddm_opt;
+    f->set_SolvOpt = hk_set_solvopt;
+    f->free_TMP_AIJ = hk_dom_none;
+    f->out_result = hk_out_result;
+    f->general_func_2nd = hk_general_none;
+    f->general_func_3rd = hk_general_none;
+    f->general_func_4th = hk_general_none;
+  }
+  f->general_func_1st = hk_general_1st;
+  f->general_func_5th = hk_general_5th;
+  f->make_HDDM_VEC = hk_make_vec;
+  f->init_bc = hk_dom_none;
+  f->make_TMP_AIJ = hk_make_tmp_aij;
+  f->renewal_bc_sp = hk_renewal;
+  f->dom_solver = mymt_CS_AIJ_Solver_COCG;
+  f->dom_solver_nmpcm = mymt_CS_AIJ_Solver_COCG_nmpcm;
+  f->makePC = hk_make_pc;
+  f->setPC = hk_set_pc;
+  f->gatherPC_s = hk_gather_pc;
+  f->solve_dom_1st = hk_solve_dom_1st;
+  f->solve_dom = hk_solve_dom;
+  f->solve_dom_last = hk_solve_dom;
+  f->COM_res_sp_1st = hk_com_res;
+  f->COM_res_sp = hk_com_res;
+  f->iteration_s_1st = hk_iteration_1st;
+  f->iteration_s = hk_iteration;
+  f->get_result_sp = hk_get_result;
+  f->post_sp = hk_post;
 }

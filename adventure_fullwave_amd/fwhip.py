@@ -33,7 +33,7 @@ EXPORTS = ("fwh_create", "fwh_destroy", "fwh_last_error", "fwh_load", "fwh_get_i
 # the reference-shaped call surface (include/fwh_ref.h)
 REF_EXPORTS = ("hddm_solver_CS_hip", "hddm_solver_CS_hip_status", "fwh_ref_desc_from_partmesh",
                "fwh_ref_desc_free", "advlas_skyline_ldl_decomposite", "advlas_skyline_ldl_solve",
-               "mymt_CS_AIJ_Solver_COCG", "mymt_CS_AIJ_Solver_COCG_nmpcm")
+               "mymt_CS_AIJ_Solver_COCG", "mymt_CS_AIJ_Solver_COCG_nmpcm", "set_hddm_func_fullwave_hip")
 
 
 class FwhOpts(ctypes.Structure):

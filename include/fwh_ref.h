@@ -385,6 +385,100 @@ typedef struct {
   int sum_ninbd;
 } PartMesh;
 
+/* ---- lib/std/types.h:566-608: a part's interface vectors --------------- */
+typedef struct {
+  int infree;
+  int outfree;
+  int tfree;
+  int total_free;
+  double *p_d;
+  Complex *p_c;
+  double *q_d;
+  Complex *q_c;
+  double *r_d;
+  Complex *r_c;
+  double *z_d;
+  Complex *z_c;
+  double *u_d;
+  Complex *u_c;
+  double r0;
+  double rz_d;
+  Complex rz_c;
+  int pc_size;
+  double *pc_d;
+  Complex *pc_c;
+} HDDM_VEC;
+
+/* ---- lib/std/types.h:611-626: a subdomain's contribution --------------- */
+typedef struct {
+  int nf;
+  double *vec_d;
+  Complex *vec_c;
+  int pc_size;
+  double *pc_d;
+  Complex *pc_c;
+} HDDM_DOM_VEC;
+
+/* ---- lib/std/types.h:629-730: the hook table --------------------------- */
+typedef int (*fwh_ref_dom_solver_fn)(AIJ_MAT *aij_mat, SolvOpt solvopt, FILE *monitor);
+typedef int (*fwh_ref_make_pc_fn)(TMP_AIJ tmp_aij, HDDM_DOM_VEC *hddm_dom_vec);
+typedef int (*fwh_ref_set_pc_fn)(HDDM_VEC *hddm_vec);
+typedef struct {
+  void (*set_HDDM_Opt)(HDDM_Opt *hddm_opt, OPTIONS options);
+  void (*set_SolvOpt)(SolvOpt *solvopt, OPTIONS options);
+  void (*make_HDDM_VEC)(HDDM_VEC *hddm_vec, PartMesh pm, HDDM_Opt hddm_opt);
+  void (*init_bc)(DomMesh *dm);
+  void (*renewal_bc_sp)(DomMesh *dm, HDDM_VEC hddm_vec);
+  void (*renewal_bc_h_send_dom)(DomMesh dm, HDDM_VEC hddm_vec, int compid, int type);
+  void (*renewal_bc_h_recv_dom)(DomMesh *dm, int compid, int type);
+  void (*make_TMP_AIJ)(DomMesh *dm, MTRL mtrl);
+  void (*free_TMP_AIJ)(DomMesh *dm);
+  fwh_ref_dom_solver_fn dom_solver;
+  fwh_ref_dom_solver_fn dom_solver_nmpcm;
+  fwh_ref_make_pc_fn makePC;
+  fwh_ref_set_pc_fn setPC;
+  void (*gatherPC_s)(OPinfo **oop, HDDM_VEC *hddm_vec);
+  void (*gatherPC_ph)(OPinfo *op, HDDM_VEC *hddm_vec);
+  void (*solve_dom_1st)(DomMesh *dm, HDDM_Opt hddm_opt, SolvOpt solvopt, fwh_ref_dom_solver_fn dom_solver,
+                        HDDM_DOM_VEC *hddm_dom_vec, fwh_ref_make_pc_fn makePC, FILE *fp_log);
+  void (*solve_dom)(DomMesh *dm, SolvOpt solvopt, fwh_ref_dom_solver_fn dom_solver_nmpcm,
+                    HDDM_DOM_VEC *hddm_dom_vec, FILE *fp_log);
+  void (*solve_dom_last)(DomMesh *dm, SolvOpt solvopt, fwh_ref_dom_solver_fn dom_solver_nmpcm,
+                         HDDM_DOM_VEC *hddm_dom_vec, FILE *fp_log);
+  void (*COM_res_sp_1st)(HDDM_VEC *hddm_vec, HDDM_DOM_VEC hddm_dom_vec, DomMesh dm, HDDM_Opt hddm_opt);
+  void (*COM_res_sp)(HDDM_VEC *hddm_vec, HDDM_DOM_VEC hddm_dom_vec, DomMesh dm, HDDM_Opt hddm_opt);
+  void (*COM_res_h_recv_1st)(HDDM_VEC *hddm_vec, DomMesh dm, HDDM_Opt hddm_opt, int compid, int type);
+  void (*COM_res_h_recv)(HDDM_VEC *hddm_vec, DomMesh dm, HDDM_Opt hddm_opt, int compid, int type);
+  void (*iteration_s_1st)(PartMesh *pm, HDDM_VEC *hddm_vec, HDDM_Opt hddm_opt, fwh_ref_set_pc_fn setPC,
+                          void (*gatherPC_s)(OPinfo **oop, HDDM_VEC *hddm_vec));
+  void (*iteration_s)(PartMesh *pm, HDDM_VEC *hddm_vec, HDDM_Opt hddm_opt, fwh_ref_set_pc_fn setPC);
+  void (*iteration_ph_1st)(PartMesh pm, HDDM_VEC *hddm_vec, HDDM_Opt hddm_opt, fwh_ref_set_pc_fn setPC,
+                           void (*gatherPC_ph)(OPinfo *op, HDDM_VEC *hddm_vec));
+  void (*iteration_ph)(PartMesh pm, HDDM_VEC *hddm_vec, HDDM_Opt hddm_opt, fwh_ref_set_pc_fn setPC);
+  void (*get_result_sp)(PartMesh *pm);
+  void (*post_sp)(PartMesh *pm);
+  void (*post_h_recv_res)(DomMesh *dm, int compid, int type);
+  void (*post_h_send_res)(DomMesh *dm, MTRL mtrl, int compid, int type);
+  void (*out_result)(int t_loop, PartMesh pm, OPTIONS options);
+  void (*general_func_1st)(PartMesh *pm, HDDM_VEC *hddm_vec, OPTIONS options, HDDM_Opt hddm_opt, SolvOpt solvopt);
+  void (*general_func_2nd)(PartMesh *pm, HDDM_VEC *hddm_vec, OPTIONS options, HDDM_Opt hddm_opt, SolvOpt solvopt);
+  void (*general_func_3rd)(PartMesh *pm, HDDM_VEC *hddm_vec, OPTIONS options, HDDM_Opt hddm_opt, SolvOpt solvopt);
+  void (*general_func_4th)(PartMesh *pm, HDDM_VEC *hddm_vec, OPTIONS options, HDDM_Opt hddm_opt, SolvOpt solvopt);
+  void (*general_func_5th)(PartMesh *pm, HDDM_VEC *hddm_vec, OPTIONS options, HDDM_Opt hddm_opt, SolvOpt solvopt);
+} HDDM_FUNC;
+
+/* ---- lib/std/types.h:732-741: the solver's state (global hddm_sw,
+ * lib/hddm/make_PartMesh.c:16) -------------------------------------------- */
+typedef struct {
+  int status;
+  int loop;
+  int ndomain;
+  int antenna;
+  int Jo;
+  int nl_loop;
+  int t_loop;
+} HDDM_SW;
+
 #endif /* FWH_USE_REFERENCE_TYPES */
 
 /* ---- entry points ------------------------------------------------------ */
@@ -411,6 +505,32 @@ typedef struct {
  * instead (0 or an FWH_E_* code, message in fwh_last_error()).            */
 void hddm_solver_CS_hip(PartMesh *pm, OPTIONS options);
 int  hddm_solver_CS_hip_status(PartMesh *pm, OPTIONS options, fwh_stats *stats);
+
+/* set_hddm_func_fullwave (main/fullwave/fullwave.h:17, fullwave.c:23-146)
+ * for the reference's own driver: hddm_solver_CS(pm, options,
+ * set_hddm_func_fullwave_hip) runs the reference's parent loop
+ * (lib/hddm/hddm_solver.h:59-183, hddm_solver_s.h:7-140) over these hooks,
+ * which batch the work on the GPU: general_func_1st loads the whole model
+ * (as hddm_solver_CS_hip does), the per-subdomain hooks of each loop
+ * (init_bc, make_TMP_AIJ, renewal_bc_sp, solve_dom_1st / solve_dom /
+ * solve_dom_last, COM_res_sp_1st / COM_res_sp) only queue, iteration_s_1st
+ * runs the first step and iteration_s the interface iterations (ahead, in
+ * hipGraphs of 16, as many as the loop asks for), setting hddm_sw.status to
+ * __HDDM_SW_LAST__ (2) at the loop where the GPU's stop test fired, so the
+ * loop runs the reference's iteration count; get_result_sp and post_sp fill
+ * dm->x_c and dm->rvalue.el_B_c / el_E_c / el_Je_c of the part, and
+ * general_func_5th releases the GPU.  make_HDDM_VEC allocates the part's
+ * host vectors (they stay zero: the iterates live on the GPU).  The
+ * subdomain solver hooks are mymt_CS_AIJ_Solver_COCG / _nmpcm of this
+ * library.  Linked into the reference, the hooks this table does not
+ * replace (set_HDDM_Opt, set_SolvOpt, out_result, free_TMP_AIJ and the
+ * p/h-mode ones) are the reference's own, taken from its
+ * set_hddm_func_fullwave; alone, set_HDDM_Opt / set_SolvOpt copy
+ * options.hddm_opt / options.solvopt and the rest do nothing.  hddm_sw is
+ * the reference's global (found as a symbol of the process).  One model
+ * at a time per process; errors print a message and exit, as emessage
+ * does.                                                                   */
+void set_hddm_func_fullwave_hip(HDDM_FUNC *hddm_func, OPTIONS options);
 
 /* The flat description libfwhip loads, built from the reference's structs
  * (no reference counterpart: the translation hddm_solver_CS_hip uses).

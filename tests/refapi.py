@@ -23,7 +23,8 @@ def lib():
     if _LIB is None:
         from adventure_fullwave_amd import fwhip
         fwhip.lib()  # the same libfwhip.so instance first
-        L = ctypes.CDLL(os.path.join(HERE, "librefdrv.so"))
+        # global symbols: the library's hook table finds the driver's hddm_sw
+        L = ctypes.CDLL(os.path.join(HERE, "librefdrv.so"), mode=ctypes.RTLD_GLOBAL)
         P = ctypes.c_void_p
         L.refdrv_desc.argtypes = [P, P, P, P]
         L.refdrv_desc.restype = ctypes.c_int
@@ -38,6 +39,8 @@ def lib():
         L.refdrv_mymt_cocg_nmpcm.argtypes = [ctypes.c_int, P, P, P, P, P, P, P, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_double, P, P, P]
         L.refdrv_mymt_cocg_nmpcm.restype = ctypes.c_int
+        L.refdrv_solve_hooks.argtypes = [P, P, P, ctypes.c_double, ctypes.c_int, ctypes.c_char_p, P, P, P, P, P]
+        L.refdrv_solve_hooks.restype = None
         _LIB = L
     return _LIB
 
@@ -116,6 +119,22 @@ def solve(model, eps=1e-8, max_loop=100000, logpath=None):
                             None if logpath is None else logpath.encode(), x.ctypes.data, B.ctypes.data,
                             E.ctypes.data, J.ctypes.data, ctypes.byref(st))
     return rc, st.as_dict(), x, B, E, J
+
+
+def solve_hooks(model, eps=1e-8, max_loop=100000, logpath=None):
+    """the reference's driver loop over set_hddm_func_fullwave_hip on the
+    reference-shaped structs of `model`: (main-loop iterations, x, B, E, J)"""
+    mats = Materials(model)
+    d = model.desc()
+    ndof = int(np.sum(model.nmp))
+    nel = int(model.el_off[-1])
+    x = np.zeros(ndof, np.complex128)
+    B, E, J = (np.zeros((nel, 3), np.complex128) for _ in range(3))
+    iters = ctypes.c_int(0)
+    lib().refdrv_solve_hooks(ctypes.byref(d), mats.flag.ctypes.data, ctypes.byref(mats.m), float(eps),
+                             int(max_loop), None if logpath is None else logpath.encode(), x.ctypes.data,
+                             B.ctypes.data, E.ctypes.data, J.ctypes.data, ctypes.byref(iters))
+    return iters.value, x, B, E, J
 
 
 def advlas(idx1, idx2, mat, rhs):

@@ -9,8 +9,10 @@
  * interface counts; per subdomain a DomMesh with nop / flag / crd / nmp,
  * dbc_Axn0 = [essential edges | interface edges], Inbc entries and one
  * ANTENNA per antenna definition (enf: local vertex -> antenna node, Jo /
- * Jor per node).  Then calls hddm_solver_CS_hip_status and copies DomMesh.x_c
- * and DomMesh.rvalue back out.
+ * Jor per node).  Then calls hddm_solver_CS_hip_status, or runs the
+ * reference's driver loop over set_hddm_func_fullwave_hip, and copies
+ * DomMesh.x_c and DomMesh.rvalue back out.  Loaded with global symbols:
+ * it defines the reference's hddm_sw, which the hook table sets.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -185,6 +187,133 @@ int refdrv_solve(const fwh_mesh_desc *f, const int *flag, const refdrv_mtrl *mt,
   }
   free_parts(pm, f->npart);
   return rc;
+}
+
+/* the reference's solver state, as lib/hddm/make_PartMesh.c:16 defines it
+ * (the hook table's iteration hooks set its status) */
+HDDM_SW hddm_sw;
+
+#define SW_FIRST 0 /* lib/hddm/libhddm.h:37-39 */
+#define SW_LAST 2
+
+static void free_vec(HDDM_VEC *v) {
+  free(v->p_c); free(v->q_c); free(v->r_c); free(v->z_c); free(v->u_c); free(v->pc_c);
+  memset(v, 0, sizeof(*v));
+}
+
+static void free_dom_vec(HDDM_DOM_VEC *v) {
+  free(v->vec_c); free(v->pc_c);
+  memset(v, 0, sizeof(*v));
+}
+
+/* the reference's s-mode parent loop over a hook table, restated for the
+ * test (lib/hddm/hddm_solver.h:59-183 with the loops of
+ * lib/hddm/hddm_solver_s.h:7-140): every part of pm in one process; the
+ * interface count is logged as the whole-call entry point logs it.
+ * Returns the number of main-loop iterations.                            */
+static int parent_loop(PartMesh *pm, int npart, OPTIONS options, void (*set_fn)(HDDM_FUNC *, OPTIONS)) {
+  HDDM_FUNC f;
+  HDDM_Opt ho;
+  SolvOpt so;
+  memset(&f, 0, sizeof(f));
+  memset(&ho, 0, sizeof(ho));
+  memset(&so, 0, sizeof(so));
+  HDDM_VEC *vec = NULL;
+  hddm_sw.status = SW_FIRST;
+  hddm_sw.loop = 0;
+  set_fn(&f, options);
+  f.general_func_1st(pm, vec, options, ho, so);
+  f.set_HDDM_Opt(&ho, options);
+  f.set_SolvOpt(&so, options);
+  vec = (HDDM_VEC *)calloc(npart, sizeof(HDDM_VEC));
+  for (int p = 0; p < npart; p++) f.make_HDDM_VEC(&vec[p], pm[p], ho);
+  if (ho.hddm_log == 0 && ho.fp) fprintf(ho.fp, "# DOF on interface %d\n", vec[0].total_free);
+  f.general_func_2nd(pm, vec, options, ho, so);
+  int iters = 0;
+  if (ho.hddm_max_loop == 0) {
+    fprintf(stdout, "HDDM Solver was not executed!!!\n");
+  } else {
+    /* LOOP_1ST */
+    for (int p = 0; p < npart; p++)
+      for (int k = 0; k < pm[p].ndomain; k++) {
+        HDDM_DOM_VEC dv;
+        memset(&dv, 0, sizeof(dv));
+        f.init_bc(&pm[p].dm[k]);
+        f.make_TMP_AIJ(&pm[p].dm[k], pm[p].mtrl);
+        f.solve_dom_1st(&pm[p].dm[k], ho, so, f.dom_solver, &dv, f.makePC, so.fp);
+        f.COM_res_sp_1st(&vec[p], dv, pm[p].dm[k], ho);
+        free_dom_vec(&dv);
+      }
+    f.iteration_s_1st(pm, vec, ho, f.setPC, f.gatherPC_s);
+    hddm_sw.loop++;
+    f.general_func_3rd(pm, vec, options, ho, so);
+    /* main loop */
+    while (hddm_sw.status != SW_LAST) {
+      for (int p = 0; p < npart; p++)
+        for (int k = 0; k < pm[p].ndomain; k++) {
+          HDDM_DOM_VEC dv;
+          memset(&dv, 0, sizeof(dv));
+          f.renewal_bc_sp(&pm[p].dm[k], vec[p]);
+          f.solve_dom(&pm[p].dm[k], so, f.dom_solver_nmpcm, &dv, so.fp);
+          f.COM_res_sp(&vec[p], dv, pm[p].dm[k], ho);
+          free_dom_vec(&dv);
+        }
+      f.iteration_s(pm, vec, ho, f.setPC);
+      iters++;
+      if (hddm_sw.loop == ho.hddm_max_loop) hddm_sw.status = SW_LAST;
+      hddm_sw.loop++;
+    }
+    f.general_func_4th(pm, vec, options, ho, so);
+    /* LOOP_LAST */
+    for (int p = 0; p < npart; p++) {
+      for (int k = 0; k < pm[p].ndomain; k++) {
+        HDDM_DOM_VEC dv;
+        memset(&dv, 0, sizeof(dv));
+        f.renewal_bc_sp(&pm[p].dm[k], vec[p]);
+        f.solve_dom_last(&pm[p].dm[k], so, f.dom_solver_nmpcm, &dv, so.fp);
+        f.COM_res_sp(&vec[p], dv, pm[p].dm[k], ho);
+        free_dom_vec(&dv);
+      }
+      f.get_result_sp(&pm[p]);
+      f.post_sp(&pm[p]);
+    }
+    f.general_func_5th(pm, vec, options, ho, so);
+    for (int p = 0; p < npart; p++) f.out_result(hddm_sw.t_loop, pm[p], options);
+  }
+  for (int p = 0; p < npart; p++) free_vec(&vec[p]);
+  free(vec);
+  return iters;
+}
+
+/* the same model through the reference's driver loop over
+ * set_hddm_func_fullwave_hip: x, B/E/J as refdrv_solve returns them; the
+ * main-loop iterations in *iters                                         */
+void refdrv_solve_hooks(const fwh_mesh_desc *f, const int *flag, const refdrv_mtrl *mt, double eps, int max_loop,
+                        const char *logpath, double *x, double *B, double *E, double *J, int *iters) {
+  PartMesh *pm = make_parts(f, flag, mt);
+  OPTIONS options;
+  memset(&options, 0, sizeof(options));
+  options.hddm_opt.hddm_eps = eps;
+  options.hddm_opt.hddm_max_loop = max_loop;
+  options.hddm_opt.hddm_pc = 1;
+  options.hddm_opt.hddm_log = logpath ? 0 : -1;
+  options.hddm_opt.fp = logpath ? fopen(logpath, "w") : NULL;
+  *iters = parent_loop(pm, f->npart, options, set_hddm_func_fullwave_hip);
+  if (options.hddm_opt.fp) fclose(options.hddm_opt.fp);
+  if (max_loop > 0) {
+    size_t xo = 0, eo = 0;
+    for (int p = 0; p < f->npart; p++)
+      for (int k = 0; k < pm[p].ndomain; k++) {
+        const DomMesh *dm = &pm[p].dm[k];
+        memcpy(x + 2 * xo, dm->x_c, dm->nmp * sizeof(Complex));
+        memcpy(B + 6 * eo, dm->rvalue.el_B_c, 3 * (size_t)dm->nel * sizeof(Complex));
+        memcpy(E + 6 * eo, dm->rvalue.el_E_c, 3 * (size_t)dm->nel * sizeof(Complex));
+        memcpy(J + 6 * eo, dm->rvalue.el_Je_c, 3 * (size_t)dm->nel * sizeof(Complex));
+        xo += dm->nmp;
+        eo += dm->nel;
+      }
+  }
+  free_parts(pm, f->npart);
 }
 
 /* advlas_skyline_ldl_decomposite then advlas_skyline_ldl_solve on one

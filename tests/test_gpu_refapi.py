@@ -34,10 +34,11 @@ def say(msg):
 @pytest.fixture()
 def pinned(tmp_path):
     """pinned kernel choices and a working directory with MODEL/ (volt.dat)"""
-    keep = {k: os.environ.get(k) for k in ("FWH_SMV_WAVES", "FWH_SMV_SPLIT", "FWH_SOLVE_DEPTH")}
+    keep = {k: os.environ.get(k) for k in ("FWH_SMV_WAVES", "FWH_SMV_SPLIT", "FWH_SOLVE_DEPTH", "FWH_SOLVE_WAVES")}
     os.environ["FWH_SMV_WAVES"] = "4"
     os.environ["FWH_SMV_SPLIT"] = "2"
     os.environ["FWH_SOLVE_DEPTH"] = "16"
+    os.environ["FWH_SOLVE_WAVES"] = "1"
     (tmp_path / "MODEL").mkdir()
     old = os.getcwd()
     os.chdir(tmp_path)
@@ -87,6 +88,37 @@ def test_hddm_solver_cs_hip_delta_gap(small_sigma_model, pinned):
                         {"lo": np.array([c - 0.6 * h, c - 0.6 * h, c - 0.1 * h]),
                          "hi": np.array([c + 0.6 * h, c + 0.6 * h, c + 1.1 * h]), "dV": 2.0}])
     check_same_bits(gap, pinned)
+
+
+def check_hooks_same_bits(model, tmp_path, eps=1e-8, max_loop=20000):
+    """the reference's driver loop over set_hddm_func_fullwave_hip against
+    the whole-call entry point: the same main-loop count, solution, element
+    fields and residual log"""
+    from adventure_fullwave_amd import fwhip
+    log_call, log_hooks = str(tmp_path / "calc_log_call"), str(tmp_path / "calc_log_hooks")
+    rc, st, x, B, E, J = refapi.solve(model, eps=eps, max_loop=max_loop, logpath=log_call)
+    assert rc == 0, fwhip.lib().fwh_last_error()
+    iters, xh, Bh, Eh, Jh = refapi.solve_hooks(model, eps=eps, max_loop=max_loop, logpath=log_hooks)
+    say(f"[gpu] {model.name} (max_loop {max_loop}): hook table {iters} loops, hddm_solver_CS_hip {st['iters']}")
+    assert iters == st["iters"]
+    assert np.array_equal(x, xh)
+    for a, b in zip((B, E, J), (Bh, Eh, Jh)):
+        assert np.array_equal(a, b)
+    assert open(log_call).read() == open(log_hooks).read()
+
+
+def test_hook_table_same_bits(small_sigma_model, cfg1_sigma_model, pinned):
+    """set_hddm_func_fullwave_hip under the reference's parent loop
+    (hddm_solver.h:59-183, restated in tests/refdrv.c): the per-subdomain
+    hooks queue, the iteration hooks run the GPU ahead in graphs and set
+    hddm_sw.status to LAST at the loop where the GPU's stop test fired; the
+    results equal hddm_solver_CS_hip's bit for bit, for a converged solve
+    (small model, cfg 1) and for one stopped by hddm_max_loop at a loop
+    that is not a multiple of the graph length."""
+    write_volt(pinned, [])
+    for model in (small_sigma_model, cfg1_sigma_model):
+        check_hooks_same_bits(model, pinned)
+    check_hooks_same_bits(small_sigma_model, pinned, max_loop=37)
 
 
 def test_hddm_solver_cs_hip_not_executed(small_sigma_model, pinned):
