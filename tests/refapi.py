@@ -41,6 +41,8 @@ def lib():
         L.refdrv_mymt_cocg_nmpcm.restype = ctypes.c_int
         L.refdrv_solve_hooks.argtypes = [P, P, P, ctypes.c_double, ctypes.c_int, ctypes.c_char_p, P, P, P, P, P]
         L.refdrv_solve_hooks.restype = None
+        L.refdrv_hooks_null.argtypes = [P]
+        L.refdrv_hooks_null.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -181,3 +183,11 @@ def mymt_cocg_nmpcm(rowptr, col, val, b1, b2, pc=3, eps=1e-9, ic_shift=1.2):
                                        x1.ctypes.data, bb2.ctypes.data, x2.ctypes.data, int(pc), float(eps),
                                        float(ic_shift), pcm.ctypes.data, ctypes.byref(size), ctypes.byref(st2))
     return st1, x1, st2.value, x2, (pcm if size.value else None)
+
+
+def hooks_null():
+    """set_hddm_func_fullwave_hip alone (no GPU call): (s-mode hooks left
+    NULL, whether set_HDDM_Opt / set_SolvOpt copy the options)"""
+    ok = ctypes.c_int(0)
+    n = lib().refdrv_hooks_null(ctypes.byref(ok))
+    return n, bool(ok.value)

@@ -285,6 +285,41 @@ static int parent_loop(PartMesh *pm, int npart, OPTIONS options, void (*set_fn)(
   return iters;
 }
 
+/* the hooks the s-mode parent loop calls (hddm_solver.h:59-183,
+ * hddm_solver_s.h:7-140), as set_hddm_func_fullwave_hip fills them alone
+ * (no GPU call): the number left NULL, and set_HDDM_Opt / set_SolvOpt
+ * copying the options (1 when they do)                                    */
+int refdrv_hooks_null(int *opts_copied) {
+  HDDM_FUNC f;
+  OPTIONS options;
+  memset(&f, 0, sizeof(f));
+  memset(&options, 0, sizeof(options));
+  options.hddm_opt.hddm_eps = 1e-8;
+  options.hddm_opt.hddm_max_loop = 123;
+  options.solvopt.solver_eps = 1e-9;
+  set_hddm_func_fullwave_hip(&f, options);
+  const void *used[] = {
+      (const void *)f.set_HDDM_Opt, (const void *)f.set_SolvOpt, (const void *)f.make_HDDM_VEC,
+      (const void *)f.init_bc, (const void *)f.renewal_bc_sp, (const void *)f.make_TMP_AIJ,
+      (const void *)f.free_TMP_AIJ, (const void *)f.dom_solver, (const void *)f.dom_solver_nmpcm,
+      (const void *)f.makePC, (const void *)f.setPC, (const void *)f.gatherPC_s, (const void *)f.solve_dom_1st,
+      (const void *)f.solve_dom, (const void *)f.solve_dom_last, (const void *)f.COM_res_sp_1st,
+      (const void *)f.COM_res_sp, (const void *)f.iteration_s_1st, (const void *)f.iteration_s,
+      (const void *)f.get_result_sp, (const void *)f.post_sp, (const void *)f.out_result,
+      (const void *)f.general_func_1st, (const void *)f.general_func_2nd, (const void *)f.general_func_3rd,
+      (const void *)f.general_func_4th, (const void *)f.general_func_5th};
+  int nnull = 0;
+  for (size_t i = 0; i < sizeof(used) / sizeof(used[0]); i++) nnull += used[i] == NULL;
+  HDDM_Opt ho;
+  SolvOpt so;
+  memset(&ho, 0, sizeof(ho));
+  memset(&so, 0, sizeof(so));
+  if (f.set_HDDM_Opt) f.set_HDDM_Opt(&ho, options);
+  if (f.set_SolvOpt) f.set_SolvOpt(&so, options);
+  *opts_copied = ho.hddm_eps == 1e-8 && ho.hddm_max_loop == 123 && so.solver_eps == 1e-9;
+  return nnull;
+}
+
 /* the same model through the reference's driver loop over
  * set_hddm_func_fullwave_hip: x, B/E/J as refdrv_solve returns them; the
  * main-loop iterations in *iters                                         */

@@ -82,3 +82,13 @@ def test_translation_refuses_missing_volt_dat(small_sigma_model, tmp_path):
             refapi.translated_desc(small_sigma_model)
     finally:
         os.chdir(old)
+
+
+def test_hook_table_fills_every_s_mode_hook():
+    """set_hddm_func_fullwave_hip, not linked into the reference, fills every
+    hook the reference's s-mode parent loop calls (hddm_solver.h:59-183,
+    hddm_solver_s.h:7-140) and its option hooks copy the options (no GPU
+    call: the GPU work starts in general_func_1st)"""
+    nnull, copied = refapi.hooks_null()
+    assert nnull == 0
+    assert copied
