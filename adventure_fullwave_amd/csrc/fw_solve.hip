@@ -342,9 +342,13 @@ __device__ __forceinline__ void load_meta(const SolveArgs &a, const SubMeta &m, 
 }
 
 // the load vector (first/last step: f_I - A_IG v_G; iteration: -A_IG p_G;
-// unit vector; right-hand side) into v
+// unit vector; right-hand side) into v.  pvs: LDS for cap interface values
+// that is free until the sweeps start (the staging buffer): when the
+// subdomain's interface values fit, they are gathered there once, so the
+// coupling loop reads them from LDS instead of gathering per coupling.
 template <int NT>
-__device__ __forceinline__ void load_rhs(const SolveArgs &a, const SubMeta &m, double2 *v, int uk) {
+__device__ __forceinline__ void load_rhs(const SolveArgs &a, const SubMeta &m, double2 *v, int uk, double2 *pvs,
+                                         int cap) {
   const bool zero_g = (a.mode == MODE_FIRST || a.mode == MODE_RHS);
   // interface values: P, or the next search direction z + beta p while the
   // interface iteration runs (formed here, not stored, see k_iter_ab)
@@ -353,6 +357,16 @@ __device__ __forceinline__ void load_rhs(const SolveArgs &a, const SubMeta &m, d
     const double2 p = a.P[gid];
     return a.Z ? zbp(a.Z[gid], beta, p) : p;
   };
+  // (uniform over the team: the same subdomain and mode)
+  const bool staged = (a.mode == MODE_ITER || a.mode == MODE_LAST) && m.ng <= cap;
+  if (staged) {
+    for (int k = team_tid<NT>(); k < m.ng; k += NT) pvs[k] = pval(a.g_gid[m.g0 + k]);
+    if (NT == WL)
+      wave_sync();  // one wave: its LDS operations complete in order
+    else
+      __syncthreads();
+  }
+  auto gval = [&](int k) -> double2 { return staged ? pvs[k] : pval(a.g_gid[m.g0 + k]); };
   for (int i = team_tid<NT>(); i < m.n; i += NT) {
     const int d = m.dof0 + i;
     double2 b;
@@ -370,14 +384,13 @@ __device__ __forceinline__ void load_rhs(const SolveArgs &a, const SubMeta &m, d
     } else {
       const int8_t fl = a.bcflag[d];
       if (fl == 1) {
-        b = zero_g ? cz() : pval(a.g_gid[m.g0 + a.kidx[d]]);
+        b = zero_g ? cz() : gval(a.kidx[d]);
       } else if (fl == 2) {
         b = cz();
       } else {
         b = (a.mode == MODE_ITER) ? cz() : a.f[d];
         if (!zero_g)
-          for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++)
-            b = cfms(b, a.cp_val[c], pval(a.g_gid[m.g0 + a.cp_k[c]]));
+          for (int c = a.cp_ptr[d]; c < a.cp_ptr[d + 1]; c++) b = cfms(b, a.cp_val[c], gval(a.cp_k[c]));
       }
     }
     v[i] = b;
@@ -446,7 +459,7 @@ __device__ __forceinline__ void solve1_body(const SolveArgs &a, const SubMeta &m
   team_sync<TEAM>();
 
   // ---- 1. load vector --------------------------------------------------------------
-  load_rhs<WL>(a, m, L.v, uk);
+  load_rhs<WL>(a, m, L.v, uk, L.stg, WL * K);  // the staging buffer is free until the forward sweep
 
   // ---- 2. forward sweep: the first two blocks go in flight once the load
   // vector's own global reads are complete ----------------------------------
@@ -693,7 +706,7 @@ __device__ __forceinline__ void solve4_body(const SolveArgs &a, const SubMeta &m
   if (done0) return;
   load_blocks<NT>(L, nblk);
   __syncthreads();
-  load_rhs<NT>(a, m, L.v, uk);
+  load_rhs<NT>(a, m, L.v, uk, L.stg, CAPQ);  // the staging buffer is free until the forward sweep
   PfSet4<KQ> pa, pb;
   issue_block4<KQ, NT>(fac, L.bt[0], minv_of(L, 0), pa);
   issue_block4<KQ, NT>(fac, L.bt[1], minv_of(L, 1), pb);
